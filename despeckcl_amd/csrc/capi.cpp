@@ -1,0 +1,289 @@
+// C ABI of libdespeckcl_amd.so: argument checks, device workspace, host<->device copies.
+#include "../../include/despeckcl_amd.h"
+
+#include "dkx_common.h"
+#include "dkx_nlinsar.h"
+#include "dkx_nlsar.h"
+
+#include <atomic>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace dkx {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+// Grow-only device workspace per device, reused across calls so that repeated calls (and
+// the benchmark's timed loop) do not allocate.  Calls on one device are serialised by the
+// mutex of that device's entry.
+struct DeviceWorkspace {
+    std::mutex mu;
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+};
+
+static std::mutex g_ws_mu;
+static std::map<int, DeviceWorkspace*> g_ws;
+
+static DeviceWorkspace* workspace_for_current_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_ws.find(dev);
+    if (it != g_ws.end()) return it->second;
+    DeviceWorkspace* w = new DeviceWorkspace();
+    g_ws[dev] = w;
+    return w;
+}
+
+static int ensure(DeviceWorkspace* w, size_t bytes)
+{
+    if (w->bytes >= bytes) return 0;
+    if (w->ptr) {
+        DKX_HIP_CHECK(hipDeviceSynchronize());
+        DKX_HIP_CHECK(hipFree(w->ptr));
+        w->ptr = nullptr;
+        w->bytes = 0;
+    }
+    hipError_t e = hipMalloc(&w->ptr, bytes);
+    if (e != hipSuccess) {
+        w->ptr = nullptr;
+        set_error("device workspace allocation of " + std::to_string(bytes) + " bytes failed: " +
+                  hipGetErrorString(e));
+        return DKX_ERESOURCE;
+    }
+    w->bytes = bytes;
+    return 0;
+}
+
+static int ensure_stream(DeviceWorkspace* w)
+{
+    if (!w->stream) DKX_HIP_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    return 0;
+}
+
+static std::atomic<int> g_tile[2] = {{0}, {0}};
+
+static int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP,
+                       const int* S, int nS, std::vector<NlsarStatsView>& views)
+{
+    // The reference looks statistics up with operator[] and builds ENL maps for every key
+    // (nlsar_filter_sub_image.cpp:224-228, 278): keys outside the patch x scale set, or
+    // missing keys, silently use empty statistics there.  Here they are rejected.
+    if (!stats || n_stats != nP * nS) {
+        set_error("nlsar: the statistics must hold one entry per (patch size, scale size) pair");
+        return DKX_ESTATS;
+    }
+    views.clear();
+    for (int i = 0; i < n_stats; i++) {
+        const dkx_nlsar_stats& s = stats[i];
+        bool known = false;
+        for (int a = 0; a < nP; a++)
+            for (int b = 0; b < nS; b++)
+                if (P[a] == s.patch_size && S[b] == s.scale_size) known = true;
+        for (int j = 0; j < i; j++)
+            if (stats[j].patch_size == s.patch_size && stats[j].scale_size == s.scale_size) known = false;
+        if (!known) {
+            set_error("nlsar: statistics key (" + std::to_string(s.patch_size) + ", " +
+                      std::to_string(s.scale_size) + ") is not in the patch x scale set");
+            return DKX_ESTATS;
+        }
+        if (s.lut_size < 2 || s.lut_size > 65536 || !s.quantilles || !s.chi2cdf_inv ||
+            !(s.dissims_max >= s.dissims_min)) {
+            set_error("nlsar: invalid statistics entry");
+            return DKX_ESTATS;
+        }
+        for (int k = 0; k < s.lut_size; k++)
+            if (!(s.quantilles[k] >= 0.0f && s.quantilles[k] < 1.0f)) {
+                set_error("nlsar: quantile LUT values must lie in [0, 1)");
+                return DKX_ESTATS;
+            }
+        views.push_back(NlsarStatsView{s.patch_size, s.scale_size, s.lut_size, s.dissims_min,
+                                       s.dissims_max, s.quantilles, s.chi2cdf_inv});
+    }
+    return 0;
+}
+
+static int nlsar_device_impl(const float* a1, const float* a2, const float* phi, float* ref,
+                             float* pha, float* coh, int height, int width, int sw, const int* P,
+                             int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                             int n_stats, float h, float c, float* enl_maps, float* alpha_maps,
+                             unsigned long long* ub, hipStream_t stream, DeviceWorkspace* w)
+{
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh || !P || !S) {
+        set_error("nlsar: null array argument");
+        return DKX_EINVAL;
+    }
+    std::vector<NlsarStatsView> views;
+    int rc = check_stats(stats, n_stats, P, nP, S, nS, views);
+    if (rc) return rc;
+    NlsarProblem pb{height, width, 0, sw, P, nP, S, nS, views.data(), (int)views.size(), h, c,
+                    -1, -1, g_tile[0].load()};
+    NlsarPlan plan;
+    rc = nlsar_plan(pb, &plan);
+    if (rc) return rc;
+    rc = ensure(w, plan.ws_bytes);
+    if (rc) return rc;
+    rc = nlsar_run(pb, plan, a1, a2, phi, nullptr, ref, pha, coh, w->ptr, enl_maps, alpha_maps,
+                   ub, stream);
+    if (rc) return rc;
+    if (ub) DKX_HIP_CHECK(hipStreamSynchronize(stream));
+    return 0;
+}
+
+static int nlinsar_device_impl(const float* a1, const float* a2, const float* phi, float* ref,
+                               float* pha, float* coh, int height, int width, int sw, int p,
+                               int niter, int lmin, unsigned long long* nsm, hipStream_t stream,
+                               DeviceWorkspace* w)
+{
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh) {
+        set_error("nlinsar: null array argument");
+        return DKX_EINVAL;
+    }
+    // nlinsar.cpp:65-78
+    float h_theo;
+    if (p == 3) h_theo = 0.58f;
+    else if (p == 7) h_theo = 0.244f;
+    else {
+        set_error("nlinsar: only patch sizes 3 and 7 are supported (the reference estimates h "
+                  "for other sizes by a non-deterministic Monte-Carlo simulation)");
+        return DKX_EUNSUPPORTED;
+    }
+    const int patch_area = p * p;
+    const float h_para = h_theo * patch_area;
+    const float T_para = (float)(2.0 / h_para * M_PI / 4 * patch_area);
+    NlinsarProblem pb{height, width, sw, p, niter, lmin, h_para, T_para, 80, g_tile[1].load()};
+    NlinsarPlan plan;
+    int rc = nlinsar_plan(pb, &plan);
+    if (rc) return rc;
+    rc = ensure(w, plan.ws_bytes);
+    if (rc) return rc;
+    return nlinsar_run(pb, plan, a1, a2, phi, ref, pha, coh, w->ptr, nsm, stream);
+}
+
+// host arrays: copy in, run on the library stream, copy out
+struct HostStage {
+    void* ptr = nullptr;
+    ~HostStage() { if (ptr) (void)hipFree(ptr); }
+};
+
+}  // namespace dkx
+
+using namespace dkx;
+
+extern "C" {
+
+const char* dkx_last_error(void) { return dkx::last_error(); }
+int dkx_version(void) { return 1; }
+
+int dkx_set_tile(int filter, int tile_index)
+{
+    if (filter < 0 || filter > 1 || tile_index < 0 || tile_index > 3) {
+        set_error("dkx_set_tile: invalid filter or tile index");
+        return DKX_EINVAL;
+    }
+    g_tile[filter].store(tile_index);
+    return 0;
+}
+
+int dkx_get_tile(int filter) { return (filter >= 0 && filter <= 1) ? g_tile[filter].load() : -1; }
+
+int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, float* ref,
+                           float* pha, float* coh, int height, int width, int sw, const int* P,
+                           int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
+                           float h, float c, float* enl_maps, float* alpha_maps,
+                           unsigned long long* ub, void* stream)
+{
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    return nlsar_device_impl(a1, a2, phi, ref, pha, coh, height, width, sw, P, nP, S, nS, stats,
+                             n_stats, h, c, enl_maps, alpha_maps, ub, (hipStream_t)stream, w);
+}
+
+int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                    float* coh, int height, int width, int sw, const int* P, int nP, const int* S,
+                    int nS, const dkx_nlsar_stats* stats, int n_stats, float h, float c)
+{
+    if (height <= 0 || width <= 0) { set_error("nlsar: empty image"); return DKX_EINVAL; }
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    int rc = ensure_stream(w);
+    if (rc) return rc;
+    const size_t n = (size_t)height * width;
+    HostStage st;
+    DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
+    float* d = (float*)st.ptr;
+    DKX_HIP_CHECK(hipMemcpyAsync(d, a1, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(d + n, a2, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(d + 2 * n, phi, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    rc = nlsar_device_impl(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width, sw,
+                           P, nP, S, nS, stats, n_stats, h, c, nullptr, nullptr, nullptr, w->stream, w);
+    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+int dkx_nlinsar_device(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                       float* coh, int height, int width, int sw, int p, int niter, int lmin,
+                       unsigned long long* nsm, void* stream)
+{
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    return nlinsar_device_impl(a1, a2, phi, ref, pha, coh, height, width, sw, p, niter, lmin, nsm,
+                               (hipStream_t)stream, w);
+}
+
+int dkx_nlinsar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                float* coh, int height, int width, int sw, int p, int niter, int lmin)
+{
+    if (height <= 0 || width <= 0) { set_error("nlinsar: empty image"); return DKX_EINVAL; }
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    int rc = ensure_stream(w);
+    if (rc) return rc;
+    const size_t n = (size_t)height * width;
+    HostStage st;
+    DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
+    float* d = (float*)st.ptr;
+    const float* src[6] = {a1, a2, phi, ref, pha, coh};
+    for (int k = 0; k < 6; k++)
+        DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    rc = nlinsar_device_impl(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width,
+                             sw, p, niter, lmin, nullptr, w->stream, w);
+    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+void dkx_release_workspace(void)
+{
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) return;
+    std::lock_guard<std::mutex> lk(w->mu);
+    if (w->ptr) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(w->ptr);
+    }
+    w->ptr = nullptr;
+    w->bytes = 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// C++ API of include/despeckcl.h implemented over the C ABI of include/despeckcl_amd.h.
+#include "../../include/despeckcl.h"
+#include "../../include/despeckcl_amd.h"
+
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+#include <stdexcept>
+
+namespace {
+
+void check(int rc)
+{
+    if (rc == DKX_OK) return;
+    const std::string msg = dkx_last_error();
+    if (rc == DKX_EINVAL || rc == DKX_ESTATS || rc == DKX_EUNSUPPORTED) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+std::vector<dkx_nlsar_stats> views(const nlsar_stats_collection& nsc)
+{
+    std::vector<dkx_nlsar_stats> v;
+    for (const auto& kv : nsc)
+        v.push_back(dkx_nlsar_stats{kv.first.patch_size, kv.first.scale_size,
+                                    (int)kv.second.lut_size, kv.second.dissims_min,
+                                    kv.second.dissims_max, kv.second.quantilles.data(),
+                                    kv.second.chi2cdf_inv.data()});
+    return v;
+}
+
+nlsar_stats_collection collect(const std::vector<dkx_nlsar_stats>& out)
+{
+    nlsar_stats_collection nsc;
+    for (const auto& s : out) {
+        nlsar::stats st;
+        st.lut_size = (unsigned int)s.lut_size;
+        st.dissims_min = s.dissims_min;
+        st.dissims_max = s.dissims_max;
+        st.quantilles.assign(s.quantilles, s.quantilles + s.lut_size);
+        st.chi2cdf_inv.assign(s.chi2cdf_inv, s.chi2cdf_inv + s.lut_size);
+        nsc.emplace(nlsar::params{s.patch_size, s.scale_size}, st);
+    }
+    return nsc;
+}
+
+}  // namespace
+
+// nlsar::stats (src/compute/nlsar/stats.cpp:28-49)
+nlsar::stats::stats() : lut_size((unsigned int)-1), dissims_min(0.f), dissims_max(0.f) {}
+
+nlsar::stats::stats(std::vector<float> dissims, unsigned int lut) : lut_size(lut)
+{
+    quantilles.resize(lut);
+    chi2cdf_inv.resize(lut);
+    const long long m = dkx_nlsar_stats_build(dissims.data(), (long long)dissims.size(), (int)lut,
+                                              &dissims_min, &dissims_max, quantilles.data(),
+                                              chi2cdf_inv.data());
+    if (m <= 0) throw std::invalid_argument("nlsar::stats: no finite dissimilarity");
+}
+
+nlsar::stats& nlsar::stats::operator=(const stats& other)
+{
+    lut_size = other.lut_size;
+    dissims_min = other.dissims_min;
+    dissims_max = other.dissims_max;
+    quantilles = other.quantilles;
+    chi2cdf_inv = other.chi2cdf_inv;
+    return *this;
+}
+
+int despeckcl::nlsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+                     float* phase_filt, float* coh_filt, const int height, const int width,
+                     const int search_window_size, const std::vector<int> patch_sizes,
+                     const std::vector<int> scale_sizes,
+                     std::map<nlsar::params, nlsar::stats> nlsar_stats, const float h_param,
+                     const float c_param, std::vector<std::string>)
+{
+    std::vector<dkx_nlsar_stats> v = views(nlsar_stats);
+    check(dkx_nlsar_insar(ampl_master, ampl_slave, phase, ref_filt, phase_filt, coh_filt, height,
+                          width, search_window_size, patch_sizes.data(), (int)patch_sizes.size(),
+                          scale_sizes.data(), (int)scale_sizes.size(), v.data(), (int)v.size(),
+                          h_param, c_param));
+    return 0;
+}
+
+int despeckcl::nlsar(float*, float*, const int, const int, const int, const int,
+                     const std::vector<int>, const std::vector<int>,
+                     std::map<nlsar::params, nlsar::stats>, const float, const float,
+                     std::vector<std::string>)
+{
+    throw std::invalid_argument("nlsar: the covariance-matrix overload is not available yet");
+}
+
+nlsar_stats_collection despeckcl::nlsar_training(float* ampl_master, float* ampl_slave,
+                                                 float* phase, const int height, const int width,
+                                                 const std::vector<int> patch_sizes,
+                                                 const std::vector<int> scale_sizes,
+                                                 std::vector<std::string>)
+{
+    const size_t n = patch_sizes.size() * scale_sizes.size();
+    std::vector<dkx_nlsar_stats> out(n);
+    std::vector<float> store(n * 2 * 256);
+    check(dkx_nlsar_training_insar(ampl_master, ampl_slave, phase, height, width,
+                                   patch_sizes.data(), (int)patch_sizes.size(), scale_sizes.data(),
+                                   (int)scale_sizes.size(), out.data(), store.data()));
+    return collect(out);
+}
+
+nlsar_stats_collection despeckcl::nlsar_training(float* covmat_raw, const int height,
+                                                 const int width, const int dim,
+                                                 const std::vector<int> patch_sizes,
+                                                 const std::vector<int> scale_sizes,
+                                                 std::vector<std::string>)
+{
+    const size_t n = patch_sizes.size() * scale_sizes.size();
+    std::vector<dkx_nlsar_stats> out(n);
+    std::vector<float> store(n * 2 * 256);
+    check(dkx_nlsar_training_covmat(covmat_raw, height, width, dim, patch_sizes.data(),
+                                    (int)patch_sizes.size(), scale_sizes.data(),
+                                    (int)scale_sizes.size(), out.data(), store.data()));
+    return collect(out);
+}
+
+// Text format: "despeckcl_amd-nlsar-stats 1", the entry count, then per entry
+// "patch scale lut_size min max" followed by the two LUTs; floats with 9 significant digits
+// so that every value reads back bit-exactly.
+void despeckcl::store_nlsar_stats_collection(nlsar_stats_collection nsc, std::string filename)
+{
+    std::ofstream f(filename);
+    if (!f) throw std::runtime_error("cannot open " + filename);
+    f << "despeckcl_amd-nlsar-stats 1\n" << nsc.size() << "\n" << std::setprecision(9);
+    for (const auto& kv : nsc) {
+        const nlsar::stats& s = kv.second;
+        f << kv.first.patch_size << " " << kv.first.scale_size << " " << s.lut_size << " "
+          << s.dissims_min << " " << s.dissims_max << "\n";
+        for (float q : s.quantilles) f << q << " ";
+        f << "\n";
+        for (float c : s.chi2cdf_inv) f << c << " ";
+        f << "\n";
+    }
+}
+
+nlsar_stats_collection despeckcl::load_nlsar_stats_collection(std::string filename)
+{
+    std::ifstream f(filename);
+    if (!f) throw std::runtime_error("cannot open " + filename);
+    std::string magic;
+    int version = 0;
+    size_t n = 0;
+    f >> magic >> version >> n;
+    if (magic != "despeckcl_amd-nlsar-stats" || version != 1) throw std::runtime_error("not a statistics file: " + filename);
+    nlsar_stats_collection nsc;
+    for (size_t i = 0; i < n; i++) {
+        nlsar::params p;
+        nlsar::stats s;
+        f >> p.patch_size >> p.scale_size >> s.lut_size >> s.dissims_min >> s.dissims_max;
+        s.quantilles.resize(s.lut_size);
+        s.chi2cdf_inv.resize(s.lut_size);
+        for (auto& q : s.quantilles) f >> q;
+        for (auto& c : s.chi2cdf_inv) f >> c;
+        if (!f) throw std::runtime_error("truncated statistics file: " + filename);
+        nsc.emplace(p, s);
+    }
+    return nsc;
+}
+
+int despeckcl::nlinsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+                       float* phase_filt, float* coh_filt, const int height, const int width,
+                       const int search_window_size, const int patch_size, const int niter,
+                       const int lmin, std::vector<std::string>)
+{
+    check(dkx_nlinsar(ampl_master, ampl_slave, phase, ref_filt, phase_filt, coh_filt, height, width,
+                      search_window_size, patch_size, niter, lmin));
+    return 0;
+}

@@ -1,0 +1,661 @@
+// NL-SAR on CDNA4: covariance preparation, Gaussian scale average, and one fused
+// "scale pass" kernel per scale that evaluates pixel dissimilarities, separable patch box
+// sums, LUT weights and the nonlocal statistics of every patch size, and (for the last
+// scale) the weighted mean, the alpha blend and the decomposition into reflectivity, phase
+// and coherence.
+//
+// Reference path replaced: src/compute/nlsar/nlsar_filter_sub_image.cpp:146-390 with the
+// kernels of src/compute/nlsar/cl_kernels (one materialised 220/441-plane buffer per stage
+// there; here every stage of one search-window offset stays in LDS).
+#include "dkx_common.h"
+#include "dkx_nlsar.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace dkx {
+
+// ------------------------------------------------------------------------------------
+// preparation kernels
+// ------------------------------------------------------------------------------------
+
+// Covariance planes of the InSAR input (covmat_create.cl:19-56) on the padded grid C:
+// plane 0 = a1^2, 1 = a2^2, 2 = Re C01 = a1*a2*cos(phi), 3 = Im C01 = a1*a2*sin(phi).
+__global__ void nlsar_prep_insar(const float* __restrict__ a1, const float* __restrict__ a2,
+                                 const float* __restrict__ phi, int H, int W, Grid C)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= C.w || gy >= C.h) return;
+    const int y = min(H - 1, max(0, gy - C.oy));
+    const int x = min(W - 1, max(0, gx - C.ox));
+    const long long i = (long long)y * W + x;
+    const float m = a1[i], s = a2[i], p = phi[i];
+    const long long o = (long long)gy * C.pitch + gx;
+    C.base[o] = m * m;
+    C.base[C.plane + o] = s * s;
+    C.base[2 * C.plane + o] = m * s * cosf(p);
+    C.base[3 * C.plane + o] = m * s * sinf(p);
+}
+
+// Covariance planes of a dim-2 covariance input (8 interlaced planes, nlsar.i:146-153):
+// the diagonal real parts and the (0,1) element, as for the InSAR input.
+__global__ void nlsar_prep_covmat2(const float* __restrict__ cov, int H, int W, Grid C)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= C.w || gy >= C.h) return;
+    const int y = min(H - 1, max(0, gy - C.oy));
+    const int x = min(W - 1, max(0, gx - C.ox));
+    const long long n = (long long)H * W;
+    const long long i = (long long)y * W + x;
+    const long long o = (long long)gy * C.pitch + gx;
+    C.base[o] = cov[i];
+    C.base[C.plane + o] = cov[6 * n + i];
+    C.base[2 * C.plane + o] = cov[2 * n + i];
+    C.base[3 * C.plane + o] = cov[3 * n + i];
+}
+
+// Gaussian s x s average of the rescaled covariance (covmat_rescale.cl:19-41 then
+// covmat_spatial_avg.cl:19-63; planes combined per pixel in the same operation order),
+// plus the log-determinant of each averaged matrix, which every pixel dissimilarity
+// involving that pixel uses (sim_measures.h:51-52).
+// Output planes of A: 0 = a00, 1 = a11, 2 = Re a01, 3 = Im a01, 4 = log det.
+__global__ void nlsar_scale_average(Grid C, Grid A, const float* __restrict__ gauss, int s,
+                                    float gamma)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= A.w || gy >= A.h) return;
+    const int y = gy - A.oy, x = gx - A.ox;  // image coordinates
+    const int ssh = (s - 1) / 2;
+    float s00 = 0.f, s11 = 0.f, sr = 0.f, si = 0.f;
+    for (int kx = 0; kx < s; kx++) {
+        const long long row = C.at(y - ssh + kx, x - ssh);
+        for (int ky = 0; ky < s; ky++) {
+            const float g = gauss[kx * s + ky];
+            const long long j = row + ky;
+            s00 = fmaf(g, C.base[j], s00);
+            s11 = fmaf(g, C.base[C.plane + j], s11);
+            const float r = C.base[2 * C.plane + j] * gamma;
+            const float im = C.base[3 * C.plane + j] * gamma;
+            sr = fmaf(g, r, sr);
+            si = fmaf(g, im, si);
+        }
+    }
+    const long long o = (long long)gy * A.pitch + gx;
+    A.base[o] = s00;
+    A.base[A.plane + o] = s11;
+    A.base[2 * A.plane + o] = sr;
+    A.base[3 * A.plane + o] = si;
+    const float det = (s00 * s11) - (sr * sr + si * si);
+    A.base[4 * A.plane + o] = logf(det);
+}
+
+// ------------------------------------------------------------------------------------
+// fused scale pass
+// ------------------------------------------------------------------------------------
+
+struct ScaleArgs {
+    Grid A;        // 5 planes, see nlsar_scale_average
+    Grid C;        // 4 planes, see nlsar_prep_insar
+    int wsh;       // search-window half width
+    int P;         // largest patch half width
+    int psh[NLSAR_MAX_NP];
+    int pidx[NLSAR_MAX_NP];        // index of (p, s) in the per-parameter maps
+    const float* lutw[NLSAR_MAX_NP];  // weight per quantile index
+    float dmin[NLSAR_MAX_NP], dmax[NLSAR_MAX_NP];
+    int lut_size[NLSAR_MAX_NP];
+    int lut_stride;                // LDS floats per LUT
+    int mean_slot;                 // slot of the largest patch (weighted mean), last scale
+    Launch L;
+    int H, W;
+    float* enl_nb;                 // [nparam][H*W] or null
+    float* alpha;                  // [nparam][H*W] or null
+    unsigned char* ubflag;         // [H*W]
+    int first_scale;               // 1: overwrite ubflag, 0: or into it
+    float* ref;                    // last scale: outputs
+    float* phase;
+    float* coh;
+    unsigned long long* ub_count;
+};
+
+constexpr int NT = 256;
+
+// 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
+__device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
+
+template <int TY, int TX, int NP, bool LAST>
+__global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
+{
+    constexpr int ROWS = NT / TX;       // thread rows
+    constexpr int PPT = TY / ROWS;      // pixels per thread
+    static_assert(NT % TX == 0 && TY % ROWS == 0, "tile must map onto 256 threads");
+
+    extern __shared__ float smem[];
+    const int wsh = a.wsh, P = a.P;
+    const int sw = 2 * wsh + 1;
+    const int DXM = TX + wsh + 2 * P;   // pitch of the dissimilarity tile
+    const int DYM = TY + wsh + 2 * P;
+    const int RXM = TX + wsh;           // pitch of row-sum / weight tiles
+    float* sD = smem;                                 // DYM * DXM
+    float* sR = sD + DYM * DXM;                       // DYM * RXM (row sums)
+    float* sW = sR + DYM * RXM;                       // NP * (TY + wsh) * RXM
+    float* sL = sW + NP * (TY + wsh) * RXM;           // NP * lut_stride
+    const int WPL = (TY + wsh) * RXM;
+
+    const int tid = threadIdx.x;
+    const int tx_ = tid % TX, ty_ = tid / TX;
+    const int ty0 = a.L.vy0 + blockIdx.y * TY;
+    const int tx0 = a.L.vx0 + blockIdx.x * TX;
+
+    for (int k = 0; k < NP; k++)
+        for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
+
+    float acc[PPT][NP][6];
+    float macc[PPT][2];
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) acc[q][k][j] = 0.f;
+        macc[q][0] = macc[q][1] = 0.f;
+    }
+
+    const float* A0 = a.A.base;
+    const float* A1 = A0 + a.A.plane;
+    const float* A2 = A0 + 2 * a.A.plane;
+    const float* A3 = A0 + 3 * a.A.plane;
+    const float* A4 = A0 + 4 * a.A.plane;
+    const float* I0 = a.C.base;
+    const float* I1 = I0 + a.C.plane;
+    const float* Cr = I0 + 2 * a.C.plane;
+    const float* Ci = I0 + 3 * a.C.plane;
+    const float K = nlsar_log2_term();
+
+    // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
+    // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
+    for (int hh = 0; hh <= wsh; hh++) {
+        const int dy = hh - wsh;                  // <= 0
+        const int wwe = (hh == wsh) ? wsh : sw;
+        for (int ww = 0; ww < wwe; ww++) {
+            const int dx = ww - wsh;
+            const int adx = dx < 0 ? -dx : dx;
+            // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+            const int RY = TY - dy, RX = TX + adx;
+            const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
+            const int DY = RY + 2 * P, DX = RX + 2 * P;
+            __syncthreads();  // previous offset's accumulation done reading sW
+            // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
+            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65)
+            {
+                const int gy0 = ty0 - P, gx0 = tx0 + rxo - P;
+                const int n = DY * DX;
+                int r = tid / DX, c = tid - r * DX;
+                const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
+                for (int i = tid; i < n; i += NT) {
+                    const long long g1 = a.A.at(gy0 + r, gx0 + c);
+                    const long long g2 = g1 + (long long)dy * a.A.pitch + dx;
+                    const float s00 = A0[g1] + A0[g2];
+                    const float s01r = A2[g1] + A2[g2];
+                    const float s01i = A3[g1] + A3[g2];
+                    const float s11 = A1[g1] + A1[g2];
+                    const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
+                    float sim = -(((K + A4[g1]) + A4[g2]) - 2.0f * logf(det));
+                    if (isnan(sim)) sim = 0.f;
+                    sD[r * DXM + c] = sim;
+                    r += sr_;
+                    c += sc_;
+                    if (c >= DX) { c -= DX; r++; }
+                }
+            }
+            for (int k = 0; k < NP; k++) {
+                const int psh = a.psh[k];
+                const int np2 = 2 * psh;
+                __syncthreads();
+                // 2. row sums (patch_similarities_row_pass.cl:52-60): left to right
+                {
+                    const int nr = RY + np2;
+                    const int n = nr * RX;
+                    const int roff = P - psh;
+                    int r = tid / RX, c = tid - r * RX;
+                    const int sr_ = NT / RX, sc_ = NT - sr_ * RX;
+                    for (int i = tid; i < n; i += NT) {
+                        const float* src = sD + (r + roff) * DXM + c + roff;
+                        float s = src[0];
+                        for (int j = 1; j <= np2; j++) s += src[j];
+                        sR[r * RXM + c] = s;
+                        r += sr_;
+                        c += sc_;
+                        if (c >= RX) { c -= RX; r++; }
+                    }
+                }
+                __syncthreads();
+                // 3. column sums (patch_similarities_col_pass.cl:52-60) and the LUT weight
+                //    (compute_weights.cl:19-56)
+                {
+                    const int n = RY * RX;
+                    const float dmin = a.dmin[k], dmax = a.dmax[k];
+                    const float rng = dmax - dmin;
+                    const float scl = (float)(a.lut_size[k] - 1);
+                    const float* lut = sL + k * a.lut_stride;
+                    float* wout = sW + k * WPL;
+                    int r = tid / RX, c = tid - r * RX;
+                    const int sr_ = NT / RX, sc_ = NT - sr_ * RX;
+                    for (int i = tid; i < n; i += NT) {
+                        const float* src = sR + r * RXM + c;
+                        float s = src[0];
+                        for (int j = 1; j <= np2; j++) s += src[j * RXM];
+                        float w;
+                        if (s > dmax) {
+                            w = 0.f;
+                        } else {
+                            float d = fmaxf(s, dmin);
+                            d = fminf(d, dmax);
+                            const float m = (d - dmin) / rng * scl;
+                            w = lut[(unsigned int)m];
+                        }
+                        wout[r * RXM + c] = w;
+                        r += sr_;
+                        c += sc_;
+                        if (c >= RX) { c -= RX; r++; }
+                    }
+                }
+            }
+            __syncthreads();
+            // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
+            //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
+            //    (compute_nl_statistics.cl:69-84) of every patch size
+#pragma unroll
+            for (int q = 0; q < PPT; q++) {
+                const int py = ty_ + q * ROWS, px = tx_;
+                const int cy = ty0 + py, cx = tx0 + px;
+                const int iu = py * RXM + (px - rxo);
+                const int il = (py - dy) * RXM + (px - dx - rxo);
+                const long long cu = a.C.at(cy + dy, cx + dx);
+                const long long cl = a.C.at(cy - dy, cx - dx);
+                const float i0u = I0[cu], i1u = I1[cu];
+                const float i0l = I0[cl], i1l = I1[cl];
+                const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
+                const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
+                float cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
+                if (LAST) {
+                    cru = Cr[cu];
+                    ciu = Ci[cu];
+                    crl = Cr[cl];
+                    cil = Ci[cl];
+                }
+#pragma unroll
+                for (int k = 0; k < NP; k++) {
+                    const float wu = sW[k * WPL + iu];
+                    const float wl = sW[k * WPL + il];
+                    acc[q][k][0] += wu;
+                    acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+                    acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
+                    acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
+                    acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
+                    acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
+                    acc[q][k][0] += wl;
+                    acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+                    acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
+                    acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
+                    acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
+                    acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
+                    if (LAST && k == a.mean_slot) {
+                        macc[q][0] = fmaf(wu, cru, macc[q][0]);
+                        macc[q][1] = fmaf(wu, ciu, macc[q][1]);
+                        macc[q][0] = fmaf(wl, crl, macc[q][0]);
+                        macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                    }
+                }
+            }
+        }
+    }
+
+    // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
+    unsigned int ubl = 0;
+#pragma unroll
+    for (int q = 0; q < PPT; q++) {
+        const int py = ty_ + q * ROWS, px = tx_;
+        const int iy = ty0 + py - a.L.vy0, ix = tx0 + px - a.L.vx0;
+        const bool valid = iy < a.L.ny && ix < a.L.nx;
+        const long long cc = a.C.at(ty0 + py, tx0 + px);
+        const float i0c = I0[cc], i1c = I1[cc];
+        const long long opix = valid ? (long long)(a.L.oy0 + iy) * a.W + (a.L.ox0 + ix) : 0;
+        bool fail = false;
+        float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            float* s = acc[q][k];
+            s[0] += 1.0f;
+            s[1] = fmaf(1.0f, 1.0f, s[1]);
+            s[2] = fmaf(1.0f, i0c, s[2]);
+            s[3] = fmaf(1.0f, i0c * i0c, s[3]);
+            s[4] = fmaf(1.0f, i1c, s[4]);
+            s[5] = fmaf(1.0f, i1c * i1c, s[5]);
+            // compute_nl_statistics.cl:86-102, compute_alphas.cl:19-44,
+            // compute_enls_nobias.cl:19-39
+            const float ws = s[0];
+            const float enl = ws * ws / s[1];
+            float alpha = 0.0f;
+            {
+                const float inl = s[2] / ws;
+                const float wv = s[3] / ws;
+                const float var = wv - inl * inl;
+                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
+            }
+            {
+                const float inl = s[4] / ws;
+                const float wv = s[5] / ws;
+                const float var = wv - inl * inl;
+                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
+            }
+            const float oma = 1.0f - alpha;
+            const float enb = enl / (oma * oma + (alpha * alpha + 2.0f * alpha * (1.0f - alpha) / ws) * enl);
+            if (!(enb > -1.0f)) fail = true;
+            if (valid) {
+                if (a.enl_nb) a.enl_nb[(long long)a.pidx[k] * a.H * a.W + opix] = enb;
+                if (a.alpha) a.alpha[(long long)a.pidx[k] * a.H * a.W + opix] = alpha;
+            }
+            if (LAST && k == a.mean_slot) {
+                alpha_mean = alpha;
+                s_mean = ws;
+                m0_mean = s[2];
+                m1_mean = s[4];
+            }
+        }
+        if (!valid) continue;
+        if (!LAST) {
+            if (a.first_scale) a.ubflag[opix] = fail ? 1 : 0;
+            else if (fail) a.ubflag[opix] = 1;
+        } else {
+            // best-parameter selection (best_params.cpp:27-53) keeps the last key of the
+            // (patch, scale) map whenever every ENL exceeds -1; pixels where the reference's
+            // selection loop is undefined are counted and also take the last key.
+            if (fail || a.ubflag[opix]) ubl++;
+            // weighted_means.cl:62-80 and covmat_decompose.cl:19-39
+            const float al = alpha_mean;
+            const float crc = Cr[cc], cic = Ci[cc];
+            const float m00 = (1 - al) * m0_mean / s_mean + al * i0c;
+            const float m11 = (1 - al) * m1_mean / s_mean + al * i1c;
+            const float mr = (1 - al) * macc[q][0] / s_mean + al * crc;
+            const float mi = (1 - al) * macc[q][1] / s_mean + al * cic;
+            const float am = sqrtf(m00), as = sqrtf(m11);
+            const float t = 0.5f * (am + as);
+            a.ref[opix] = t * t;
+            a.phase[opix] = atan2f(mi, mr);
+            a.coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
+        }
+    }
+    if (LAST && ubl) atomicAdd(a.ub_count, (unsigned long long)ubl);
+}
+
+// ------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------
+
+// covmat_spatial_avg::gen_gauss, cl_wrappers/covmat_spatial_avg.cpp:35-50
+std::vector<float> gen_gauss(int scale_size)
+{
+    std::vector<float> g;
+    const int ssh = (scale_size - 1) / 2;
+    for (int x = -ssh; x <= ssh; x++)
+        for (int y = -ssh; y <= ssh; y++)
+            g.push_back((float)std::exp(-M_PI * (x * x + y * y) / (double)std::pow(ssh + 0.5f, 2.0f)));
+    float K = 0.0f;
+    for (float v : g) K += v;
+    for (float& v : g) v = v / K;
+    return g;
+}
+
+// Weight of every quantile-LUT index: exp(-|chi2inv(q) - c| / h), the value compute_weights.cl
+// (lines 47-51) derives from the index; built once per call on the host.
+std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c)
+{
+    std::vector<float> w(st.lut_size);
+    const float scl = (float)(st.lut_size - 1);
+    for (int i = 0; i < st.lut_size; i++) {
+        const float q = st.quantilles[i];
+        const float x = st.chi2cdf_inv[(unsigned int)(q * scl)];
+        w[i] = std::exp(-std::fabs(x - c) / h);
+    }
+    return w;
+}
+
+static int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+template <int TY, int TX, int NP, bool LAST>
+static size_t scale_pass_lds(int wsh, int P, int lut_stride)
+{
+    const int DXM = TX + wsh + 2 * P, DYM = TY + wsh + 2 * P, RXM = TX + wsh;
+    return sizeof(float) * ((size_t)DYM * DXM + (size_t)DYM * RXM + (size_t)NP * (TY + wsh) * RXM +
+                            (size_t)NP * lut_stride);
+}
+
+template <int TY, int TX, int NP, bool LAST>
+static int launch_scale_pass(const ScaleArgs& base, const Launch* L, int nl, int lut_stride,
+                             hipStream_t stream)
+{
+    const size_t lds = scale_pass_lds<TY, TX, NP, LAST>(base.wsh, base.P, lut_stride);
+    if (lds > 160 * 1024) {
+        set_error("nlsar: search window / patch sizes exceed the LDS budget");
+        return -2;
+    }
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    for (int i = 0; i < nl; i++) {
+        if (L[i].ny <= 0 || L[i].nx <= 0) continue;
+        ScaleArgs a = base;
+        a.L = L[i];
+        dim3 grid((L[i].nx + TX - 1) / TX, (L[i].ny + TY - 1) / TY);
+        hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST>), grid, dim3(NT), lds, stream, a);
+        DKX_HIP_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+template <int NP, bool LAST>
+static int dispatch_np(const ScaleArgs& a, const Launch* L, int nl, int lut_stride, int tile,
+                       hipStream_t stream)
+{
+    switch (tile) {
+    case 0: return launch_scale_pass<32, 32, NP, LAST>(a, L, nl, lut_stride, stream);
+    case 1: return launch_scale_pass<16, 32, NP, LAST>(a, L, nl, lut_stride, stream);
+    case 2: return launch_scale_pass<32, 64, NP, LAST>(a, L, nl, lut_stride, stream);
+    default: return launch_scale_pass<16, 64, NP, LAST>(a, L, nl, lut_stride, stream);
+    }
+}
+
+static int dispatch(const ScaleArgs& a, int np, bool last, const Launch* L, int nl,
+                    int lut_stride, int tile, hipStream_t stream)
+{
+    switch (np) {
+    case 1: return last ? dispatch_np<1, true>(a, L, nl, lut_stride, tile, stream)
+                        : dispatch_np<1, false>(a, L, nl, lut_stride, tile, stream);
+    case 2: return last ? dispatch_np<2, true>(a, L, nl, lut_stride, tile, stream)
+                        : dispatch_np<2, false>(a, L, nl, lut_stride, tile, stream);
+    case 3: return last ? dispatch_np<3, true>(a, L, nl, lut_stride, tile, stream)
+                        : dispatch_np<3, false>(a, L, nl, lut_stride, tile, stream);
+    case 4: return last ? dispatch_np<4, true>(a, L, nl, lut_stride, tile, stream)
+                        : dispatch_np<4, false>(a, L, nl, lut_stride, tile, stream);
+    default: set_error("nlsar: at most 4 patch sizes are supported"); return -1;
+    }
+}
+
+static const int kTileTY[4] = {32, 16, 32, 16};
+static const int kTileTX[4] = {32, 32, 64, 64};
+
+int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
+{
+    if (pb.H <= 0 || pb.W <= 0) { set_error("nlsar: empty image"); return -1; }
+    if (pb.sw < 3 || pb.sw % 2 == 0) { set_error("nlsar: search window size must be odd and >= 3"); return -1; }
+    if (pb.nP < 1 || pb.nP > NLSAR_MAX_NP || pb.nS < 1 || pb.nS > NLSAR_MAX_NS) {
+        set_error("nlsar: 1-4 patch sizes and 1-8 scale sizes are supported");
+        return -1;
+    }
+    int pmax = 0, smax = 0;
+    for (int i = 0; i < pb.nP; i++) {
+        if (pb.P[i] < 1 || pb.P[i] % 2 == 0) { set_error("nlsar: patch sizes must be odd"); return -1; }
+        pmax = std::max(pmax, pb.P[i]);
+    }
+    for (int i = 0; i < pb.nS; i++) {
+        if (pb.S[i] < 1 || pb.S[i] % 2 == 0 || pb.S[i] > 31) { set_error("nlsar: scale sizes must be odd and <= 31"); return -1; }
+        smax = std::max(smax, pb.S[i]);
+    }
+    for (int i = 0; i < pb.nP; i++)
+        for (int j = i + 1; j < pb.nP; j++)
+            if (pb.P[i] == pb.P[j]) { set_error("nlsar: duplicate patch size"); return -1; }
+    for (int i = 0; i < pb.nS; i++)
+        for (int j = i + 1; j < pb.nS; j++)
+            if (pb.S[i] == pb.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
+    plan->tile = std::min(std::max(pb.tile, 0), 3);
+    plan->TY = kTileTY[plan->tile];
+    plan->TX = kTileTX[plan->tile];
+    plan->wsh = (pb.sw - 1) / 2;
+    plan->P = (pmax - 1) / 2;
+    plan->smax = smax;
+    plan->overlap = plan->P + plan->wsh + (smax - 1) / 2;
+    plan->eh = pb.edge_h < 0 ? plan->overlap : pb.edge_h;
+    plan->ew = pb.edge_w < 0 ? plan->overlap : pb.edge_w;
+    const int H = pb.H, W = pb.W;
+    // virtual extents covered by the four launch mappings
+    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->TY), H - 1 + plan->eh + plan->TY);
+    const int Wx = std::max(round_up(std::max(W - 1, 1), plan->TX), W - 1 + plan->ew + plan->TX);
+    const int ra = plan->wsh + plan->P;          // A grid padding
+    const int rc = ra + (smax - 1) / 2;          // C grid padding
+    plan->C = Grid{nullptr, 0, round_up(Wx + 2 * rc, 64), rc, rc, Hx + 2 * rc, Wx + 2 * rc};
+    plan->C.plane = (long long)plan->C.pitch * plan->C.h;
+    plan->A = Grid{nullptr, 0, round_up(Wx + 2 * ra, 64), ra, ra, Hx + 2 * ra, Wx + 2 * ra};
+    plan->A.plane = (long long)plan->A.pitch * plan->A.h;
+    int maxlut = 0;
+    for (int i = 0; i < pb.nst; i++) maxlut = std::max(maxlut, pb.stats[i].lut_size);
+    plan->lut_stride = maxlut;
+    plan->nparam = pb.nP * pb.nS;
+    // workspace layout (bytes, 256-aligned)
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
+    plan->off_C = take(sizeof(float) * 4 * plan->C.plane);
+    plan->off_A = take(sizeof(float) * 5 * plan->A.plane);
+    plan->off_lut = take(sizeof(float) * (size_t)plan->nparam * maxlut);
+    plan->off_gauss = take(sizeof(float) * (size_t)NLSAR_MAX_NS * 31 * 31);
+    plan->off_flag = take((size_t)H * W);
+    plan->off_ub = take(sizeof(unsigned long long));
+    plan->ws_bytes = off;
+    return 0;
+}
+
+int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, const float* d_a2,
+              const float* d_phi, const float* d_cov, float* d_ref, float* d_phase, float* d_coh,
+              void* d_ws, float* d_enl_maps, float* d_alpha_maps, unsigned long long* ub_host,
+              hipStream_t stream)
+{
+    char* ws = (char*)d_ws;
+    Grid C = plan.C, A = plan.A;
+    C.base = (float*)(ws + plan.off_C);
+    A.base = (float*)(ws + plan.off_A);
+    float* d_lut = (float*)(ws + plan.off_lut);
+    float* d_gauss = (float*)(ws + plan.off_gauss);
+    unsigned char* d_flag = (unsigned char*)(ws + plan.off_flag);
+    unsigned long long* d_ub = (unsigned long long*)(ws + plan.off_ub);
+    const int H = pb.H, W = pb.W;
+
+    // per-parameter weight LUTs, parameter index = position in the scale-major loop
+    std::vector<float> hl((size_t)plan.nparam * plan.lut_stride, 0.f);
+    std::vector<const NlsarStatsView*> pst(plan.nparam);
+    for (int si = 0; si < pb.nS; si++)
+        for (int pi = 0; pi < pb.nP; pi++) {
+            const int idx = si * pb.nP + pi;
+            const NlsarStatsView* st = nullptr;
+            for (int k = 0; k < pb.nst; k++)
+                if (pb.stats[k].patch_size == pb.P[pi] && pb.stats[k].scale_size == pb.S[si]) st = &pb.stats[k];
+            if (!st) { set_error("nlsar: statistics missing for a (patch, scale) pair"); return -3; }
+            pst[idx] = st;
+            std::vector<float> w = weight_lut(*st, pb.h, pb.c);
+            std::copy(w.begin(), w.end(), hl.begin() + (size_t)idx * plan.lut_stride);
+        }
+    DKX_HIP_CHECK(hipMemcpyAsync(d_lut, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    std::vector<float> hg((size_t)NLSAR_MAX_NS * 31 * 31, 0.f);
+    for (int si = 0; si < pb.nS; si++) {
+        std::vector<float> g = gen_gauss(pb.S[si]);
+        std::copy(g.begin(), g.end(), hg.begin() + (size_t)si * 31 * 31);
+    }
+    DKX_HIP_CHECK(hipMemcpyAsync(d_gauss, hg.data(), hg.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    DKX_HIP_CHECK(hipMemsetAsync(d_ub, 0, sizeof(unsigned long long), stream));
+
+    {
+        dim3 blk(256), grd((C.w + 255) / 256, C.h);
+        if (pb.dim == 0)
+            hipLaunchKernelGGL(nlsar_prep_insar, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, C);
+        else
+            hipLaunchKernelGGL(nlsar_prep_covmat2, grd, blk, 0, stream, d_cov, H, W, C);
+        DKX_HIP_CHECK(hipGetLastError());
+    }
+    // covmat_rescale.cl:24-25 with nlooks = 1 and dimension 2
+    const float gamma = std::pow(std::min(1.0f / 2, 1.0f), 0.33333f);
+
+    Launch L[4];
+    L[0] = Launch{0, 0, H - 1, W - 1, 0, 0};
+    L[1] = Launch{H - 1 + plan.eh, 0, 1, W - 1, H - 1, 0};
+    L[2] = Launch{0, W - 1 + plan.ew, H - 1, 1, 0, W - 1};
+    L[3] = Launch{H - 1 + plan.eh, W - 1 + plan.ew, 1, 1, H - 1, W - 1};
+
+    // scale order: every scale but the largest, then the largest (its pass also forms
+    // the weighted mean with the largest patch size, the key the selection keeps)
+    std::vector<int> order;
+    for (int si = 0; si < pb.nS; si++) if (pb.S[si] != plan.smax) order.push_back(si);
+    for (int si = 0; si < pb.nS; si++) if (pb.S[si] == plan.smax) order.push_back(si);
+    int pmax_slot = 0;
+    for (int pi = 0; pi < pb.nP; pi++) if (pb.P[pi] > pb.P[pmax_slot]) pmax_slot = pi;
+
+    for (size_t oi = 0; oi < order.size(); oi++) {
+        const int si = order[oi];
+        const int s = pb.S[si];
+        dim3 blk(256), grd((A.w + 255) / 256, A.h);
+        hipLaunchKernelGGL(nlsar_scale_average, grd, blk, 0, stream, C, A,
+                           (const float*)(d_gauss + (size_t)si * 31 * 31), s, gamma);
+        DKX_HIP_CHECK(hipGetLastError());
+        ScaleArgs sa;
+        memset(&sa, 0, sizeof(sa));
+        sa.A = A;
+        sa.C = C;
+        sa.wsh = plan.wsh;
+        sa.P = plan.P;
+        for (int pi = 0; pi < pb.nP; pi++) {
+            const int idx = si * pb.nP + pi;
+            sa.psh[pi] = (pb.P[pi] - 1) / 2;
+            sa.pidx[pi] = idx;
+            sa.lutw[pi] = d_lut + (size_t)idx * plan.lut_stride;
+            sa.dmin[pi] = pst[idx]->dissims_min;
+            sa.dmax[pi] = pst[idx]->dissims_max;
+            sa.lut_size[pi] = pst[idx]->lut_size;
+        }
+        sa.lut_stride = plan.lut_stride;
+        sa.mean_slot = pmax_slot;
+        sa.H = H;
+        sa.W = W;
+        sa.enl_nb = d_enl_maps;
+        sa.alpha = d_alpha_maps;
+        sa.ubflag = d_flag;
+        sa.first_scale = oi == 0 ? 1 : 0;
+        sa.ref = d_ref;
+        sa.phase = d_phase;
+        sa.coh = d_coh;
+        sa.ub_count = d_ub;
+        const bool last = oi + 1 == order.size();
+        if (last && oi == 0) {
+            // a single scale: nothing else wrote the flags
+            DKX_HIP_CHECK(hipMemsetAsync(d_flag, 0, (size_t)H * W, stream));
+        }
+        int rc = dispatch(sa, pb.nP, last, L, 4, plan.lut_stride, plan.tile, stream);
+        if (rc) return rc;
+    }
+    if (ub_host) {
+        DKX_HIP_CHECK(hipMemcpyAsync(ub_host, d_ub, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    }
+    return 0;
+}
+
+}  // namespace dkx

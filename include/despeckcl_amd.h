@@ -1,0 +1,121 @@
+/*
+ * despeckcl_amd.h -- C ABI of the MI355X NL-SAR / NL-InSAR library (libdespeckcl_amd.so).
+ *
+ * This is the drop-in boundary for despeckCL's nonlocal filters.  Every entry point takes
+ * plain pointers and sizes; nothing here depends on C++ or on PyTorch.  The C++ API of
+ * include/despeckcl.h and the Python module despeckcl_amd are thin layers over it.
+ *
+ * Conventions
+ *  - images are row-major height x width float32 planes, as in the reference
+ *    (swig/python/despeckcl_typemaps.i:15-24); covariance matrices are 2*dim*dim planes,
+ *    real and imaginary parts interlaced per matrix element in row-major (r, c) order
+ *    (swig/python/nlsar.i:146-153);
+ *  - host-pointer entry points copy in, filter on the current HIP device and copy out,
+ *    returning when the result is in the caller's arrays;
+ *  - *_device entry points take device pointers and a hipStream_t (passed as void*),
+ *    enqueue the work on that stream and return without synchronising;
+ *  - return value 0 on success, a negative DKX_E* code otherwise; dkx_last_error() gives
+ *    the message of the calling thread's last failure.  (The reference terminates the
+ *    process on device errors, src/compute/nlsar/nlsar_filter_sub_image.cpp:74-78.)
+ */
+#ifndef DESPECKCL_AMD_H
+#define DESPECKCL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DKX_OK 0
+#define DKX_EINVAL (-1)       /* invalid sizes or parameters */
+#define DKX_ERESOURCE (-2)    /* parameters exceed a kernel resource (LDS) or memory */
+#define DKX_ESTATS (-3)       /* statistics keys do not match the patch x scale set */
+#define DKX_EUNSUPPORTED (-4) /* valid in the reference, not supported here */
+#define DKX_EHIP (-5)         /* HIP runtime error */
+
+/* One entry of std::map<nlsar::params, nlsar::stats>
+ * (include/compute/nlsar/parameters.h:24-56, include/compute/nlsar/stats.h:27-55). */
+typedef struct {
+    int patch_size;
+    int scale_size;
+    int lut_size;
+    float dissims_min;
+    float dissims_max;
+    const float* quantilles;  /* lut_size floats */
+    const float* chi2cdf_inv; /* lut_size floats */
+} dkx_nlsar_stats;
+
+const char* dkx_last_error(void);
+int dkx_version(void);
+
+/* despeckcl::nlsar, InSAR overload (include/despeckcl.h:45-59; src/compute/nlsar/nlsar.cpp:139-167).
+ * The statistics must hold exactly one entry per (patch size, scale size) pair. */
+int dkx_nlsar_insar(const float* ampl_master, const float* ampl_slave, const float* phase,
+                    float* ref_filt, float* phase_filt, float* coh_filt, int height, int width,
+                    int search_window_size, const int* patch_sizes, int n_patch_sizes,
+                    const int* scale_sizes, int n_scale_sizes, const dkx_nlsar_stats* stats,
+                    int n_stats, float h_param, float c_param);
+
+/* Same filter on device buffers, enqueued on `stream` (a hipStream_t; NULL = default).
+ * enl_maps / alpha_maps: optional device buffers of n_patch*n_scale*height*width floats that
+ * receive the bias-corrected ENL and alpha of every (patch, scale) pair, parameter index
+ * = scale_index*n_patch_sizes + patch_index.  ub_pixels: optional host pointer; when given
+ * the call synchronises and stores the number of pixels for which the reference's best
+ * parameter selection is undefined (src/compute/nlsar/best_params.cpp:27-53). */
+int dkx_nlsar_insar_device(const float* d_ampl_master, const float* d_ampl_slave,
+                           const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                           float* d_coh_filt, int height, int width, int search_window_size,
+                           const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                           int n_scale_sizes, const dkx_nlsar_stats* stats, int n_stats,
+                           float h_param, float c_param, float* d_enl_maps, float* d_alpha_maps,
+                           unsigned long long* ub_pixels, void* stream);
+
+/* despeckcl::nlinsar (include/despeckcl.h:97-109; src/compute/nlinsar/nlinsar.cpp:42-146).
+ * ref_filt / phase_filt / coh_filt are read as the first iteration's estimates and
+ * overwritten with the result, like the reference (nlinsar.cpp:59-61).  Patch sizes 3 and
+ * 7 only (the reference draws h at random for the others, insarsim_simu.cpp:62-82). */
+int dkx_nlinsar(const float* ampl_master, const float* ampl_slave, const float* phase,
+                float* ref_filt, float* phase_filt, float* coh_filt, int height, int width,
+                int search_window_size, int patch_size, int niter, int lmin);
+
+/* Same on device buffers.  n_smoothed: optional host pointer (synchronises) receiving the
+ * number of pixel-iterations that went through the lmin smoothing. */
+int dkx_nlinsar_device(const float* d_ampl_master, const float* d_ampl_slave,
+                       const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                       float* d_coh_filt, int height, int width, int search_window_size,
+                       int patch_size, int niter, int lmin, unsigned long long* n_smoothed,
+                       void* stream);
+
+/* despeckcl::nlsar_training, InSAR overload (include/despeckcl.h:75-83;
+ * src/compute/nlsar/training/nlsar_training.cpp:57-79).  Writes n_patch*n_scale entries to
+ * `out` in std::map order (patch size, then scale size); their LUT arrays point into
+ * `lut_storage`, which must hold n_patch*n_scale*2*256 floats. */
+int dkx_nlsar_training_insar(const float* ampl_master, const float* ampl_slave,
+                             const float* phase, int height, int width, const int* patch_sizes,
+                             int n_patch_sizes, const int* scale_sizes, int n_scale_sizes,
+                             dkx_nlsar_stats* out, float* lut_storage);
+
+/* despeckcl::nlsar_training, covariance overload (include/despeckcl.h:85-92). */
+int dkx_nlsar_training_covmat(const float* covmat, int height, int width, int dim,
+                              const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                              int n_scale_sizes, dkx_nlsar_stats* out, float* lut_storage);
+
+/* nlsar::stats::stats(dissims, lut_size) (src/compute/nlsar/stats.cpp:28-80): quantile LUT
+ * and inverse chi-square(49) LUT.  Returns the number of finite dissimilarities (0: none). */
+long long dkx_nlsar_stats_build(const float* dissims, long long n, int lut_size, float* dissims_min,
+                                float* dissims_max, float* quantilles, float* chi2cdf_inv);
+
+/* Inverse CDF of the chi-square distribution (the reference uses GSL's gsl_cdf_chisq_Pinv,
+ * src/compute/nlsar/stats.cpp:76). */
+double dkx_chi2_inv(double p, double nu);
+
+/* Tile shape used by the NL-SAR / NL-InSAR kernels (chosen by the autotuner). */
+int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
+int dkx_get_tile(int filter);
+
+/* Releases the device workspace cached by the library on the current device. */
+void dkx_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+/*
+ * oracle.h -- CPU restatement of despeckCL's NL-SAR / NL-InSAR hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP
+ * product path and the "port" CPU baseline of bench.py.  Nothing in the
+ * product (despeckcl_amd/) may link, load or call it.
+ *
+ * Every function restates the reference arithmetic in the reference's
+ * operation order; the file:line it follows is cited at each definition
+ * (paths relative to the despeckCL source tree).
+ */
+#ifndef DESPECKLE_ORACLE_H
+#define DESPECKLE_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One entry of the reference's std::map<nlsar::params, nlsar::stats>
+ * (include/compute/nlsar/parameters.h:24-56, include/compute/nlsar/stats.h:27-55). */
+typedef struct {
+    int patch_size;
+    int scale_size;
+    int lut_size;
+    float dissims_min;
+    float dissims_max;
+    const float* quantilles;   /* lut_size floats */
+    const float* chi2cdf_inv;  /* lut_size floats */
+} orc_stats;
+
+/* ---- single formulas (exposed for the ported closed-form unit tests) ---- */
+float orc_det_covmat_2x2(float el_00, float el_01real, float el_01imag, float el_11);
+float orc_det_covmat_3x3(float a00, float a11, float a22, float a01r, float a01i,
+                         float a02r, float a02i, float a12r, float a12i);
+float orc_pixel_similarity_2x2(const float p1[4], const float p2[4], int nlooks);
+float orc_pixel_similarity_3x3(const float p1[9], const float p2[9], int nlooks);
+void  orc_gen_gauss(int scale_size, float* out);
+float orc_enl_nobias(float enl, float alpha, float wsum);
+float orc_nlinsar_pixel_similarity(float am1, float as1, float dp1, float am2, float as2, float dp2);
+float orc_nlinsar_pixel_kl(float ref1, float dp1, float coh1, float ref2, float dp2, float coh2);
+double orc_chi2_inv(double p, double nu);
+double orc_chi2_cdf(double x, double nu);
+
+/* ---- per-kernel restatements on caller buffers (ported kernel tests) ---- */
+void orc_covmat_create(const float* a1, const float* a2, const float* phi, float* covmat, int h, int w);
+void orc_covmat_rescale(float* covmat, int dim, int nlooks, int h, int w);
+void orc_covmat_spatial_avg(const float* in, float* out, int dim, int h_overlap, int w_overlap,
+                            int scale_size, int scale_size_max);
+void orc_patch_similarities(const float* pix, float* pat, int h_sim, int w_sim, int sw,
+                            int patch_size, int patch_size_max);
+void orc_copy_symm_weights(const float* wsymm, float* wfull, int h_ori, int w_ori, int sw);
+void orc_nl_statistics(const float* covmat_ori, const float* weights, float* int_nl, float* wvar,
+                       float* wsums, float* enl, int h_ori, int w_ori, int sw, int pmax, int smax,
+                       int dim);
+void orc_alphas(const float* int_nl, const float* wvar, float* alphas, int h_ori, int w_ori,
+                int dim, int nlooks);
+void orc_weighted_means(const float* cin, float* cout, const float* weights, const float* alphas,
+                        int h_ori, int w_ori, int sw, int pmax, int smax, int dim);
+void orc_covmat_decompose(const float* covmat, float* ref, float* phase, float* coh, int h, int w);
+
+/* ---- NL-SAR statistics / LUT ---- */
+/* nlsar::stats::stats(dissims, lut_size), stats.cpp:28-80.  Returns the number of finite
+ * dissimilarities kept. */
+int orc_stats_build(const float* dissims, int n, int lut_size, float* dmin, float* dmax,
+                    float* quantilles, float* chi2cdf_inv);
+
+/* ---- whole filters ---- */
+/* despeckcl::nlsar (InSAR overload), nlsar.cpp:139-167 + map_filter_tiles.h:37-104.
+ * tile_h/tile_w: reference tile dims (0 = one tile covering the image plus an overlap-wide
+ * virtual margin, the reference's result with its 2048-pixel tiles).  Returns 0 on success;
+ * *ub_pixels counts pixels where the reference's best-parameter loop is undefined. */
+int orc_nlsar_insar(const float* a1, const float* a2, const float* phi,
+                    float* ref_filt, float* phase_filt, float* coh_filt,
+                    int height, int width, int search_window_size,
+                    const int* patch_sizes, int n_patch, const int* scale_sizes, int n_scale,
+                    const orc_stats* stats, int n_stats, float h_param, float c_param,
+                    int tile_h, int tile_w, long long* ub_pixels);
+
+/* despeckcl::nlsar (covariance overload), nlsar.cpp:171-194; covmats are 2*dim*dim planes. */
+int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, int width, int dim,
+                     int search_window_size, const int* patch_sizes, int n_patch,
+                     const int* scale_sizes, int n_scale, const orc_stats* stats, int n_stats,
+                     float h_param, float c_param, int tile_h, int tile_w, long long* ub_pixels);
+
+/* despeckcl::nlsar_training, nlsar_training.cpp:35-96; out must hold n_patch*n_scale entries
+ * (ordered patch-major like the reference's std::map); quantilles/chi2 storage is supplied
+ * by the caller in qbuf/cbuf (n_patch*n_scale*256 floats each). */
+int orc_nlsar_training_covmat(const float* covmat, int height, int width, int dim,
+                              const int* patch_sizes, int n_patch, const int* scale_sizes,
+                              int n_scale, orc_stats* out, float* qbuf, float* cbuf);
+int orc_nlsar_training_insar(const float* a1, const float* a2, const float* phi, int height,
+                             int width, const int* patch_sizes, int n_patch,
+                             const int* scale_sizes, int n_scale, orc_stats* out, float* qbuf,
+                             float* cbuf);
+/* all unordered patch dissimilarities of one (p, s), get_dissims.cpp:32-117; returns count,
+ * writes up to cap values */
+long long orc_training_dissims(const float* covmat, int height, int width, int dim,
+                               int patch_size, int scale_size, float* out, long long cap);
+
+/* despeckcl::nlinsar, nlinsar.cpp:42-146.  ref/phase/coh are in/out: their input values are
+ * the first iteration's estimates (the Python API passes zeros).  tile: reference tile side
+ * (80).  *n_smoothed counts pixels that went through the lmin smoothing. */
+int orc_nlinsar(const float* a1, const float* a2, const float* phi,
+                float* ref_filt, float* phase_filt, float* coh_filt,
+                int height, int width, int search_window_size, int patch_size, int niter,
+                int lmin, int tile, long long* n_smoothed);
+/* h and T used by the NL-InSAR weights, nlinsar.cpp:65-78 (patch 3 and 7 only) */
+int orc_nlinsar_params(int patch_size, float* h_para, float* T_para);
+/* search_window_smoothing on one window, smoothing.cpp:78-129 */
+void orc_search_window_smoothing(const float* ampl_master, float* weights, int h, int w,
+                                 int width_overlap, int patch_size, int search_window_size,
+                                 int lmin);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,229 @@
+"""ctypes front end of the CPU oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py, always as the checker.  The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("patch_size", C.c_int), ("scale_size", C.c_int), ("lut_size", C.c_int),
+                ("dissims_min", C.c_float), ("dissims_max", C.c_float),
+                ("quantilles", C.POINTER(C.c_float)), ("chi2cdf_inv", C.POINTER(C.c_float))]
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile (gcc)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_chi2_inv.restype = C.c_double
+        L.orc_chi2_inv.argtypes = [C.c_double, C.c_double]
+        L.orc_chi2_cdf.restype = C.c_double
+        L.orc_chi2_cdf.argtypes = [C.c_double, C.c_double]
+        for name in ("orc_det_covmat_2x2",):
+            getattr(L, name).restype = C.c_float
+            getattr(L, name).argtypes = [C.c_float] * 4
+        L.orc_det_covmat_3x3.restype = C.c_float
+        L.orc_det_covmat_3x3.argtypes = [C.c_float] * 9
+        L.orc_pixel_similarity_2x2.restype = C.c_float
+        L.orc_pixel_similarity_2x2.argtypes = [_f32p, _f32p, C.c_int]
+        L.orc_pixel_similarity_3x3.restype = C.c_float
+        L.orc_pixel_similarity_3x3.argtypes = [_f32p, _f32p, C.c_int]
+        L.orc_enl_nobias.restype = C.c_float
+        L.orc_enl_nobias.argtypes = [C.c_float] * 3
+        L.orc_nlinsar_pixel_similarity.restype = C.c_float
+        L.orc_nlinsar_pixel_similarity.argtypes = [C.c_float] * 6
+        L.orc_nlinsar_pixel_kl.restype = C.c_float
+        L.orc_nlinsar_pixel_kl.argtypes = [C.c_float] * 6
+        L.orc_training_dissims.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _stats_array(stats):
+    """stats: mapping (patch, scale) -> object with lut_size, dissims_min, dissims_max,
+    quantilles, chi2cdf_inv.  Returns (ctypes array, keepalive list)."""
+    keep = []
+    arr = (OrcStats * len(stats))()
+    for i, ((p, s), st) in enumerate(sorted(stats.items())):
+        q = _f32(st.quantilles)
+        c = _f32(st.chi2cdf_inv)
+        keep += [q, c]
+        arr[i] = OrcStats(p, s, int(st.lut_size), float(st.dissims_min), float(st.dissims_max),
+                          q.ctypes.data_as(C.POINTER(C.c_float)),
+                          c.ctypes.data_as(C.POINTER(C.c_float)))
+    return arr, keep
+
+
+def _ints(v):
+    a = (C.c_int * len(v))(*[int(x) for x in v])
+    return a
+
+
+class _Stats:
+    __slots__ = ("lut_size", "dissims_min", "dissims_max", "quantilles", "chi2cdf_inv")
+
+    def __init__(self, lut_size, dmin, dmax, q, c):
+        self.lut_size, self.dissims_min, self.dissims_max = lut_size, dmin, dmax
+        self.quantilles, self.chi2cdf_inv = q, c
+
+
+def chi2_inv(p, nu=49.0):
+    return lib().orc_chi2_inv(float(p), float(nu))
+
+
+def chi2_cdf(x, nu=49.0):
+    return lib().orc_chi2_cdf(float(x), float(nu))
+
+
+def stats_build(dissims, lut_size=256):
+    d = _f32(dissims)
+    q = np.zeros(lut_size, np.float32)
+    c = np.zeros(lut_size, np.float32)
+    dmin, dmax = C.c_float(), C.c_float()
+    m = lib().orc_stats_build(d.ctypes.data_as(C.c_void_p), C.c_int(d.size), C.c_int(lut_size),
+                              C.byref(dmin), C.byref(dmax), q.ctypes.data_as(C.c_void_p),
+                              c.ctypes.data_as(C.c_void_p))
+    return _Stats(lut_size, dmin.value, dmax.value, q, c), m
+
+
+def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats,
+                h=15.0, c=49.0, tile=(0, 0), init=None):
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    out = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
+    arr, keep = _stats_array(stats)
+    ub = C.c_longlong(0)
+    rc = lib().orc_nlsar_insar(
+        a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
+        out[0].ctypes.data_as(C.c_void_p), out[1].ctypes.data_as(C.c_void_p),
+        out[2].ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+        _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)),
+        arr, C.c_int(len(stats)), C.c_float(h), C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]),
+        C.byref(ub))
+    if rc != 0:
+        raise ValueError(f"oracle nlsar_insar failed ({rc})")
+    return out[0], out[1], out[2], ub.value
+
+
+def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stats,
+                 h=15.0, c=49.0, tile=(0, 0)):
+    cov = _f32(covmat)
+    H, W = cov.shape[-2:]
+    out = np.zeros_like(cov)
+    arr, keep = _stats_array(stats)
+    ub = C.c_longlong(0)
+    rc = lib().orc_nlsar_covmat(
+        cov.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W),
+        C.c_int(dim), C.c_int(search_window_size), _ints(patch_sizes), C.c_int(len(patch_sizes)),
+        _ints(scale_sizes), C.c_int(len(scale_sizes)), arr, C.c_int(len(stats)), C.c_float(h),
+        C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]), C.byref(ub))
+    if rc != 0:
+        raise ValueError(f"oracle nlsar_covmat failed ({rc})")
+    return out, ub.value
+
+
+def _training_out(n, stats_arr, qbuf, cbuf):
+    res = {}
+    for i in range(n):
+        s = stats_arr[i]
+        res[(s.patch_size, s.scale_size)] = _Stats(s.lut_size, s.dissims_min, s.dissims_max,
+                                                   qbuf[i].copy(), cbuf[i].copy())
+    return res
+
+
+def nlsar_train_insar(a1, a2, phi, patch_sizes, scale_sizes):
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    n = len(patch_sizes) * len(scale_sizes)
+    arr = (OrcStats * n)()
+    qbuf = np.zeros((n, 256), np.float32)
+    cbuf = np.zeros((n, 256), np.float32)
+    rc = lib().orc_nlsar_training_insar(
+        a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
+        C.c_int(H), C.c_int(W), _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes),
+        C.c_int(len(scale_sizes)), arr, qbuf.ctypes.data_as(C.c_void_p), cbuf.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise ValueError(f"oracle training failed ({rc})")
+    return _training_out(n, arr, qbuf, cbuf)
+
+
+def nlsar_train_covmat(covmat, dim, patch_sizes, scale_sizes):
+    cov = _f32(covmat)
+    H, W = cov.shape[-2:]
+    n = len(patch_sizes) * len(scale_sizes)
+    arr = (OrcStats * n)()
+    qbuf = np.zeros((n, 256), np.float32)
+    cbuf = np.zeros((n, 256), np.float32)
+    rc = lib().orc_nlsar_training_covmat(
+        cov.ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W), C.c_int(dim), _ints(patch_sizes),
+        C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)), arr,
+        qbuf.ctypes.data_as(C.c_void_p), cbuf.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise ValueError(f"oracle training failed ({rc})")
+    return _training_out(n, arr, qbuf, cbuf)
+
+
+def training_dissims(covmat, dim, patch_size, scale_size):
+    cov = _f32(covmat)
+    H, W = cov.shape[-2:]
+    h_ov, w_ov = H - scale_size + 1, W - scale_size + 1
+    npatch = -(-h_ov // patch_size) * -(-w_ov // patch_size)
+    cap = npatch * (npatch - 1) // 2
+    out = np.zeros(max(cap, 1), np.float32)
+    n = lib().orc_training_dissims(cov.ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W),
+                                   C.c_int(dim), C.c_int(patch_size), C.c_int(scale_size),
+                                   out.ctypes.data_as(C.c_void_p), C.c_longlong(cap))
+    return out[:n]
+
+
+def nlinsar(a1, a2, phi, search_window_size, patch_size, niter, lmin, init=None, tile=80):
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    out = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
+    ns = C.c_longlong(0)
+    rc = lib().orc_nlinsar(
+        a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
+        out[0].ctypes.data_as(C.c_void_p), out[1].ctypes.data_as(C.c_void_p),
+        out[2].ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+        C.c_int(patch_size), C.c_int(niter), C.c_int(lmin), C.c_int(tile), C.byref(ns))
+    if rc != 0:
+        raise ValueError(f"oracle nlinsar failed ({rc})")
+    return out[0], out[1], out[2], ns.value
+
+
+def call(name, *args):
+    """Call a per-kernel restatement with numpy arrays / ints."""
+    conv = []
+    for a in args:
+        if isinstance(a, np.ndarray):
+            conv.append(a.ctypes.data_as(C.c_void_p))
+        elif isinstance(a, float):
+            conv.append(C.c_float(a))
+        else:
+            conv.append(C.c_int(int(a)))
+    return getattr(lib(), name)(*conv)
