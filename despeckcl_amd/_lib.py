@@ -47,6 +47,8 @@ SIGNATURES = {
     "dkx_chi2_inv": (C.c_double, [C.c_double, C.c_double]),
     "dkx_set_tile": (_I, [_I, _I]),
     "dkx_get_tile": (_I, [_I]),
+    "dkx_set_timing": (_I, [_I]),
+    "dkx_stage_times": (_I, [_P, _P, _I]),
     "dkx_release_workspace": (None, []),
 }
 

@@ -5,6 +5,7 @@
 #include "dkx_nlinsar.h"
 #include "dkx_nlsar.h"
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <map>
@@ -24,8 +25,51 @@ struct DeviceWorkspace {
     std::mutex mu;
     void* ptr = nullptr;
     size_t bytes = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // the library's own stream (host-array entry points)
+    // device constants (weight LUTs, Gaussian taps) and the host image of their content
+    float* consts = nullptr;
+    size_t consts_cap = 0;
+    std::vector<float> consts_host;
+    // completion of the last call that used this workspace, and its stream: a call on
+    // another stream waits for it, so calls sharing the workspace never overlap on the GPU
+    hipEvent_t done = nullptr;
+    hipStream_t done_stream = nullptr;
+    bool have_done = false;
 };
+
+// make `stream` wait for the previous user of the workspace
+static int workspace_acquire(DeviceWorkspace* w, hipStream_t stream)
+{
+    if (w->have_done && w->done_stream != stream) DKX_HIP_CHECK(hipStreamWaitEvent(stream, w->done, 0));
+    return 0;
+}
+
+static int workspace_release(DeviceWorkspace* w, hipStream_t stream)
+{
+    if (!w->done) DKX_HIP_CHECK(hipEventCreateWithFlags(&w->done, hipEventDisableTiming));
+    DKX_HIP_CHECK(hipEventRecord(w->done, stream));
+    w->done_stream = stream;
+    w->have_done = true;
+    return 0;
+}
+
+// Upload the constants only when they differ from what the device holds; a change waits
+// for all work in flight (which may still read the previous constants).
+static int set_constants(DeviceWorkspace* w, const std::vector<float>& blob)
+{
+    if (w->consts && w->consts_host == blob) return 0;
+    DKX_HIP_CHECK(hipDeviceSynchronize());
+    if (w->consts_cap < blob.size()) {
+        if (w->consts) DKX_HIP_CHECK(hipFree(w->consts));
+        w->consts = nullptr;
+        w->consts_cap = 0;
+        DKX_HIP_CHECK(hipMalloc(&w->consts, std::max<size_t>(blob.size(), 1) * sizeof(float)));
+        w->consts_cap = blob.size();
+    }
+    DKX_HIP_CHECK(hipMemcpy(w->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice));
+    w->consts_host = blob;
+    return 0;
+}
 
 static std::mutex g_ws_mu;
 static std::map<int, DeviceWorkspace*> g_ws;
@@ -70,6 +114,35 @@ static int ensure_stream(DeviceWorkspace* w)
 
 static std::atomic<int> g_tile[2] = {{0}, {0}};
 
+static std::atomic<bool> g_timing{false};
+static std::mutex g_timing_mu;
+struct Mark { int kind; hipEvent_t a, b; };
+static std::vector<Mark> g_marks;
+static std::vector<std::pair<int, float>> g_times;
+
+bool timing_on() { return g_timing.load(); }
+
+void timing_push(int kind, hipEvent_t start, hipEvent_t stop)
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_marks.push_back(Mark{kind, start, stop});
+}
+
+// resolve the events recorded by the last call (after it synchronised)
+static void timing_collect()
+{
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_times.clear();
+    for (const Mark& m : g_marks) {
+        float ms = -1.f;
+        if (hipEventSynchronize(m.b) == hipSuccess && hipEventElapsedTime(&ms, m.a, m.b) != hipSuccess) ms = -1.f;
+        g_times.emplace_back(m.kind, ms);
+        (void)hipEventDestroy(m.a);
+        (void)hipEventDestroy(m.b);
+    }
+    g_marks.clear();
+}
+
 static int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP,
                        const int* S, int nS, std::vector<NlsarStatsView>& views)
 {
@@ -110,42 +183,57 @@ static int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, 
     return 0;
 }
 
-static int nlsar_device_impl(const float* a1, const float* a2, const float* phi, float* ref,
-                             float* pha, float* coh, int height, int width, int sw, const int* P,
-                             int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
-                             int n_stats, float h, float c, float* enl_maps, float* alpha_maps,
-                             unsigned long long* ub, hipStream_t stream, DeviceWorkspace* w)
-{
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh || !P || !S) {
-        set_error("nlsar: null array argument");
-        return DKX_EINVAL;
-    }
+// Everything that can be checked without a device: argument checks, statistics, plan.
+struct NlsarCall {
     std::vector<NlsarStatsView> views;
-    int rc = check_stats(stats, n_stats, P, nP, S, nS, views);
-    if (rc) return rc;
-    NlsarProblem pb{height, width, 0, sw, P, nP, S, nS, views.data(), (int)views.size(), h, c,
-                    -1, -1, g_tile[0].load()};
+    NlsarProblem pb;
     NlsarPlan plan;
-    rc = nlsar_plan(pb, &plan);
+};
+
+static int nlsar_prepare(NlsarCall& call, int height, int width, int sw, const int* P, int nP,
+                         const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats, float h,
+                         float c)
+{
+    if (!P || !S) { set_error("nlsar: null size list"); return DKX_EINVAL; }
+    if (nP < 1 || nS < 1) { set_error("nlsar: empty patch or scale size list"); return DKX_EINVAL; }
+    int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views);
     if (rc) return rc;
-    rc = ensure(w, plan.ws_bytes);
+    call.pb = NlsarProblem{height, width, 0, sw, P, nP, S, nS, call.views.data(),
+                           (int)call.views.size(), h, c, -1, -1, g_tile[0].load()};
+    return nlsar_plan(call.pb, &call.plan);
+}
+
+static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, const float* phi,
+                             float* ref, float* pha, float* coh, float* enl_maps,
+                             float* alpha_maps, unsigned long long* ub, hipStream_t stream,
+                             DeviceWorkspace* w)
+{
+    std::vector<float> blob;
+    int rc = nlsar_build_constants(call.pb, call.plan, blob);
     if (rc) return rc;
-    rc = nlsar_run(pb, plan, a1, a2, phi, nullptr, ref, pha, coh, w->ptr, enl_maps, alpha_maps,
-                   ub, stream);
+    rc = ensure(w, call.plan.ws_bytes);
+    if (rc) return rc;
+    rc = set_constants(w, blob);
+    if (rc) return rc;
+    rc = workspace_acquire(w, stream);
+    if (rc) return rc;
+    rc = nlsar_run(call.pb, call.plan, a1, a2, phi, nullptr, ref, pha, coh, w->ptr, w->consts,
+                   enl_maps, alpha_maps, ub, stream);
+    if (rc) return rc;
+    rc = workspace_release(w, stream);
     if (rc) return rc;
     if (ub) DKX_HIP_CHECK(hipStreamSynchronize(stream));
     return 0;
 }
 
-static int nlinsar_device_impl(const float* a1, const float* a2, const float* phi, float* ref,
-                               float* pha, float* coh, int height, int width, int sw, int p,
-                               int niter, int lmin, unsigned long long* nsm, hipStream_t stream,
-                               DeviceWorkspace* w)
+struct NlinsarCall {
+    NlinsarProblem pb;
+    NlinsarPlan plan;
+};
+
+static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int p, int niter,
+                           int lmin)
 {
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) {
-        set_error("nlinsar: null array argument");
-        return DKX_EINVAL;
-    }
     // nlinsar.cpp:65-78
     float h_theo;
     if (p == 3) h_theo = 0.58f;
@@ -158,13 +246,21 @@ static int nlinsar_device_impl(const float* a1, const float* a2, const float* ph
     const int patch_area = p * p;
     const float h_para = h_theo * patch_area;
     const float T_para = (float)(2.0 / h_para * M_PI / 4 * patch_area);
-    NlinsarProblem pb{height, width, sw, p, niter, lmin, h_para, T_para, 80, g_tile[1].load()};
-    NlinsarPlan plan;
-    int rc = nlinsar_plan(pb, &plan);
+    call.pb = NlinsarProblem{height, width, sw, p, niter, lmin, h_para, T_para, 80, g_tile[1].load()};
+    return nlinsar_plan(call.pb, &call.plan);
+}
+
+static int nlinsar_device_impl(NlinsarCall& call, const float* a1, const float* a2,
+                               const float* phi, float* ref, float* pha, float* coh,
+                               unsigned long long* nsm, hipStream_t stream, DeviceWorkspace* w)
+{
+    int rc = ensure(w, call.plan.ws_bytes);
     if (rc) return rc;
-    rc = ensure(w, plan.ws_bytes);
+    rc = workspace_acquire(w, stream);
     if (rc) return rc;
-    return nlinsar_run(pb, plan, a1, a2, phi, ref, pha, coh, w->ptr, nsm, stream);
+    rc = nlinsar_run(call.pb, call.plan, a1, a2, phi, ref, pha, coh, w->ptr, nsm, stream);
+    if (rc) return rc;
+    return workspace_release(w, stream);
 }
 
 // host arrays: copy in, run on the library stream, copy out
@@ -194,29 +290,53 @@ int dkx_set_tile(int filter, int tile_index)
 
 int dkx_get_tile(int filter) { return (filter >= 0 && filter <= 1) ? g_tile[filter].load() : -1; }
 
+int dkx_set_timing(int enabled)
+{
+    g_timing.store(enabled != 0);
+    return 0;
+}
+
+int dkx_stage_times(int* kinds, float* ms, int max_entries)
+{
+    timing_collect();
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    const int n = (int)g_times.size();
+    for (int i = 0; i < n && i < max_entries; i++) {
+        if (kinds) kinds[i] = g_times[i].first;
+        if (ms) ms[i] = g_times[i].second;
+    }
+    return n;
+}
+
 int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, float* ref,
                            float* pha, float* coh, int height, int width, int sw, const int* P,
                            int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
                            float h, float c, float* enl_maps, float* alpha_maps,
                            unsigned long long* ub, void* stream)
 {
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+    NlsarCall call;
+    int rc = nlsar_prepare(call, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
     std::lock_guard<std::mutex> lk(w->mu);
-    return nlsar_device_impl(a1, a2, phi, ref, pha, coh, height, width, sw, P, nP, S, nS, stats,
-                             n_stats, h, c, enl_maps, alpha_maps, ub, (hipStream_t)stream, w);
+    return nlsar_device_impl(call, a1, a2, phi, ref, pha, coh, enl_maps, alpha_maps, ub,
+                             (hipStream_t)stream, w);
 }
 
 int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                     float* coh, int height, int width, int sw, const int* P, int nP, const int* S,
                     int nS, const dkx_nlsar_stats* stats, int n_stats, float h, float c)
 {
-    if (height <= 0 || width <= 0) { set_error("nlsar: empty image"); return DKX_EINVAL; }
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+    NlsarCall call;
+    int rc = nlsar_prepare(call, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
     std::lock_guard<std::mutex> lk(w->mu);
-    int rc = ensure_stream(w);
+    rc = ensure_stream(w);
     if (rc) return rc;
     const size_t n = (size_t)height * width;
     HostStage st;
@@ -225,8 +345,8 @@ int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
     DKX_HIP_CHECK(hipMemcpyAsync(d, a1, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(d + n, a2, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(d + 2 * n, phi, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlsar_device_impl(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width, sw,
-                           P, nP, S, nS, stats, n_stats, h, c, nullptr, nullptr, nullptr, w->stream, w);
+    rc = nlsar_device_impl(call, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, nullptr,
+                           nullptr, nullptr, w->stream, w);
     if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
     DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
@@ -239,22 +359,27 @@ int dkx_nlinsar_device(const float* a1, const float* a2, const float* phi, float
                        float* coh, int height, int width, int sw, int p, int niter, int lmin,
                        unsigned long long* nsm, void* stream)
 {
+    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+    NlinsarCall call;
+    int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
+    if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
     std::lock_guard<std::mutex> lk(w->mu);
-    return nlinsar_device_impl(a1, a2, phi, ref, pha, coh, height, width, sw, p, niter, lmin, nsm,
-                               (hipStream_t)stream, w);
+    return nlinsar_device_impl(call, a1, a2, phi, ref, pha, coh, nsm, (hipStream_t)stream, w);
 }
 
 int dkx_nlinsar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                 float* coh, int height, int width, int sw, int p, int niter, int lmin)
 {
-    if (height <= 0 || width <= 0) { set_error("nlinsar: empty image"); return DKX_EINVAL; }
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+    NlinsarCall call;
+    int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
+    if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
     std::lock_guard<std::mutex> lk(w->mu);
-    int rc = ensure_stream(w);
+    rc = ensure_stream(w);
     if (rc) return rc;
     const size_t n = (size_t)height * width;
     HostStage st;
@@ -263,8 +388,8 @@ int dkx_nlinsar(const float* a1, const float* a2, const float* phi, float* ref, 
     const float* src[6] = {a1, a2, phi, ref, pha, coh};
     for (int k = 0; k < 6; k++)
         DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlinsar_device_impl(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width,
-                             sw, p, niter, lmin, nullptr, w->stream, w);
+    rc = nlinsar_device_impl(call, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, nullptr,
+                             w->stream, w);
     if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
     DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));

@@ -39,6 +39,17 @@ struct Launch {
 void set_error(const std::string& msg);
 const char* last_error();
 
+// Stage timing (off by default): when enabled, the main launch of each fused stage is
+// bracketed by HIP events on its stream; dkx_stage_times() reports the durations.
+enum StageKind {
+    STAGE_NLSAR_SCALE = 1,       // NL-SAR scale pass, not the last
+    STAGE_NLSAR_LAST = 2,        // NL-SAR last pass (statistics + weighted mean + outputs)
+    STAGE_NLINSAR_WEIGHTS = 11,  // NL-InSAR weights / ENL / estimates
+    STAGE_NLINSAR_SMOOTH = 12,   // NL-InSAR lmin smoothing
+};
+bool timing_on();
+void timing_push(int kind, hipEvent_t start, hipEvent_t stop);
+
 }  // namespace dkx
 
 #define DKX_HIP_CHECK(expr)                                                                  \

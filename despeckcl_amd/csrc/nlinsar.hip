@@ -122,7 +122,8 @@ struct WeightArgs {
 
 constexpr int NTW = 256;
 
-template <int TY, int TX>
+// EDGE: separate kernel symbol for the thin last-row / last-column launches
+template <int TY, int TX, bool EDGE>
 __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
 {
     constexpr int NPIX = TY * TX;
@@ -138,8 +139,18 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     float* rK = rS + PY * TX;
 
     const int tid = threadIdx.x;
-    const int ty0 = a.L.vy0 + blockIdx.y * TY;
-    const int tx0 = a.L.vx0 + blockIdx.x * TX;
+    // Blocks are sub-tiles of the interiors (step x step) of the reference's tiles, so every
+    // output of a block belongs to one reference tile: block row by = reference tile row
+    // * nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile of a reference
+    // tile is cut at the tile's end.  One-row / one-column launches have a single block
+    // row / column starting at the launch origin.
+    const int nsy = (a.step + TY - 1) / TY, nsx = (a.step + TX - 1) / TX;
+    const int ry = blockIdx.y / nsy, sy = blockIdx.y - ry * nsy;
+    const int rx = blockIdx.x / nsx, sx_ = blockIdx.x - rx * nsx;
+    const int ty0 = a.L.vy0 + ry * a.step + sy * TY;
+    const int tx0 = a.L.vx0 + rx * a.step + sx_ * TX;
+    const int rows_lim = min(TY, a.step - sy * TY);
+    const int cols_lim = min(TX, a.step - sx_ * TX);
     // interior origin of the reference tile that computes this block's outputs
     const int Y0 = (ty0 / a.step) * a.step;
     const int X0 = (tx0 / a.step) * a.step;
@@ -154,7 +165,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
         const int pq = tid + q * NTW;
         const int py = pq / TX, px = pq - (pq / TX) * TX;
         const int iy = ty0 + py - a.L.vy0, ix = tx0 + px - a.L.vx0;
-        valid[q] = pq < NPIX && iy < a.L.ny && ix < a.L.nx;
+        valid[q] = pq < NPIX && py < rows_lim && px < cols_lim && iy < a.L.ny && ix < a.L.nx;
         opix[q] = valid[q] ? (long long)(a.L.oy0 + iy) * a.W + (a.L.ox0 + ix) : 0;
     }
 
@@ -425,17 +436,9 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->overlap = plan->wsh + plan->psh;
     plan->step = pb.ref_tile - 2 * plan->overlap;
     if (plan->step <= 0) { set_error("nlinsar: search window and patch too large for the 80-pixel tiles"); return -1; }
-    if (plan->step != 54 && pb.tile >= 0) {
-        // block shapes below divide 54; other window/patch sizes use one row of tiles of
-        // the reference interior width (always aligned)
-    }
     plan->tile = std::min(std::max(pb.tile, 0), 2);
     plan->TY = kTY[plan->tile];
     plan->TX = kTX[plan->tile];
-    if (plan->step % plan->TY != 0 || plan->step % plan->TX != 0) {
-        set_error("nlinsar: block shape must divide the reference tile interior (54 for sw 21, p 7)");
-        return -1;
-    }
     // virtual position written to the last row/column by the reference's 80x80 tiling
     // (tile_iterator.cpp:37-44, sub_images.cpp:47-69): the last tile's last interior row
     auto vmax = [&](int n) {
@@ -446,8 +449,8 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->eh = vmax(pb.H) - (pb.H - 1);
     plan->ew = vmax(pb.W) - (pb.W - 1);
     const int H = pb.H, W = pb.W;
-    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->TY), H - 1 + plan->eh + plan->TY);
-    const int Wx = std::max(round_up(std::max(W - 1, 1), plan->TX), W - 1 + plan->ew + plan->TX);
+    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->step) + plan->TY, H - 1 + plan->eh + plan->TY);
+    const int Wx = std::max(round_up(std::max(W - 1, 1), plan->step) + plan->TX, W - 1 + plan->ew + plan->TX);
     const int o = plan->overlap;
     plan->R = Grid{nullptr, 0, round_up(Wx + 2 * o, 64), o, o, Hx + 2 * o, Wx + 2 * o};
     plan->R.plane = (long long)plan->R.pitch * plan->R.h;
@@ -480,15 +483,36 @@ static int launch_weights(const WeightArgs& base, const Launch* L, int nl, hipSt
     const int np2 = 2 * base.psh;
     const size_t lds = sizeof(float) * (2 * (size_t)(TY + np2) * (TX + np2) + 2 * (size_t)(TY + np2) * TX);
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, true>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int i = 0; i < nl; i++) {
         if (L[i].ny <= 0 || L[i].nx <= 0) continue;
         WeightArgs a = base;
         a.L = L[i];
-        dim3 grid((L[i].nx + TX - 1) / TX, (L[i].ny + TY - 1) / TY);
-        hipLaunchKernelGGL((nlinsar_weights<TY, TX>), grid, dim3(NTW), lds, stream, a);
+        // reference tiles covered, times sub-tiles per reference tile; thin launches
+        // (one row / column) use a single block row / column
+        const int nsy = (base.step + TY - 1) / TY, nsx = (base.step + TX - 1) / TX;
+        const int gy = L[i].ny == 1 ? 1 : ((L[i].ny + base.step - 1) / base.step) * nsy;
+        const int gx = L[i].nx == 1 ? 1 : ((L[i].nx + base.step - 1) / base.step) * nsx;
+        dim3 grid(gx, gy);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool timed = i == 0 && timing_on();
+        if (timed) {
+            DKX_HIP_CHECK(hipEventCreate(&e0));
+            DKX_HIP_CHECK(hipEventCreate(&e1));
+            DKX_HIP_CHECK(hipEventRecord(e0, stream));
+        }
+        if (i == 0)
+            hipLaunchKernelGGL((nlinsar_weights<TY, TX, false>), grid, dim3(NTW), lds, stream, a);
+        else
+            hipLaunchKernelGGL((nlinsar_weights<TY, TX, true>), grid, dim3(NTW), lds, stream, a);
         DKX_HIP_CHECK(hipGetLastError());
+        if (timed) {
+            DKX_HIP_CHECK(hipEventRecord(e1, stream));
+            timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
+        }
     }
     return 0;
 }
@@ -566,8 +590,18 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         const int nfull = pb.sw * pb.sw;
         const size_t lds = (size_t)plan.heap_block * (2 * pb.lmin + (nfull + 31) / 32) * 4;
         const int blocks = std::max(1, std::min((int)((n + plan.heap_block - 1) / plan.heap_block), ncu * 8));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing_on()) {
+            DKX_HIP_CHECK(hipEventCreate(&e0));
+            DKX_HIP_CHECK(hipEventCreate(&e1));
+            DKX_HIP_CHECK(hipEventRecord(e0, stream));
+        }
         hipLaunchKernelGGL(nlinsar_smooth, dim3(blocks), dim3(plan.heap_block), lds, stream, sa);
         DKX_HIP_CHECK(hipGetLastError());
+        if (e0) {
+            DKX_HIP_CHECK(hipEventRecord(e1, stream));
+            timing_push(STAGE_NLINSAR_SMOOTH, e0, e1);
+        }
         // the estimates just written become the next iteration's input
         for (int k = 0; k < 3; k++) {
             float* t = nxt[k];

@@ -127,7 +127,9 @@ constexpr int NT = 256;
 // 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
 __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
 
-template <int TY, int TX, int NP, bool LAST>
+// EDGE only gives the launches of the last row / column their own kernel symbol, so that
+// per-kernel profiles of the main launch are not mixed with the thin edge launches.
+template <int TY, int TX, int NP, bool LAST, bool EDGE>
 __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
 {
     constexpr int ROWS = NT / TX;       // thread rows
@@ -365,6 +367,9 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
                 s_mean = ws;
                 m0_mean = s[2];
                 m1_mean = s[4];
+                // centre weight 1 on the off-diagonal sums as well
+                macc[q][0] = fmaf(1.0f, Cr[cc], macc[q][0]);
+                macc[q][1] = fmaf(1.0f, Ci[cc], macc[q][1]);
             }
         }
         if (!valid) continue;
@@ -444,15 +449,31 @@ static int launch_scale_pass(const ScaleArgs& base, const Launch* L, int nl, int
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
     }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST, false>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST, true>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     for (int i = 0; i < nl; i++) {
         if (L[i].ny <= 0 || L[i].nx <= 0) continue;
         ScaleArgs a = base;
         a.L = L[i];
         dim3 grid((L[i].nx + TX - 1) / TX, (L[i].ny + TY - 1) / TY);
-        hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST>), grid, dim3(NT), lds, stream, a);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool timed = i == 0 && timing_on();
+        if (timed) {
+            DKX_HIP_CHECK(hipEventCreate(&e0));
+            DKX_HIP_CHECK(hipEventCreate(&e1));
+            DKX_HIP_CHECK(hipEventRecord(e0, stream));
+        }
+        if (i == 0)
+            hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST, false>), grid, dim3(NT), lds, stream, a);
+        else
+            hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST, true>), grid, dim3(NT), lds, stream, a);
         DKX_HIP_CHECK(hipGetLastError());
+        if (timed) {
+            DKX_HIP_CHECK(hipEventRecord(e1, stream));
+            timing_push(LAST ? STAGE_NLSAR_LAST : STAGE_NLSAR_SCALE, e0, e1);
+        }
     }
     return 0;
 }
@@ -479,9 +500,7 @@ static int dispatch(const ScaleArgs& a, int np, bool last, const Launch* L, int 
                         : dispatch_np<2, false>(a, L, nl, lut_stride, tile, stream);
     case 3: return last ? dispatch_np<3, true>(a, L, nl, lut_stride, tile, stream)
                         : dispatch_np<3, false>(a, L, nl, lut_stride, tile, stream);
-    case 4: return last ? dispatch_np<4, true>(a, L, nl, lut_stride, tile, stream)
-                        : dispatch_np<4, false>(a, L, nl, lut_stride, tile, stream);
-    default: set_error("nlsar: at most 4 patch sizes are supported"); return -1;
+    default: set_error("nlsar: internal error, more than 3 patch sizes in one pass"); return -1;
     }
 }
 
@@ -492,8 +511,8 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
 {
     if (pb.H <= 0 || pb.W <= 0) { set_error("nlsar: empty image"); return -1; }
     if (pb.sw < 3 || pb.sw % 2 == 0) { set_error("nlsar: search window size must be odd and >= 3"); return -1; }
-    if (pb.nP < 1 || pb.nP > NLSAR_MAX_NP || pb.nS < 1 || pb.nS > NLSAR_MAX_NS) {
-        set_error("nlsar: 1-4 patch sizes and 1-8 scale sizes are supported");
+    if (pb.nP < 1 || pb.nP > NLSAR_MAX_PATCHES || pb.nS < 1 || pb.nS > NLSAR_MAX_NS) {
+        set_error("nlsar: 1-16 patch sizes and 1-8 scale sizes are supported");
         return -1;
     }
     int pmax = 0, smax = 0;
@@ -539,50 +558,63 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
     plan->off_C = take(sizeof(float) * 4 * plan->C.plane);
     plan->off_A = take(sizeof(float) * 5 * plan->A.plane);
-    plan->off_lut = take(sizeof(float) * (size_t)plan->nparam * maxlut);
-    plan->off_gauss = take(sizeof(float) * (size_t)NLSAR_MAX_NS * 31 * 31);
+    plan->const_floats = (size_t)plan->nparam * maxlut + (size_t)NLSAR_MAX_NS * 31 * 31;
     plan->off_flag = take((size_t)H * W);
     plan->off_ub = take(sizeof(unsigned long long));
     plan->ws_bytes = off;
     return 0;
 }
 
+// statistics entry of every parameter index (scale-major loop order)
+int nlsar_param_stats(const NlsarProblem& pb, const NlsarPlan& plan,
+                      std::vector<const NlsarStatsView*>& pst)
+{
+    pst.assign(plan.nparam, nullptr);
+    for (int si = 0; si < pb.nS; si++)
+        for (int pi = 0; pi < pb.nP; pi++) {
+            const int idx = si * pb.nP + pi;
+            for (int k = 0; k < pb.nst; k++)
+                if (pb.stats[k].patch_size == pb.P[pi] && pb.stats[k].scale_size == pb.S[si]) pst[idx] = &pb.stats[k];
+            if (!pst[idx]) { set_error("nlsar: statistics missing for a (patch, scale) pair"); return -3; }
+        }
+    return 0;
+}
+
+// Host image of the call's device constants: the weight LUT of every parameter index,
+// then the Gaussian taps of every scale (31*31 floats each).
+int nlsar_build_constants(const NlsarProblem& pb, const NlsarPlan& plan, std::vector<float>& blob)
+{
+    std::vector<const NlsarStatsView*> pst;
+    if (nlsar_param_stats(pb, plan, pst)) return -3;
+    blob.assign(plan.const_floats, 0.f);
+    for (int idx = 0; idx < plan.nparam; idx++) {
+        std::vector<float> w = weight_lut(*pst[idx], pb.h, pb.c);
+        std::copy(w.begin(), w.end(), blob.begin() + (size_t)idx * plan.lut_stride);
+    }
+    const size_t goff = (size_t)plan.nparam * plan.lut_stride;
+    for (int si = 0; si < pb.nS; si++) {
+        std::vector<float> g = gen_gauss(pb.S[si]);
+        std::copy(g.begin(), g.end(), blob.begin() + goff + (size_t)si * 31 * 31);
+    }
+    return 0;
+}
+
 int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, const float* d_a2,
               const float* d_phi, const float* d_cov, float* d_ref, float* d_phase, float* d_coh,
-              void* d_ws, float* d_enl_maps, float* d_alpha_maps, unsigned long long* ub_host,
-              hipStream_t stream)
+              void* d_ws, const float* d_const, float* d_enl_maps, float* d_alpha_maps,
+              unsigned long long* ub_host, hipStream_t stream)
 {
     char* ws = (char*)d_ws;
     Grid C = plan.C, A = plan.A;
     C.base = (float*)(ws + plan.off_C);
     A.base = (float*)(ws + plan.off_A);
-    float* d_lut = (float*)(ws + plan.off_lut);
-    float* d_gauss = (float*)(ws + plan.off_gauss);
+    const float* d_lut = d_const;
+    const float* d_gauss = d_const + (size_t)plan.nparam * plan.lut_stride;
     unsigned char* d_flag = (unsigned char*)(ws + plan.off_flag);
     unsigned long long* d_ub = (unsigned long long*)(ws + plan.off_ub);
     const int H = pb.H, W = pb.W;
-
-    // per-parameter weight LUTs, parameter index = position in the scale-major loop
-    std::vector<float> hl((size_t)plan.nparam * plan.lut_stride, 0.f);
-    std::vector<const NlsarStatsView*> pst(plan.nparam);
-    for (int si = 0; si < pb.nS; si++)
-        for (int pi = 0; pi < pb.nP; pi++) {
-            const int idx = si * pb.nP + pi;
-            const NlsarStatsView* st = nullptr;
-            for (int k = 0; k < pb.nst; k++)
-                if (pb.stats[k].patch_size == pb.P[pi] && pb.stats[k].scale_size == pb.S[si]) st = &pb.stats[k];
-            if (!st) { set_error("nlsar: statistics missing for a (patch, scale) pair"); return -3; }
-            pst[idx] = st;
-            std::vector<float> w = weight_lut(*st, pb.h, pb.c);
-            std::copy(w.begin(), w.end(), hl.begin() + (size_t)idx * plan.lut_stride);
-        }
-    DKX_HIP_CHECK(hipMemcpyAsync(d_lut, hl.data(), hl.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-    std::vector<float> hg((size_t)NLSAR_MAX_NS * 31 * 31, 0.f);
-    for (int si = 0; si < pb.nS; si++) {
-        std::vector<float> g = gen_gauss(pb.S[si]);
-        std::copy(g.begin(), g.end(), hg.begin() + (size_t)si * 31 * 31);
-    }
-    DKX_HIP_CHECK(hipMemcpyAsync(d_gauss, hg.data(), hg.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    std::vector<const NlsarStatsView*> pst;
+    if (nlsar_param_stats(pb, plan, pst)) return -3;
     DKX_HIP_CHECK(hipMemsetAsync(d_ub, 0, sizeof(unsigned long long), stream));
 
     {
@@ -602,38 +634,64 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
     L[2] = Launch{0, W - 1 + plan.ew, H - 1, 1, 0, W - 1};
     L[3] = Launch{H - 1 + plan.eh, W - 1 + plan.ew, 1, 1, H - 1, W - 1};
 
-    // scale order: every scale but the largest, then the largest (its pass also forms
-    // the weighted mean with the largest patch size, the key the selection keeps)
-    std::vector<int> order;
-    for (int si = 0; si < pb.nS; si++) if (pb.S[si] != plan.smax) order.push_back(si);
-    for (int si = 0; si < pb.nS; si++) if (pb.S[si] == plan.smax) order.push_back(si);
-    int pmax_slot = 0;
-    for (int pi = 0; pi < pb.nP; pi++) if (pb.P[pi] > pb.P[pmax_slot]) pmax_slot = pi;
+    // Passes: each handles one scale and up to three patch sizes (one dissimilarity field
+    // feeds all of them).  The pass holding the largest patch at the largest scale runs last:
+    // it also forms the weighted mean with that (patch, scale), the key the selection keeps.
+    int pmax_i = 0;
+    for (int pi = 0; pi < pb.nP; pi++) if (pb.P[pi] > pb.P[pmax_i]) pmax_i = pi;
+    int smax_i = 0;
+    for (int si = 0; si < pb.nS; si++) if (pb.S[si] > pb.S[smax_i]) smax_i = si;
+    struct Pass { int si; std::vector<int> pis; bool last; };
+    std::vector<Pass> passes;
+    for (int si = 0; si < pb.nS; si++) {
+        std::vector<int> rest;
+        for (int pi = 0; pi < pb.nP; pi++) if (!(si == smax_i && pi == pmax_i)) rest.push_back(pi);
+        if (si == smax_i) {
+            // the final chunk of this scale: the largest patch plus up to two others
+            std::vector<int> fin{pmax_i};
+            while (fin.size() < 3 && !rest.empty()) { fin.insert(fin.begin(), rest.back()); rest.pop_back(); }
+            for (size_t k = 0; k < rest.size(); k += 3)
+                passes.push_back(Pass{si, std::vector<int>(rest.begin() + k, rest.begin() + std::min(rest.size(), k + 3)), false});
+            passes.push_back(Pass{si, fin, true});
+        } else {
+            for (size_t k = 0; k < rest.size(); k += 3)
+                passes.push_back(Pass{si, std::vector<int>(rest.begin() + k, rest.begin() + std::min(rest.size(), k + 3)), false});
+        }
+    }
+    // the final pass must come last
+    std::stable_partition(passes.begin(), passes.end(), [](const Pass& q) { return !q.last; });
 
-    for (size_t oi = 0; oi < order.size(); oi++) {
-        const int si = order[oi];
+    int cur_scale = -1;
+    for (size_t oi = 0; oi < passes.size(); oi++) {
+        const Pass& ps = passes[oi];
+        const int si = ps.si;
         const int s = pb.S[si];
-        dim3 blk(256), grd((A.w + 255) / 256, A.h);
-        hipLaunchKernelGGL(nlsar_scale_average, grd, blk, 0, stream, C, A,
-                           (const float*)(d_gauss + (size_t)si * 31 * 31), s, gamma);
-        DKX_HIP_CHECK(hipGetLastError());
+        if (si != cur_scale) {
+            dim3 blk(256), grd((A.w + 255) / 256, A.h);
+            hipLaunchKernelGGL(nlsar_scale_average, grd, blk, 0, stream, C, A,
+                               (const float*)(d_gauss + (size_t)si * 31 * 31), s, gamma);
+            DKX_HIP_CHECK(hipGetLastError());
+            cur_scale = si;
+        }
         ScaleArgs sa;
         memset(&sa, 0, sizeof(sa));
         sa.A = A;
         sa.C = C;
         sa.wsh = plan.wsh;
         sa.P = plan.P;
-        for (int pi = 0; pi < pb.nP; pi++) {
+        sa.mean_slot = -1;
+        for (size_t k = 0; k < ps.pis.size(); k++) {
+            const int pi = ps.pis[k];
             const int idx = si * pb.nP + pi;
-            sa.psh[pi] = (pb.P[pi] - 1) / 2;
-            sa.pidx[pi] = idx;
-            sa.lutw[pi] = d_lut + (size_t)idx * plan.lut_stride;
-            sa.dmin[pi] = pst[idx]->dissims_min;
-            sa.dmax[pi] = pst[idx]->dissims_max;
-            sa.lut_size[pi] = pst[idx]->lut_size;
+            sa.psh[k] = (pb.P[pi] - 1) / 2;
+            sa.pidx[k] = idx;
+            sa.lutw[k] = d_lut + (size_t)idx * plan.lut_stride;
+            sa.dmin[k] = pst[idx]->dissims_min;
+            sa.dmax[k] = pst[idx]->dissims_max;
+            sa.lut_size[k] = pst[idx]->lut_size;
+            if (ps.last && pi == pmax_i) sa.mean_slot = (int)k;
         }
         sa.lut_stride = plan.lut_stride;
-        sa.mean_slot = pmax_slot;
         sa.H = H;
         sa.W = W;
         sa.enl_nb = d_enl_maps;
@@ -644,12 +702,11 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
         sa.phase = d_phase;
         sa.coh = d_coh;
         sa.ub_count = d_ub;
-        const bool last = oi + 1 == order.size();
-        if (last && oi == 0) {
-            // a single scale: nothing else wrote the flags
+        if (ps.last && oi == 0) {
+            // a single pass: nothing else wrote the flags
             DKX_HIP_CHECK(hipMemsetAsync(d_flag, 0, (size_t)H * W, stream));
         }
-        int rc = dispatch(sa, pb.nP, last, L, 4, plan.lut_stride, plan.tile, stream);
+        int rc = dispatch(sa, (int)ps.pis.size(), ps.last, L, 4, plan.lut_stride, plan.tile, stream);
         if (rc) return rc;
     }
     if (ub_host) {

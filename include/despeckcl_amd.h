@@ -112,6 +112,15 @@ double dkx_chi2_inv(double p, double nu);
 int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
 int dkx_get_tile(int filter);
 
+/* Stage timing.  When enabled, the main launch of every fused stage is bracketed by HIP
+ * events on the caller's stream.  dkx_stage_times() synchronises with those events and
+ * returns the stages of the calls since the previous dkx_stage_times() call: kinds
+ * 1 = NL-SAR scale pass, 2 = NL-SAR last scale pass (statistics, weighted mean, outputs),
+ * 11 = NL-InSAR weights, 12 = NL-InSAR smoothing; durations in milliseconds.  Returns the
+ * number of entries (entries beyond max_entries are dropped). */
+int dkx_set_timing(int enabled);
+int dkx_stage_times(int* kinds, float* ms, int max_entries);
+
 /* Releases the device workspace cached by the library on the current device. */
 void dkx_release_workspace(void);
 

@@ -629,9 +629,11 @@ static int params_less(int p1, int s1, int p2, int s2)
 }
 
 /* covmat_ori: 2*dim*dim planes of th x tw; covmat_filt receives the interior */
+/* cond (optional, th x tw, interior written): min over the diagonal of var / I_nl^2 of the
+ * selected parameter; alpha = 1 - I_nl^2 / var is ill-conditioned where it is small */
 static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int tw, int dim,
                       int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
-                      int nst, float hpar, float cpar, long long* ub_pixels)
+                      int nst, float hpar, float cpar, long long* ub_pixels, float* cond)
 {
     int pmax = 0, smax = 0;
     for (int i = 0; i < nP; i++) pmax = imax(pmax, P[i]);
@@ -656,6 +658,7 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
     float* all_weights = (float*)malloc(sizeof(float) * (size_t)nparam * nfull * n_ori);
     float* enls_nobias = (float*)malloc(sizeof(float) * (size_t)nparam * n_ori);
     float* alphas = (float*)malloc(sizeof(float) * (size_t)nparam * n_ori);
+    float* conds = (float*)malloc(sizeof(float) * (size_t)nparam * n_ori);
     int* keyp = (int*)malloc(sizeof(int) * nparam);
     int* keys = (int*)malloc(sizeof(int) * nparam);
 
@@ -692,6 +695,15 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
                               smax, dim);
             float* al = alphas + (size_t)idx * n_ori;
             orc_alphas(int_nl, wvar, al, h_ori, w_ori, dim, nlooks);
+            for (size_t i = 0; i < n_ori; i++) {
+                double cmin = INFINITY;
+                for (int d = 0; d < dim; d++) {
+                    const double v = wvar[d * n_ori + i], m = int_nl[d * n_ori + i];
+                    const double q = v / (m * m);
+                    if (!(q >= cmin)) cmin = q;
+                }
+                conds[(size_t)idx * n_ori + i] = (float)cmin;
+            }
             float* en = enls_nobias + (size_t)idx * n_ori;
             for (size_t i = 0; i < n_ori; i++) en[i] = orc_enl_nobias(enl[i], al[i], wsums[i]);
         }
@@ -731,10 +743,18 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
     }
     orc_weighted_means(covmat_ori, covmat_filt, best_w, best_alpha, h_ori, w_ori, sw, pmax, smax,
                        dim);
+    if (cond) {
+        const int o = wsh + psh + (smax - 1) / 2;
+        for (int y = 0; y < h_ori; y++)
+            for (int x = 0; x < w_ori; x++) {
+                const size_t i = (size_t)y * w_ori + x;
+                cond[(size_t)(y + o) * tw + x + o] = conds[(size_t)best[i] * n_ori + i];
+            }
+    }
 
     free(best_w); free(best_alpha); free(best); free(order);
     free(enl); free(wsums); free(wvar); free(int_nl); free(pat); free(pix); free(avg);
-    free(keys); free(keyp); free(alphas); free(enls_nobias); free(all_weights); free(rescaled);
+    free(keys); free(keyp); free(conds); free(alphas); free(enls_nobias); free(all_weights); free(rescaled);
     return 0;
 }
 
@@ -793,6 +813,7 @@ typedef struct {
     const orc_stats* st;
     float h, c;
     long long ub;
+    int want_cond;
 } nlsar_ctx;
 
 static int nlsar_insar_tile(const float* tin, float* tout, int th, int tw, void* vctx)
@@ -804,7 +825,7 @@ static int nlsar_insar_tile(const float* tin, float* tout, int th, int tw, void*
     /* data_to_covmat, conversion.cpp:22-62 */
     orc_covmat_create(tin, tin + n, tin + 2 * n, cov, th, tw);
     int rc = nlsar_tile(cov, filt, th, tw, 2, cx->sw, cx->P, cx->nP, cx->S, cx->nS, cx->st,
-                        cx->nst, cx->h, cx->c, &cx->ub);
+                        cx->nst, cx->h, cx->c, &cx->ub, cx->want_cond ? tout + 3 * n : NULL);
     /* covmat_to_data, conversion.cpp:88-142 */
     if (rc == 0) orc_covmat_decompose(filt, tout, tout + n, tout + 2 * n, th, tw);
     free(filt);
@@ -816,13 +837,23 @@ static int nlsar_covmat_tile(const float* tin, float* tout, int th, int tw, void
 {
     nlsar_ctx* cx = (nlsar_ctx*)vctx;
     return nlsar_tile(tin, tout, th, tw, cx->dim, cx->sw, cx->P, cx->nP, cx->S, cx->nS, cx->st,
-                      cx->nst, cx->h, cx->c, &cx->ub);
+                      cx->nst, cx->h, cx->c, &cx->ub, NULL);
 }
 
 int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref_filt,
                     float* phase_filt, float* coh_filt, int height, int width, int sw,
                     const int* P, int nP, const int* S, int nS, const orc_stats* st, int nst,
                     float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
+{
+    return orc_nlsar_insar_cond(a1, a2, phi, ref_filt, phase_filt, coh_filt, NULL, height, width,
+                                sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
+}
+
+int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                         float* phase_filt, float* coh_filt, float* cond, int height, int width,
+                         int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                         int nst, float hpar, float cpar, int tile_h, int tile_w,
+                         long long* ub_pixels)
 {
     int rc = check_nlsar_args(height, width, 2, sw, P, nP, S, nS, st, nst);
     if (rc) return rc;
@@ -832,19 +863,21 @@ int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
     if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
     const size_t n = (size_t)height * width;
     float* in = (float*)malloc(sizeof(float) * 3 * n);
-    float* out = (float*)malloc(sizeof(float) * 3 * n);
+    float* out = (float*)calloc(4 * n, sizeof(float));
     memcpy(in, a1, n * sizeof(float));
     memcpy(in + n, a2, n * sizeof(float));
     memcpy(in + 2 * n, phi, n * sizeof(float));
     memcpy(out, ref_filt, n * sizeof(float));
     memcpy(out + n, phase_filt, n * sizeof(float));
     memcpy(out + 2 * n, coh_filt, n * sizeof(float));
-    nlsar_ctx cx = {2, sw, nP, nS, nst, P, S, st, hpar, cpar, 0};
-    rc = map_tiles(in, out, height, width, 3, 3, tile_h, tile_w, overlap, nlsar_insar_tile, &cx);
+    nlsar_ctx cx = {2, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, cond != NULL};
+    rc = map_tiles(in, out, height, width, 3, cond ? 4 : 3, tile_h, tile_w, overlap,
+                   nlsar_insar_tile, &cx);
     if (rc == 0) {
         memcpy(ref_filt, out, n * sizeof(float));
         memcpy(phase_filt, out + n, n * sizeof(float));
         memcpy(coh_filt, out + 2 * n, n * sizeof(float));
+        if (cond) memcpy(cond, out + 3 * n, n * sizeof(float));
     }
     if (ub_pixels) *ub_pixels = cx.ub;
     free(in);
@@ -862,7 +895,7 @@ int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, in
     if (tile_h <= 0) tile_h = height + 3 * overlap;
     if (tile_w <= 0) tile_w = width + 3 * overlap;
     if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
-    nlsar_ctx cx = {dim, sw, nP, nS, nst, P, S, st, hpar, cpar, 0};
+    nlsar_ctx cx = {dim, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, 0};
     rc = map_tiles(covmat_raw, covmat_filt, height, width, 2 * dim * dim, 2 * dim * dim, tile_h,
                    tile_w, overlap, nlsar_covmat_tile, &cx);
     if (ub_pixels) *ub_pixels = cx.ub;

@@ -76,6 +76,14 @@ int orc_nlsar_insar(const float* a1, const float* a2, const float* phi,
                     const orc_stats* stats, int n_stats, float h_param, float c_param,
                     int tile_h, int tile_w, long long* ub_pixels);
 
+/* same, also writing cond (H x W): per pixel min over the diagonal of var / I_nl^2 of the
+ * selected parameter (alpha = 1 - I_nl^2/var is rounding noise where this is tiny) */
+int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                         float* phase_filt, float* coh_filt, float* cond, int height, int width,
+                         int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                         int nst, float hpar, float cpar, int tile_h, int tile_w,
+                         long long* ub_pixels);
+
 /* despeckcl::nlsar (covariance overload), nlsar.cpp:171-194; covmats are 2*dim*dim planes. */
 int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, int width, int dim,
                      int search_window_size, const int* patch_sizes, int n_patch,

@@ -112,21 +112,27 @@ def stats_build(dissims, lut_size=256):
 
 
 def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats,
-                h=15.0, c=49.0, tile=(0, 0), init=None):
+                h=15.0, c=49.0, tile=(0, 0), init=None, want_cond=False):
+    """Returns (ref, phase, coh, ub_pixels), plus the conditioning map when want_cond
+    (per pixel min over the diagonal of var / I_nl^2 of the selected parameter)."""
     a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
     H, W = a1.shape
     out = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
+    cond = np.zeros((H, W), np.float32)
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
-    rc = lib().orc_nlsar_insar(
+    rc = lib().orc_nlsar_insar_cond(
         a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
         out[0].ctypes.data_as(C.c_void_p), out[1].ctypes.data_as(C.c_void_p),
-        out[2].ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+        out[2].ctypes.data_as(C.c_void_p), cond.ctypes.data_as(C.c_void_p) if want_cond else None,
+        C.c_int(H), C.c_int(W), C.c_int(search_window_size),
         _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)),
         arr, C.c_int(len(stats)), C.c_float(h), C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]),
         C.byref(ub))
     if rc != 0:
         raise ValueError(f"oracle nlsar_insar failed ({rc})")
+    if want_cond:
+        return out[0], out[1], out[2], ub.value, cond
     return out[0], out[1], out[2], ub.value
 
 

@@ -1,0 +1,168 @@
+"""Host-side checks of the product library that need no GPU: the C ABI loads and exports
+every declared symbol, argument errors are reported before any device work, and the host
+parts (training, statistics, the chi-square LUT, statistics files) match the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def D():
+    import despeckcl_amd
+    despeckcl_amd.lib()
+    return despeckcl_amd
+
+
+def _declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if not f.endswith(".h"):
+            continue
+        txt = open(os.path.join(inc, f)).read()
+        if 'extern "C"' not in txt:
+            continue
+        names |= set(re.findall(r"\b(dkx_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(D):
+    lib = D.lib()
+    names = _declared_symbols()
+    assert len(names) >= 13
+    for n in names:
+        assert hasattr(lib, n), n
+    from despeckcl_amd._lib import SIGNATURES
+    assert set(SIGNATURES) == set(names)
+
+
+def test_cxx_api_symbols_exported(D):
+    syms = os.popen(f"nm -D --defined-only {D._lib.LIB_PATH}").read()
+    for mangled_part in ("despeckcl5nlsar", "despeckcl7nlinsar", "despeckcl14nlsar_training",
+                         "store_nlsar_stats_collection", "load_nlsar_stats_collection"):
+        assert mangled_part in syms, mangled_part
+
+
+def _stats_for(D, keys):
+    coll = D.nlsar_stats_collection()
+    for p, s in keys:
+        st = D.stats(lut_size=256)
+        st.dissims_min, st.dissims_max = 1.0, 10.0
+        st.quantilles = np.linspace(0, 0.99, 256).astype(np.float32)
+        st.chi2cdf_inv = np.linspace(0, 100, 256).astype(np.float32)
+        coll[D.params(p, s)] = st
+    return coll
+
+
+def test_argument_errors_without_device(D):
+    a = np.ones((16, 16), np.float32)
+    st = _stats_for(D, [(3, 1), (5, 1)])
+    with pytest.raises(ValueError, match="odd"):
+        D.nlsar_insar(a, a, a, 20, [3, 5], [1], st)
+    with pytest.raises(ValueError, match="patch x scale|one entry"):
+        D.nlsar_insar(a, a, a, 21, [3, 5, 7], [1], st)
+    with pytest.raises(ValueError, match="not in the patch x scale set"):
+        D.nlsar_insar(a, a, a, 21, [3, 7], [1], st)
+    with pytest.raises(ValueError, match="odd"):
+        D.nlsar_insar(a, a, a, 21, [3, 5], [2], _stats_for(D, [(3, 2), (5, 2)]))
+    with pytest.raises(ValueError, match="shape"):
+        D.nlsar_insar(a, a[:8], a, 21, [3, 5], [1], st)
+    with pytest.raises(ValueError, match="empty"):
+        z = np.zeros((0, 5), np.float32)
+        D.nlsar_insar(z, z, z, 21, [3, 5], [1], st)
+    bad = _stats_for(D, [(3, 1), (5, 1)])
+    bad[D.params(3, 1)].quantilles[4] = 1.5
+    with pytest.raises(ValueError, match="quantile"):
+        D.nlsar_insar(a, a, a, 21, [3, 5], [1], bad)
+    with pytest.raises(ValueError, match="patch sizes 3 and 7"):
+        D.nlinsar(a, a, a, 21, 5, 2, 10)
+    with pytest.raises(ValueError, match="odd"):
+        D.nlinsar(a, a, a, 22, 7, 2, 10)
+    with pytest.raises(ValueError, match="lmin"):
+        D.nlinsar(a, a, a, 21, 7, 2, 0)
+
+
+def test_training_matches_oracle(D, oracle):
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.homogeneous_area(32, S.SEED_BASE + 100)
+    got = D.nlsar_train_insar(a1, a2, ph, [3, 5, 7], [1, 3])
+    want = oracle.nlsar_train_insar(a1, a2, ph, [3, 5, 7], [1, 3])
+    assert list(got.keys()) == sorted(want.keys())
+    for k in got:
+        g, w = got[k], want[(k.patch_size, k.scale_size)]
+        assert g.lut_size == 256
+        assert g.dissims_min == w.dissims_min and g.dissims_max == w.dissims_max
+        assert np.array_equal(g.quantilles, w.quantilles)
+        assert np.array_equal(g.chi2cdf_inv, w.chi2cdf_inv)
+
+
+def test_training_covmat_matches_oracle(D, oracle):
+    rng = np.random.default_rng(3)
+    for dim in (2, 3):
+        z = (rng.standard_normal((dim, 20, 20)) + 1j * rng.standard_normal((dim, 20, 20))) / np.sqrt(2)
+        cov = np.einsum("ahw,bhw->abhw", z, np.conj(z)).astype(np.complex64)
+        inter = np.empty((2 * dim * dim, 20, 20), np.float32)
+        inter[::2] = cov.real.reshape(-1, 20, 20)
+        inter[1::2] = cov.imag.reshape(-1, 20, 20)
+        got = D.nlsar_train(cov, [3, 5], [1, 3])
+        want = oracle.nlsar_train_covmat(inter, dim, [3, 5], [1, 3])
+        for k in got:
+            g, w = got[k], want[(k.patch_size, k.scale_size)]
+            assert g.dissims_min == w.dissims_min and g.dissims_max == w.dissims_max
+            assert np.array_equal(g.quantilles, w.quantilles)
+
+
+def test_stats_constructor_matches_oracle(D, oracle):
+    rng = np.random.default_rng(4)
+    d = rng.gamma(2.0, 2.0, 5000).astype(np.float32)
+    d[::97] = np.nan
+    s = D.stats(d, 256)
+    o, _ = oracle.stats_build(d, 256)
+    assert s.dissims_min == o.dissims_min and s.dissims_max == o.dissims_max
+    assert np.array_equal(s.quantilles, o.quantilles)
+    assert np.array_equal(s.chi2cdf_inv, o.chi2cdf_inv)
+
+
+def test_chi2_matches_scipy_fixture(D):
+    ref = np.load(os.path.join(GOLDEN, "chi2_49_lut256.npy"))
+    got = np.array([np.float32(D.lib().dkx_chi2_inv(i / 256.0, 49.0)) for i in range(255)])
+    assert np.array_equal(got, ref)
+
+
+def test_stats_file_roundtrip(D, tmp_path):
+    fx = D.load_nlsar_stats_collection(os.path.join(GOLDEN, "train32_stats.txt"))
+    assert sorted(fx.keys()) == [(3, 1), (3, 3), (5, 1), (5, 3), (7, 1), (7, 3)]
+    f = tmp_path / "s.txt"
+    D.store_nlsar_stats_collection(fx, str(f))
+    back = D.load_nlsar_stats_collection(str(f))
+    for k in fx:
+        assert back[k].dissims_min == fx[k].dissims_min
+        assert np.array_equal(back[k].quantilles, fx[k].quantilles)
+        assert np.array_equal(back[k].chi2cdf_inv, fx[k].chi2cdf_inv)
+
+
+def test_synthetic_is_seeded():
+    from despeckcl_amd import synthetic as S
+    a = S.interferogram(64, 48, 5)
+    b = S.interferogram(64, 48, 5)
+    c = S.interferogram(64, 48, 6)
+    for x, y in zip(a, b):
+        assert x.dtype == np.float32 and np.array_equal(x, y)
+    assert not np.array_equal(a[0], c[0])
+    assert np.all(a[0] >= 0) and np.all(np.abs(a[2]) <= np.pi)
+
+
+def test_no_oracle_in_product():
+    """The product package never imports or loads the oracle."""
+    pkg = os.path.join(ROOT, "despeckcl_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.replace("# oracle", ""), f
