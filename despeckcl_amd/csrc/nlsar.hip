@@ -102,64 +102,125 @@ struct ScaleArgs {
     Grid A;        // 5 planes, see nlsar_scale_average
     Grid C;        // 4 planes, see nlsar_prep_insar
     int wsh;       // search-window half width
-    int P;         // largest patch half width
+    int P;         // largest patch half width (also a template parameter of the kernel)
     int psh[NLSAR_MAX_NP];
-    int pidx[NLSAR_MAX_NP];        // index of (p, s) in the per-parameter maps
+    int pidx[NLSAR_MAX_NP];           // index of (p, s) in the per-parameter maps
     const float* lutw[NLSAR_MAX_NP];  // weight per quantile index
     float dmin[NLSAR_MAX_NP], dmax[NLSAR_MAX_NP];
     int lut_size[NLSAR_MAX_NP];
     int lut_stride;                // LDS floats per LUT
-    int mean_slot;                 // slot of the largest patch (weighted mean), last scale
-    Launch L;
+    int mean_slot;                 // slot of the largest patch (weighted mean), last pass
+    Launch L[4];                   // interior, last row, last column, corner
+    int gx[4];                     // block columns of each mapping
+    int bstart[5];                 // first linear block of each mapping, bstart[4] = total
     int H, W;
     float* enl_nb;                 // [nparam][H*W] or null
     float* alpha;                  // [nparam][H*W] or null
     unsigned char* ubflag;         // [H*W]
     int first_scale;               // 1: overwrite ubflag, 0: or into it
-    float* ref;                    // last scale: outputs
+    float* ref;                    // last pass: outputs
     float* phase;
     float* coh;
     unsigned long long* ub_count;
 };
 
-constexpr int NT = 256;
+constexpr int NLSAR_TY = 32;   // tile rows; each thread owns a column run of 4 of them
 
 // 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
 __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
 
-// EDGE only gives the launches of the last row / column their own kernel symbol, so that
-// per-kernel profiles of the main launch are not mixed with the thin edge launches.
-template <int TY, int TX, int NP, bool LAST, bool EDGE>
-__global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
+// LUT weight of a patch dissimilarity (compute_weights.cl:37-52): 0 above the trained
+// maximum, else the table entry of index (uint)((d - min) / (max - min) * (lut - 1)).  The
+// index is first formed with a multiplication by s = (lut - 1) / (max - min); it differs from
+// the reference's division by at most 3 roundings (6.1e-5 in index units over 256 entries),
+// so it can only change the truncated index where its fraction lies within 1e-4 of an
+// integer, and there the reference expression itself is evaluated.
+__device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, float rng,
+                                            float scl, float mul, const float* lut)
 {
-    constexpr int ROWS = NT / TX;       // thread rows
-    constexpr int PPT = TY / ROWS;      // pixels per thread
-    static_assert(NT % TX == 0 && TY % ROWS == 0, "tile must map onto 256 threads");
+    if (s > dmax) return 0.f;
+    float d = fmaxf(s, dmin);
+    d = fminf(d, dmax);
+    const float a = d - dmin;
+    const float t = a * mul;
+    const float fr = t - floorf(t);
+    unsigned int idx;
+    if (fr < 1e-4f || fr > 0.9999f) idx = (unsigned int)(a / rng * scl);
+    else idx = (unsigned int)t;
+    return lut[idx];
+}
 
+// Block -> tile: the four launch mappings are laid end to end in one grid; linear block
+// ids are remapped so that the blocks the hardware sends to one XCD (id mod 8) cover a
+// contiguous run of tiles, which then share their halos in that XCD's L2.
+__device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, int& m, int& ty0,
+                                                 int& tx0)
+{
+    const int nb = a.bstart[4];
+    const int b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, k = b >> 3;
+    const int t = xcd * q + min(xcd, r) + k;
+    m = 0;
+    while (m < 3 && t >= a.bstart[m + 1]) m++;
+    const int local = t - a.bstart[m];
+    const int by = local / a.gx[m], bx = local - by * a.gx[m];
+    ty0 = a.L[m].vy0 + by * NLSAR_TY;
+    tx0 = a.L[m].vx0 + bx * TX;
+}
+
+// LDS layout (floats) shared by the kernel and its launcher
+template <int TX, int NP, int P>
+struct ScaleLds {
+    int wsh, lut_stride;
+    __host__ __device__ int DXM() const { return TX + wsh + 2 * P + 4; }
+    __host__ __device__ int DYM() const { return NLSAR_TY + wsh + 2 * P + 4; }
+    __host__ __device__ int RXM() const { return TX + wsh + 4; }
+    __host__ __device__ int WRows() const { return NLSAR_TY + wsh + 4; }
+    __host__ __device__ int off_R() const { return DYM() * DXM(); }
+    __host__ __device__ int off_W() const { return off_R() + NP * DYM() * RXM(); }
+    __host__ __device__ int off_L() const { return off_W() + NP * WRows() * RXM(); }
+    __host__ __device__ int total() const { return off_L() + NP * lut_stride; }
+};
+
+template <int TX, int NP, bool LAST, int P>
+__global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
+{
+    constexpr int NT = 8 * TX;
+    constexpr int TY = NLSAR_TY;
+    constexpr int W2 = 4 + 2 * P;         // register window of a 4-run plus the patch halo
     extern __shared__ float smem[];
-    const int wsh = a.wsh, P = a.P;
+    const int wsh = a.wsh;
     const int sw = 2 * wsh + 1;
-    const int DXM = TX + wsh + 2 * P;   // pitch of the dissimilarity tile
-    const int DYM = TY + wsh + 2 * P;
-    const int RXM = TX + wsh;           // pitch of row-sum / weight tiles
-    float* sD = smem;                                 // DYM * DXM
-    float* sR = sD + DYM * DXM;                       // DYM * RXM (row sums)
-    float* sW = sR + DYM * RXM;                       // NP * (TY + wsh) * RXM
-    float* sL = sW + NP * (TY + wsh) * RXM;           // NP * lut_stride
-    const int WPL = (TY + wsh) * RXM;
+    const ScaleLds<TX, NP, P> lay{wsh, a.lut_stride};
+    const int DXM = lay.DXM(), RXM = lay.RXM(), DYM = lay.DYM();
+    float* sD = smem;
+    float* sR = smem + lay.off_R();
+    float* sW = smem + lay.off_W();
+    float* sL = smem + lay.off_L();
+    const int RPL = DYM * RXM;            // row-sum plane
+    const int WPL = lay.WRows() * RXM;    // weight plane
 
     const int tid = threadIdx.x;
-    const int tx_ = tid % TX, ty_ = tid / TX;
-    const int ty0 = a.L.vy0 + blockIdx.y * TY;
-    const int tx0 = a.L.vx0 + blockIdx.x * TX;
+    const int px = tid % TX, g = tid / TX;  // column, group of 4 rows
+    int m_, ty0, tx0;
+    nlsar_block_tile(a, TX, m_, ty0, tx0);
+    const Launch L = a.L[m_];
 
     for (int k = 0; k < NP; k++)
         for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
-
-    float acc[PPT][NP][6];
-    float macc[PPT][2];
+    float mul[NP], rng[NP], scl[NP];
 #pragma unroll
-    for (int q = 0; q < PPT; q++) {
+    for (int k = 0; k < NP; k++) {
+        rng[k] = a.dmax[k] - a.dmin[k];
+        scl[k] = (float)(a.lut_size[k] - 1);
+        mul[k] = scl[k] / rng[k];
+    }
+
+    float acc[4][NP][6];
+    float macc[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
 #pragma unroll
         for (int k = 0; k < NP; k++)
 #pragma unroll
@@ -177,6 +238,7 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
     const float* Cr = I0 + 2 * a.C.plane;
     const float* Ci = I0 + 3 * a.C.plane;
     const float K = nlsar_log2_term();
+    const int cpitch = a.C.pitch;
 
     // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
     // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
@@ -190,7 +252,6 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
             const int RY = TY - dy, RX = TX + adx;
             const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
             const int DY = RY + 2 * P, DX = RX + 2 * P;
-            __syncthreads();  // previous offset's accumulation done reading sW
             // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
             //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65)
             {
@@ -198,9 +259,10 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
                 const int n = DY * DX;
                 int r = tid / DX, c = tid - r * DX;
                 const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
+                const long long dofs = (long long)dy * a.A.pitch + dx;
                 for (int i = tid; i < n; i += NT) {
                     const long long g1 = a.A.at(gy0 + r, gx0 + c);
-                    const long long g2 = g1 + (long long)dy * a.A.pitch + dx;
+                    const long long g2 = g1 + dofs;
                     const float s00 = A0[g1] + A0[g2];
                     const float s01r = A2[g1] + A2[g2];
                     const float s01i = A3[g1] + A3[g2];
@@ -214,56 +276,70 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
                     if (c >= DX) { c -= DX; r++; }
                 }
             }
-            for (int k = 0; k < NP; k++) {
-                const int psh = a.psh[k];
-                const int np2 = 2 * psh;
-                __syncthreads();
-                // 2. row sums (patch_similarities_row_pass.cl:52-60): left to right
-                {
-                    const int nr = RY + np2;
-                    const int n = nr * RX;
-                    const int roff = P - psh;
-                    int r = tid / RX, c = tid - r * RX;
-                    const int sr_ = NT / RX, sc_ = NT - sr_ * RX;
-                    for (int i = tid; i < n; i += NT) {
-                        const float* src = sD + (r + roff) * DXM + c + roff;
-                        float s = src[0];
-                        for (int j = 1; j <= np2; j++) s += src[j];
-                        sR[r * RXM + c] = s;
-                        r += sr_;
-                        c += sc_;
-                        if (c >= RX) { c -= RX; r++; }
+            __syncthreads();
+            // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
+            //    patch size, for runs of 4 columns held in registers
+            {
+                const int nrx = (RX + 3) >> 2;
+                const int n = DY * nrx;
+                for (int i = tid; i < n; i += NT) {
+                    const int r = i / nrx, x0 = (i - r * nrx) << 2;
+                    float w[W2];
+                    const float* src = sD + r * DXM + x0;
+#pragma unroll
+                    for (int t = 0; t < W2; t++) w[t] = src[t];
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const int psh = a.psh[k];
+                        float* dst = sR + k * RPL + r * RXM + x0;
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            float acc_s = 0.f;
+                            bool first = true;
+#pragma unroll
+                            for (int jj = -P; jj <= P; jj++) {
+                                if (jj >= -psh && jj <= psh) {
+                                    const float v = w[m + P + jj];
+                                    acc_s = first ? v : acc_s + v;
+                                    first = false;
+                                }
+                            }
+                            dst[m] = acc_s;
+                        }
                     }
                 }
-                __syncthreads();
-                // 3. column sums (patch_similarities_col_pass.cl:52-60) and the LUT weight
-                //    (compute_weights.cl:19-56)
-                {
-                    const int n = RY * RX;
-                    const float dmin = a.dmin[k], dmax = a.dmax[k];
-                    const float rng = dmax - dmin;
-                    const float scl = (float)(a.lut_size[k] - 1);
-                    const float* lut = sL + k * a.lut_stride;
-                    float* wout = sW + k * WPL;
-                    int r = tid / RX, c = tid - r * RX;
-                    const int sr_ = NT / RX, sc_ = NT - sr_ * RX;
-                    for (int i = tid; i < n; i += NT) {
-                        const float* src = sR + r * RXM + c;
-                        float s = src[0];
-                        for (int j = 1; j <= np2; j++) s += src[j * RXM];
-                        float w;
-                        if (s > dmax) {
-                            w = 0.f;
-                        } else {
-                            float d = fmaxf(s, dmin);
-                            d = fminf(d, dmax);
-                            const float m = (d - dmin) / rng * scl;
-                            w = lut[(unsigned int)m];
+            }
+            __syncthreads();
+            // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
+            //    runs of 4 rows, and the LUT weight (compute_weights.cl:19-56)
+            {
+                const int nry = (RY + 3) >> 2;
+                const int n = nry * RX;
+                for (int i = tid; i < n; i += NT) {
+                    const int ry = i / RX, x = i - ry * RX, y0 = ry << 2;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const int psh = a.psh[k];
+                        const float* src = sR + k * RPL + y0 * RXM + x;
+                        float w[W2];
+#pragma unroll
+                        for (int t = 0; t < W2; t++) w[t] = src[t * RXM];
+                        const float* lut = sL + k * a.lut_stride;
+                        float* dst = sW + k * WPL + y0 * RXM + x;
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            float acc_s = 0.f;
+                            bool first = true;
+#pragma unroll
+                            for (int jj = -P; jj <= P; jj++) {
+                                if (jj >= -psh && jj <= psh) {
+                                    const float v = w[m + P + jj];
+                                    acc_s = first ? v : acc_s + v;
+                                    first = false;
+                                }
+                            }
+                            dst[m * RXM] = lut_weight(acc_s, a.dmin[k], a.dmax[k], rng[k], scl[k], mul[k], lut);
                         }
-                        wout[r * RXM + c] = w;
-                        r += sr_;
-                        c += sc_;
-                        if (c >= RX) { c -= RX; r++; }
                     }
                 }
             }
@@ -271,62 +347,68 @@ __global__ __launch_bounds__(NT) void nlsar_scale_pass(const ScaleArgs a)
             // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
             //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
             //    (compute_nl_statistics.cl:69-84) of every patch size
+            {
+                const int py0 = g * 4;
+                const int cy = ty0 + py0, cx = tx0 + px;
+                const long long cu0 = a.C.at(cy + dy, cx + dx);
+                const long long cl0 = a.C.at(cy - dy, cx - dx);
+                const int iu0 = py0 * RXM + (px - rxo);
+                const int il0 = (py0 - dy) * RXM + (px - dx - rxo);
 #pragma unroll
-            for (int q = 0; q < PPT; q++) {
-                const int py = ty_ + q * ROWS, px = tx_;
-                const int cy = ty0 + py, cx = tx0 + px;
-                const int iu = py * RXM + (px - rxo);
-                const int il = (py - dy) * RXM + (px - dx - rxo);
-                const long long cu = a.C.at(cy + dy, cx + dx);
-                const long long cl = a.C.at(cy - dy, cx - dx);
-                const float i0u = I0[cu], i1u = I1[cu];
-                const float i0l = I0[cl], i1l = I1[cl];
-                const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
-                const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
-                float cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
-                if (LAST) {
-                    cru = Cr[cu];
-                    ciu = Ci[cu];
-                    crl = Cr[cl];
-                    cil = Ci[cl];
-                }
+                for (int q = 0; q < 4; q++) {
+                    const long long cu = cu0 + (long long)q * cpitch;
+                    const long long cl = cl0 + (long long)q * cpitch;
+                    const float i0u = I0[cu], i1u = I1[cu];
+                    const float i0l = I0[cl], i1l = I1[cl];
+                    const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
+                    const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
+                    float cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
+                    if (LAST) {
+                        cru = Cr[cu];
+                        ciu = Ci[cu];
+                        crl = Cr[cl];
+                        cil = Ci[cl];
+                    }
 #pragma unroll
-                for (int k = 0; k < NP; k++) {
-                    const float wu = sW[k * WPL + iu];
-                    const float wl = sW[k * WPL + il];
-                    acc[q][k][0] += wu;
-                    acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
-                    acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
-                    acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
-                    acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
-                    acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
-                    acc[q][k][0] += wl;
-                    acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
-                    acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
-                    acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
-                    acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
-                    acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
-                    if (LAST && k == a.mean_slot) {
-                        macc[q][0] = fmaf(wu, cru, macc[q][0]);
-                        macc[q][1] = fmaf(wu, ciu, macc[q][1]);
-                        macc[q][0] = fmaf(wl, crl, macc[q][0]);
-                        macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                    for (int k = 0; k < NP; k++) {
+                        const float wu = sW[k * WPL + iu0 + q * RXM];
+                        const float wl = sW[k * WPL + il0 + q * RXM];
+                        acc[q][k][0] += wu;
+                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
+                        acc[q][k][0] += wl;
+                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
+                        if (LAST && k == a.mean_slot) {
+                            macc[q][0] = fmaf(wu, cru, macc[q][0]);
+                            macc[q][1] = fmaf(wu, ciu, macc[q][1]);
+                            macc[q][0] = fmaf(wl, crl, macc[q][0]);
+                            macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                        }
                     }
                 }
             }
+            // the next offset's step 1 writes sD only; its barrier orders step 2/3 writes
+            // after this step's reads of sW
         }
     }
 
     // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
     unsigned int ubl = 0;
 #pragma unroll
-    for (int q = 0; q < PPT; q++) {
-        const int py = ty_ + q * ROWS, px = tx_;
-        const int iy = ty0 + py - a.L.vy0, ix = tx0 + px - a.L.vx0;
-        const bool valid = iy < a.L.ny && ix < a.L.nx;
+    for (int q = 0; q < 4; q++) {
+        const int py = g * 4 + q;
+        const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
+        const bool valid = iy < L.ny && ix < L.nx;
         const long long cc = a.C.at(ty0 + py, tx0 + px);
         const float i0c = I0[cc], i1c = I1[cc];
-        const long long opix = valid ? (long long)(a.L.oy0 + iy) * a.W + (a.L.ox0 + ix) : 0;
+        const long long opix = valid ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
         bool fail = false;
         float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
 #pragma unroll
@@ -432,80 +514,85 @@ std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c)
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-template <int TY, int TX, int NP, bool LAST>
-static size_t scale_pass_lds(int wsh, int P, int lut_stride)
+template <int TX, int NP, bool LAST, int P>
+static int launch_scale_pass(ScaleArgs a, const Launch* L, int lut_stride, hipStream_t stream)
 {
-    const int DXM = TX + wsh + 2 * P, DYM = TY + wsh + 2 * P, RXM = TX + wsh;
-    return sizeof(float) * ((size_t)DYM * DXM + (size_t)DYM * RXM + (size_t)NP * (TY + wsh) * RXM +
-                            (size_t)NP * lut_stride);
-}
-
-template <int TY, int TX, int NP, bool LAST>
-static int launch_scale_pass(const ScaleArgs& base, const Launch* L, int nl, int lut_stride,
-                             hipStream_t stream)
-{
-    const size_t lds = scale_pass_lds<TY, TX, NP, LAST>(base.wsh, base.P, lut_stride);
+    const ScaleLds<TX, NP, P> lay{a.wsh, lut_stride};
+    const size_t lds = sizeof(float) * (size_t)lay.total();
     if (lds > 160 * 1024) {
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
     }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST, false>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, NP, LAST, P>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TY, TX, NP, LAST, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    for (int i = 0; i < nl; i++) {
-        if (L[i].ny <= 0 || L[i].nx <= 0) continue;
-        ScaleArgs a = base;
-        a.L = L[i];
-        dim3 grid((L[i].nx + TX - 1) / TX, (L[i].ny + TY - 1) / TY);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        const bool timed = i == 0 && timing_on();
-        if (timed) {
-            DKX_HIP_CHECK(hipEventCreate(&e0));
-            DKX_HIP_CHECK(hipEventCreate(&e1));
-            DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    int total = 0;
+    for (int i = 0; i < 4; i++) {
+        a.L[i] = L[i];
+        a.bstart[i] = total;
+        if (L[i].ny <= 0 || L[i].nx <= 0) {
+            a.gx[i] = 1;
+            continue;
         }
-        if (i == 0)
-            hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST, false>), grid, dim3(NT), lds, stream, a);
-        else
-            hipLaunchKernelGGL((nlsar_scale_pass<TY, TX, NP, LAST, true>), grid, dim3(NT), lds, stream, a);
-        DKX_HIP_CHECK(hipGetLastError());
-        if (timed) {
-            DKX_HIP_CHECK(hipEventRecord(e1, stream));
-            timing_push(LAST ? STAGE_NLSAR_LAST : STAGE_NLSAR_SCALE, e0, e1);
-        }
+        a.gx[i] = (L[i].nx + TX - 1) / TX;
+        total += a.gx[i] * ((L[i].ny + NLSAR_TY - 1) / NLSAR_TY);
+    }
+    a.bstart[4] = total;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timing_on();
+    if (timed) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    hipLaunchKernelGGL((nlsar_scale_pass<TX, NP, LAST, P>), dim3(total), dim3(8 * TX), lds, stream, a);
+    DKX_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(LAST ? STAGE_NLSAR_LAST : STAGE_NLSAR_SCALE, e0, e1);
     }
     return 0;
 }
 
-template <int NP, bool LAST>
-static int dispatch_np(const ScaleArgs& a, const Launch* L, int nl, int lut_stride, int tile,
-                       hipStream_t stream)
+template <int NP, bool LAST, int P>
+static int dispatch_tile(const ScaleArgs& a, const Launch* L, int lut_stride, int tile,
+                         hipStream_t stream)
 {
-    switch (tile) {
-    case 0: return launch_scale_pass<32, 32, NP, LAST>(a, L, nl, lut_stride, stream);
-    case 1: return launch_scale_pass<16, 32, NP, LAST>(a, L, nl, lut_stride, stream);
-    case 2: return launch_scale_pass<32, 64, NP, LAST>(a, L, nl, lut_stride, stream);
-    default: return launch_scale_pass<16, 64, NP, LAST>(a, L, nl, lut_stride, stream);
+    if (tile == 1) return launch_scale_pass<64, NP, LAST, P>(a, L, lut_stride, stream);
+    return launch_scale_pass<32, NP, LAST, P>(a, L, lut_stride, stream);
+}
+
+template <int NP, bool LAST>
+static int dispatch_p(const ScaleArgs& a, const Launch* L, int lut_stride, int tile,
+                      hipStream_t stream)
+{
+    switch (a.P) {
+    case 0: return dispatch_tile<NP, LAST, 0>(a, L, lut_stride, tile, stream);
+    case 1: return dispatch_tile<NP, LAST, 1>(a, L, lut_stride, tile, stream);
+    case 2: return dispatch_tile<NP, LAST, 2>(a, L, lut_stride, tile, stream);
+    case 3: return dispatch_tile<NP, LAST, 3>(a, L, lut_stride, tile, stream);
+    case 4: return dispatch_tile<NP, LAST, 4>(a, L, lut_stride, tile, stream);
+    case 5: return dispatch_tile<NP, LAST, 5>(a, L, lut_stride, tile, stream);
+    default: set_error("nlsar: patch sizes above 11 are not supported"); return -4;
     }
 }
 
 static int dispatch(const ScaleArgs& a, int np, bool last, const Launch* L, int nl,
                     int lut_stride, int tile, hipStream_t stream)
 {
+    (void)nl;
     switch (np) {
-    case 1: return last ? dispatch_np<1, true>(a, L, nl, lut_stride, tile, stream)
-                        : dispatch_np<1, false>(a, L, nl, lut_stride, tile, stream);
-    case 2: return last ? dispatch_np<2, true>(a, L, nl, lut_stride, tile, stream)
-                        : dispatch_np<2, false>(a, L, nl, lut_stride, tile, stream);
-    case 3: return last ? dispatch_np<3, true>(a, L, nl, lut_stride, tile, stream)
-                        : dispatch_np<3, false>(a, L, nl, lut_stride, tile, stream);
+    case 1: return last ? dispatch_p<1, true>(a, L, lut_stride, tile, stream)
+                        : dispatch_p<1, false>(a, L, lut_stride, tile, stream);
+    case 2: return last ? dispatch_p<2, true>(a, L, lut_stride, tile, stream)
+                        : dispatch_p<2, false>(a, L, lut_stride, tile, stream);
+    case 3: return last ? dispatch_p<3, true>(a, L, lut_stride, tile, stream)
+                        : dispatch_p<3, false>(a, L, lut_stride, tile, stream);
     default: set_error("nlsar: internal error, more than 3 patch sizes in one pass"); return -1;
     }
 }
 
-static const int kTileTY[4] = {32, 16, 32, 16};
-static const int kTileTX[4] = {32, 32, 64, 64};
+static const int kTileTY[2] = {32, 32};
+static const int kTileTX[2] = {32, 64};
 
 int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
 {
@@ -518,6 +605,7 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
     int pmax = 0, smax = 0;
     for (int i = 0; i < pb.nP; i++) {
         if (pb.P[i] < 1 || pb.P[i] % 2 == 0) { set_error("nlsar: patch sizes must be odd"); return -1; }
+        if (pb.P[i] > 11) { set_error("nlsar: patch sizes above 11 are not supported"); return -4; }
         pmax = std::max(pmax, pb.P[i]);
     }
     for (int i = 0; i < pb.nS; i++) {
@@ -530,7 +618,7 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
     for (int i = 0; i < pb.nS; i++)
         for (int j = i + 1; j < pb.nS; j++)
             if (pb.S[i] == pb.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
-    plan->tile = std::min(std::max(pb.tile, 0), 3);
+    plan->tile = std::min(std::max(pb.tile, 0), 1);
     plan->TY = kTileTY[plan->tile];
     plan->TX = kTileTX[plan->tile];
     plan->wsh = (pb.sw - 1) / 2;
