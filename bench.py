@@ -26,6 +26,10 @@ H = W = 2048
 SW, PATCHES, SCALES = 21, [3, 5, 7], [1, 3]
 HPAR, CPAR = 15.0, 49.0
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per launch of the last scale pass: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# this benchmark (KiB per dispatch), scaled by what the same counters read for a 256 MiB
+# device-to-device copy (FETCH x 2.000, WRITE x 1.000); profiles/nlsar_2048_scale_pass_pmc.md
+TRAFFIC_LAST_PASS_BYTES = round((2276456 * 2.000 + 49344 * 1.000) * 1024)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),
@@ -137,7 +141,8 @@ def main():
                    "scale_sizes": SCALES, "h": HPAR, "c": CPAR, "form": "plain per-call",
                    "tile": DV.get_tile(0)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": TRAFFIC_LAST_PASS_BYTES,
                      "kernel": "nlsar_scale_pass (last scale, 3 patch sizes)",
                      "kernel_ms": round(last_ms, 4),
                      "note": "compute roof = FP32 vector peak (no MFMA on this path); "
