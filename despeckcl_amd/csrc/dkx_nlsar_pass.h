@@ -1,0 +1,483 @@
+// The fused NL-SAR scale pass: for one scale and up to three patch sizes, evaluates per
+// search-window offset the pixel dissimilarities, the separable patch box sums, the LUT
+// weights and the nonlocal statistics; the last pass also forms the weighted mean, the
+// alpha blend and the decomposition.  Included by the translation units that instantiate
+// it (one per (last pass, tile width) pair, so that they compile in parallel).
+//
+// Reference path replaced: src/compute/nlsar/nlsar_filter_sub_image.cpp:146-390 with the
+// kernels of src/compute/nlsar/cl_kernels (one materialised 220/441-plane buffer per stage
+// there; here every stage of one search-window offset stays in LDS).
+#pragma once
+
+#include "dkx_common.h"
+#include "dkx_nlsar.h"
+
+namespace dkx {
+
+// Patch sets are compile-time bit masks of patch half widths (bit k: patch size 2k+1), so
+// that every box sum is a straight chain of additions.
+__host__ __device__ constexpr int pmask_count(int m) { return m ? (m & 1) + pmask_count(m >> 1) : 0; }
+__host__ __device__ constexpr int pmask_top(int m) { return m > 1 ? 1 + pmask_top(m >> 1) : 0; }
+// half width of slot k (slots in ascending patch size)
+__host__ __device__ constexpr int pmask_bit(int m, int k, int b = 0)
+{
+    return (m & 1) ? (k == 0 ? b : pmask_bit(m >> 1, k - 1, b + 1)) : pmask_bit(m >> 1, k, b + 1);
+}
+
+constexpr int NLSAR_TY = 32;    // tile rows; each thread owns a column run of 4 of them
+constexpr int NLSAR_BAND = 8;   // tile rows per band of the block order
+
+struct ScaleArgs {
+    const float4* A4;   // averaged covariance per grid element: a00, a11, Re a01, Im a01
+    const float* LD;    // log det of the averaged covariance, same grid
+    const float4* C4;   // covariance of the input pixel: a1^2, a2^2, Re C01, Im C01
+    int apitch, aoy, aox;
+    int cpitch, coy, cox;
+    int wsh;                             // search-window half width
+    int pidx[NLSAR_MAX_NP];              // index of (p, s) in the per-parameter maps
+    const float* lutw[NLSAR_MAX_NP];     // weight per quantile index
+    float dmin[NLSAR_MAX_NP], dmax[NLSAR_MAX_NP];
+    float rng[NLSAR_MAX_NP];             // dmax - dmin
+    float rcp[NLSAR_MAX_NP];             // 1 / rng, correctly rounded
+    float scl[NLSAR_MAX_NP];             // lut_size - 1
+    int lut_size[NLSAR_MAX_NP];
+    int lut_stride;                      // LDS floats per LUT
+    Launch L[4];                         // interior, last row, last column, corner
+    int gx[4], gy[4];                    // block columns / rows of each mapping
+    int bstart[5];                       // first linear block of each mapping, [4] = total
+    int H, W;
+    float* enl_nb;                       // [nparam][H*W] or null
+    float* alpha;                        // [nparam][H*W] or null
+    unsigned char* ubflag;               // [H*W]
+    int first_scale;                     // 1: first pass (ubflag not yet written)
+    float* ref;                          // last pass: outputs
+    float* phase;
+    float* coh;
+    unsigned long long* ub_count;
+};
+
+// LDS layout (floats) shared by the kernel and its launcher.  XU/YU: the union region of a
+// tile and the tile shifted by an offset, rounded up to runs of 4.
+template <int TX, int P>
+struct PassLds {
+    int wsh, lut_stride, np;
+    static constexpr int W2 = 4 + 2 * P;          // a run of 4 plus the patch halo
+    static constexpr int W2R = (W2 + 3) & ~3;     // the same in float4 loads
+    __host__ __device__ int XU() const { return (TX + wsh + 3) & ~3; }
+    __host__ __device__ int YU() const { return (NLSAR_TY + wsh + 3) & ~3; }
+    __host__ __device__ int DXM() const { return XU() + W2R; }          // dissimilarity row
+    __host__ __device__ int DYM() const { return NLSAR_TY + wsh + 2 * P; }
+    __host__ __device__ int RXM() const { return XU(); }                // row-sum / weight row
+    __host__ __device__ int RRows() const { return YU() + 2 * P; }
+    __host__ __device__ int WRows() const { return YU(); }
+    __host__ __device__ int off_R() const { return DYM() * DXM(); }
+    __host__ __device__ int off_W() const { return off_R() + np * RRows() * RXM(); }
+    __host__ __device__ int off_L() const { return off_W() + np * WRows() * RXM(); }
+    __host__ __device__ int total() const { return off_L() + np * lut_stride; }
+};
+
+// 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
+__device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
+
+// LUT weight of a patch dissimilarity (compute_weights.cl:37-52): 0 above the trained
+// maximum, else the table entry of index (uint)((d - min) / (max - min) * (lut - 1)).  The
+// quotient is formed as q0 = a * (1/rng) followed by one FMA residual correction, which
+// gives the correctly rounded quotient for every normal result (Markstein); subnormal
+// quotients can differ by an ulp but truncate to index 0 either way.
+__device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, float rng,
+                                            float rcp, float scl, const float* lut)
+{
+    const float d = fminf(fmaxf(s, dmin), dmax);
+    const float a = d - dmin;
+    const float q0 = a * rcp;
+    const float e = fmaf(-q0, rng, a);
+    const float q = fmaf(e, rcp, q0);
+    const float w = lut[(unsigned int)(q * scl)];
+    return s > dmax ? 0.f : w;
+}
+
+// Block -> tile.  The four launch mappings are laid end to end in one grid.  Linear block
+// ids are remapped so that the blocks the hardware sends to one XCD (id mod 8) take a
+// contiguous run of tiles, and tiles are ordered column by column within bands of
+// NLSAR_BAND tile rows, so the blocks resident on one XCD cover a compact 2-D patch whose
+// halos overlap in that XCD's L2.
+__device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, int& m, int& ty0,
+                                                 int& tx0)
+{
+    const int nb = a.bstart[4];
+    const int b = blockIdx.x;
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, k = b >> 3;
+    const int t = xcd * q + min(xcd, r) + k;
+    m = 0;
+    while (m < 3 && t >= a.bstart[m + 1]) m++;
+    const int local = t - a.bstart[m];
+    const int gx = a.gx[m], gy = a.gy[m];
+    const int band = local / (NLSAR_BAND * gx);
+    const int within = local - band * NLSAR_BAND * gx;
+    const int bh = min(NLSAR_BAND, gy - band * NLSAR_BAND);
+    const int bx = within / bh;
+    const int by = band * NLSAR_BAND + (within - bx * bh);
+    ty0 = a.L[m].vy0 + by * NLSAR_TY;
+    tx0 = a.L[m].vx0 + bx * TX;
+}
+
+template <int TX, int PMASK, bool LAST>
+__global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
+{
+    constexpr int NT = 8 * TX;
+    constexpr int TY = NLSAR_TY;
+    constexpr int NP = pmask_count(PMASK);
+    constexpr int P = pmask_top(PMASK);
+    constexpr int W2 = PassLds<TX, P>::W2;
+    constexpr int W2R = PassLds<TX, P>::W2R;
+    extern __shared__ float4 smem4[];
+    float* smem = reinterpret_cast<float*>(smem4);
+    const int wsh = a.wsh;
+    const int sw = 2 * wsh + 1;
+    const PassLds<TX, P> lay{wsh, a.lut_stride, NP};
+    const int DXM = lay.DXM(), RXM = lay.RXM();
+    float* sD = smem;
+    float* sR = smem + lay.off_R();
+    float* sW = smem + lay.off_W();
+    float* sL = smem + lay.off_L();
+    const int RPL = lay.RRows() * RXM;   // row-sum plane
+    const int WPL = lay.WRows() * RXM;   // weight plane
+
+    const int tid = threadIdx.x;
+    const int px = tid % TX, g = tid / TX;  // column, group of 4 rows
+    int m_, ty0, tx0;
+    nlsar_block_tile(a, TX, m_, ty0, tx0);
+    const Launch L = a.L[m_];
+
+#pragma unroll
+    for (int k = 0; k < NP; k++)
+        for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
+
+    float acc[4][NP][6];
+    float macc[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int k = 0; k < NP; k++)
+#pragma unroll
+            for (int j = 0; j < 6; j++) acc[q][k][j] = 0.f;
+        macc[q][0] = macc[q][1] = 0.f;
+    }
+
+    const float K = nlsar_log2_term();
+    const int apitch = a.apitch, cpitch = a.cpitch;
+
+    // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
+    // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
+    for (int hh = 0; hh <= wsh; hh++) {
+        const int dy = hh - wsh;                  // <= 0
+        const int wwe = (hh == wsh) ? wsh : sw;
+        for (int ww = 0; ww < wwe; ww++) {
+            const int dx = ww - wsh;
+            const int adx = dx < 0 ? -dx : dx;
+            // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+            const int RY = TY - dy, RX = TX + adx;
+            const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
+            const int DY = RY + 2 * P, DX = RX + 2 * P;
+            // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
+            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65)
+            {
+                const int n = DY * DX;
+                int r = tid / DX, c = tid - r * DX;
+                const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
+                const int base = (ty0 - P + a.aoy) * apitch + (tx0 + rxo - P + a.aox);
+                const int dofs = dy * apitch + dx;
+                for (int i = tid; i < n; i += NT) {
+                    const int g1 = base + r * apitch + c;
+                    const int g2 = g1 + dofs;
+                    const float4 u = a.A4[g1];
+                    const float4 v = a.A4[g2];
+                    const float s00 = u.x + v.x;
+                    const float s01r = u.z + v.z;
+                    const float s01i = u.w + v.w;
+                    const float s11 = u.y + v.y;
+                    const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
+                    float sim = -(((K + a.LD[g1]) + a.LD[g2]) - 2.0f * logf(det));
+                    if (isnan(sim)) sim = 0.f;
+                    sD[r * DXM + c] = sim;
+                    r += sr_;
+                    c += sc_;
+                    if (c >= DX) { c -= DX; r++; }
+                }
+            }
+            __syncthreads();
+            // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
+            //    patch size, for runs of 4 columns held in registers
+            {
+                const int nrx = (RX + 3) >> 2;
+                const int n = DY * nrx;
+                for (int i = tid; i < n; i += NT) {
+                    const int r = i / nrx, x0 = (i - r * nrx) << 2;
+                    float w[W2R];
+                    const float4* src = reinterpret_cast<const float4*>(sD + r * DXM + x0);
+#pragma unroll
+                    for (int t = 0; t < W2R / 4; t++) {
+                        const float4 v = src[t];
+                        w[4 * t] = v.x;
+                        w[4 * t + 1] = v.y;
+                        w[4 * t + 2] = v.z;
+                        w[4 * t + 3] = v.w;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const int psh = pmask_bit(PMASK, k);
+                        float o[4];
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            float s = w[m + P - psh];
+#pragma unroll
+                            for (int jj = -P + 1; jj <= P; jj++)
+                                if (jj > -psh && jj <= psh) s = s + w[m + P + jj];
+                            o[m] = s;
+                        }
+                        *reinterpret_cast<float4*>(sR + k * RPL + r * RXM + x0) =
+                            make_float4(o[0], o[1], o[2], o[3]);
+                    }
+                }
+            }
+            __syncthreads();
+            // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
+            //    runs of 4 rows in pairs of columns, and the LUT weight
+            //    (compute_weights.cl:19-56)
+            {
+                const int nrx = (RX + 1) >> 1;
+                const int nry = (RY + 3) >> 2;
+                const int n = nry * nrx;
+                for (int i = tid; i < n; i += NT) {
+                    const int ry = i / nrx, x = (i - ry * nrx) << 1, y0 = ry << 2;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const int psh = pmask_bit(PMASK, k);
+                        const float* src = sR + k * RPL + y0 * RXM + x;
+                        float2 w[W2];
+#pragma unroll
+                        for (int t = 0; t < W2; t++) w[t] = *reinterpret_cast<const float2*>(src + t * RXM);
+                        const float* lut = sL + k * a.lut_stride;
+                        float* dst = sW + k * WPL + y0 * RXM + x;
+#pragma unroll
+                        for (int m = 0; m < 4; m++) {
+                            float s0 = w[m + P - psh].x, s1 = w[m + P - psh].y;
+#pragma unroll
+                            for (int jj = -P + 1; jj <= P; jj++)
+                                if (jj > -psh && jj <= psh) {
+                                    s0 = s0 + w[m + P + jj].x;
+                                    s1 = s1 + w[m + P + jj].y;
+                                }
+                            const float w0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                            const float w1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                            *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(w0, w1);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
+            //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
+            //    (compute_nl_statistics.cl:69-84) of every patch size
+            {
+                const int py0 = g * 4;
+                const int cy = ty0 + py0, cx = tx0 + px;
+                const int cu0 = (cy + dy + a.coy) * cpitch + (cx + dx + a.cox);
+                const int cl0 = (cy - dy + a.coy) * cpitch + (cx - dx + a.cox);
+                const int iu0 = py0 * RXM + (px - rxo);
+                const int il0 = (py0 - dy) * RXM + (px - dx - rxo);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
+                    if (LAST) {
+                        const float4 u = a.C4[cu0 + q * cpitch];
+                        const float4 l = a.C4[cl0 + q * cpitch];
+                        i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
+                        i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
+                    } else {
+                        const float2 u = reinterpret_cast<const float2*>(a.C4)[2 * (cu0 + q * cpitch)];
+                        const float2 l = reinterpret_cast<const float2*>(a.C4)[2 * (cl0 + q * cpitch)];
+                        i0u = u.x; i1u = u.y;
+                        i0l = l.x; i1l = l.y;
+                    }
+                    const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
+                    const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const float wu = sW[k * WPL + iu0 + q * RXM];
+                        const float wl = sW[k * WPL + il0 + q * RXM];
+                        acc[q][k][0] += wu;
+                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
+                        acc[q][k][0] += wl;
+                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
+                        if (LAST && k == NP - 1) {
+                            macc[q][0] = fmaf(wu, cru, macc[q][0]);
+                            macc[q][1] = fmaf(wu, ciu, macc[q][1]);
+                            macc[q][0] = fmaf(wl, crl, macc[q][0]);
+                            macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                        }
+                    }
+                }
+            }
+            // the next offset's step 1 writes sD only; its barrier orders step 2/3 writes
+            // after this step's reads of sW
+        }
+    }
+
+    // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
+    unsigned int ubl = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int py = g * 4 + q;
+        const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
+        const bool valid = iy < L.ny && ix < L.nx;
+        const float4 cc = a.C4[(ty0 + py + a.coy) * cpitch + (tx0 + px + a.cox)];
+        const float i0c = cc.x, i1c = cc.y;
+        const long long opix = valid ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
+        bool fail = false;
+        float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
+#pragma unroll
+        for (int k = 0; k < NP; k++) {
+            float* s = acc[q][k];
+            s[0] += 1.0f;
+            s[1] = fmaf(1.0f, 1.0f, s[1]);
+            s[2] = fmaf(1.0f, i0c, s[2]);
+            s[3] = fmaf(1.0f, i0c * i0c, s[3]);
+            s[4] = fmaf(1.0f, i1c, s[4]);
+            s[5] = fmaf(1.0f, i1c * i1c, s[5]);
+            // compute_nl_statistics.cl:86-102, compute_alphas.cl:19-44,
+            // compute_enls_nobias.cl:19-39
+            const float ws = s[0];
+            const float enl = ws * ws / s[1];
+            float alpha = 0.0f;
+            {
+                const float inl = s[2] / ws;
+                const float wv = s[3] / ws;
+                const float var = wv - inl * inl;
+                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
+            }
+            {
+                const float inl = s[4] / ws;
+                const float wv = s[5] / ws;
+                const float var = wv - inl * inl;
+                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
+            }
+            const float oma = 1.0f - alpha;
+            const float enb = enl / (oma * oma + (alpha * alpha + 2.0f * alpha * (1.0f - alpha) / ws) * enl);
+            if (!(enb > -1.0f)) fail = true;
+            if (valid) {
+                if (a.enl_nb) a.enl_nb[(long long)a.pidx[k] * a.H * a.W + opix] = enb;
+                if (a.alpha) a.alpha[(long long)a.pidx[k] * a.H * a.W + opix] = alpha;
+            }
+            if (LAST && k == NP - 1) {
+                alpha_mean = alpha;
+                s_mean = ws;
+                m0_mean = s[2];
+                m1_mean = s[4];
+                // centre weight 1 on the off-diagonal sums as well
+                macc[q][0] = fmaf(1.0f, cc.z, macc[q][0]);
+                macc[q][1] = fmaf(1.0f, cc.w, macc[q][1]);
+            }
+        }
+        if (!valid) continue;
+        if (!LAST) {
+            if (a.first_scale) a.ubflag[opix] = fail ? 1 : 0;
+            else if (fail) a.ubflag[opix] = 1;
+        } else {
+            // best-parameter selection (best_params.cpp:27-53) keeps the last key of the
+            // (patch, scale) map whenever every ENL exceeds -1; pixels where the reference's
+            // selection loop is undefined are counted and also take the last key.
+            if (fail || (!a.first_scale && a.ubflag[opix])) ubl++;
+            // weighted_means.cl:62-80 and covmat_decompose.cl:19-39
+            const float al = alpha_mean;
+            const float m00 = (1 - al) * m0_mean / s_mean + al * i0c;
+            const float m11 = (1 - al) * m1_mean / s_mean + al * i1c;
+            const float mr = (1 - al) * macc[q][0] / s_mean + al * cc.z;
+            const float mi = (1 - al) * macc[q][1] / s_mean + al * cc.w;
+            const float am = sqrtf(m00), as = sqrtf(m11);
+            const float t = 0.5f * (am + as);
+            a.ref[opix] = t * t;
+            a.phase[opix] = atan2f(mi, mr);
+            a.coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
+        }
+    }
+    if (LAST && ubl) atomicAdd(a.ub_count, (unsigned long long)ubl);
+}
+
+// Launch of one pass; defined in the instantiating translation units.
+template <bool LAST, int TX>
+int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
+
+// Every patch set of one to three half widths in [0, 5].
+#define DKX_NLSAR_PMASKS(X)                                                                  \
+    X(1) X(2) X(4) X(8) X(16) X(32)                                                          \
+    X(3) X(5) X(9) X(17) X(33) X(6) X(10) X(18) X(34) X(12) X(20) X(36) X(24) X(40) X(48)   \
+    X(7) X(11) X(19) X(35) X(13) X(21) X(37) X(25) X(41) X(49) X(14) X(22) X(38) X(26)      \
+    X(42) X(50) X(28) X(44) X(52) X(56)
+
+template <bool LAST, int TX, int PMASK>
+int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
+{
+    constexpr int NP = pmask_count(PMASK);
+    constexpr int P = pmask_top(PMASK);
+    const PassLds<TX, P> lay{a.wsh, a.lut_stride, NP};
+    const size_t lds = sizeof(float) * (size_t)lay.total();
+    if (lds > 160 * 1024) {
+        set_error("nlsar: search window / patch sizes exceed the LDS budget");
+        return -2;
+    }
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, PMASK, LAST>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int total = 0;
+    for (int i = 0; i < 4; i++) {
+        a.bstart[i] = total;
+        if (a.L[i].ny <= 0 || a.L[i].nx <= 0) {
+            a.gx[i] = a.gy[i] = 1;
+            continue;
+        }
+        a.gx[i] = (a.L[i].nx + TX - 1) / TX;
+        a.gy[i] = (a.L[i].ny + NLSAR_TY - 1) / NLSAR_TY;
+        total += a.gx[i] * a.gy[i];
+    }
+    a.bstart[4] = total;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timing_on();
+    if (timed) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    hipLaunchKernelGGL((nlsar_scale_pass<TX, PMASK, LAST>), dim3(total), dim3(8 * TX), lds, stream, a);
+    DKX_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(LAST ? STAGE_NLSAR_LAST : STAGE_NLSAR_SCALE, e0, e1);
+    }
+    return 0;
+}
+
+#define DKX_NLSAR_PASS_CASE(M) case M: return nlsar_pass_launch_mask<LAST_, TX_, M>(a, stream);
+#define DKX_NLSAR_PASS_INSTANTIATE(LAST_V, TX_V)                                              \
+    template <>                                                                              \
+    int nlsar_pass_launch<LAST_V, TX_V>(int pmask, const ScaleArgs& a, hipStream_t stream)   \
+    {                                                                                        \
+        constexpr bool LAST_ = LAST_V;                                                       \
+        constexpr int TX_ = TX_V;                                                            \
+        switch (pmask) {                                                                     \
+            DKX_NLSAR_PMASKS(DKX_NLSAR_PASS_CASE)                                            \
+        default:                                                                             \
+            set_error("nlsar: internal error, unsupported patch set");                      \
+            return -1;                                                                       \
+        }                                                                                    \
+    }
+
+}  // namespace dkx

@@ -1,17 +1,15 @@
-// NL-SAR on CDNA4: covariance preparation, Gaussian scale average, and one fused
-// "scale pass" kernel per scale that evaluates pixel dissimilarities, separable patch box
-// sums, LUT weights and the nonlocal statistics of every patch size, and (for the last
-// scale) the weighted mean, the alpha blend and the decomposition into reflectivity, phase
-// and coherence.
+// NL-SAR on CDNA4: covariance preparation, Gaussian scale average, planning and the pass
+// sequence.  The fused scale pass itself lives in dkx_nlsar_pass.h.
 //
 // Reference path replaced: src/compute/nlsar/nlsar_filter_sub_image.cpp:146-390 with the
-// kernels of src/compute/nlsar/cl_kernels (one materialised 220/441-plane buffer per stage
-// there; here every stage of one search-window offset stays in LDS).
+// kernels of src/compute/nlsar/cl_kernels.
 #include "dkx_common.h"
 #include "dkx_nlsar.h"
+#include "dkx_nlsar_pass.h"
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -21,8 +19,8 @@ namespace dkx {
 // preparation kernels
 // ------------------------------------------------------------------------------------
 
-// Covariance planes of the InSAR input (covmat_create.cl:19-56) on the padded grid C:
-// plane 0 = a1^2, 1 = a2^2, 2 = Re C01 = a1*a2*cos(phi), 3 = Im C01 = a1*a2*sin(phi).
+// Covariance of the InSAR input (covmat_create.cl:19-56) on the padded grid C, one float4
+// per element: a1^2, a2^2, Re C01 = a1*a2*cos(phi), Im C01 = a1*a2*sin(phi).
 __global__ void nlsar_prep_insar(const float* __restrict__ a1, const float* __restrict__ a2,
                                  const float* __restrict__ phi, int H, int W, Grid C)
 {
@@ -34,14 +32,11 @@ __global__ void nlsar_prep_insar(const float* __restrict__ a1, const float* __re
     const long long i = (long long)y * W + x;
     const float m = a1[i], s = a2[i], p = phi[i];
     const long long o = (long long)gy * C.pitch + gx;
-    C.base[o] = m * m;
-    C.base[C.plane + o] = s * s;
-    C.base[2 * C.plane + o] = m * s * cosf(p);
-    C.base[3 * C.plane + o] = m * s * sinf(p);
+    reinterpret_cast<float4*>(C.base)[o] = make_float4(m * m, s * s, m * s * cosf(p), m * s * sinf(p));
 }
 
-// Covariance planes of a dim-2 covariance input (8 interlaced planes, nlsar.i:146-153):
-// the diagonal real parts and the (0,1) element, as for the InSAR input.
+// Covariance of a dim-2 covariance input (8 interlaced planes, nlsar.i:146-153): the
+// diagonal real parts and the (0,1) element, laid out as for the InSAR input.
 __global__ void nlsar_prep_covmat2(const float* __restrict__ cov, int H, int W, Grid C)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,17 +47,14 @@ __global__ void nlsar_prep_covmat2(const float* __restrict__ cov, int H, int W, 
     const long long n = (long long)H * W;
     const long long i = (long long)y * W + x;
     const long long o = (long long)gy * C.pitch + gx;
-    C.base[o] = cov[i];
-    C.base[C.plane + o] = cov[6 * n + i];
-    C.base[2 * C.plane + o] = cov[2 * n + i];
-    C.base[3 * C.plane + o] = cov[3 * n + i];
+    reinterpret_cast<float4*>(C.base)[o] = make_float4(cov[i], cov[6 * n + i], cov[2 * n + i], cov[3 * n + i]);
 }
 
 // Gaussian s x s average of the rescaled covariance (covmat_rescale.cl:19-41 then
 // covmat_spatial_avg.cl:19-63; planes combined per pixel in the same operation order),
 // plus the log-determinant of each averaged matrix, which every pixel dissimilarity
 // involving that pixel uses (sim_measures.h:51-52).
-// Output planes of A: 0 = a00, 1 = a11, 2 = Re a01, 3 = Im a01, 4 = log det.
+// A holds one float4 per element (a00, a11, Re a01, Im a01) followed by the log det plane.
 __global__ void nlsar_scale_average(Grid C, Grid A, const float* __restrict__ gauss, int s,
                                     float gamma)
 {
@@ -71,413 +63,32 @@ __global__ void nlsar_scale_average(Grid C, Grid A, const float* __restrict__ ga
     if (gx >= A.w || gy >= A.h) return;
     const int y = gy - A.oy, x = gx - A.ox;  // image coordinates
     const int ssh = (s - 1) / 2;
+    const float4* C4 = reinterpret_cast<const float4*>(C.base);
     float s00 = 0.f, s11 = 0.f, sr = 0.f, si = 0.f;
     for (int kx = 0; kx < s; kx++) {
         const long long row = C.at(y - ssh + kx, x - ssh);
         for (int ky = 0; ky < s; ky++) {
             const float g = gauss[kx * s + ky];
-            const long long j = row + ky;
-            s00 = fmaf(g, C.base[j], s00);
-            s11 = fmaf(g, C.base[C.plane + j], s11);
-            const float r = C.base[2 * C.plane + j] * gamma;
-            const float im = C.base[3 * C.plane + j] * gamma;
+            const float4 c = C4[row + ky];
+            s00 = fmaf(g, c.x, s00);
+            s11 = fmaf(g, c.y, s11);
+            const float r = c.z * gamma;
+            const float im = c.w * gamma;
             sr = fmaf(g, r, sr);
             si = fmaf(g, im, si);
         }
     }
     const long long o = (long long)gy * A.pitch + gx;
-    A.base[o] = s00;
-    A.base[A.plane + o] = s11;
-    A.base[2 * A.plane + o] = sr;
-    A.base[3 * A.plane + o] = si;
+    reinterpret_cast<float4*>(A.base)[o] = make_float4(s00, s11, sr, si);
     const float det = (s00 * s11) - (sr * sr + si * si);
     A.base[4 * A.plane + o] = logf(det);
 }
 
-// ------------------------------------------------------------------------------------
-// fused scale pass
-// ------------------------------------------------------------------------------------
-
-struct ScaleArgs {
-    Grid A;        // 5 planes, see nlsar_scale_average
-    Grid C;        // 4 planes, see nlsar_prep_insar
-    int wsh;       // search-window half width
-    int P;         // largest patch half width (also a template parameter of the kernel)
-    int psh[NLSAR_MAX_NP];
-    int pidx[NLSAR_MAX_NP];           // index of (p, s) in the per-parameter maps
-    const float* lutw[NLSAR_MAX_NP];  // weight per quantile index
-    float dmin[NLSAR_MAX_NP], dmax[NLSAR_MAX_NP];
-    int lut_size[NLSAR_MAX_NP];
-    int lut_stride;                // LDS floats per LUT
-    int mean_slot;                 // slot of the largest patch (weighted mean), last pass
-    Launch L[4];                   // interior, last row, last column, corner
-    int gx[4];                     // block columns of each mapping
-    int bstart[5];                 // first linear block of each mapping, bstart[4] = total
-    int H, W;
-    float* enl_nb;                 // [nparam][H*W] or null
-    float* alpha;                  // [nparam][H*W] or null
-    unsigned char* ubflag;         // [H*W]
-    int first_scale;               // 1: overwrite ubflag, 0: or into it
-    float* ref;                    // last pass: outputs
-    float* phase;
-    float* coh;
-    unsigned long long* ub_count;
-};
-
-constexpr int NLSAR_TY = 32;   // tile rows; each thread owns a column run of 4 of them
-
-// 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
-__device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
-
-// LUT weight of a patch dissimilarity (compute_weights.cl:37-52): 0 above the trained
-// maximum, else the table entry of index (uint)((d - min) / (max - min) * (lut - 1)).  The
-// index is first formed with a multiplication by s = (lut - 1) / (max - min); it differs from
-// the reference's division by at most 3 roundings (6.1e-5 in index units over 256 entries),
-// so it can only change the truncated index where its fraction lies within 1e-4 of an
-// integer, and there the reference expression itself is evaluated.
-__device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, float rng,
-                                            float scl, float mul, const float* lut)
+// Clears the undefined-selection counter (a kernel, not a memset node, so that the call
+// can be captured in a HIP graph).
+__global__ void nlsar_zero_count(unsigned long long* c)
 {
-    if (s > dmax) return 0.f;
-    float d = fmaxf(s, dmin);
-    d = fminf(d, dmax);
-    const float a = d - dmin;
-    const float t = a * mul;
-    const float fr = t - floorf(t);
-    unsigned int idx;
-    if (fr < 1e-4f || fr > 0.9999f) idx = (unsigned int)(a / rng * scl);
-    else idx = (unsigned int)t;
-    return lut[idx];
-}
-
-// Block -> tile: the four launch mappings are laid end to end in one grid; linear block
-// ids are remapped so that the blocks the hardware sends to one XCD (id mod 8) cover a
-// contiguous run of tiles, which then share their halos in that XCD's L2.
-__device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, int& m, int& ty0,
-                                                 int& tx0)
-{
-    const int nb = a.bstart[4];
-    const int b = blockIdx.x;
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, k = b >> 3;
-    const int t = xcd * q + min(xcd, r) + k;
-    m = 0;
-    while (m < 3 && t >= a.bstart[m + 1]) m++;
-    const int local = t - a.bstart[m];
-    const int by = local / a.gx[m], bx = local - by * a.gx[m];
-    ty0 = a.L[m].vy0 + by * NLSAR_TY;
-    tx0 = a.L[m].vx0 + bx * TX;
-}
-
-// LDS layout (floats) shared by the kernel and its launcher
-template <int TX, int NP, int P>
-struct ScaleLds {
-    int wsh, lut_stride;
-    __host__ __device__ int DXM() const { return TX + wsh + 2 * P + 4; }
-    __host__ __device__ int DYM() const { return NLSAR_TY + wsh + 2 * P + 4; }
-    __host__ __device__ int RXM() const { return TX + wsh + 4; }
-    __host__ __device__ int WRows() const { return NLSAR_TY + wsh + 4; }
-    __host__ __device__ int off_R() const { return DYM() * DXM(); }
-    __host__ __device__ int off_W() const { return off_R() + NP * DYM() * RXM(); }
-    __host__ __device__ int off_L() const { return off_W() + NP * WRows() * RXM(); }
-    __host__ __device__ int total() const { return off_L() + NP * lut_stride; }
-};
-
-template <int TX, int NP, bool LAST, int P>
-__global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
-{
-    constexpr int NT = 8 * TX;
-    constexpr int TY = NLSAR_TY;
-    constexpr int W2 = 4 + 2 * P;         // register window of a 4-run plus the patch halo
-    extern __shared__ float smem[];
-    const int wsh = a.wsh;
-    const int sw = 2 * wsh + 1;
-    const ScaleLds<TX, NP, P> lay{wsh, a.lut_stride};
-    const int DXM = lay.DXM(), RXM = lay.RXM(), DYM = lay.DYM();
-    float* sD = smem;
-    float* sR = smem + lay.off_R();
-    float* sW = smem + lay.off_W();
-    float* sL = smem + lay.off_L();
-    const int RPL = DYM * RXM;            // row-sum plane
-    const int WPL = lay.WRows() * RXM;    // weight plane
-
-    const int tid = threadIdx.x;
-    const int px = tid % TX, g = tid / TX;  // column, group of 4 rows
-    int m_, ty0, tx0;
-    nlsar_block_tile(a, TX, m_, ty0, tx0);
-    const Launch L = a.L[m_];
-
-    for (int k = 0; k < NP; k++)
-        for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
-    float mul[NP], rng[NP], scl[NP];
-#pragma unroll
-    for (int k = 0; k < NP; k++) {
-        rng[k] = a.dmax[k] - a.dmin[k];
-        scl[k] = (float)(a.lut_size[k] - 1);
-        mul[k] = scl[k] / rng[k];
-    }
-
-    float acc[4][NP][6];
-    float macc[4][2];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-#pragma unroll
-        for (int k = 0; k < NP; k++)
-#pragma unroll
-            for (int j = 0; j < 6; j++) acc[q][k][j] = 0.f;
-        macc[q][0] = macc[q][1] = 0.f;
-    }
-
-    const float* A0 = a.A.base;
-    const float* A1 = A0 + a.A.plane;
-    const float* A2 = A0 + 2 * a.A.plane;
-    const float* A3 = A0 + 3 * a.A.plane;
-    const float* A4 = A0 + 4 * a.A.plane;
-    const float* I0 = a.C.base;
-    const float* I1 = I0 + a.C.plane;
-    const float* Cr = I0 + 2 * a.C.plane;
-    const float* Ci = I0 + 3 * a.C.plane;
-    const float K = nlsar_log2_term();
-    const int cpitch = a.C.pitch;
-
-    // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
-    // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
-    for (int hh = 0; hh <= wsh; hh++) {
-        const int dy = hh - wsh;                  // <= 0
-        const int wwe = (hh == wsh) ? wsh : sw;
-        for (int ww = 0; ww < wwe; ww++) {
-            const int dx = ww - wsh;
-            const int adx = dx < 0 ? -dx : dx;
-            // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
-            const int RY = TY - dy, RX = TX + adx;
-            const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
-            const int DY = RY + 2 * P, DX = RX + 2 * P;
-            // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
-            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65)
-            {
-                const int gy0 = ty0 - P, gx0 = tx0 + rxo - P;
-                const int n = DY * DX;
-                int r = tid / DX, c = tid - r * DX;
-                const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
-                const long long dofs = (long long)dy * a.A.pitch + dx;
-                for (int i = tid; i < n; i += NT) {
-                    const long long g1 = a.A.at(gy0 + r, gx0 + c);
-                    const long long g2 = g1 + dofs;
-                    const float s00 = A0[g1] + A0[g2];
-                    const float s01r = A2[g1] + A2[g2];
-                    const float s01i = A3[g1] + A3[g2];
-                    const float s11 = A1[g1] + A1[g2];
-                    const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
-                    float sim = -(((K + A4[g1]) + A4[g2]) - 2.0f * logf(det));
-                    if (isnan(sim)) sim = 0.f;
-                    sD[r * DXM + c] = sim;
-                    r += sr_;
-                    c += sc_;
-                    if (c >= DX) { c -= DX; r++; }
-                }
-            }
-            __syncthreads();
-            // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
-            //    patch size, for runs of 4 columns held in registers
-            {
-                const int nrx = (RX + 3) >> 2;
-                const int n = DY * nrx;
-                for (int i = tid; i < n; i += NT) {
-                    const int r = i / nrx, x0 = (i - r * nrx) << 2;
-                    float w[W2];
-                    const float* src = sD + r * DXM + x0;
-#pragma unroll
-                    for (int t = 0; t < W2; t++) w[t] = src[t];
-#pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const int psh = a.psh[k];
-                        float* dst = sR + k * RPL + r * RXM + x0;
-#pragma unroll
-                        for (int m = 0; m < 4; m++) {
-                            float acc_s = 0.f;
-                            bool first = true;
-#pragma unroll
-                            for (int jj = -P; jj <= P; jj++) {
-                                if (jj >= -psh && jj <= psh) {
-                                    const float v = w[m + P + jj];
-                                    acc_s = first ? v : acc_s + v;
-                                    first = false;
-                                }
-                            }
-                            dst[m] = acc_s;
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
-            //    runs of 4 rows, and the LUT weight (compute_weights.cl:19-56)
-            {
-                const int nry = (RY + 3) >> 2;
-                const int n = nry * RX;
-                for (int i = tid; i < n; i += NT) {
-                    const int ry = i / RX, x = i - ry * RX, y0 = ry << 2;
-#pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const int psh = a.psh[k];
-                        const float* src = sR + k * RPL + y0 * RXM + x;
-                        float w[W2];
-#pragma unroll
-                        for (int t = 0; t < W2; t++) w[t] = src[t * RXM];
-                        const float* lut = sL + k * a.lut_stride;
-                        float* dst = sW + k * WPL + y0 * RXM + x;
-#pragma unroll
-                        for (int m = 0; m < 4; m++) {
-                            float acc_s = 0.f;
-                            bool first = true;
-#pragma unroll
-                            for (int jj = -P; jj <= P; jj++) {
-                                if (jj >= -psh && jj <= psh) {
-                                    const float v = w[m + P + jj];
-                                    acc_s = first ? v : acc_s + v;
-                                    first = false;
-                                }
-                            }
-                            dst[m * RXM] = lut_weight(acc_s, a.dmin[k], a.dmax[k], rng[k], scl[k], mul[k], lut);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
-            //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
-            //    (compute_nl_statistics.cl:69-84) of every patch size
-            {
-                const int py0 = g * 4;
-                const int cy = ty0 + py0, cx = tx0 + px;
-                const long long cu0 = a.C.at(cy + dy, cx + dx);
-                const long long cl0 = a.C.at(cy - dy, cx - dx);
-                const int iu0 = py0 * RXM + (px - rxo);
-                const int il0 = (py0 - dy) * RXM + (px - dx - rxo);
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const long long cu = cu0 + (long long)q * cpitch;
-                    const long long cl = cl0 + (long long)q * cpitch;
-                    const float i0u = I0[cu], i1u = I1[cu];
-                    const float i0l = I0[cl], i1l = I1[cl];
-                    const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
-                    const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
-                    float cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
-                    if (LAST) {
-                        cru = Cr[cu];
-                        ciu = Ci[cu];
-                        crl = Cr[cl];
-                        cil = Ci[cl];
-                    }
-#pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const float wu = sW[k * WPL + iu0 + q * RXM];
-                        const float wl = sW[k * WPL + il0 + q * RXM];
-                        acc[q][k][0] += wu;
-                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
-                        acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
-                        acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
-                        acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
-                        acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
-                        acc[q][k][0] += wl;
-                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
-                        acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
-                        acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
-                        acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
-                        acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
-                        if (LAST && k == a.mean_slot) {
-                            macc[q][0] = fmaf(wu, cru, macc[q][0]);
-                            macc[q][1] = fmaf(wu, ciu, macc[q][1]);
-                            macc[q][0] = fmaf(wl, crl, macc[q][0]);
-                            macc[q][1] = fmaf(wl, cil, macc[q][1]);
-                        }
-                    }
-                }
-            }
-            // the next offset's step 1 writes sD only; its barrier orders step 2/3 writes
-            // after this step's reads of sW
-        }
-    }
-
-    // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
-    unsigned int ubl = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int py = g * 4 + q;
-        const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
-        const bool valid = iy < L.ny && ix < L.nx;
-        const long long cc = a.C.at(ty0 + py, tx0 + px);
-        const float i0c = I0[cc], i1c = I1[cc];
-        const long long opix = valid ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
-        bool fail = false;
-        float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
-#pragma unroll
-        for (int k = 0; k < NP; k++) {
-            float* s = acc[q][k];
-            s[0] += 1.0f;
-            s[1] = fmaf(1.0f, 1.0f, s[1]);
-            s[2] = fmaf(1.0f, i0c, s[2]);
-            s[3] = fmaf(1.0f, i0c * i0c, s[3]);
-            s[4] = fmaf(1.0f, i1c, s[4]);
-            s[5] = fmaf(1.0f, i1c * i1c, s[5]);
-            // compute_nl_statistics.cl:86-102, compute_alphas.cl:19-44,
-            // compute_enls_nobias.cl:19-39
-            const float ws = s[0];
-            const float enl = ws * ws / s[1];
-            float alpha = 0.0f;
-            {
-                const float inl = s[2] / ws;
-                const float wv = s[3] / ws;
-                const float var = wv - inl * inl;
-                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
-            }
-            {
-                const float inl = s[4] / ws;
-                const float wv = s[5] / ws;
-                const float var = wv - inl * inl;
-                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
-            }
-            const float oma = 1.0f - alpha;
-            const float enb = enl / (oma * oma + (alpha * alpha + 2.0f * alpha * (1.0f - alpha) / ws) * enl);
-            if (!(enb > -1.0f)) fail = true;
-            if (valid) {
-                if (a.enl_nb) a.enl_nb[(long long)a.pidx[k] * a.H * a.W + opix] = enb;
-                if (a.alpha) a.alpha[(long long)a.pidx[k] * a.H * a.W + opix] = alpha;
-            }
-            if (LAST && k == a.mean_slot) {
-                alpha_mean = alpha;
-                s_mean = ws;
-                m0_mean = s[2];
-                m1_mean = s[4];
-                // centre weight 1 on the off-diagonal sums as well
-                macc[q][0] = fmaf(1.0f, Cr[cc], macc[q][0]);
-                macc[q][1] = fmaf(1.0f, Ci[cc], macc[q][1]);
-            }
-        }
-        if (!valid) continue;
-        if (!LAST) {
-            if (a.first_scale) a.ubflag[opix] = fail ? 1 : 0;
-            else if (fail) a.ubflag[opix] = 1;
-        } else {
-            // best-parameter selection (best_params.cpp:27-53) keeps the last key of the
-            // (patch, scale) map whenever every ENL exceeds -1; pixels where the reference's
-            // selection loop is undefined are counted and also take the last key.
-            if (fail || a.ubflag[opix]) ubl++;
-            // weighted_means.cl:62-80 and covmat_decompose.cl:19-39
-            const float al = alpha_mean;
-            const float crc = Cr[cc], cic = Ci[cc];
-            const float m00 = (1 - al) * m0_mean / s_mean + al * i0c;
-            const float m11 = (1 - al) * m1_mean / s_mean + al * i1c;
-            const float mr = (1 - al) * macc[q][0] / s_mean + al * crc;
-            const float mi = (1 - al) * macc[q][1] / s_mean + al * cic;
-            const float am = sqrtf(m00), as = sqrtf(m11);
-            const float t = 0.5f * (am + as);
-            a.ref[opix] = t * t;
-            a.phase[opix] = atan2f(mi, mr);
-            a.coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
-        }
-    }
-    if (LAST && ubl) atomicAdd(a.ub_count, (unsigned long long)ubl);
+    if (threadIdx.x == 0) *c = 0ull;
 }
 
 // ------------------------------------------------------------------------------------
@@ -513,83 +124,6 @@ std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c)
 }
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
-template <int TX, int NP, bool LAST, int P>
-static int launch_scale_pass(ScaleArgs a, const Launch* L, int lut_stride, hipStream_t stream)
-{
-    const ScaleLds<TX, NP, P> lay{a.wsh, lut_stride};
-    const size_t lds = sizeof(float) * (size_t)lay.total();
-    if (lds > 160 * 1024) {
-        set_error("nlsar: search window / patch sizes exceed the LDS budget");
-        return -2;
-    }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, NP, LAST, P>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int total = 0;
-    for (int i = 0; i < 4; i++) {
-        a.L[i] = L[i];
-        a.bstart[i] = total;
-        if (L[i].ny <= 0 || L[i].nx <= 0) {
-            a.gx[i] = 1;
-            continue;
-        }
-        a.gx[i] = (L[i].nx + TX - 1) / TX;
-        total += a.gx[i] * ((L[i].ny + NLSAR_TY - 1) / NLSAR_TY);
-    }
-    a.bstart[4] = total;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    const bool timed = timing_on();
-    if (timed) {
-        DKX_HIP_CHECK(hipEventCreate(&e0));
-        DKX_HIP_CHECK(hipEventCreate(&e1));
-        DKX_HIP_CHECK(hipEventRecord(e0, stream));
-    }
-    hipLaunchKernelGGL((nlsar_scale_pass<TX, NP, LAST, P>), dim3(total), dim3(8 * TX), lds, stream, a);
-    DKX_HIP_CHECK(hipGetLastError());
-    if (timed) {
-        DKX_HIP_CHECK(hipEventRecord(e1, stream));
-        timing_push(LAST ? STAGE_NLSAR_LAST : STAGE_NLSAR_SCALE, e0, e1);
-    }
-    return 0;
-}
-
-template <int NP, bool LAST, int P>
-static int dispatch_tile(const ScaleArgs& a, const Launch* L, int lut_stride, int tile,
-                         hipStream_t stream)
-{
-    if (tile == 1) return launch_scale_pass<64, NP, LAST, P>(a, L, lut_stride, stream);
-    return launch_scale_pass<32, NP, LAST, P>(a, L, lut_stride, stream);
-}
-
-template <int NP, bool LAST>
-static int dispatch_p(const ScaleArgs& a, const Launch* L, int lut_stride, int tile,
-                      hipStream_t stream)
-{
-    switch (a.P) {
-    case 0: return dispatch_tile<NP, LAST, 0>(a, L, lut_stride, tile, stream);
-    case 1: return dispatch_tile<NP, LAST, 1>(a, L, lut_stride, tile, stream);
-    case 2: return dispatch_tile<NP, LAST, 2>(a, L, lut_stride, tile, stream);
-    case 3: return dispatch_tile<NP, LAST, 3>(a, L, lut_stride, tile, stream);
-    case 4: return dispatch_tile<NP, LAST, 4>(a, L, lut_stride, tile, stream);
-    case 5: return dispatch_tile<NP, LAST, 5>(a, L, lut_stride, tile, stream);
-    default: set_error("nlsar: patch sizes above 11 are not supported"); return -4;
-    }
-}
-
-static int dispatch(const ScaleArgs& a, int np, bool last, const Launch* L, int nl,
-                    int lut_stride, int tile, hipStream_t stream)
-{
-    (void)nl;
-    switch (np) {
-    case 1: return last ? dispatch_p<1, true>(a, L, lut_stride, tile, stream)
-                        : dispatch_p<1, false>(a, L, lut_stride, tile, stream);
-    case 2: return last ? dispatch_p<2, true>(a, L, lut_stride, tile, stream)
-                        : dispatch_p<2, false>(a, L, lut_stride, tile, stream);
-    case 3: return last ? dispatch_p<3, true>(a, L, lut_stride, tile, stream)
-                        : dispatch_p<3, false>(a, L, lut_stride, tile, stream);
-    default: set_error("nlsar: internal error, more than 3 patch sizes in one pass"); return -1;
-    }
-}
 
 static const int kTileTY[2] = {32, 32};
 static const int kTileTX[2] = {32, 64};
@@ -637,6 +171,11 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
     plan->C.plane = (long long)plan->C.pitch * plan->C.h;
     plan->A = Grid{nullptr, 0, round_up(Wx + 2 * ra, 64), ra, ra, Hx + 2 * ra, Wx + 2 * ra};
     plan->A.plane = (long long)plan->A.pitch * plan->A.h;
+    // kernels index grid elements with 32-bit offsets
+    if (plan->C.plane >= INT_MAX / 2 || plan->A.plane >= INT_MAX / 2) {
+        set_error("nlsar: image too large for one call (split it into bands)");
+        return -4;
+    }
     int maxlut = 0;
     for (int i = 0; i < pb.nst; i++) maxlut = std::max(maxlut, pb.stats[i].lut_size);
     plan->lut_stride = maxlut;
@@ -703,7 +242,8 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
     const int H = pb.H, W = pb.W;
     std::vector<const NlsarStatsView*> pst;
     if (nlsar_param_stats(pb, plan, pst)) return -3;
-    DKX_HIP_CHECK(hipMemsetAsync(d_ub, 0, sizeof(unsigned long long), stream));
+    hipLaunchKernelGGL(nlsar_zero_count, dim3(1), dim3(64), 0, stream, d_ub);
+    DKX_HIP_CHECK(hipGetLastError());
 
     {
         dim3 blk(256), grd((C.w + 255) / 256, C.h);
@@ -761,24 +301,42 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
             DKX_HIP_CHECK(hipGetLastError());
             cur_scale = si;
         }
+        // slots in ascending patch size: the pass's patch set is a compile-time mask of
+        // half widths, and the last pass's largest patch (the mean's) is its last slot
+        std::vector<int> pis = ps.pis;
+        std::sort(pis.begin(), pis.end(), [&](int x, int y) { return pb.P[x] < pb.P[y]; });
         ScaleArgs sa;
         memset(&sa, 0, sizeof(sa));
-        sa.A = A;
-        sa.C = C;
+        sa.A4 = reinterpret_cast<const float4*>(A.base);
+        sa.LD = A.base + 4 * A.plane;
+        sa.C4 = reinterpret_cast<const float4*>(C.base);
+        sa.apitch = A.pitch;
+        sa.aoy = A.oy;
+        sa.aox = A.ox;
+        sa.cpitch = C.pitch;
+        sa.coy = C.oy;
+        sa.cox = C.ox;
         sa.wsh = plan.wsh;
-        sa.P = plan.P;
-        sa.mean_slot = -1;
-        for (size_t k = 0; k < ps.pis.size(); k++) {
-            const int pi = ps.pis[k];
+        int pmask = 0;
+        for (size_t k = 0; k < pis.size(); k++) {
+            const int pi = pis[k];
             const int idx = si * pb.nP + pi;
-            sa.psh[k] = (pb.P[pi] - 1) / 2;
+            pmask |= 1 << ((pb.P[pi] - 1) / 2);
             sa.pidx[k] = idx;
             sa.lutw[k] = d_lut + (size_t)idx * plan.lut_stride;
-            sa.dmin[k] = pst[idx]->dissims_min;
-            sa.dmax[k] = pst[idx]->dissims_max;
+            const float dmin = pst[idx]->dissims_min, dmax = pst[idx]->dissims_max;
+            sa.dmin[k] = dmin;
+            sa.dmax[k] = dmax;
+            sa.rng[k] = dmax - dmin;
+            sa.rcp[k] = 1.0f / sa.rng[k];
+            sa.scl[k] = (float)(pst[idx]->lut_size - 1);
             sa.lut_size[k] = pst[idx]->lut_size;
-            if (ps.last && pi == pmax_i) sa.mean_slot = (int)k;
         }
+        if (ps.last && pis.back() != pmax_i) {
+            set_error("nlsar: internal error, the last pass must end with the largest patch");
+            return -1;
+        }
+        for (int i = 0; i < 4; i++) sa.L[i] = L[i];
         sa.lut_stride = plan.lut_stride;
         sa.H = H;
         sa.W = W;
@@ -790,11 +348,13 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
         sa.phase = d_phase;
         sa.coh = d_coh;
         sa.ub_count = d_ub;
-        if (ps.last && oi == 0) {
-            // a single pass: nothing else wrote the flags
-            DKX_HIP_CHECK(hipMemsetAsync(d_flag, 0, (size_t)H * W, stream));
-        }
-        int rc = dispatch(sa, (int)ps.pis.size(), ps.last, L, 4, plan.lut_stride, plan.tile, stream);
+        int rc;
+        if (ps.last)
+            rc = plan.tile == 1 ? nlsar_pass_launch<true, 64>(pmask, sa, stream)
+                                : nlsar_pass_launch<true, 32>(pmask, sa, stream);
+        else
+            rc = plan.tile == 1 ? nlsar_pass_launch<false, 64>(pmask, sa, stream)
+                                : nlsar_pass_launch<false, 32>(pmask, sa, stream);
         if (rc) return rc;
     }
     if (ub_host) {

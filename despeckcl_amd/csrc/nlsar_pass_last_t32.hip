@@ -1,0 +1,6 @@
+// Instantiation of the NL-SAR scale pass for the last pass, tiles of 32 x 32.
+#include "dkx_nlsar_pass.h"
+
+namespace dkx {
+DKX_NLSAR_PASS_INSTANTIATE(true, 32)
+}  // namespace dkx
