@@ -24,7 +24,7 @@ __host__ __device__ constexpr int pmask_bit(int m, int k, int b = 0)
     return (m & 1) ? (k == 0 ? b : pmask_bit(m >> 1, k - 1, b + 1)) : pmask_bit(m >> 1, k, b + 1);
 }
 
-constexpr int NLSAR_TY = 32;    // tile rows; each thread owns a column run of 4 of them
+constexpr int NLSAR_TY = 32;    // tile rows
 constexpr int NLSAR_BAND = 8;   // tile rows per band of the block order
 
 struct ScaleArgs {
@@ -56,25 +56,34 @@ struct ScaleArgs {
     unsigned long long* ub_count;
 };
 
-// LDS layout (floats) shared by the kernel and its launcher.  XU/YU: the union region of a
-// tile and the tile shifted by an offset, rounded up to runs of 4.
-template <int TX, int P>
+// LDS layout (floats) shared by the kernel and its launcher, fixed at compile time for
+// search windows of half width up to WMAX so that every LDS access takes an immediate
+// offset.  XU/YU: the union region of a tile and the tile shifted by an offset, rounded up
+// to runs of 4.
+template <int TX, int P, int WMAX>
 struct PassLds {
-    int wsh, lut_stride, np;
     static constexpr int W2 = 4 + 2 * P;          // a run of 4 plus the patch halo
     static constexpr int W2R = (W2 + 3) & ~3;     // the same in float4 loads
-    __host__ __device__ int XU() const { return (TX + wsh + 3) & ~3; }
-    __host__ __device__ int YU() const { return (NLSAR_TY + wsh + 3) & ~3; }
-    __host__ __device__ int DXM() const { return XU() + W2R; }          // dissimilarity row
-    __host__ __device__ int DYM() const { return NLSAR_TY + wsh + 2 * P; }
-    __host__ __device__ int RXM() const { return XU(); }                // row-sum / weight row
-    __host__ __device__ int RRows() const { return YU() + 2 * P; }
-    __host__ __device__ int WRows() const { return YU(); }
-    __host__ __device__ int off_R() const { return DYM() * DXM(); }
-    __host__ __device__ int off_W() const { return off_R() + np * RRows() * RXM(); }
-    __host__ __device__ int off_L() const { return off_W() + np * WRows() * RXM(); }
-    __host__ __device__ int total() const { return off_L() + np * lut_stride; }
+    static constexpr int XU = (TX + WMAX + 3) & ~3;
+    static constexpr int YU = (NLSAR_TY + WMAX + 3) & ~3;
+    static constexpr int DXM = XU + W2R;           // dissimilarity row
+    static constexpr int DYM = NLSAR_TY + WMAX + 2 * P;
+    static constexpr int RXM = XU;                 // row-sum / weight row
+    static constexpr int RRows = YU + 2 * P;
+    static constexpr int WRows = YU;
+    static constexpr int RPL = RRows * RXM;        // row-sum plane
+    static constexpr int WPL = WRows * RXM;        // weight plane
+    // the dissimilarities (steps 1-2) and the weights (steps 3-4) share one region
+    static constexpr int shared_floats(int np) { return DYM * DXM > np * WPL ? DYM * DXM : np * WPL; }
+    static constexpr int off_W = 0;
+    static constexpr int off_R(int np) { return shared_floats(np); }
+    static constexpr int off_L(int np) { return off_R(np) + np * RPL; }
+    static constexpr int total(int np, int lut_stride) { return off_L(np) + np * lut_stride; }
 };
+
+// Window classes: the largest search-window half width each compiled layout serves.
+constexpr int NLSAR_WMAX_SMALL = 10;   // search windows up to 21
+constexpr int NLSAR_WMAX_LARGE = 20;   // search windows up to 41
 
 // 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
 __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
@@ -122,42 +131,45 @@ __device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, int
     tx0 = a.L[m].vx0 + bx * TX;
 }
 
-template <int TX, int PMASK, bool LAST>
-__global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
+// Occupancy target: blocks per CU (three fit the LDS budget of the small window class; two
+// for single-row threads, whose 16-wave blocks would otherwise leave 40 registers), which
+// bounds the registers the compiler may use.
+constexpr int nlsar_blocks_per_cu(int RPT, int WMAX) { return WMAX > NLSAR_WMAX_SMALL ? 1 : RPT == 1 ? 2 : 3; }
+
+template <int TX, int RPT, int PMASK, bool LAST, int WMAX>
+__global__ __launch_bounds__(TX * NLSAR_TY / RPT, nlsar_blocks_per_cu(RPT, WMAX) * TX * NLSAR_TY / RPT / 256)
+void nlsar_scale_pass(const ScaleArgs a)
 {
-    constexpr int NT = 8 * TX;
+    constexpr int NT = TX * NLSAR_TY / RPT;   // each thread owns RPT rows of one column
     constexpr int TY = NLSAR_TY;
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
-    constexpr int W2 = PassLds<TX, P>::W2;
-    constexpr int W2R = PassLds<TX, P>::W2R;
+    using Lay = PassLds<TX, P, WMAX>;
+    constexpr int W2 = Lay::W2, W2R = Lay::W2R;
+    constexpr int DXM = Lay::DXM, RXM = Lay::RXM, RPL = Lay::RPL, WPL = Lay::WPL;
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     const int wsh = a.wsh;
     const int sw = 2 * wsh + 1;
-    const PassLds<TX, P> lay{wsh, a.lut_stride, NP};
-    const int DXM = lay.DXM(), RXM = lay.RXM();
     float* sD = smem;
-    float* sR = smem + lay.off_R();
-    float* sW = smem + lay.off_W();
-    float* sL = smem + lay.off_L();
-    const int RPL = lay.RRows() * RXM;   // row-sum plane
-    const int WPL = lay.WRows() * RXM;   // weight plane
+    float* sW = smem + Lay::off_W;
+    float* sR = smem + Lay::off_R(NP);
+    float* sL = smem + Lay::off_L(NP);
 
     const int tid = threadIdx.x;
-    const int px = tid % TX, g = tid / TX;  // column, group of 4 rows
+    const int px = tid % TX, g = tid / TX;  // column, group of RPT rows
     int m_, ty0, tx0;
     nlsar_block_tile(a, TX, m_, ty0, tx0);
-    const Launch L = a.L[m_];
+    const Launch L = m_ == 0 ? a.L[0] : m_ == 1 ? a.L[1] : m_ == 2 ? a.L[2] : a.L[3];
 
 #pragma unroll
     for (int k = 0; k < NP; k++)
         for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
 
-    float acc[4][NP][6];
-    float macc[4][2];
+    float acc[RPT][NP][6];
+    float macc[RPT][2];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < RPT; q++) {
 #pragma unroll
         for (int k = 0; k < NP; k++)
 #pragma unroll
@@ -167,6 +179,9 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
 
     const float K = nlsar_log2_term();
     const int apitch = a.apitch, cpitch = a.cpitch;
+    const char* A4b = reinterpret_cast<const char*>(a.A4);
+    const char* LDb = reinterpret_cast<const char*>(a.LD);
+    const char* C4b = reinterpret_cast<const char*>(a.C4);
 
     // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
     // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
@@ -181,29 +196,41 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
             const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
             const int DY = RY + 2 * P, DX = RX + 2 * P;
             // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
-            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65)
+            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65).  Element i of the
+            //    DY x DX region: row r, column c; byte offsets advance incrementally.
             {
                 const int n = DY * DX;
-                int r = tid / DX, c = tid - r * DX;
                 const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
-                const int base = (ty0 - P + a.aoy) * apitch + (tx0 + rxo - P + a.aox);
-                const int dofs = dy * apitch + dx;
+                int r = tid / DX, c = tid - r * DX;
+                const unsigned int gb0 = (unsigned int)((ty0 - P + a.aoy) * apitch + (tx0 + rxo - P + a.aox));
+                unsigned int g1 = gb0 + (unsigned int)(r * apitch + c);      // element index
+                const unsigned int dofs = (unsigned int)(dy * apitch + dx);
+                const unsigned int gstep = (unsigned int)(sr_ * apitch + sc_);
+                const unsigned int gwrap = (unsigned int)(apitch - DX);
+                int so = r * DXM + c;
+                const int sstep = sr_ * DXM + sc_;
                 for (int i = tid; i < n; i += NT) {
-                    const int g1 = base + r * apitch + c;
-                    const int g2 = g1 + dofs;
-                    const float4 u = a.A4[g1];
-                    const float4 v = a.A4[g2];
+                    const unsigned int g2 = g1 + dofs;
+                    const float4 u = *reinterpret_cast<const float4*>(A4b + 16u * g1);
+                    const float4 v = *reinterpret_cast<const float4*>(A4b + 16u * g2);
+                    const float l1 = *reinterpret_cast<const float*>(LDb + 4u * g1);
+                    const float l2 = *reinterpret_cast<const float*>(LDb + 4u * g2);
                     const float s00 = u.x + v.x;
                     const float s01r = u.z + v.z;
                     const float s01i = u.w + v.w;
                     const float s11 = u.y + v.y;
                     const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
-                    float sim = -(((K + a.LD[g1]) + a.LD[g2]) - 2.0f * logf(det));
+                    float sim = -(((K + l1) + l2) - 2.0f * logf(det));
                     if (isnan(sim)) sim = 0.f;
-                    sD[r * DXM + c] = sim;
-                    r += sr_;
+                    sD[so] = sim;
                     c += sc_;
-                    if (c >= DX) { c -= DX; r++; }
+                    g1 += gstep;
+                    so += sstep;
+                    if (c >= DX) {
+                        c -= DX;
+                        g1 += gwrap;
+                        so += DXM - DX;
+                    }
                 }
             }
             __syncthreads();
@@ -212,8 +239,10 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
             {
                 const int nrx = (RX + 3) >> 2;
                 const int n = DY * nrx;
+                const int qs = NT / nrx, qr = NT - qs * nrx;
+                int r = tid / nrx, xi = tid - r * nrx;
                 for (int i = tid; i < n; i += NT) {
-                    const int r = i / nrx, x0 = (i - r * nrx) << 2;
+                    const int x0 = xi << 2;
                     float w[W2R];
                     const float4* src = reinterpret_cast<const float4*>(sD + r * DXM + x0);
 #pragma unroll
@@ -224,20 +253,26 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
                         w[4 * t + 2] = v.z;
                         w[4 * t + 3] = v.w;
                     }
+                    float* dst = sR + r * RXM + x0;
 #pragma unroll
                     for (int k = 0; k < NP; k++) {
                         const int psh = pmask_bit(PMASK, k);
                         float o[4];
 #pragma unroll
                         for (int m = 0; m < 4; m++) {
-                            float s = w[m + P - psh];
+                            float sum = w[m + P - psh];
 #pragma unroll
                             for (int jj = -P + 1; jj <= P; jj++)
-                                if (jj > -psh && jj <= psh) s = s + w[m + P + jj];
-                            o[m] = s;
+                                if (jj > -psh && jj <= psh) sum = sum + w[m + P + jj];
+                            o[m] = sum;
                         }
-                        *reinterpret_cast<float4*>(sR + k * RPL + r * RXM + x0) =
-                            make_float4(o[0], o[1], o[2], o[3]);
+                        *reinterpret_cast<float4*>(dst + k * RPL) = make_float4(o[0], o[1], o[2], o[3]);
+                    }
+                    xi += qr;
+                    r += qs;
+                    if (xi >= nrx) {
+                        xi -= nrx;
+                        r++;
                     }
                 }
             }
@@ -249,30 +284,42 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
                 const int nrx = (RX + 1) >> 1;
                 const int nry = (RY + 3) >> 2;
                 const int n = nry * nrx;
+                const int qs = NT / nrx, qr = NT - qs * nrx;
+                int ry = tid / nrx, xi = tid - ry * nrx;
                 for (int i = tid; i < n; i += NT) {
-                    const int ry = i / nrx, x = (i - ry * nrx) << 1, y0 = ry << 2;
+                    const int off = (ry << 2) * RXM + (xi << 1);
 #pragma unroll
                     for (int k = 0; k < NP; k++) {
                         const int psh = pmask_bit(PMASK, k);
-                        const float* src = sR + k * RPL + y0 * RXM + x;
-                        float2 w[W2];
+                        const float* src = sR + k * RPL + off;
+                        float w0[W2], w1[W2];
 #pragma unroll
-                        for (int t = 0; t < W2; t++) w[t] = *reinterpret_cast<const float2*>(src + t * RXM);
+                        for (int t = 0; t < W2; t++) {
+                            const float2 v = *reinterpret_cast<const float2*>(src + t * RXM);
+                            w0[t] = v.x;
+                            w1[t] = v.y;
+                        }
                         const float* lut = sL + k * a.lut_stride;
-                        float* dst = sW + k * WPL + y0 * RXM + x;
+                        float* dst = sW + k * WPL + off;
 #pragma unroll
                         for (int m = 0; m < 4; m++) {
-                            float s0 = w[m + P - psh].x, s1 = w[m + P - psh].y;
+                            float s0 = w0[m + P - psh], s1 = w1[m + P - psh];
 #pragma unroll
                             for (int jj = -P + 1; jj <= P; jj++)
                                 if (jj > -psh && jj <= psh) {
-                                    s0 = s0 + w[m + P + jj].x;
-                                    s1 = s1 + w[m + P + jj].y;
+                                    s0 = s0 + w0[m + P + jj];
+                                    s1 = s1 + w1[m + P + jj];
                                 }
-                            const float w0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
-                            const float w1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
-                            *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(w0, w1);
+                            const float x0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                            const float x1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                            *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(x0, x1);
                         }
+                    }
+                    xi += qr;
+                    ry += qs;
+                    if (xi >= nrx) {
+                        xi -= nrx;
+                        ry++;
                     }
                 }
             }
@@ -281,23 +328,25 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
             //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
             //    (compute_nl_statistics.cl:69-84) of every patch size
             {
-                const int py0 = g * 4;
+                const int py0 = g * RPT;
                 const int cy = ty0 + py0, cx = tx0 + px;
-                const int cu0 = (cy + dy + a.coy) * cpitch + (cx + dx + a.cox);
-                const int cl0 = (cy - dy + a.coy) * cpitch + (cx - dx + a.cox);
-                const int iu0 = py0 * RXM + (px - rxo);
-                const int il0 = (py0 - dy) * RXM + (px - dx - rxo);
+                const unsigned int cu0 = (unsigned int)((cy + dy + a.coy) * cpitch + (cx + dx + a.cox));
+                const unsigned int cl0 = (unsigned int)((cy - dy + a.coy) * cpitch + (cx - dx + a.cox));
+                const float* wu0 = sW + py0 * RXM + (px - rxo);
+                const float* wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < RPT; q++) {
+                    const unsigned int cu = cu0 + (unsigned int)(q * cpitch);
+                    const unsigned int cl = cl0 + (unsigned int)(q * cpitch);
                     float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
                     if (LAST) {
-                        const float4 u = a.C4[cu0 + q * cpitch];
-                        const float4 l = a.C4[cl0 + q * cpitch];
+                        const float4 u = *reinterpret_cast<const float4*>(C4b + 16u * cu);
+                        const float4 l = *reinterpret_cast<const float4*>(C4b + 16u * cl);
                         i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
                         i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
                     } else {
-                        const float2 u = reinterpret_cast<const float2*>(a.C4)[2 * (cu0 + q * cpitch)];
-                        const float2 l = reinterpret_cast<const float2*>(a.C4)[2 * (cl0 + q * cpitch)];
+                        const float2 u = *reinterpret_cast<const float2*>(C4b + 16u * cu);
+                        const float2 l = *reinterpret_cast<const float2*>(C4b + 16u * cl);
                         i0u = u.x; i1u = u.y;
                         i0l = l.x; i1l = l.y;
                     }
@@ -305,8 +354,8 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
                     const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
 #pragma unroll
                     for (int k = 0; k < NP; k++) {
-                        const float wu = sW[k * WPL + iu0 + q * RXM];
-                        const float wl = sW[k * WPL + il0 + q * RXM];
+                        const float wu = wu0[k * WPL + q * RXM];
+                        const float wl = wl0[k * WPL + q * RXM];
                         acc[q][k][0] += wu;
                         acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
                         acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
@@ -328,19 +377,20 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
                     }
                 }
             }
-            // the next offset's step 1 writes sD only; its barrier orders step 2/3 writes
-            // after this step's reads of sW
+            // the next offset's step 1 overwrites this step's weights
+            __syncthreads();
         }
     }
 
     // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
     unsigned int ubl = 0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int py = g * 4 + q;
+    for (int q = 0; q < RPT; q++) {
+        const int py = g * RPT + q;
         const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
         const bool valid = iy < L.ny && ix < L.nx;
-        const float4 cc = a.C4[(ty0 + py + a.coy) * cpitch + (tx0 + px + a.cox)];
+        const float4 cc = *reinterpret_cast<const float4*>(
+            C4b + 16u * (unsigned int)((ty0 + py + a.coy) * cpitch + (tx0 + px + a.cox)));
         const float i0c = cc.x, i1c = cc.y;
         const long long opix = valid ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
         bool fail = false;
@@ -414,7 +464,7 @@ __global__ __launch_bounds__(8 * TX) void nlsar_scale_pass(const ScaleArgs a)
 }
 
 // Launch of one pass; defined in the instantiating translation units.
-template <bool LAST, int TX>
+template <bool LAST, int TX, int RPT, int WMAX>
 int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
 
 // Every patch set of one to three half widths in [0, 5].
@@ -424,18 +474,18 @@ int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
     X(7) X(11) X(19) X(35) X(13) X(21) X(37) X(25) X(41) X(49) X(14) X(22) X(38) X(26)      \
     X(42) X(50) X(28) X(44) X(52) X(56)
 
-template <bool LAST, int TX, int PMASK>
+template <bool LAST, int TX, int RPT, int WMAX, int PMASK>
 int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
 {
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
-    const PassLds<TX, P> lay{a.wsh, a.lut_stride, NP};
-    const size_t lds = sizeof(float) * (size_t)lay.total();
-    if (lds > 160 * 1024) {
+    using Lay = PassLds<TX, P, WMAX>;
+    const size_t lds = sizeof(float) * (size_t)Lay::total(NP, a.lut_stride);
+    if (a.wsh > WMAX || lds > 160 * 1024) {
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
     }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, PMASK, LAST>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int total = 0;
     for (int i = 0; i < 4; i++) {
@@ -456,7 +506,8 @@ int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
         DKX_HIP_CHECK(hipEventCreate(&e1));
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
-    hipLaunchKernelGGL((nlsar_scale_pass<TX, PMASK, LAST>), dim3(total), dim3(8 * TX), lds, stream, a);
+    hipLaunchKernelGGL((nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX>), dim3(total), dim3(TX * NLSAR_TY / RPT),
+                       lds, stream, a);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
         DKX_HIP_CHECK(hipEventRecord(e1, stream));
@@ -465,13 +516,16 @@ int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
     return 0;
 }
 
-#define DKX_NLSAR_PASS_CASE(M) case M: return nlsar_pass_launch_mask<LAST_, TX_, M>(a, stream);
-#define DKX_NLSAR_PASS_INSTANTIATE(LAST_V, TX_V)                                              \
+#define DKX_NLSAR_PASS_CASE(M) case M: return nlsar_pass_launch_mask<LAST_, TX_, RPT_, WMAX_, M>(a, stream);
+#define DKX_NLSAR_PASS_INSTANTIATE(LAST_V, TX_V, RPT_V, WMAX_V)                               \
     template <>                                                                              \
-    int nlsar_pass_launch<LAST_V, TX_V>(int pmask, const ScaleArgs& a, hipStream_t stream)   \
+    int nlsar_pass_launch<LAST_V, TX_V, RPT_V, WMAX_V>(int pmask, const ScaleArgs& a,       \
+                                                       hipStream_t stream)                   \
     {                                                                                        \
         constexpr bool LAST_ = LAST_V;                                                       \
         constexpr int TX_ = TX_V;                                                            \
+        constexpr int RPT_ = RPT_V;                                                          \
+        constexpr int WMAX_ = WMAX_V;                                                        \
         switch (pmask) {                                                                     \
             DKX_NLSAR_PMASKS(DKX_NLSAR_PASS_CASE)                                            \
         default:                                                                             \

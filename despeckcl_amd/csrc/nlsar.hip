@@ -125,13 +125,15 @@ std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c)
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-static const int kTileTY[2] = {32, 32};
-static const int kTileTX[2] = {32, 64};
+// tile shapes (rows x columns); tiles 0 / 1 / 2 give each thread 4 / 2 / 1 rows
+static const int kTileTY[3] = {32, 32, 32};
+static const int kTileTX[3] = {32, 32, 32};
 
 int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
 {
     if (pb.H <= 0 || pb.W <= 0) { set_error("nlsar: empty image"); return -1; }
     if (pb.sw < 3 || pb.sw % 2 == 0) { set_error("nlsar: search window size must be odd and >= 3"); return -1; }
+    if (pb.sw > 2 * NLSAR_WMAX_LARGE + 1) { set_error("nlsar: search windows above 41 are not supported"); return -4; }
     if (pb.nP < 1 || pb.nP > NLSAR_MAX_PATCHES || pb.nS < 1 || pb.nS > NLSAR_MAX_NS) {
         set_error("nlsar: 1-16 patch sizes and 1-8 scale sizes are supported");
         return -1;
@@ -152,7 +154,7 @@ int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
     for (int i = 0; i < pb.nS; i++)
         for (int j = i + 1; j < pb.nS; j++)
             if (pb.S[i] == pb.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
-    plan->tile = std::min(std::max(pb.tile, 0), 1);
+    plan->tile = std::min(std::max(pb.tile, 0), 2);
     plan->TY = kTileTY[plan->tile];
     plan->TX = kTileTX[plan->tile];
     plan->wsh = (pb.sw - 1) / 2;
@@ -349,12 +351,25 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
         sa.coh = d_coh;
         sa.ub_count = d_ub;
         int rc;
-        if (ps.last)
-            rc = plan.tile == 1 ? nlsar_pass_launch<true, 64>(pmask, sa, stream)
-                                : nlsar_pass_launch<true, 32>(pmask, sa, stream);
-        else
-            rc = plan.tile == 1 ? nlsar_pass_launch<false, 64>(pmask, sa, stream)
-                                : nlsar_pass_launch<false, 32>(pmask, sa, stream);
+        if (plan.wsh <= NLSAR_WMAX_SMALL) {
+            switch (plan.tile) {
+            case 0:
+                rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream);
+                break;
+            case 2:
+                rc = ps.last ? nlsar_pass_launch<true, 32, 1, NLSAR_WMAX_SMALL>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 1, NLSAR_WMAX_SMALL>(pmask, sa, stream);
+                break;
+            default:
+                rc = ps.last ? nlsar_pass_launch<true, 32, 2, NLSAR_WMAX_SMALL>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 2, NLSAR_WMAX_SMALL>(pmask, sa, stream);
+                break;
+            }
+        } else {
+            rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_LARGE>(pmask, sa, stream)
+                         : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE>(pmask, sa, stream);
+        }
         if (rc) return rc;
     }
     if (ub_host) {
