@@ -4,6 +4,8 @@ parts (training, statistics, the chi-square LUT, statistics files) match the ora
 import ctypes as C
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -166,3 +168,72 @@ def test_no_oracle_in_product():
             if f.endswith((".py", ".cpp", ".hip", ".h")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in txt.replace("# oracle", ""), f
+
+
+LUT_INDEX_C = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+static float bits(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t ubits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+/* the scale pass's index: q0 = a * (1/rng), e = fma(-q0, rng, a), q = fma(e, 1/rng, q0) */
+static unsigned idx_fma(float a, float rng, float r, float scl) {
+    float q0 = a * r, e = fmaf(-q0, rng, a), q = fmaf(e, r, q0);
+    return (unsigned)(q * scl);
+}
+/* the reference's: (unsigned int)((d - min) / (max - min) * (lut_size - 1)) */
+static unsigned idx_div(float a, float rng, float scl) { return (unsigned)(a / rng * scl); }
+int main(void) {
+    const float dmins[] = {0.0f, -3.25f, 1.5f, 12.0f, -0.001f, 100.0f};
+    const float dmaxs[] = {1.0f, 7.3421f, 255.0f, 164.77f, 0.099f, 98865.4f};
+    long long n = 0, bad = 0;
+    uint64_t s = 88172645463325252ull;
+    for (int k = 0; k < 6; k++) {
+        const float dmin = dmins[k], dmax = dmaxs[k], rng = dmax - dmin, r = 1.0f / rng;
+        for (int lut = 2; lut <= 1024; lut *= 2) {
+            const float scl = (float)(lut - 1);
+            /* +-64 ulps around every index boundary i * rng / scl */
+            for (int i = 0; i <= lut; i++) {
+                const float b = (float)i * rng / scl;
+                const uint32_t c = ubits(b);
+                for (int dd = -64; dd <= 64; dd++) {
+                    if (c < 64 && dd < 0 && (uint32_t)(-dd) > c) continue;
+                    const float a = bits(c + dd);
+                    if (!(a >= 0.0f && a <= rng)) continue;
+                    n++;
+                    if (idx_fma(a, rng, r, scl) != idx_div(a, rng, scl)) bad++;
+                }
+            }
+            /* random differences in [0, rng] */
+            for (int t = 0; t < 200000; t++) {
+                s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+                const float d = dmin + (float)((s >> 11) * (1.0 / 9007199254740992.0)) * rng;
+                const float dc = fminf(fmaxf(d, dmin), dmax), a = dc - dmin;
+                n++;
+                if (idx_fma(a, rng, r, scl) != idx_div(a, rng, scl)) bad++;
+            }
+        }
+    }
+    printf("%lld %lld\n", n, bad);
+    return 0;
+}
+"""
+
+
+def test_lut_index_fma_quotient_equals_division(tmp_path):
+    """The scale pass forms the LUT index from the FMA-corrected quotient (Markstein) instead
+    of a division; it must give the reference expression's index for every difference: at
+    +-64 ulps around every index boundary and on random differences, for several trained
+    ranges and LUT sizes.  (An exhaustive run over all floats of ten ranges agrees too.)"""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "lutidx.c"
+    src.write_text(LUT_INDEX_C)
+    exe = tmp_path / "lutidx"
+    subprocess.run([gcc, "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-o", str(exe), str(src), "-lm"],
+                   check=True)
+    n, bad = map(int, subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split())
+    assert n > 1_000_000
+    assert bad == 0
