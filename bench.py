@@ -59,6 +59,74 @@ def cpu_baseline(a1, a2, ph, stats, size=1536):
                       f"{dt:.1f} s wall on {cores} OpenMP threads"}
 
 
+def batched(args):
+    """Batched form: 64 seeded interferograms with sides log-uniform in [256, 2048] (aspect
+    <= 4:1), the same NL-SAR parameters; plain per-call loop vs one batch (eager launches
+    and HIP-graph replay), on one stream with inputs resident in HBM."""
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    stats = D.load_nlsar_stats_collection(os.path.join(ROOT, "tests", "golden", "train32_stats.txt"))
+    shapes = S.tile_shapes(args.tiles, S.SEED_BASE + 3)
+    stream = HIP.Stream()
+    imgs = []
+    for i, (h, w) in enumerate(shapes):
+        host = S.interferogram(h, w, S.SEED_BASE + 3000 + i, n_regions=16, coh_range=(0.2, 0.95))
+        inp = [HIP.DeviceArray.from_host(x) for x in host]
+        out = [HIP.DeviceArray((h, w)) for _ in range(3)]
+        imgs.append((inp, out, h, w))
+    log(f"batched: {len(imgs)} images generated, {sum(h * w for _, _, h, w in imgs) / 1e6:.1f} Mpixel")
+    px = sum(h * w for _, _, h, w in imgs)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        stream.synchronize()
+        e0, e1 = HIP.Event(), HIP.Event()
+        e0.record(stream.handle)
+        for _ in range(steps):
+            fn()
+        e1.record(stream.handle)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    b = DV.NlsarBatch()
+    b.set([(*[d.ptr for d in inp], *[d.ptr for d in out], h, w) for inp, out, h, w in imgs],
+          SW, PATCHES, SCALES, stats, HPAR, CPAR)
+    tile, tune_ms = b.autotune(stream.handle, reps=1)
+    log(f"batched: autotune picked tile {tile} ({tune_ms})")
+    DV.set_tile(0, tile)
+
+    def plain():
+        for inp, out, h, w in imgs:
+            DV.nlsar_insar(*[d.ptr for d in inp], *[d.ptr for d in out], h, w, SW, PATCHES, SCALES,
+                           stats, HPAR, CPAR, stream=stream.handle)
+
+    plain_ms = timed(plain, args.steps)
+    eager_ms = timed(lambda: b.enqueue(stream.handle), args.steps)
+    graph_ms = timed(lambda: b.graph_launch(stream.handle), args.steps)
+    best = min(eager_ms, graph_ms)
+    out = {
+        "metric": "Mpixel/s batched NL-SAR (64 interferograms, sides 256-2048, 1 GPU)",
+        "value": round(px / 1e6 / (best / 1e3), 3), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(best, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded speckle model of insarsim_simu.cpp, 16 regions, coherence U[0.2,0.95])",
+        "config": {"workload": f"NL-SAR InSAR, {len(imgs)} interferograms per batch", "pixels": px,
+                   "shapes_seed": S.SEED_BASE + 3, "search_window": SW, "patch_sizes": PATCHES,
+                   "scale_sizes": SCALES, "tile": tile},
+        "plain_per_call": {"ms": round(plain_ms, 3), "mpx_s": round(px / 1e6 / (plain_ms / 1e3), 3)},
+        "batched_eager": {"ms": round(eager_ms, 3), "mpx_s": round(px / 1e6 / (eager_ms / 1e3), 3)},
+        "batched_graph": {"ms": round(graph_ms, 3), "mpx_s": round(px / 1e6 / (graph_ms / 1e3), 3)},
+        "speedup_vs_plain": round(plain_ms / best, 3),
+        "autotune_ms_per_tile": [round(x, 3) for x in tune_ms],
+    }
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,11 +134,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
+    ap.add_argument("--form", choices=("plain", "batched"), default="plain")
+    ap.add_argument("--tiles", type=int, default=64, help="images per batch (--form batched)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU NL-SAR path "
               "(multi-GPU sharding is by image bands and is not part of this build)", file=sys.stderr)
         sys.exit(2)
+    if args.form == "batched":
+        return batched(args)
 
     import despeckcl_amd as D
     from despeckcl_amd import device as DV

@@ -25,6 +25,12 @@ class DkxStats(C.Structure):
                 ("quantilles", C.POINTER(C.c_float)), ("chi2cdf_inv", C.POINTER(C.c_float))]
 
 
+class DkxImage(C.Structure):
+    _fields_ = [("ampl_master", C.c_void_p), ("ampl_slave", C.c_void_p), ("phase", C.c_void_p),
+                ("ref_filt", C.c_void_p), ("phase_filt", C.c_void_p), ("coh_filt", C.c_void_p),
+                ("height", C.c_int), ("width", C.c_int)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -50,6 +56,15 @@ SIGNATURES = {
     "dkx_set_timing": (_I, [_I]),
     "dkx_stage_times": (_I, [_P, _P, _I]),
     "dkx_release_workspace": (None, []),
+    "dkx_nlsar_batch_create": (_I, [C.POINTER(C.c_void_p)]),
+    "dkx_nlsar_batch_destroy": (None, [_P]),
+    "dkx_nlsar_batch_set": (_I, [_P, C.POINTER(DkxImage), _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F]),
+    "dkx_nlsar_batch_enqueue": (_I, [_P, _P]),
+    "dkx_nlsar_batch_graph_launch": (_I, [_P, _P]),
+    "dkx_nlsar_batch_set_tile": (_I, [_P, _I]),
+    "dkx_nlsar_batch_get_tile": (_I, [_P]),
+    "dkx_nlsar_batch_autotune": (_I, [_P, _P, _I, _IP, _P]),
+    "dkx_nlsar_batch_ub_pixels": (_I, [_P, _ULLP]),
 }
 
 _lib = None

@@ -143,8 +143,8 @@ static void timing_collect()
     g_marks.clear();
 }
 
-static int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP,
-                       const int* S, int nS, std::vector<NlsarStatsView>& views)
+int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
+                int nS, std::vector<NlsarStatsView>& views)
 {
     // The reference looks statistics up with operator[] and builds ENL maps for every key
     // (nlsar_filter_sub_image.cpp:224-228, 278): keys outside the patch x scale set, or
@@ -186,8 +186,9 @@ static int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, 
 // Everything that can be checked without a device: argument checks, statistics, plan.
 struct NlsarCall {
     std::vector<NlsarStatsView> views;
-    NlsarProblem pb;
-    NlsarPlan plan;
+    NlsarParams pp;
+    NlsarCommon cm;
+    NlsarImagePlan ip;
 };
 
 static int nlsar_prepare(NlsarCall& call, int height, int width, int sw, const int* P, int nP,
@@ -198,9 +199,11 @@ static int nlsar_prepare(NlsarCall& call, int height, int width, int sw, const i
     if (nP < 1 || nS < 1) { set_error("nlsar: empty patch or scale size list"); return DKX_EINVAL; }
     int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views);
     if (rc) return rc;
-    call.pb = NlsarProblem{height, width, 0, sw, P, nP, S, nS, call.views.data(),
-                           (int)call.views.size(), h, c, -1, -1, g_tile[0].load()};
-    return nlsar_plan(call.pb, &call.plan);
+    call.pp = NlsarParams{0, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
+                          g_tile[0].load()};
+    rc = nlsar_common(call.pp, &call.cm);
+    if (rc) return rc;
+    return nlsar_image_plan(call.cm, height, width, -1, -1, &call.ip);
 }
 
 static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, const float* phi,
@@ -209,17 +212,19 @@ static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, 
                              DeviceWorkspace* w)
 {
     std::vector<float> blob;
-    int rc = nlsar_build_constants(call.pb, call.plan, blob);
+    int rc = nlsar_build_constants(call.pp, call.cm, blob);
     if (rc) return rc;
-    rc = ensure(w, call.plan.ws_bytes);
+    rc = ensure(w, call.ip.bytes);
     if (rc) return rc;
     rc = set_constants(w, blob);
     if (rc) return rc;
     rc = workspace_acquire(w, stream);
     if (rc) return rc;
-    rc = nlsar_run(call.pb, call.plan, a1, a2, phi, nullptr, ref, pha, coh, w->ptr, w->consts,
-                   enl_maps, alpha_maps, ub, stream);
+    NlsarImageDesc d;
+    nlsar_image_desc(call.ip, (char*)w->ptr, a1, a2, phi, nullptr, ref, pha, coh, enl_maps, alpha_maps, &d);
+    rc = nlsar_enqueue(call.pp, call.cm, &d, 1, nullptr, w->consts, stream);
     if (rc) return rc;
+    if (ub) DKX_HIP_CHECK(hipMemcpyAsync(ub, d.ub_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     rc = workspace_release(w, stream);
     if (rc) return rc;
     if (ub) DKX_HIP_CHECK(hipStreamSynchronize(stream));

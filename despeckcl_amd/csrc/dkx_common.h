@@ -9,6 +9,33 @@
 
 namespace dkx {
 
+// Pointers read from descriptors in memory reach the kernels as generic pointers; these casts
+// give their loads and stores the global address space (global_load with a scalar base
+// instead of flat_load, which also waits on LDS traffic).
+#define DKX_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const DKX_GLOBAL T* gptr(const T* p) { return (const DKX_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ DKX_GLOBAL T* gptr(T* p) { return (DKX_GLOBAL T*)p; }
+// vector loads / stores through such pointers (HIP's float4 class has no constructor from
+// an address-space-qualified object; the native vector types do)
+typedef float dkx_f4 __attribute__((ext_vector_type(4)));
+typedef float dkx_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 gload4(const DKX_GLOBAL void* p)
+{
+    const dkx_f4 v = *(const DKX_GLOBAL dkx_f4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 gload2(const DKX_GLOBAL void* p)
+{
+    const dkx_f2 v = *(const DKX_GLOBAL dkx_f2*)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gstore4(DKX_GLOBAL void* p, float4 v)
+{
+    *(DKX_GLOBAL dkx_f4*)p = dkx_f4{v.x, v.y, v.z, v.w};
+}
+
 // A padded planar grid in image coordinates: element (y, x) of plane k lives at
 // base[k * plane + (y + oy) * pitch + (x + ox)], for y in [-oy, h - oy), x in [-ox, w - ox).
 // Grids are filled with clamp-to-edge replication of the image, which is how the reference

@@ -1,4 +1,6 @@
-// Host interface of the NL-SAR HIP path (planning + launch on device buffers).
+// Host interface of the NL-SAR HIP path: parameter validation, per-image workspace plans,
+// image descriptors and the launch sequence, for one image or a batch of images that share
+// the filter parameters.
 #pragma once
 
 #include "dkx_common.h"
@@ -10,6 +12,8 @@ namespace dkx {
 constexpr int NLSAR_MAX_NP = 3;        // patch sizes per kernel pass
 constexpr int NLSAR_MAX_PATCHES = 16;  // patch sizes per call
 constexpr int NLSAR_MAX_NS = 8;
+constexpr int NLSAR_TILE = 32;         // every pass variant computes 32 x 32 output tiles
+constexpr int NLSAR_NTILES = 3;        // pass variants: 4, 2 or 1 rows per thread
 
 // One (patch, scale) entry of the dissimilarity statistics (include/compute/nlsar/stats.h:27-55)
 struct NlsarStatsView {
@@ -19,40 +23,72 @@ struct NlsarStatsView {
     const float* chi2cdf_inv;
 };
 
-struct NlsarProblem {
-    int H, W;
+// The filter parameters of a call, or of every image of a batch.
+struct NlsarParams {
     int dim;          // 0: InSAR input (a1, a2, phase); 2: dim-2 covariance input
     int sw;
     const int* P; int nP;
     const int* S; int nS;
     const NlsarStatsView* stats; int nst;
     float h, c;
-    int edge_h, edge_w;  // virtual distance of the last row/column; < 0: the overlap
-    int tile;            // tile shape index (autotuned)
+    int tile;         // pass variant (0..NLSAR_NTILES-1)
 };
 
-struct NlsarPlan {
-    int tile, TY, TX;
-    int wsh, P, smax, overlap, eh, ew;
+// Validated parameters and what follows from them.
+struct NlsarCommon {
+    int tile;
+    int wsh, P, smax, overlap;
     int nparam, lut_stride;
+    size_t const_floats;   // weight LUTs + Gaussian taps, see nlsar_build_constants
+};
+
+// Workspace plan of one image: padded grids C (input covariance) and A (scale average).
+struct NlsarImagePlan {
+    int H, W, eh, ew;
     Grid C, A;
-    size_t off_C, off_A, off_flag, off_ub, ws_bytes;
-    size_t const_floats;  // weight LUTs + Gaussian taps, see nlsar_build_constants
+    size_t off_C, off_A, off_flag, off_ub, bytes;   // relative to the image's workspace
+};
+
+// What the kernels read about one image (device-visible; a batch keeps a table of them in
+// device memory, a single call passes one by value).
+struct NlsarImageDesc {
+    const float* a1; const float* a2; const float* phi; const float* cov;
+    float* ref; float* phase; float* coh;
+    float* enl_nb; float* alpha;         // [nparam][H*W] or null
+    float* C;                            // float4 per element
+    float* A;                            // float4 per element, then the log det plane
+    unsigned char* ubflag;               // [H*W]
+    unsigned long long* ub_count;
+    int H, W;
+    int cpitch, coy, cox, ch, cw;
+    int apitch, aoy, aox, ah, aw, aplane;
+    Launch L[4];                         // interior, last row, last column, corner
+    int gx[4], gy[4], bstart[5];         // pass blocks of each mapping, [4] = image total
+    int row0_c, row0_a, block0;          // first row / pass block of this image in a batch
 };
 
 std::vector<float> gen_gauss(int scale_size);
 std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c);
 
-int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan);
-int nlsar_param_stats(const NlsarProblem& pb, const NlsarPlan& plan,
+int nlsar_common(const NlsarParams& pp, NlsarCommon* cm);
+// edge_h / edge_w: virtual distance of the last row / column (< 0: the overlap)
+int nlsar_image_plan(const NlsarCommon& cm, int H, int W, int edge_h, int edge_w,
+                     NlsarImagePlan* ip);
+int nlsar_param_stats(const NlsarParams& pp, const NlsarCommon& cm,
                       std::vector<const NlsarStatsView*>& pst);
-// host image of the device constants of a call (weight LUTs and Gaussian taps)
-int nlsar_build_constants(const NlsarProblem& pb, const NlsarPlan& plan, std::vector<float>& blob);
-// d_const: device copy of nlsar_build_constants' blob, which must stay unchanged until the
-// enqueued work has run
-int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, const float* d_a2,
-              const float* d_phi, const float* d_cov, float* d_ref, float* d_phase, float* d_coh,
-              void* d_ws, const float* d_const, float* d_enl_maps, float* d_alpha_maps,
-              unsigned long long* ub_host, hipStream_t stream);
+// host image of the device constants (weight LUTs and Gaussian taps)
+int nlsar_build_constants(const NlsarParams& pp, const NlsarCommon& cm, std::vector<float>& blob);
+// descriptor of one image whose workspace starts at ws_image
+void nlsar_image_desc(const NlsarImagePlan& ip, char* ws_image, const float* a1, const float* a2,
+                      const float* phi, const float* cov, float* ref, float* phase, float* coh,
+                      float* enl_maps, float* alpha_maps, NlsarImageDesc* d);
+// numbers the rows and pass blocks of a batch; returns the totals
+void nlsar_batch_layout(NlsarImageDesc* d, int n, int* rows_c, int* rows_a, int* blocks,
+                        int* max_cw, int* max_aw);
+// Enqueues the filter on `stream`.  d_table: device copy of descs[0..n) (after
+// nlsar_batch_layout), or null for a single image (n == 1), which is passed by value.
+// d_const: device copy of nlsar_build_constants' blob, unchanged until the work has run.
+int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImageDesc* descs,
+                  int n, const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream);
 
 }  // namespace dkx

@@ -28,11 +28,10 @@ constexpr int NLSAR_TY = 32;    // tile rows
 constexpr int NLSAR_BAND = 8;   // tile rows per band of the block order
 
 struct ScaleArgs {
-    const float4* A4;   // averaged covariance per grid element: a00, a11, Re a01, Im a01
-    const float* LD;    // log det of the averaged covariance, same grid
-    const float4* C4;   // covariance of the input pixel: a1^2, a2^2, Re C01, Im C01
-    int apitch, aoy, aox;
-    int cpitch, coy, cox;
+    const NlsarImageDesc* table;         // batch: descriptors in device memory, else null
+    NlsarImageDesc one;                  // the image of a single call (table == null)
+    int nimg;
+    int total_blocks;                    // blocks of all images
     int wsh;                             // search-window half width
     int pidx[NLSAR_MAX_NP];              // index of (p, s) in the per-parameter maps
     const float* lutw[NLSAR_MAX_NP];     // weight per quantile index
@@ -42,18 +41,7 @@ struct ScaleArgs {
     float scl[NLSAR_MAX_NP];             // lut_size - 1
     int lut_size[NLSAR_MAX_NP];
     int lut_stride;                      // LDS floats per LUT
-    Launch L[4];                         // interior, last row, last column, corner
-    int gx[4], gy[4];                    // block columns / rows of each mapping
-    int bstart[5];                       // first linear block of each mapping, [4] = total
-    int H, W;
-    float* enl_nb;                       // [nparam][H*W] or null
-    float* alpha;                        // [nparam][H*W] or null
-    unsigned char* ubflag;               // [H*W]
     int first_scale;                     // 1: first pass (ubflag not yet written)
-    float* ref;                          // last pass: outputs
-    float* phase;
-    float* coh;
-    unsigned long long* ub_count;
 };
 
 // LDS layout (floats) shared by the kernel and its launcher, fixed at compile time for
@@ -105,30 +93,77 @@ __device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, flo
     return s > dmax ? 0.f : w;
 }
 
-// Block -> tile.  The four launch mappings are laid end to end in one grid.  Linear block
-// ids are remapped so that the blocks the hardware sends to one XCD (id mod 8) take a
-// contiguous run of tiles, and tiles are ordered column by column within bands of
-// NLSAR_BAND tile rows, so the blocks resident on one XCD cover a compact 2-D patch whose
-// halos overlap in that XCD's L2.
-__device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, int& m, int& ty0,
+// Values every lane of a wave holds alike, moved to scalar registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)u);
+    const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(u >> 32));
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ const NlsarImageDesc& nlsar_image(const ScaleArgs& a, int img)
+{
+    return a.table ? a.table[img] : a.one;
+}
+
+// What the offset loop of one block needs to know about its image.
+struct PassImage {
+    const DKX_GLOBAL char* A4b;   // A grid (float4 per element)
+    const DKX_GLOBAL char* LDb;   // log det plane of the A grid
+    const DKX_GLOBAL char* C4b;   // C grid (float4 per element)
+    int apitch, aorg;     // aorg: element offset of image position (0, 0) in the A grid
+    int cpitch, corg;
+    int img, m;           // image index, launch mapping of the tile
+};
+
+// Block -> (image, tile).  The images' blocks are laid end to end, each image's four launch
+// mappings end to end within it.  Linear block ids are remapped so that the blocks the
+// hardware sends to one XCD (id mod 8) take a contiguous run of tiles, and tiles are ordered
+// column by column within bands of NLSAR_BAND tile rows, so the blocks resident on one XCD
+// cover a compact 2-D patch whose halos overlap in that XCD's L2.
+__device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, PassImage& v, int& ty0,
                                                  int& tx0)
 {
-    const int nb = a.bstart[4];
+    const int nb = a.total_blocks;
     const int b = blockIdx.x;
     const int q = nb >> 3, r = nb & 7;
     const int xcd = b & 7, k = b >> 3;
     const int t = xcd * q + min(xcd, r) + k;
-    m = 0;
-    while (m < 3 && t >= a.bstart[m + 1]) m++;
-    const int local = t - a.bstart[m];
-    const int gx = a.gx[m], gy = a.gy[m];
+    int img = 0;
+    if (a.table) {
+        int lo = 0, hi = a.nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (uni(a.table[mid].block0) <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        img = lo;
+    }
+    const NlsarImageDesc& d = nlsar_image(a, img);
+    const int tl = t - uni(d.block0);
+    int m = 0;
+    while (m < 3 && tl >= uni(d.bstart[m + 1])) m++;
+    const int local = tl - uni(d.bstart[m]);
+    const int gx = uni(d.gx[m]), gy = uni(d.gy[m]);
     const int band = local / (NLSAR_BAND * gx);
     const int within = local - band * NLSAR_BAND * gx;
     const int bh = min(NLSAR_BAND, gy - band * NLSAR_BAND);
     const int bx = within / bh;
     const int by = band * NLSAR_BAND + (within - bx * bh);
-    ty0 = a.L[m].vy0 + by * NLSAR_TY;
-    tx0 = a.L[m].vx0 + bx * TX;
+    ty0 = uni(d.L[m].vy0) + by * NLSAR_TY;
+    tx0 = uni(d.L[m].vx0) + bx * TX;
+    v.img = img;
+    v.m = m;
+    v.A4b = gptr(uni(reinterpret_cast<const char*>(d.A)));
+    v.LDb = v.A4b + 16ull * (unsigned long long)uni(d.aplane);
+    v.C4b = gptr(uni(reinterpret_cast<const char*>(d.C)));
+    v.apitch = uni(d.apitch);
+    v.aorg = uni(d.aoy) * v.apitch + uni(d.aox);
+    v.cpitch = uni(d.cpitch);
+    v.corg = uni(d.coy) * v.cpitch + uni(d.cox);
 }
 
 // Occupancy target: blocks per CU (three fit the LDS budget of the small window class; two
@@ -158,9 +193,9 @@ void nlsar_scale_pass(const ScaleArgs a)
 
     const int tid = threadIdx.x;
     const int px = tid % TX, g = tid / TX;  // column, group of RPT rows
-    int m_, ty0, tx0;
-    nlsar_block_tile(a, TX, m_, ty0, tx0);
-    const Launch L = m_ == 0 ? a.L[0] : m_ == 1 ? a.L[1] : m_ == 2 ? a.L[2] : a.L[3];
+    PassImage im;
+    int ty0, tx0;
+    nlsar_block_tile(a, TX, im, ty0, tx0);
 
 #pragma unroll
     for (int k = 0; k < NP; k++)
@@ -178,10 +213,10 @@ void nlsar_scale_pass(const ScaleArgs a)
     }
 
     const float K = nlsar_log2_term();
-    const int apitch = a.apitch, cpitch = a.cpitch;
-    const char* A4b = reinterpret_cast<const char*>(a.A4);
-    const char* LDb = reinterpret_cast<const char*>(a.LD);
-    const char* C4b = reinterpret_cast<const char*>(a.C4);
+    const int apitch = im.apitch, cpitch = im.cpitch;
+    const DKX_GLOBAL char* A4b = im.A4b;
+    const DKX_GLOBAL char* LDb = im.LDb;
+    const DKX_GLOBAL char* C4b = im.C4b;
 
     // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
     // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
@@ -202,7 +237,7 @@ void nlsar_scale_pass(const ScaleArgs a)
                 const int n = DY * DX;
                 const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
                 int r = tid / DX, c = tid - r * DX;
-                const unsigned int gb0 = (unsigned int)((ty0 - P + a.aoy) * apitch + (tx0 + rxo - P + a.aox));
+                const unsigned int gb0 = (unsigned int)(im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P));
                 unsigned int g1 = gb0 + (unsigned int)(r * apitch + c);      // element index
                 const unsigned int dofs = (unsigned int)(dy * apitch + dx);
                 const unsigned int gstep = (unsigned int)(sr_ * apitch + sc_);
@@ -211,10 +246,10 @@ void nlsar_scale_pass(const ScaleArgs a)
                 const int sstep = sr_ * DXM + sc_;
                 for (int i = tid; i < n; i += NT) {
                     const unsigned int g2 = g1 + dofs;
-                    const float4 u = *reinterpret_cast<const float4*>(A4b + 16u * g1);
-                    const float4 v = *reinterpret_cast<const float4*>(A4b + 16u * g2);
-                    const float l1 = *reinterpret_cast<const float*>(LDb + 4u * g1);
-                    const float l2 = *reinterpret_cast<const float*>(LDb + 4u * g2);
+                    const float4 u = gload4(A4b + 16u * g1);
+                    const float4 v = gload4(A4b + 16u * g2);
+                    const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g1);
+                    const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g2);
                     const float s00 = u.x + v.x;
                     const float s01r = u.z + v.z;
                     const float s01i = u.w + v.w;
@@ -330,8 +365,8 @@ void nlsar_scale_pass(const ScaleArgs a)
             {
                 const int py0 = g * RPT;
                 const int cy = ty0 + py0, cx = tx0 + px;
-                const unsigned int cu0 = (unsigned int)((cy + dy + a.coy) * cpitch + (cx + dx + a.cox));
-                const unsigned int cl0 = (unsigned int)((cy - dy + a.coy) * cpitch + (cx - dx + a.cox));
+                const unsigned int cu0 = (unsigned int)(im.corg + (cy + dy) * cpitch + (cx + dx));
+                const unsigned int cl0 = (unsigned int)(im.corg + (cy - dy) * cpitch + (cx - dx));
                 const float* wu0 = sW + py0 * RXM + (px - rxo);
                 const float* wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
 #pragma unroll
@@ -340,13 +375,13 @@ void nlsar_scale_pass(const ScaleArgs a)
                     const unsigned int cl = cl0 + (unsigned int)(q * cpitch);
                     float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
                     if (LAST) {
-                        const float4 u = *reinterpret_cast<const float4*>(C4b + 16u * cu);
-                        const float4 l = *reinterpret_cast<const float4*>(C4b + 16u * cl);
+                        const float4 u = gload4(C4b + 16u * cu);
+                        const float4 l = gload4(C4b + 16u * cl);
                         i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
                         i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
                     } else {
-                        const float2 u = *reinterpret_cast<const float2*>(C4b + 16u * cu);
-                        const float2 l = *reinterpret_cast<const float2*>(C4b + 16u * cl);
+                        const float2 u = gload2(C4b + 16u * cu);
+                        const float2 l = gload2(C4b + 16u * cl);
                         i0u = u.x; i1u = u.y;
                         i0l = l.x; i1l = l.y;
                     }
@@ -383,16 +418,27 @@ void nlsar_scale_pass(const ScaleArgs a)
     }
 
     // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
+    const NlsarImageDesc& dsc = nlsar_image(a, im.img);
+    const int mm = im.m;
+    const Launch L{uni(dsc.L[mm].vy0), uni(dsc.L[mm].vx0), uni(dsc.L[mm].ny), uni(dsc.L[mm].nx),
+                   uni(dsc.L[mm].oy0), uni(dsc.L[mm].ox0)};
+    const int imH = uni(dsc.H), imW = uni(dsc.W);
+    DKX_GLOBAL float* const o_enl = gptr(uni(dsc.enl_nb));
+    DKX_GLOBAL float* const o_alpha = gptr(uni(dsc.alpha));
+    DKX_GLOBAL unsigned char* const o_flag = gptr(uni(dsc.ubflag));
+    DKX_GLOBAL float* const o_ref = gptr(uni(dsc.ref));
+    DKX_GLOBAL float* const o_phase = gptr(uni(dsc.phase));
+    DKX_GLOBAL float* const o_coh = gptr(uni(dsc.coh));
+    unsigned long long* const o_ub = uni(dsc.ub_count);
     unsigned int ubl = 0;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int py = g * RPT + q;
         const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
         const bool valid = iy < L.ny && ix < L.nx;
-        const float4 cc = *reinterpret_cast<const float4*>(
-            C4b + 16u * (unsigned int)((ty0 + py + a.coy) * cpitch + (tx0 + px + a.cox)));
+        const float4 cc = gload4(C4b + 16u * (unsigned int)(im.corg + (ty0 + py) * cpitch + (tx0 + px)));
         const float i0c = cc.x, i1c = cc.y;
-        const long long opix = valid ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
+        const long long opix = valid ? (long long)(L.oy0 + iy) * imW + (L.ox0 + ix) : 0;
         bool fail = false;
         float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
 #pragma unroll
@@ -425,8 +471,8 @@ void nlsar_scale_pass(const ScaleArgs a)
             const float enb = enl / (oma * oma + (alpha * alpha + 2.0f * alpha * (1.0f - alpha) / ws) * enl);
             if (!(enb > -1.0f)) fail = true;
             if (valid) {
-                if (a.enl_nb) a.enl_nb[(long long)a.pidx[k] * a.H * a.W + opix] = enb;
-                if (a.alpha) a.alpha[(long long)a.pidx[k] * a.H * a.W + opix] = alpha;
+                if (o_enl) o_enl[(long long)a.pidx[k] * imH * imW + opix] = enb;
+                if (o_alpha) o_alpha[(long long)a.pidx[k] * imH * imW + opix] = alpha;
             }
             if (LAST && k == NP - 1) {
                 alpha_mean = alpha;
@@ -440,13 +486,13 @@ void nlsar_scale_pass(const ScaleArgs a)
         }
         if (!valid) continue;
         if (!LAST) {
-            if (a.first_scale) a.ubflag[opix] = fail ? 1 : 0;
-            else if (fail) a.ubflag[opix] = 1;
+            if (a.first_scale) o_flag[opix] = fail ? 1 : 0;
+            else if (fail) o_flag[opix] = 1;
         } else {
             // best-parameter selection (best_params.cpp:27-53) keeps the last key of the
             // (patch, scale) map whenever every ENL exceeds -1; pixels where the reference's
             // selection loop is undefined are counted and also take the last key.
-            if (fail || (!a.first_scale && a.ubflag[opix])) ubl++;
+            if (fail || (!a.first_scale && o_flag[opix])) ubl++;
             // weighted_means.cl:62-80 and covmat_decompose.cl:19-39
             const float al = alpha_mean;
             const float m00 = (1 - al) * m0_mean / s_mean + al * i0c;
@@ -455,12 +501,12 @@ void nlsar_scale_pass(const ScaleArgs a)
             const float mi = (1 - al) * macc[q][1] / s_mean + al * cc.w;
             const float am = sqrtf(m00), as = sqrtf(m11);
             const float t = 0.5f * (am + as);
-            a.ref[opix] = t * t;
-            a.phase[opix] = atan2f(mi, mr);
-            a.coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
+            o_ref[opix] = t * t;
+            o_phase[opix] = atan2f(mi, mr);
+            o_coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
         }
     }
-    if (LAST && ubl) atomicAdd(a.ub_count, (unsigned long long)ubl);
+    if (LAST && ubl) atomicAdd(o_ub, (unsigned long long)ubl);
 }
 
 // Launch of one pass; defined in the instantiating translation units.
@@ -475,7 +521,7 @@ int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
     X(42) X(50) X(28) X(44) X(52) X(56)
 
 template <bool LAST, int TX, int RPT, int WMAX, int PMASK>
-int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
+int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
 {
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
@@ -487,18 +533,9 @@ int nlsar_pass_launch_mask(ScaleArgs a, hipStream_t stream)
     }
     DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int total = 0;
-    for (int i = 0; i < 4; i++) {
-        a.bstart[i] = total;
-        if (a.L[i].ny <= 0 || a.L[i].nx <= 0) {
-            a.gx[i] = a.gy[i] = 1;
-            continue;
-        }
-        a.gx[i] = (a.L[i].nx + TX - 1) / TX;
-        a.gy[i] = (a.L[i].ny + NLSAR_TY - 1) / NLSAR_TY;
-        total += a.gx[i] * a.gy[i];
-    }
-    a.bstart[4] = total;
+    static_assert(TX == NLSAR_TILE && NLSAR_TY == NLSAR_TILE, "block counts assume 32 x 32 tiles");
+    const int total = a.total_blocks;
+    if (total <= 0) return 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = timing_on();
     if (timed) {

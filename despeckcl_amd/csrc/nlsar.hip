@@ -16,38 +16,67 @@
 namespace dkx {
 
 // ------------------------------------------------------------------------------------
-// preparation kernels
+// preparation kernels (one thread per grid element; blocks cover the rows of every image)
 // ------------------------------------------------------------------------------------
 
-// Covariance of the InSAR input (covmat_create.cl:19-56) on the padded grid C, one float4
-// per element: a1^2, a2^2, Re C01 = a1*a2*cos(phi), Im C01 = a1*a2*sin(phi).
-__global__ void nlsar_prep_insar(const float* __restrict__ a1, const float* __restrict__ a2,
-                                 const float* __restrict__ phi, int H, int W, Grid C)
+struct PrepArgs {
+    const NlsarImageDesc* table;   // batch: descriptors in device memory, else null
+    NlsarImageDesc one;            // the image of a single call
+    int nimg;
+    int cblocks;                   // blocks per grid row
+    const float* gauss;            // scale average: s x s taps
+    int s;                         // scale size
+    float gamma;                   // off-diagonal rescaling factor
+};
+
+// image owning batch row `row` of the C grids (A: of the A grids)
+__device__ __forceinline__ const NlsarImageDesc& prep_image(const PrepArgs& a, int row, bool A)
 {
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gy = blockIdx.y;
-    if (gx >= C.w || gy >= C.h) return;
-    const int y = min(H - 1, max(0, gy - C.oy));
-    const int x = min(W - 1, max(0, gx - C.ox));
-    const long long i = (long long)y * W + x;
-    const float m = a1[i], s = a2[i], p = phi[i];
-    const long long o = (long long)gy * C.pitch + gx;
-    reinterpret_cast<float4*>(C.base)[o] = make_float4(m * m, s * s, m * s * cosf(p), m * s * sinf(p));
+    if (!a.table) return a.one;
+    int lo = 0, hi = a.nimg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((A ? a.table[mid].row0_a : a.table[mid].row0_c) <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return a.table[lo];
 }
 
-// Covariance of a dim-2 covariance input (8 interlaced planes, nlsar.i:146-153): the
-// diagonal real parts and the (0,1) element, laid out as for the InSAR input.
-__global__ void nlsar_prep_covmat2(const float* __restrict__ cov, int H, int W, Grid C)
+__global__ void nlsar_zero_counts(const PrepArgs a)
 {
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gy = blockIdx.y;
-    if (gx >= C.w || gy >= C.h) return;
-    const int y = min(H - 1, max(0, gy - C.oy));
-    const int x = min(W - 1, max(0, gx - C.ox));
-    const long long n = (long long)H * W;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (!a.table) {
+        if (i == 0) *a.one.ub_count = 0ull;
+    } else if (i < a.nimg) {
+        *a.table[i].ub_count = 0ull;
+    }
+}
+
+// Covariance of the input pixel on the padded grid C, one float4 per element:
+// InSAR input (covmat_create.cl:19-56): a1^2, a2^2, a1*a2*cos(phi), a1*a2*sin(phi);
+// dim-2 covariance input (8 interlaced planes, nlsar.i:146-153): the diagonal real parts and
+// the (0,1) element.
+__global__ void nlsar_prep(const PrepArgs a)
+{
+    const int row = blockIdx.x / a.cblocks;
+    const int gx = (blockIdx.x - row * a.cblocks) * blockDim.x + threadIdx.x;
+    const NlsarImageDesc& d = prep_image(a, row, false);
+    const int gy = row - d.row0_c;
+    if (gx >= d.cw || gy >= d.ch) return;
+    const int H = d.H, W = d.W;
+    const int y = min(H - 1, max(0, gy - d.coy));
+    const int x = min(W - 1, max(0, gx - d.cox));
     const long long i = (long long)y * W + x;
-    const long long o = (long long)gy * C.pitch + gx;
-    reinterpret_cast<float4*>(C.base)[o] = make_float4(cov[i], cov[6 * n + i], cov[2 * n + i], cov[3 * n + i]);
+    float4 v;
+    if (d.cov) {
+        const DKX_GLOBAL float* cov = gptr(d.cov);
+        const long long n = (long long)H * W;
+        v = make_float4(cov[i], cov[6 * n + i], cov[2 * n + i], cov[3 * n + i]);
+    } else {
+        const float m = gptr(d.a1)[i], sl = gptr(d.a2)[i], p = gptr(d.phi)[i];
+        v = make_float4(m * m, sl * sl, m * sl * cosf(p), m * sl * sinf(p));
+    }
+    gstore4(gptr(d.C) + 4 * ((long long)gy * d.cpitch + gx), v);
 }
 
 // Gaussian s x s average of the rescaled covariance (covmat_rescale.cl:19-41 then
@@ -55,40 +84,37 @@ __global__ void nlsar_prep_covmat2(const float* __restrict__ cov, int H, int W, 
 // plus the log-determinant of each averaged matrix, which every pixel dissimilarity
 // involving that pixel uses (sim_measures.h:51-52).
 // A holds one float4 per element (a00, a11, Re a01, Im a01) followed by the log det plane.
-__global__ void nlsar_scale_average(Grid C, Grid A, const float* __restrict__ gauss, int s,
-                                    float gamma)
+__global__ void nlsar_scale_average(const PrepArgs a)
 {
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gy = blockIdx.y;
-    if (gx >= A.w || gy >= A.h) return;
-    const int y = gy - A.oy, x = gx - A.ox;  // image coordinates
+    const int row = blockIdx.x / a.cblocks;
+    const int gx = (blockIdx.x - row * a.cblocks) * blockDim.x + threadIdx.x;
+    const NlsarImageDesc& d = prep_image(a, row, true);
+    const int gy = row - d.row0_a;
+    if (gx >= d.aw || gy >= d.ah) return;
+    const int y = gy - d.aoy, x = gx - d.aox;  // image coordinates
+    const int s = a.s;
     const int ssh = (s - 1) / 2;
-    const float4* C4 = reinterpret_cast<const float4*>(C.base);
+    const DKX_GLOBAL float* C4 = gptr(d.C);
+    const DKX_GLOBAL float* gauss = gptr(a.gauss);
     float s00 = 0.f, s11 = 0.f, sr = 0.f, si = 0.f;
     for (int kx = 0; kx < s; kx++) {
-        const long long row = C.at(y - ssh + kx, x - ssh);
+        const long long rowc = (long long)(y - ssh + kx + d.coy) * d.cpitch + (x - ssh + d.cox);
         for (int ky = 0; ky < s; ky++) {
             const float g = gauss[kx * s + ky];
-            const float4 c = C4[row + ky];
+            const float4 c = gload4(C4 + 4 * (rowc + ky));
             s00 = fmaf(g, c.x, s00);
             s11 = fmaf(g, c.y, s11);
-            const float r = c.z * gamma;
-            const float im = c.w * gamma;
+            const float r = c.z * a.gamma;
+            const float im = c.w * a.gamma;
             sr = fmaf(g, r, sr);
             si = fmaf(g, im, si);
         }
     }
-    const long long o = (long long)gy * A.pitch + gx;
-    reinterpret_cast<float4*>(A.base)[o] = make_float4(s00, s11, sr, si);
+    const long long o = (long long)gy * d.apitch + gx;
+    DKX_GLOBAL float* A = gptr(d.A);
+    gstore4(A + 4 * o, make_float4(s00, s11, sr, si));
     const float det = (s00 * s11) - (sr * sr + si * si);
-    A.base[4 * A.plane + o] = logf(det);
-}
-
-// Clears the undefined-selection counter (a kernel, not a memset node, so that the call
-// can be captured in a HIP graph).
-__global__ void nlsar_zero_count(unsigned long long* c)
-{
-    if (threadIdx.x == 0) *c = 0ull;
+    A[4ll * d.aplane + o] = logf(det);
 }
 
 // ------------------------------------------------------------------------------------
@@ -125,85 +151,86 @@ std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c)
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
-// tile shapes (rows x columns); tiles 0 / 1 / 2 give each thread 4 / 2 / 1 rows
-static const int kTileTY[3] = {32, 32, 32};
-static const int kTileTX[3] = {32, 32, 32};
 
-int nlsar_plan(const NlsarProblem& pb, NlsarPlan* plan)
+int nlsar_common(const NlsarParams& pp, NlsarCommon* cm)
 {
-    if (pb.H <= 0 || pb.W <= 0) { set_error("nlsar: empty image"); return -1; }
-    if (pb.sw < 3 || pb.sw % 2 == 0) { set_error("nlsar: search window size must be odd and >= 3"); return -1; }
-    if (pb.sw > 2 * NLSAR_WMAX_LARGE + 1) { set_error("nlsar: search windows above 41 are not supported"); return -4; }
-    if (pb.nP < 1 || pb.nP > NLSAR_MAX_PATCHES || pb.nS < 1 || pb.nS > NLSAR_MAX_NS) {
+    if (pp.sw < 3 || pp.sw % 2 == 0) { set_error("nlsar: search window size must be odd and >= 3"); return -1; }
+    if (pp.sw > 2 * NLSAR_WMAX_LARGE + 1) { set_error("nlsar: search windows above 41 are not supported"); return -4; }
+    if (pp.nP < 1 || pp.nP > NLSAR_MAX_PATCHES || pp.nS < 1 || pp.nS > NLSAR_MAX_NS) {
         set_error("nlsar: 1-16 patch sizes and 1-8 scale sizes are supported");
         return -1;
     }
     int pmax = 0, smax = 0;
-    for (int i = 0; i < pb.nP; i++) {
-        if (pb.P[i] < 1 || pb.P[i] % 2 == 0) { set_error("nlsar: patch sizes must be odd"); return -1; }
-        if (pb.P[i] > 11) { set_error("nlsar: patch sizes above 11 are not supported"); return -4; }
-        pmax = std::max(pmax, pb.P[i]);
+    for (int i = 0; i < pp.nP; i++) {
+        if (pp.P[i] < 1 || pp.P[i] % 2 == 0) { set_error("nlsar: patch sizes must be odd"); return -1; }
+        if (pp.P[i] > 11) { set_error("nlsar: patch sizes above 11 are not supported"); return -4; }
+        pmax = std::max(pmax, pp.P[i]);
     }
-    for (int i = 0; i < pb.nS; i++) {
-        if (pb.S[i] < 1 || pb.S[i] % 2 == 0 || pb.S[i] > 31) { set_error("nlsar: scale sizes must be odd and <= 31"); return -1; }
-        smax = std::max(smax, pb.S[i]);
+    for (int i = 0; i < pp.nS; i++) {
+        if (pp.S[i] < 1 || pp.S[i] % 2 == 0 || pp.S[i] > 31) { set_error("nlsar: scale sizes must be odd and <= 31"); return -1; }
+        smax = std::max(smax, pp.S[i]);
     }
-    for (int i = 0; i < pb.nP; i++)
-        for (int j = i + 1; j < pb.nP; j++)
-            if (pb.P[i] == pb.P[j]) { set_error("nlsar: duplicate patch size"); return -1; }
-    for (int i = 0; i < pb.nS; i++)
-        for (int j = i + 1; j < pb.nS; j++)
-            if (pb.S[i] == pb.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
-    plan->tile = std::min(std::max(pb.tile, 0), 2);
-    plan->TY = kTileTY[plan->tile];
-    plan->TX = kTileTX[plan->tile];
-    plan->wsh = (pb.sw - 1) / 2;
-    plan->P = (pmax - 1) / 2;
-    plan->smax = smax;
-    plan->overlap = plan->P + plan->wsh + (smax - 1) / 2;
-    plan->eh = pb.edge_h < 0 ? plan->overlap : pb.edge_h;
-    plan->ew = pb.edge_w < 0 ? plan->overlap : pb.edge_w;
-    const int H = pb.H, W = pb.W;
+    for (int i = 0; i < pp.nP; i++)
+        for (int j = i + 1; j < pp.nP; j++)
+            if (pp.P[i] == pp.P[j]) { set_error("nlsar: duplicate patch size"); return -1; }
+    for (int i = 0; i < pp.nS; i++)
+        for (int j = i + 1; j < pp.nS; j++)
+            if (pp.S[i] == pp.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
+    cm->tile = std::min(std::max(pp.tile, 0), NLSAR_NTILES - 1);
+    cm->wsh = (pp.sw - 1) / 2;
+    cm->P = (pmax - 1) / 2;
+    cm->smax = smax;
+    cm->overlap = cm->P + cm->wsh + (smax - 1) / 2;
+    int maxlut = 0;
+    for (int i = 0; i < pp.nst; i++) maxlut = std::max(maxlut, pp.stats[i].lut_size);
+    cm->lut_stride = maxlut;
+    cm->nparam = pp.nP * pp.nS;
+    cm->const_floats = (size_t)cm->nparam * maxlut + (size_t)NLSAR_MAX_NS * 31 * 31;
+    return 0;
+}
+
+int nlsar_image_plan(const NlsarCommon& cm, int H, int W, int edge_h, int edge_w, NlsarImagePlan* ip)
+{
+    if (H <= 0 || W <= 0) { set_error("nlsar: empty image"); return -1; }
+    ip->H = H;
+    ip->W = W;
+    ip->eh = edge_h < 0 ? cm.overlap : edge_h;
+    ip->ew = edge_w < 0 ? cm.overlap : edge_w;
+    const int T = NLSAR_TILE;
     // virtual extents covered by the four launch mappings
-    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->TY), H - 1 + plan->eh + plan->TY);
-    const int Wx = std::max(round_up(std::max(W - 1, 1), plan->TX), W - 1 + plan->ew + plan->TX);
-    const int ra = plan->wsh + plan->P;          // A grid padding
-    const int rc = ra + (smax - 1) / 2;          // C grid padding
-    plan->C = Grid{nullptr, 0, round_up(Wx + 2 * rc, 64), rc, rc, Hx + 2 * rc, Wx + 2 * rc};
-    plan->C.plane = (long long)plan->C.pitch * plan->C.h;
-    plan->A = Grid{nullptr, 0, round_up(Wx + 2 * ra, 64), ra, ra, Hx + 2 * ra, Wx + 2 * ra};
-    plan->A.plane = (long long)plan->A.pitch * plan->A.h;
-    // kernels index grid elements with 32-bit offsets
-    if (plan->C.plane >= INT_MAX / 2 || plan->A.plane >= INT_MAX / 2) {
+    const int Hx = std::max(round_up(std::max(H - 1, 1), T), H - 1 + ip->eh + T);
+    const int Wx = std::max(round_up(std::max(W - 1, 1), T), W - 1 + ip->ew + T);
+    const int ra = cm.wsh + cm.P;               // A grid padding
+    const int rc = ra + (cm.smax - 1) / 2;      // C grid padding
+    ip->C = Grid{nullptr, 0, round_up(Wx + 2 * rc, 64), rc, rc, Hx + 2 * rc, Wx + 2 * rc};
+    ip->C.plane = (long long)ip->C.pitch * ip->C.h;
+    ip->A = Grid{nullptr, 0, round_up(Wx + 2 * ra, 64), ra, ra, Hx + 2 * ra, Wx + 2 * ra};
+    ip->A.plane = (long long)ip->A.pitch * ip->A.h;
+    // kernels address grids with 32-bit byte offsets (16 bytes per element)
+    if (ip->C.plane >= (1ll << 28) || ip->A.plane >= (1ll << 28)) {
         set_error("nlsar: image too large for one call (split it into bands)");
         return -4;
     }
-    int maxlut = 0;
-    for (int i = 0; i < pb.nst; i++) maxlut = std::max(maxlut, pb.stats[i].lut_size);
-    plan->lut_stride = maxlut;
-    plan->nparam = pb.nP * pb.nS;
-    // workspace layout (bytes, 256-aligned)
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
-    plan->off_C = take(sizeof(float) * 4 * plan->C.plane);
-    plan->off_A = take(sizeof(float) * 5 * plan->A.plane);
-    plan->const_floats = (size_t)plan->nparam * maxlut + (size_t)NLSAR_MAX_NS * 31 * 31;
-    plan->off_flag = take((size_t)H * W);
-    plan->off_ub = take(sizeof(unsigned long long));
-    plan->ws_bytes = off;
+    ip->off_C = take(sizeof(float) * 4 * ip->C.plane);
+    ip->off_A = take(sizeof(float) * 5 * ip->A.plane);
+    ip->off_flag = take((size_t)H * W);
+    ip->off_ub = take(sizeof(unsigned long long));
+    ip->bytes = off;
     return 0;
 }
 
 // statistics entry of every parameter index (scale-major loop order)
-int nlsar_param_stats(const NlsarProblem& pb, const NlsarPlan& plan,
+int nlsar_param_stats(const NlsarParams& pp, const NlsarCommon& cm,
                       std::vector<const NlsarStatsView*>& pst)
 {
-    pst.assign(plan.nparam, nullptr);
-    for (int si = 0; si < pb.nS; si++)
-        for (int pi = 0; pi < pb.nP; pi++) {
-            const int idx = si * pb.nP + pi;
-            for (int k = 0; k < pb.nst; k++)
-                if (pb.stats[k].patch_size == pb.P[pi] && pb.stats[k].scale_size == pb.S[si]) pst[idx] = &pb.stats[k];
+    pst.assign(cm.nparam, nullptr);
+    for (int si = 0; si < pp.nS; si++)
+        for (int pi = 0; pi < pp.nP; pi++) {
+            const int idx = si * pp.nP + pi;
+            for (int k = 0; k < pp.nst; k++)
+                if (pp.stats[k].patch_size == pp.P[pi] && pp.stats[k].scale_size == pp.S[si]) pst[idx] = &pp.stats[k];
             if (!pst[idx]) { set_error("nlsar: statistics missing for a (patch, scale) pair"); return -3; }
         }
     return 0;
@@ -211,71 +238,130 @@ int nlsar_param_stats(const NlsarProblem& pb, const NlsarPlan& plan,
 
 // Host image of the call's device constants: the weight LUT of every parameter index,
 // then the Gaussian taps of every scale (31*31 floats each).
-int nlsar_build_constants(const NlsarProblem& pb, const NlsarPlan& plan, std::vector<float>& blob)
+int nlsar_build_constants(const NlsarParams& pp, const NlsarCommon& cm, std::vector<float>& blob)
 {
     std::vector<const NlsarStatsView*> pst;
-    if (nlsar_param_stats(pb, plan, pst)) return -3;
-    blob.assign(plan.const_floats, 0.f);
-    for (int idx = 0; idx < plan.nparam; idx++) {
-        std::vector<float> w = weight_lut(*pst[idx], pb.h, pb.c);
-        std::copy(w.begin(), w.end(), blob.begin() + (size_t)idx * plan.lut_stride);
+    if (nlsar_param_stats(pp, cm, pst)) return -3;
+    blob.assign(cm.const_floats, 0.f);
+    for (int idx = 0; idx < cm.nparam; idx++) {
+        std::vector<float> w = weight_lut(*pst[idx], pp.h, pp.c);
+        std::copy(w.begin(), w.end(), blob.begin() + (size_t)idx * cm.lut_stride);
     }
-    const size_t goff = (size_t)plan.nparam * plan.lut_stride;
-    for (int si = 0; si < pb.nS; si++) {
-        std::vector<float> g = gen_gauss(pb.S[si]);
+    const size_t goff = (size_t)cm.nparam * cm.lut_stride;
+    for (int si = 0; si < pp.nS; si++) {
+        std::vector<float> g = gen_gauss(pp.S[si]);
         std::copy(g.begin(), g.end(), blob.begin() + goff + (size_t)si * 31 * 31);
     }
     return 0;
 }
 
-int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, const float* d_a2,
-              const float* d_phi, const float* d_cov, float* d_ref, float* d_phase, float* d_coh,
-              void* d_ws, const float* d_const, float* d_enl_maps, float* d_alpha_maps,
-              unsigned long long* ub_host, hipStream_t stream)
+void nlsar_image_desc(const NlsarImagePlan& ip, char* ws, const float* a1, const float* a2,
+                      const float* phi, const float* cov, float* ref, float* phase, float* coh,
+                      float* enl_maps, float* alpha_maps, NlsarImageDesc* d)
 {
-    char* ws = (char*)d_ws;
-    Grid C = plan.C, A = plan.A;
-    C.base = (float*)(ws + plan.off_C);
-    A.base = (float*)(ws + plan.off_A);
-    const float* d_lut = d_const;
-    const float* d_gauss = d_const + (size_t)plan.nparam * plan.lut_stride;
-    unsigned char* d_flag = (unsigned char*)(ws + plan.off_flag);
-    unsigned long long* d_ub = (unsigned long long*)(ws + plan.off_ub);
-    const int H = pb.H, W = pb.W;
-    std::vector<const NlsarStatsView*> pst;
-    if (nlsar_param_stats(pb, plan, pst)) return -3;
-    hipLaunchKernelGGL(nlsar_zero_count, dim3(1), dim3(64), 0, stream, d_ub);
-    DKX_HIP_CHECK(hipGetLastError());
-
-    {
-        dim3 blk(256), grd((C.w + 255) / 256, C.h);
-        if (pb.dim == 0)
-            hipLaunchKernelGGL(nlsar_prep_insar, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, C);
-        else
-            hipLaunchKernelGGL(nlsar_prep_covmat2, grd, blk, 0, stream, d_cov, H, W, C);
-        DKX_HIP_CHECK(hipGetLastError());
+    memset(d, 0, sizeof(*d));
+    d->a1 = a1;
+    d->a2 = a2;
+    d->phi = phi;
+    d->cov = cov;
+    d->ref = ref;
+    d->phase = phase;
+    d->coh = coh;
+    d->enl_nb = enl_maps;
+    d->alpha = alpha_maps;
+    d->C = (float*)(ws + ip.off_C);
+    d->A = (float*)(ws + ip.off_A);
+    d->ubflag = (unsigned char*)(ws + ip.off_flag);
+    d->ub_count = (unsigned long long*)(ws + ip.off_ub);
+    d->H = ip.H;
+    d->W = ip.W;
+    d->cpitch = ip.C.pitch; d->coy = ip.C.oy; d->cox = ip.C.ox; d->ch = ip.C.h; d->cw = ip.C.w;
+    d->apitch = ip.A.pitch; d->aoy = ip.A.oy; d->aox = ip.A.ox; d->ah = ip.A.h; d->aw = ip.A.w;
+    d->aplane = (int)ip.A.plane;
+    const int H = ip.H, W = ip.W;
+    d->L[0] = Launch{0, 0, H - 1, W - 1, 0, 0};
+    d->L[1] = Launch{H - 1 + ip.eh, 0, 1, W - 1, H - 1, 0};
+    d->L[2] = Launch{0, W - 1 + ip.ew, H - 1, 1, 0, W - 1};
+    d->L[3] = Launch{H - 1 + ip.eh, W - 1 + ip.ew, 1, 1, H - 1, W - 1};
+    int total = 0;
+    for (int i = 0; i < 4; i++) {
+        d->bstart[i] = total;
+        if (d->L[i].ny <= 0 || d->L[i].nx <= 0) {
+            d->gx[i] = d->gy[i] = 1;
+            continue;
+        }
+        d->gx[i] = (d->L[i].nx + NLSAR_TILE - 1) / NLSAR_TILE;
+        d->gy[i] = (d->L[i].ny + NLSAR_TILE - 1) / NLSAR_TILE;
+        total += d->gx[i] * d->gy[i];
     }
-    // covmat_rescale.cl:24-25 with nlooks = 1 and dimension 2
-    const float gamma = std::pow(std::min(1.0f / 2, 1.0f), 0.33333f);
+    d->bstart[4] = total;
+}
 
-    Launch L[4];
-    L[0] = Launch{0, 0, H - 1, W - 1, 0, 0};
-    L[1] = Launch{H - 1 + plan.eh, 0, 1, W - 1, H - 1, 0};
-    L[2] = Launch{0, W - 1 + plan.ew, H - 1, 1, 0, W - 1};
-    L[3] = Launch{H - 1 + plan.eh, W - 1 + plan.ew, 1, 1, H - 1, W - 1};
+void nlsar_batch_layout(NlsarImageDesc* d, int n, int* rows_c, int* rows_a, int* blocks, int* max_cw,
+                        int* max_aw)
+{
+    int rc = 0, ra = 0, nb = 0, cw = 0, aw = 0;
+    for (int i = 0; i < n; i++) {
+        d[i].row0_c = rc;
+        d[i].row0_a = ra;
+        d[i].block0 = nb;
+        rc += d[i].ch;
+        ra += d[i].ah;
+        nb += d[i].bstart[4];
+        cw = std::max(cw, d[i].cw);
+        aw = std::max(aw, d[i].aw);
+    }
+    *rows_c = rc;
+    *rows_a = ra;
+    *blocks = nb;
+    *max_cw = cw;
+    *max_aw = aw;
+}
+
+int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImageDesc* descs, int n,
+                  const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream)
+{
+    if (n < 1 || (!d_table && n != 1)) { set_error("nlsar: internal error, bad image set"); return -1; }
+    const float* d_lut = d_const;
+    const float* d_gauss = d_const + (size_t)cm.nparam * cm.lut_stride;
+    std::vector<const NlsarStatsView*> pst;
+    if (nlsar_param_stats(pp, cm, pst)) return -3;
+    // totals (the batch table was numbered by nlsar_batch_layout)
+    const NlsarImageDesc& last = descs[n - 1];
+    const int rows_c = last.row0_c + last.ch, rows_a = last.row0_a + last.ah;
+    const int blocks = last.block0 + last.bstart[4];
+    int max_cw = 0, max_aw = 0;
+    for (int i = 0; i < n; i++) {
+        max_cw = std::max(max_cw, descs[i].cw);
+        max_aw = std::max(max_aw, descs[i].aw);
+    }
+
+    PrepArgs pa;
+    memset(&pa, 0, sizeof(pa));
+    pa.table = d_table;
+    if (!d_table) pa.one = descs[0];
+    pa.nimg = n;
+    hipLaunchKernelGGL(nlsar_zero_counts, dim3((n + 255) / 256), dim3(256), 0, stream, pa);
+    DKX_HIP_CHECK(hipGetLastError());
+    pa.cblocks = (max_cw + 255) / 256;
+    hipLaunchKernelGGL(nlsar_prep, dim3((unsigned)rows_c * pa.cblocks), dim3(256), 0, stream, pa);
+    DKX_HIP_CHECK(hipGetLastError());
+    // covmat_rescale.cl:24-25 with nlooks = 1 and dimension 2
+    pa.gamma = std::pow(std::min(1.0f / 2, 1.0f), 0.33333f);
+    pa.cblocks = (max_aw + 255) / 256;
 
     // Passes: each handles one scale and up to three patch sizes (one dissimilarity field
     // feeds all of them).  The pass holding the largest patch at the largest scale runs last:
     // it also forms the weighted mean with that (patch, scale), the key the selection keeps.
     int pmax_i = 0;
-    for (int pi = 0; pi < pb.nP; pi++) if (pb.P[pi] > pb.P[pmax_i]) pmax_i = pi;
+    for (int pi = 0; pi < pp.nP; pi++) if (pp.P[pi] > pp.P[pmax_i]) pmax_i = pi;
     int smax_i = 0;
-    for (int si = 0; si < pb.nS; si++) if (pb.S[si] > pb.S[smax_i]) smax_i = si;
+    for (int si = 0; si < pp.nS; si++) if (pp.S[si] > pp.S[smax_i]) smax_i = si;
     struct Pass { int si; std::vector<int> pis; bool last; };
     std::vector<Pass> passes;
-    for (int si = 0; si < pb.nS; si++) {
+    for (int si = 0; si < pp.nS; si++) {
         std::vector<int> rest;
-        for (int pi = 0; pi < pb.nP; pi++) if (!(si == smax_i && pi == pmax_i)) rest.push_back(pi);
+        for (int pi = 0; pi < pp.nP; pi++) if (!(si == smax_i && pi == pmax_i)) rest.push_back(pi);
         if (si == smax_i) {
             // the final chunk of this scale: the largest patch plus up to two others
             std::vector<int> fin{pmax_i};
@@ -295,37 +381,31 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
     for (size_t oi = 0; oi < passes.size(); oi++) {
         const Pass& ps = passes[oi];
         const int si = ps.si;
-        const int s = pb.S[si];
         if (si != cur_scale) {
-            dim3 blk(256), grd((A.w + 255) / 256, A.h);
-            hipLaunchKernelGGL(nlsar_scale_average, grd, blk, 0, stream, C, A,
-                               (const float*)(d_gauss + (size_t)si * 31 * 31), s, gamma);
+            pa.gauss = d_gauss + (size_t)si * 31 * 31;
+            pa.s = pp.S[si];
+            hipLaunchKernelGGL(nlsar_scale_average, dim3((unsigned)rows_a * pa.cblocks), dim3(256), 0, stream, pa);
             DKX_HIP_CHECK(hipGetLastError());
             cur_scale = si;
         }
         // slots in ascending patch size: the pass's patch set is a compile-time mask of
         // half widths, and the last pass's largest patch (the mean's) is its last slot
         std::vector<int> pis = ps.pis;
-        std::sort(pis.begin(), pis.end(), [&](int x, int y) { return pb.P[x] < pb.P[y]; });
+        std::sort(pis.begin(), pis.end(), [&](int x, int y) { return pp.P[x] < pp.P[y]; });
         ScaleArgs sa;
         memset(&sa, 0, sizeof(sa));
-        sa.A4 = reinterpret_cast<const float4*>(A.base);
-        sa.LD = A.base + 4 * A.plane;
-        sa.C4 = reinterpret_cast<const float4*>(C.base);
-        sa.apitch = A.pitch;
-        sa.aoy = A.oy;
-        sa.aox = A.ox;
-        sa.cpitch = C.pitch;
-        sa.coy = C.oy;
-        sa.cox = C.ox;
-        sa.wsh = plan.wsh;
+        sa.table = d_table;
+        if (!d_table) sa.one = descs[0];
+        sa.nimg = n;
+        sa.total_blocks = blocks;
+        sa.wsh = cm.wsh;
         int pmask = 0;
         for (size_t k = 0; k < pis.size(); k++) {
             const int pi = pis[k];
-            const int idx = si * pb.nP + pi;
-            pmask |= 1 << ((pb.P[pi] - 1) / 2);
+            const int idx = si * pp.nP + pi;
+            pmask |= 1 << ((pp.P[pi] - 1) / 2);
             sa.pidx[k] = idx;
-            sa.lutw[k] = d_lut + (size_t)idx * plan.lut_stride;
+            sa.lutw[k] = d_lut + (size_t)idx * cm.lut_stride;
             const float dmin = pst[idx]->dissims_min, dmax = pst[idx]->dissims_max;
             sa.dmin[k] = dmin;
             sa.dmax[k] = dmax;
@@ -338,21 +418,11 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
             set_error("nlsar: internal error, the last pass must end with the largest patch");
             return -1;
         }
-        for (int i = 0; i < 4; i++) sa.L[i] = L[i];
-        sa.lut_stride = plan.lut_stride;
-        sa.H = H;
-        sa.W = W;
-        sa.enl_nb = d_enl_maps;
-        sa.alpha = d_alpha_maps;
-        sa.ubflag = d_flag;
+        sa.lut_stride = cm.lut_stride;
         sa.first_scale = oi == 0 ? 1 : 0;
-        sa.ref = d_ref;
-        sa.phase = d_phase;
-        sa.coh = d_coh;
-        sa.ub_count = d_ub;
         int rc;
-        if (plan.wsh <= NLSAR_WMAX_SMALL) {
-            switch (plan.tile) {
+        if (cm.wsh <= NLSAR_WMAX_SMALL) {
+            switch (cm.tile) {
             case 0:
                 rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream)
                              : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream);
@@ -371,9 +441,6 @@ int nlsar_run(const NlsarProblem& pb, const NlsarPlan& plan, const float* d_a1, 
                          : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE>(pmask, sa, stream);
         }
         if (rc) return rc;
-    }
-    if (ub_host) {
-        DKX_HIP_CHECK(hipMemcpyAsync(ub_host, d_ub, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     }
     return 0;
 }

@@ -45,3 +45,66 @@ def set_tile(filter_index, tile_index):
 
 def get_tile(filter_index):
     return lib().dkx_get_tile(int(filter_index))
+
+
+class NlsarBatch:
+    """Batched NL-SAR on device buffers: many images of any size, one parameter set, one
+    launch per stage over the tiles of all images (dkx_nlsar_batch_* in the C ABI).
+
+        b = NlsarBatch()
+        b.set(images, 21, [3, 5, 7], [1, 3], stats)   # images: (a1, a2, phase, ref, phase_filt, coh, h, w)
+        b.enqueue(stream)          # or b.graph_launch(stream) to capture once and replay
+    """
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().dkx_nlsar_batch_create(C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().dkx_nlsar_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, images, search_window_size, patch_sizes, scale_sizes, nlsar_stats, h=15.0, c=49.0):
+        from . import _stats_structs
+        from ._lib import DkxImage
+        arr = (DkxImage * len(images))()
+        for i, (a1, a2, ph, ref, phf, coh, hh, ww) in enumerate(images):
+            arr[i] = DkxImage(a1, a2, ph, ref, phf, coh, int(hh), int(ww))
+        P, nP = int_array(patch_sizes)
+        S, nS = int_array(scale_sizes)
+        st, ns, keep = _stats_structs(nlsar_stats)
+        check(lib().dkx_nlsar_batch_set(self._h, arr, len(images), int(search_window_size), P, nP, S, nS,
+                                        st, ns, float(h), float(c)))
+        self.n = len(images)
+
+    def enqueue(self, stream=0):
+        check(lib().dkx_nlsar_batch_enqueue(self._h, stream or None))
+
+    def graph_launch(self, stream):
+        check(lib().dkx_nlsar_batch_graph_launch(self._h, stream))
+
+    def set_tile(self, tile):
+        check(lib().dkx_nlsar_batch_set_tile(self._h, int(tile)))
+
+    def tile(self):
+        return lib().dkx_nlsar_batch_get_tile(self._h)
+
+    def autotune(self, stream=0, reps=2):
+        t = C.c_int(0)
+        ms = (C.c_float * 8)()
+        check(lib().dkx_nlsar_batch_autotune(self._h, stream or None, int(reps), C.byref(t), ms))
+        return t.value, [ms[i] for i in range(3)]
+
+    def ub_pixels(self):
+        out = (C.c_ulonglong * max(1, self.n))()
+        check(lib().dkx_nlsar_batch_ub_pixels(self._h, out))
+        return [out[i] for i in range(self.n)]

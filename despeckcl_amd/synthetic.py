@@ -73,3 +73,16 @@ def homogeneous_area(size, seed, reflectivity=10.0, coherence=0.6, phase=0.5):
     return (np.ascontiguousarray(np.abs(z1).astype(np.float32)),
             np.ascontiguousarray(np.abs(z2).astype(np.float32)),
             np.ascontiguousarray(np.angle(z1 * np.conj(z2)).astype(np.float32)))
+
+
+def tile_shapes(n, seed, lo=256, hi=2048, max_aspect=4.0):
+    """n (height, width) pairs with both sides log-uniform in [lo, hi] and aspect ratio at
+    most max_aspect (pairs beyond it are redrawn)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    while len(out) < n:
+        h, w = np.exp(rng.uniform(np.log(lo), np.log(hi), 2))
+        h, w = int(round(h)), int(round(w))
+        if max(h, w) <= max_aspect * min(h, w):
+            out.append((h, w))
+    return out

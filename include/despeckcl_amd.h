@@ -124,6 +124,56 @@ int dkx_stage_times(int* kinds, float* ms, int max_entries);
 /* Releases the device workspace cached by the library on the current device. */
 void dkx_release_workspace(void);
 
+/* ---- batched NL-SAR ----------------------------------------------------------------------
+ * No counterpart in the reference, which filters one image per call (despeckcl::nlsar,
+ * include/despeckcl.h:45-59) in fixed tiles of up to 2048^2 pixels, one tile after another
+ * (src/compute/nlsar/nlsar.cpp:81-136, map_filter_tiles.h:37-104).  For callers that filter
+ * many interferograms of any size with the same parameters: every stage runs as one launch
+ * over the tiles of all images, and the whole stage sequence can be captured in a HIP graph.
+ * Each image's result equals the result of dkx_nlsar_insar_device on that image alone with
+ * the same pass variant (dkx_set_tile / dkx_nlsar_batch_set_tile). */
+
+/* One image of a batch: device pointers, row-major height x width float32 planes. */
+typedef struct {
+    const float* ampl_master;
+    const float* ampl_slave;
+    const float* phase;
+    float* ref_filt;
+    float* phase_filt;
+    float* coh_filt;
+    int height;
+    int width;
+} dkx_nlsar_image;
+
+typedef struct dkx_nlsar_batch_s* dkx_nlsar_batch;
+
+int dkx_nlsar_batch_create(dkx_nlsar_batch* batch);
+void dkx_nlsar_batch_destroy(dkx_nlsar_batch batch);
+/* (Re)defines the images and the parameters (same meaning and checks as dkx_nlsar_insar).
+ * Waits for the batch's previous run, grows its device workspace if needed and uploads the
+ * image table and the weight LUTs (synchronous).  The pass variant is the one the autotuner
+ * chose for this parameter set and total pixel count, or 1. */
+int dkx_nlsar_batch_set(dkx_nlsar_batch batch, const dkx_nlsar_image* images, int n_images,
+                        int search_window_size, const int* patch_sizes, int n_patch_sizes,
+                        const int* scale_sizes, int n_scale_sizes, const dkx_nlsar_stats* stats,
+                        int n_stats, float h_param, float c_param);
+/* Enqueues the whole batch on `stream` (a hipStream_t); capturable in a HIP graph. */
+int dkx_nlsar_batch_enqueue(dkx_nlsar_batch batch, void* stream);
+/* Captures the batch's stage sequence into a HIP graph on first use (after each _set), then
+ * replays it on `stream` (not the null stream). */
+int dkx_nlsar_batch_graph_launch(dkx_nlsar_batch batch, void* stream);
+/* Pass variant: -1 = the autotuner's choice, 0/1/2 = 4/2/1 rows per thread. */
+int dkx_nlsar_batch_set_tile(dkx_nlsar_batch batch, int tile);
+int dkx_nlsar_batch_get_tile(dkx_nlsar_batch batch);
+/* Times every pass variant on the batch (reps runs each, after one warm-up run), records the
+ * fastest for this parameter set and total pixel count, and selects it.  ms (optional)
+ * receives the mean time of each variant. */
+int dkx_nlsar_batch_autotune(dkx_nlsar_batch batch, void* stream, int reps, int* tile,
+                             float* ms);
+/* Waits for the last run and reads each image's count of pixels where the reference's best
+ * parameter selection is undefined (see dkx_nlsar_insar_device). */
+int dkx_nlsar_batch_ub_pixels(dkx_nlsar_batch batch, unsigned long long* counts);
+
 #ifdef __cplusplus
 }
 #endif
