@@ -127,6 +127,85 @@ def batched(args):
     print(json.dumps(out), flush=True)
 
 
+def nlinsar_form(args):
+    """NL-InSAR on one seeded interferogram per call (default: 4096^2, 32 regions plus 5 %
+    point scatterers at x50, search window 21, patch 7, 10 iterations, lmin 10), inputs and
+    the estimates resident in HBM; the estimates start from zero, as the Python API's."""
+    import ctypes as C
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    n = args.size
+    big = n >= 1024
+    a1, a2, ph = S.interferogram(n, n, S.SEED_BASE + (2 if big else 0), n_regions=32 if big else 8,
+                                 coh_range=(0.3, 0.9), scatterer_fraction=0.05 if big else 0.0)
+    stream = HIP.Stream()
+    d_in = [HIP.DeviceArray.from_host(x) for x in (a1, a2, ph)]
+    d_out = [HIP.DeviceArray((n, n)) for _ in range(3)]
+
+    def run(want=False):
+        for d in d_out:
+            d.zero()
+        return DV.nlinsar(*[d.ptr for d in d_in], *[d.ptr for d in d_out], n, n, 21, 7, args.niter, 10,
+                          stream=stream.handle, want_count=want)
+
+    nsm = run(True)
+    stream.synchronize()
+    lib = D.lib()
+    for _ in range(args.warmup):
+        run()
+    stream.synchronize()
+    HIP.synchronize()
+    e0, e1 = HIP.Event(), HIP.Event()
+    e0.record(stream.handle)
+    for _ in range(args.steps):
+        run()
+    e1.record(stream.handle)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    lib.dkx_set_timing(1)
+    lib.dkx_stage_times(None, None, 0)
+    run()
+    stream.synchronize()
+    kinds = (C.c_int * 256)()
+    times = (C.c_float * 256)()
+    k = lib.dkx_stage_times(kinds, times, 256)
+    lib.dkx_set_timing(0)
+    wts = [times[i] for i in range(k) if kinds[i] == 11]
+    sms = [times[i] for i in range(k) if kinds[i] == 12]
+    px = n * n
+    flop = 31791.0 * px * args.niter
+    out = {
+        "metric": f"Mpixel/s NL-InSAR {n}x{n}, {args.niter} iterations (1 GPU)",
+        "value": round(px / 1e6 / (ms / 1e3), 3), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded speckle model of insarsim_simu.cpp" +
+                (", 32 regions, 5% point scatterers x50)" if big else ", 8 regions)"),
+        "config": {"workload": f"NL-InSAR, one {n}x{n} interferogram per call", "search_window": 21,
+                   "patch": 7, "niter": args.niter, "lmin": 10, "tile": DV.get_tile(1)},
+        "mpx_iter_s": round(px * args.niter / 1e6 / (ms / 1e3), 3),
+        "tflops": round(flop / (ms * 1e-3) / 1e12, 3),
+        "weights_ms": [round(x, 3) for x in wts], "smooth_ms": [round(x, 3) for x in sms],
+        "smoothed_pixel_iterations": int(nsm),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        c = 192
+        t0 = time.perf_counter()
+        O.nlinsar(a1[:c, :c], a2[:c, :c], ph[:c, :c], 21, 7, 2, 10)
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(c * c * 2 / 1e6 / dt, 4), "unit": "Mpixel-iterations/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"NL-InSAR, 2 iterations on the {c}x{c} top-left crop, {dt:.1f} s "
+                                         f"on {cores} OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +213,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
-    ap.add_argument("--form", choices=("plain", "batched"), default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "nlinsar"), default="plain")
+    ap.add_argument("--size", type=int, default=4096, help="image side (--form nlinsar)")
+    ap.add_argument("--niter", type=int, default=10, help="iterations (--form nlinsar)")
     ap.add_argument("--tiles", type=int, default=64, help="images per batch (--form batched)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -143,6 +224,8 @@ def main():
         sys.exit(2)
     if args.form == "batched":
         return batched(args)
+    if args.form == "nlinsar":
+        return nlinsar_form(args)
 
     import despeckcl_amd as D
     from despeckcl_amd import device as DV

@@ -112,7 +112,7 @@ static int ensure_stream(DeviceWorkspace* w)
     return 0;
 }
 
-static std::atomic<int> g_tile[2] = {{1}, {0}};  // NL-SAR: 2 rows per thread
+static std::atomic<int> g_tile[2] = {{1}, {1}};  // NL-SAR: 2 rows per thread; NL-InSAR: 18 x 54
 
 static std::atomic<bool> g_timing{false};
 static std::mutex g_timing_mu;

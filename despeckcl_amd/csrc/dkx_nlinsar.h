@@ -18,9 +18,11 @@ struct NlinsarPlan {
     int tile, TY, TX;
     int wsh, psh, overlap, step;   // step = ref_tile - 2*overlap (interior of a reference tile)
     int eh, ew;                    // virtual distance of the last row / column
-    Grid R, E, F;                  // raw (a1, a2, phase), estimates, filter values (a, Re x, Im x)
+    Grid R, E, F;                  // float4 grids: raw (a1, a2, phase), estimates (ref, phase,
+                                   // coh), filter values (a, Re x, Im x)
     int heap_block;                // threads per block of the smoothing kernel
-    size_t off_R, off_E, off_F, off_est, off_w, off_list, off_cnt, ws_bytes;
+    int blocks;                    // blocks of the weights kernel (weight / cache chunks)
+    size_t off_R, off_E, off_F, off_est, off_w, off_s, off_list, off_cnt, ws_bytes;
 };
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);

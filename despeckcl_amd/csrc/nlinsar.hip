@@ -17,9 +17,11 @@
 
 namespace dkx {
 
-// replicate an H x W image set (nplanes planes) into a padded grid, clamping coordinates
-__global__ void grid_fill(const float* __restrict__ p0, const float* __restrict__ p1,
-                          const float* __restrict__ p2, int H, int W, Grid G)
+// Replicates an H x W image set (three planes) into a padded grid of float4 (the fourth
+// component is 0), clamping coordinates like the reference's tile padding
+// (src/data/sub_images.cpp:37-40).
+__global__ void grid_fill4(const float* __restrict__ p0, const float* __restrict__ p1,
+                           const float* __restrict__ p2, int H, int W, Grid G)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
@@ -27,10 +29,24 @@ __global__ void grid_fill(const float* __restrict__ p0, const float* __restrict_
     const int y = min(H - 1, max(0, gy - G.oy));
     const int x = min(W - 1, max(0, gx - G.ox));
     const long long i = (long long)y * W + x;
-    const long long o = (long long)gy * G.pitch + gx;
-    G.base[o] = p0[i];
-    G.base[G.plane + o] = p1[i];
-    G.base[2 * G.plane + o] = p2[i];
+    reinterpret_cast<float4*>(G.base)[(long long)gy * G.pitch + gx] = make_float4(p0[i], p1[i], p2[i], 0.f);
+}
+
+// The estimates as the KL divergence uses them (nlinsar/sim_measures.h:42-49): per element
+// of the padded grid E, float4 (ref, coh * cos phase, coh * sin phase,
+// u = 1 / (ref * (1 - coh^2))).
+__global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const float* __restrict__ pha,
+                                       const float* __restrict__ coh, int H, int W, Grid E)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= E.w || gy >= E.h) return;
+    const int y = min(H - 1, max(0, gy - E.oy));
+    const int x = min(W - 1, max(0, gx - E.ox));
+    const long long i = (long long)y * W + x;
+    const float r = ref[i], p = pha[i], c = coh[i];
+    const long long o = (long long)gy * E.pitch + gx;
+    reinterpret_cast<float4*>(E.base)[o] = make_float4(r, c * cosf(p), c * sinf(p), 1.0f / (r * (1.0f - c * c)));
 }
 
 // precompute_filter_values.cl:19-39: a = (a1^2 + a2^2)/2, x = a1*a2*exp(-i*phase)
@@ -39,12 +55,29 @@ __global__ void nlinsar_filter_values(Grid R, Grid F)
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
     if (gx >= F.w || gy >= F.h) return;
-    const long long r = R.at(gy - F.oy, gx - F.ox);
-    const float a1 = R.base[r], a2 = R.base[R.plane + r], dp = R.base[2 * R.plane + r];
-    const long long o = (long long)gy * F.pitch + gx;
-    F.base[o] = 0.5f * (a1 * a1 + a2 * a2);
-    F.base[F.plane + o] = a1 * a2 * cosf(-dp);
-    F.base[2 * F.plane + o] = a1 * a2 * sinf(-dp);
+    const float4 r = reinterpret_cast<const float4*>(R.base)[R.at(gy - F.oy, gx - F.ox)];
+    const float a1 = r.x, a2 = r.y, dp = r.z;
+    reinterpret_cast<float4*>(F.base)[(long long)gy * F.pitch + gx] =
+        make_float4(0.5f * (a1 * a1 + a2 * a2), a1 * a2 * cosf(-dp), a1 * a2 * sinf(-dp), 0.f);
+}
+
+// flag[0] = 1 if any reflectivity estimate is nonzero: otherwise every KL divergence is
+// -4/pi * (1 + 1 - 2) = -0 (nlinsar/sim_measures.h:42-49 with ref2 == 0) and is skipped.
+__global__ void nlinsar_any_estimate(const float* __restrict__ ref, long long n, unsigned int* flag)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool nz = false;
+    for (long long k = i; k < n; k += (long long)gridDim.x * blockDim.x) nz |= ref[k] != 0.0f;
+    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+__global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, int clear_total)
+{
+    if (threadIdx.x == 0) {
+        cnt[0] = 0u;
+        if (clear_total) cnt[1] = 0u;
+        *flag = 0u;
+    }
 }
 
 // pixel_similarity, nlinsar/sim_measures.h:27-40
@@ -61,14 +94,19 @@ __device__ __forceinline__ float nl_pixel_similarity(float am1, float as1, float
     return logf(f_el);
 }
 
-// pixel_kullback_leibler, nlinsar/sim_measures.h:42-49
-__device__ __forceinline__ float nl_pixel_kl(float ref1, float dp1, float coh1, float ref2,
-                                             float dp2, float coh2)
+// pixel_kullback_leibler (nlinsar/sim_measures.h:42-49) from the per-pixel values of
+// nlinsar_fill_estimates:
+//   a1 = ref1 / ref2 * (1 - coh1*coh2*cos(dp1 - dp2)) / (1 - coh2^2) = ref1 * r * u2,
+//   a2 = ref2 / ref1 * r / (1 - coh1^2)                               = ref2 * r * u1,
+// with coh1*coh2*cos(dp1 - dp2) = (coh1 cos1)(coh2 cos2) + (coh1 sin1)(coh2 sin2) (a few
+// ulps from the reference's expression, inside OpenCL's bounds for cos and division);
+// ref2 == 0 gives a1 = a2 = 1, i.e. -0.
+__device__ __forceinline__ float nl_pixel_kl_pre(float4 e1, float4 e2)
 {
-    const float cs = cosf(dp1 - dp2);
-    const float a1 = (ref2 != 0.0f) ? ref1 / ref2 * (1.0f - coh1 * coh2 * cs) / (1.0f - coh2 * coh2) : 1.0f;
-    const float a2 = (ref2 != 0.0f) ? ref2 / ref1 * (1.0f - coh1 * coh2 * cs) / (1.0f - coh1 * coh1) : 1.0f;
-    return -4.0f / 3.14159274101257324219f * (a1 + a2 - 2.0f);
+    const float r = 1.0f - fmaf(e1.y, e2.y, e1.z * e2.z);
+    const float a1 = e1.x * r * e2.w;
+    const float a2 = e2.x * r * e1.w;
+    return e2.x != 0.0f ? -4.0f / 3.14159274101257324219f * (a1 + a2 - 2.0f) : -0.0f;
 }
 
 __device__ __forceinline__ bool nl_is_valid(float x) { return !(isnan(x) || isinf(x) || x == 0); }
@@ -103,13 +141,23 @@ __device__ __forceinline__ void nl_estimate(float N, float at, float xr, float x
 }
 
 struct WeightArgs {
-    Grid R, E, F;
-    int wsh, psh, step, wsim;
-    float h, T;
+    const float4* R4;     // raw data: a1, a2, phase (padded by the overlap)
+    const float4* E4;     // estimates, see nlinsar_fill_estimates (same grid)
+    const float4* F4;     // filter values: a, Re x, Im x (padded by the window)
+    int rpitch, rorg;     // rorg: element of image position (0, 0)
+    int fpitch, forg;
+    int wsh, step, wsim;
+    float rh, rT;         // 1 / h, 1 / T
     float lmin;
-    Launch L;
+    Launch L[4];          // interior, last row, last column, corner
+    int gx[4], nsx[4], nsy[4];   // block columns / sub-tiles per reference tile, per mapping
+    int bstart[5];        // first linear block of each mapping, [4] = total
     int H, W;
-    float* wt;  // [sw*sw][H*W]
+    // Per block, one contiguous chunk of [sw*sw][TY*TX] floats (block-major, so that every
+    // block streams through its own memory):
+    float* scache;        // patch similarities (written by the first iteration)
+    float* wt;            // weights (read by the smoothing)
+    const unsigned int* kl_on;
     const float* old_ref;
     const float* old_pha;
     const float* old_coh;
@@ -120,64 +168,103 @@ struct WeightArgs {
     unsigned int* cnt;
 };
 
-constexpr int NTW = 256;
+constexpr int NTW = 512;
+constexpr int NLI_BAND = 4;   // block rows per band of the block order
 
-// EDGE: separate kernel symbol for the thin last-row / last-column launches
-template <int TY, int TX, bool EDGE>
+// FIRST: evaluate the likelihood similarities and keep their patch sums (they depend on the
+// raw data only); later iterations read them back and evaluate only the KL divergences.
+template <int TY, int TX, int PSH, bool FIRST>
 __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
 {
     constexpr int NPIX = TY * TX;
     constexpr int PPT = (NPIX + NTW - 1) / NTW;
-    extern __shared__ float smem[];
-    const int wsh = a.wsh, psh = a.psh;
+    constexpr int NP2 = 2 * PSH;
+    constexpr int PY = TY + NP2, PX = TX + NP2;
+    extern __shared__ float4 smem4[];
+    const int wsh = a.wsh;
     const int sw = 2 * wsh + 1;
-    const int np2 = 2 * psh;
-    const int PY = TY + np2, PX = TX + np2;
-    float* sS = smem;
-    float* sK = sS + PY * PX;
-    float* rS = sK + PY * PX;
-    float* rK = rS + PY * TX;
+    // LDS: the records of the block's whole neighbourhood (raw data in the first iteration,
+    // estimates after), staged once; then the per-offset similarity / KL fields and sums
+    const int o = wsh + PSH;
+    const int EW = TX + 2 * o, EH = TY + 2 * o;
+    float4* stg = smem4;
+    float* sS = reinterpret_cast<float*>(smem4 + EH * EW);
+    float* rS = sS + (FIRST ? PY * PX : 0);
+    float* sK = rS + (FIRST ? PY * TX : 0);
+    float* rK = sK + PY * PX;
 
     const int tid = threadIdx.x;
-    // Blocks are sub-tiles of the interiors (step x step) of the reference's tiles, so every
-    // output of a block belongs to one reference tile: block row by = reference tile row
-    // * nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile of a reference
-    // tile is cut at the tile's end.  One-row / one-column launches have a single block
-    // row / column starting at the launch origin.
-    const int nsy = (a.step + TY - 1) / TY, nsx = (a.step + TX - 1) / TX;
-    const int ry = blockIdx.y / nsy, sy = blockIdx.y - ry * nsy;
-    const int rx = blockIdx.x / nsx, sx_ = blockIdx.x - rx * nsx;
-    const int ty0 = a.L.vy0 + ry * a.step + sy * TY;
-    const int tx0 = a.L.vx0 + rx * a.step + sx_ * TX;
+    const bool kl = *a.kl_on != 0u;
+    // The four launch mappings (interior, last row, last column, corner) are laid end to end
+    // in one grid.  Blocks are sub-tiles of the interiors (step x step) of the reference's
+    // tiles, so every output of a block belongs to one reference tile: block row by =
+    // reference tile row * nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile
+    // of a reference tile is cut at the tile's end.  The one-row / one-column mappings have a
+    // single block row / column starting at the mapping's origin.
+    // Linear block ids are remapped so that the blocks the hardware sends to one XCD (id
+    // mod 8) take a contiguous run of tiles, ordered column by column within bands of
+    // NLI_BAND block rows: the blocks resident on one XCD then cover a compact patch of the
+    // image, whose neighbourhoods overlap in that XCD's L2.
+    const int nb = a.bstart[4];
+    const int bq = nb >> 3, br = nb & 7;
+    const int xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    const int t = xcd * bq + min(xcd, br) + kk;   // tile index (also the chunk index)
+    int m = 0;
+    while (m < 3 && t >= a.bstart[m + 1]) m++;
+    const Launch L = m == 0 ? a.L[0] : m == 1 ? a.L[1] : m == 2 ? a.L[2] : a.L[3];
+    const int nsy = a.nsy[m], nsx = a.nsx[m];
+    const int local = t - a.bstart[m];
+    const int gxm = a.gx[m];
+    const int gym = (a.bstart[m + 1] - a.bstart[m]) / gxm;
+    const int band = local / (NLI_BAND * gxm);
+    const int within = local - band * NLI_BAND * gxm;
+    const int bh = min(NLI_BAND, gym - band * NLI_BAND);
+    const int blx = within / bh;
+    const int bly = band * NLI_BAND + (within - blx * bh);
+    const int ry = bly / nsy, sy = bly - ry * nsy;
+    const int rx = blx / nsx, sx_ = blx - rx * nsx;
+    const int ty0 = L.vy0 + ry * a.step + sy * TY;
+    const int tx0 = L.vx0 + rx * a.step + sx_ * TX;
     const int rows_lim = min(TY, a.step - sy * TY);
     const int cols_lim = min(TX, a.step - sx_ * TX);
     // interior origin of the reference tile that computes this block's outputs
     const int Y0 = (ty0 / a.step) * a.step;
     const int X0 = (tx0 / a.step) * a.step;
 
-    const long long HW = (long long)a.H * a.W;
+    // this block's chunk of the similarity cache and of the weights
+    const long long chunk = (long long)t * (sw * sw) * NPIX;
     float acc[PPT][5];
     long long opix[PPT];
     bool valid[PPT];
+    unsigned int fofs[PPT];   // F grid element of the pixel (offset 0)
+    // the next offset's cached similarity sum and filter values, loaded one offset ahead so
+    // that their latency overlaps the KL divergences of the current offset
+    float sv[PPT];
+    float4 fv[PPT];
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
         acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = acc[q][4] = 0.f;
         const int pq = tid + q * NTW;
         const int py = pq / TX, px = pq - (pq / TX) * TX;
-        const int iy = ty0 + py - a.L.vy0, ix = tx0 + px - a.L.vx0;
-        valid[q] = pq < NPIX && py < rows_lim && px < cols_lim && iy < a.L.ny && ix < a.L.nx;
-        opix[q] = valid[q] ? (long long)(a.L.oy0 + iy) * a.W + (a.L.ox0 + ix) : 0;
+        const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
+        valid[q] = pq < NPIX && py < rows_lim && px < cols_lim && iy < L.ny && ix < L.nx;
+        opix[q] = valid[q] ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
+        fofs[q] = (unsigned int)(a.forg + (ty0 + py) * a.fpitch + (tx0 + px));
+        // offset plane 0: (-wsh, -wsh)
+        sv[q] = (!FIRST && valid[q]) ? a.scache[chunk + pq] : 0.f;
+        fv[q] = a.F4[fofs[q] - wsh * a.fpitch - wsh];
     }
 
-    const float* Ra1 = a.R.base;
-    const float* Ra2 = Ra1 + a.R.plane;
-    const float* Rdp = Ra1 + 2 * a.R.plane;
-    const float* Eref = a.E.base;
-    const float* Epha = Eref + a.E.plane;
-    const float* Ecoh = Eref + 2 * a.E.plane;
-    const float* Fa = a.F.base;
-    const float* Fxr = Fa + a.F.plane;
-    const float* Fxi = Fa + 2 * a.F.plane;
+    // stage the neighbourhood: rows ty0 - o .. ty0 + TY + o, columns tx0 - o .. tx0 + TX + o
+    {
+        const float4* src = FIRST ? a.R4 : a.E4;
+        const int n = EH * EW;
+        for (int i = tid; i < n; i += NTW) {
+            const int r = i / EW, c = i - r * EW;
+            stg[i] = src[(unsigned int)(a.rorg + (ty0 - o + r) * a.rpitch + (tx0 - o + c))];
+        }
+    }
+    const int sorg = -(ty0 - o) * EW - (tx0 - o);   // staged index of image position (0, 0)
 
     const int centre = wsh * sw + wsh;
     for (int hh = 0; hh < sw; hh++) {
@@ -189,96 +276,141 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                 // centre weight 0 (nlinsar_sub_image.cpp:189-197); adding it changes no sum
 #pragma unroll
                 for (int q = 0; q < PPT; q++)
-                    if (valid[q]) a.wt[(long long)plane * HW + opix[q]] = 0.f;
+                    if (valid[q]) a.wt[chunk + (long long)plane * NPIX + tid + q * NTW] = 0.f;
                 continue;
             }
-            __syncthreads();
-            // 1. similarity and KL at every patch position of the block.  Offsets with
-            //    dy >= 0 are evaluated centre-first everywhere; offsets with dy < 0 are the
-            //    mirrored values (the pixel at +d first) at positions inside the reference
-            //    tile's similarity border, centre-first on the border
-            //    (precompute_similarities_1st_pass.cl.in:36-42, _2nd_pass.cl:30-48).
-            {
-                const int n = PY * PX;
-                int r = tid / PX, c = tid - r * PX;
-                const int sr_ = NTW / PX, sc_ = NTW - sr_ * PX;
-                for (int i = tid; i < n; i += NTW) {
-                    const int gy = ty0 - psh + r, gx = tx0 - psh + c;
-                    const int sh = gy - Y0 + psh, sx = gx - X0 + psh;
-                    const bool interior = sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
-                    const bool swap = dy < 0 && interior;
-                    const int y1 = swap ? gy + dy : gy, x1 = swap ? gx + dx : gx;
-                    const int y2 = swap ? gy : gy + dy, x2 = swap ? gx : gx + dx;
-                    const long long p1 = a.R.at(y1, x1), p2 = a.R.at(y2, x2);
-                    sS[r * PX + c] = nl_pixel_similarity(Ra1[p1], Ra2[p1], Rdp[p1], Ra1[p2], Ra2[p2], Rdp[p2]);
-                    const long long e1 = a.E.at(y1, x1), e2 = a.E.at(y2, x2);
-                    sK[r * PX + c] = nl_pixel_kl(Eref[e1], Epha[e1], Ecoh[e1], Eref[e2], Epha[e2], Ecoh[e2]);
-                    r += sr_;
-                    c += sc_;
-                    if (c >= PX) { c -= PX; r++; }
-                }
-            }
-            __syncthreads();
-            // 2. row sums along x (precompute_patch_similarities.cl:45-50)
-            {
-                const int n = PY * TX;
-                int r = tid / TX, c = tid - r * TX;
-                const int sr_ = NTW / TX, sc_ = NTW - sr_ * TX;
-                for (int i = tid; i < n; i += NTW) {
-                    const float* s1 = sS + r * PX + c;
-                    const float* s2 = sK + r * PX + c;
-                    float u = s1[0], v = s2[0];
-                    for (int j = 1; j <= np2; j++) {
-                        u += s1[j];
-                        v += s2[j];
+            if (FIRST || kl) {
+                __syncthreads();
+                // 1. similarity and KL at every patch position of the block.  Offsets with
+                //    dy >= 0 are evaluated centre-first everywhere; offsets with dy < 0 are the
+                //    mirrored values (the pixel at +d first) at positions inside the reference
+                //    tile's similarity border, centre-first on the border
+                //    (precompute_similarities_1st_pass.cl.in:36-42, _2nd_pass.cl:30-48).
+                {
+                    constexpr int n = PY * PX;
+                    int r = tid / PX, c = tid - r * PX;
+                    constexpr int sr_ = NTW / PX, sc_ = NTW - sr_ * PX;
+                    for (int i = tid; i < n; i += NTW) {
+                        const int gy = ty0 - PSH + r, gx = tx0 - PSH + c;
+                        const int sh = gy - Y0 + PSH, sx = gx - X0 + PSH;
+                        const bool interior = sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
+                        const bool swap = dy < 0 && interior;
+                        const int y1 = swap ? gy + dy : gy, x1 = swap ? gx + dx : gx;
+                        const int y2 = swap ? gy : gy + dy, x2 = swap ? gx : gx + dx;
+                        const int s1 = sorg + y1 * EW + x1, s2 = sorg + y2 * EW + x2;
+                        if (FIRST) {
+                            const float4 u = stg[s1], v = stg[s2];
+                            sS[r * PX + c] = nl_pixel_similarity(u.x, u.y, u.z, v.x, v.y, v.z);
+                            if (kl) {
+                                // nonzero initial estimates: read them from the grid
+                                const unsigned int p1 = (unsigned int)(a.rorg + y1 * a.rpitch + x1);
+                                const unsigned int p2 = (unsigned int)(a.rorg + y2 * a.rpitch + x2);
+                                sK[r * PX + c] = nl_pixel_kl_pre(a.E4[p1], a.E4[p2]);
+                            }
+                        } else {
+                            sK[r * PX + c] = nl_pixel_kl_pre(stg[s1], stg[s2]);
+                        }
+                        r += sr_;
+                        c += sc_;
+                        if (c >= PX) { c -= PX; r++; }
                     }
-                    rS[r * TX + c] = u;
-                    rK[r * TX + c] = v;
-                    r += sr_;
-                    c += sc_;
-                    if (c >= TX) { c -= TX; r++; }
                 }
+                __syncthreads();
+                // 2. row sums along x (precompute_patch_similarities.cl:45-50)
+                {
+                    constexpr int n = PY * TX;
+                    int r = tid / TX, c = tid - r * TX;
+                    constexpr int sr_ = NTW / TX, sc_ = NTW - sr_ * TX;
+                    for (int i = tid; i < n; i += NTW) {
+                        if (FIRST) {
+                            const float* s1 = sS + r * PX + c;
+                            float u = s1[0];
+#pragma unroll
+                            for (int j = 1; j <= NP2; j++) u += s1[j];
+                            rS[r * TX + c] = u;
+                        }
+                        if (kl) {
+                            const float* s2 = sK + r * PX + c;
+                            float v = s2[0];
+#pragma unroll
+                            for (int j = 1; j <= NP2; j++) v += s2[j];
+                            rK[r * TX + c] = v;
+                        }
+                        r += sr_;
+                        c += sc_;
+                        if (c >= TX) { c -= TX; r++; }
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
             // 3. rows accumulated top to bottom (precompute_patch_similarities.cl:53-58),
             //    weight (compute_weights.cl:19-30), ENL sums (compute_number_of_looks.cl)
             //    and weighted means (compute_insar.cl:81-93) in plane order
+            // next plane, skipping the centre
+            int nplane = plane + 1;
+            if (nplane == centre) nplane++;
+            const bool has_next = nplane < sw * sw;
+            const int nfd = (nplane / sw - wsh) * a.fpitch + (nplane - (nplane / sw) * sw - wsh);
 #pragma unroll
             for (int q = 0; q < PPT; q++) {
                 const int pq = tid + q * NTW;
                 if (pq >= NPIX) continue;
                 const int py = pq / TX, px = pq - (pq / TX) * TX;
-                const float* s1 = rS + py * TX + px;
-                const float* s2 = rK + py * TX + px;
-                float u = s1[0], v = s2[0];
-                for (int j = 1; j <= np2; j++) {
-                    u += s1[j * TX];
-                    v += s2[j * TX];
+                float u;
+                if (FIRST) {
+                    const float* s1 = rS + py * TX + px;
+                    u = s1[0];
+#pragma unroll
+                    for (int j = 1; j <= NP2; j++) u += s1[j * TX];
+                    if (valid[q]) a.scache[chunk + (long long)plane * NPIX + pq] = u;
+                } else {
+                    u = sv[q];
                 }
-                float w = expf(u / a.h + v / a.T);
+                float v = -0.0f;
+                if (kl) {
+                    const float* s2 = rK + py * TX + px;
+                    v = s2[0];
+#pragma unroll
+                    for (int j = 1; j <= NP2; j++) v += s2[j * TX];
+                }
+                float w = expf(u * a.rh + v * a.rT);
                 if (isnan(w) || isinf(w)) w = 0.0f;
-                if (valid[q]) a.wt[(long long)plane * HW + opix[q]] = w;
-                const long long f = a.F.at(ty0 + py + dy, tx0 + px + dx);
+                if (valid[q]) a.wt[chunk + (long long)plane * NPIX + pq] = w;
+                const float4 f = fv[q];
+                if (has_next) {
+                    if (!FIRST) sv[q] = valid[q] ? a.scache[chunk + (long long)nplane * NPIX + pq] : 0.f;
+                    fv[q] = a.F4[fofs[q] + nfd];
+                }
                 acc[q][0] += w;
                 acc[q][1] = fmaf(w, w, acc[q][1]);
-                acc[q][2] = fmaf(w, Fa[f], acc[q][2]);
-                acc[q][3] = fmaf(w, Fxr[f], acc[q][3]);
-                acc[q][4] = fmaf(w, Fxi[f], acc[q][4]);
+                acc[q][2] = fmaf(w, f.x, acc[q][2]);
+                acc[q][3] = fmaf(w, f.y, acc[q][3]);
+                acc[q][4] = fmaf(w, f.z, acc[q][4]);
             }
         }
     }
+    // pixels whose ENL is below lmin are queued for the smoothing, a wave at a time (one
+    // atomic per wave keeps a wave's pixels, which are neighbours, together in the queue)
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int q = 0; q < PPT; q++) {
-        if (!valid[q]) continue;
         const float ws = acc[q][0];
         const float nol = ws * ws / acc[q][1];
-        if (nol < a.lmin) {
-            const unsigned int slot = atomicAdd(a.cnt, 1u);
-            a.list[slot] = (int)opix[q];
-        } else {
+        const bool queued = valid[q] && nol < a.lmin;
+        const unsigned long long qmask = __ballot(queued);
+        if (qmask) {
+            unsigned int base = 0;
+            if (lane == 0) base = atomicAdd(a.cnt, (unsigned int)__popcll(qmask));
+            base = __shfl(base, 0);
+            if (queued) {
+                const unsigned int k = base + __popcll(qmask & ((1ull << lane) - 1ull));
+                a.list[2 * k] = (int)opix[q];
+                a.list[2 * k + 1] = t * NPIX + tid + q * NTW;   // weight slot
+            }
+        }
+        if (valid[q] && !queued)
             nl_estimate(ws, acc[q][2], acc[q][3], acc[q][4], opix[q], a.old_ref, a.old_pha,
                         a.old_coh, a.new_ref, a.new_pha, a.new_coh);
-        }
     }
 }
 
@@ -287,6 +419,7 @@ struct SmoothArgs {
     int wsh;
     int lmin;
     int H, W, eh, ew;
+    int npix;           // weight slots per block (TY * TX of the weights kernel)
     const float* wt;
     const float* old_ref;
     const float* old_pha;
@@ -294,7 +427,7 @@ struct SmoothArgs {
     float* new_ref;
     float* new_pha;
     float* new_coh;
-    const int* list;
+    const int* list;    // (output pixel, weight slot) pairs
     unsigned int* cnt;  // [0] queued this iteration, [1] running total
 };
 
@@ -364,21 +497,25 @@ __global__ void nlinsar_smooth(const SmoothArgs a)
     unsigned int* mask = (unsigned int*)(hi + a.lmin);
     const unsigned int total = a.cnt[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
-    const long long HW = (long long)a.H * a.W;
     for (unsigned int li = blockIdx.x * blockDim.x + threadIdx.x; li < total;
          li += gridDim.x * blockDim.x) {
-        const long long opix = a.list[li];
+        const long long opix = a.list[2 * li];
+        const int slot = a.list[2 * li + 1];
+        // the pixel's weights: plane k at wts[k * npix]
+        const float* wts = a.wt + ((long long)(slot / a.npix) * nfull * a.npix + slot % a.npix);
         const int oy = (int)(opix / a.W), ox = (int)(opix - (long long)oy * a.W);
         const int vy = oy == a.H - 1 ? oy + a.eh : oy;
         const int vx = ox == a.W - 1 ? ox + a.ew : ox;
-        const double as = a.R.base[a.R.at(vy, vx)];
+        const float4* R4 = reinterpret_cast<const float4*>(a.R.base);
+        const float4* F4 = reinterpret_cast<const float4*>(a.F.base);
+        const double as = R4[a.R.at(vy, vx)].x;
         int n = 0;
         for (int hh = 0; hh < sw; hh++)
             for (int ww = 0; ww < sw; ww++) {
-                const float at = a.R.base[a.R.at(vy + hh - a.wsh, vx + ww - a.wsh)];
+                const float at = R4[a.R.at(vy + hh - a.wsh, vx + ww - a.wsh)].x;
                 if ((double)at < 2.0 * as) {
                     const int idx = hh * sw + ww;
-                    const float w = a.wt[(long long)idx * HW + opix];
+                    const float w = wts[(long long)idx * a.npix];
                     if (n < a.lmin) {
                         hp_push(hw, hi, n, w, idx);
                     } else if (hw[0] < w) {
@@ -398,17 +535,14 @@ __global__ void nlinsar_smooth(const SmoothArgs a)
         }
         wsum /= (float)m;
         float N = 0.f, at_ = 0.f, xr = 0.f, xi = 0.f;
-        const float* Fa = a.F.base;
-        const float* Fxr = Fa + a.F.plane;
-        const float* Fxi = Fa + 2 * a.F.plane;
         for (int k = 0; k < nfull; k++) {
-            const float w = ((mask[k >> 5] >> (k & 31)) & 1u) ? wsum : a.wt[(long long)k * HW + opix];
+            const float w = ((mask[k >> 5] >> (k & 31)) & 1u) ? wsum : wts[(long long)k * a.npix];
             const int hh = k / sw, ww = k - (k / sw) * sw;
-            const long long f = a.F.at(vy + hh - a.wsh, vx + ww - a.wsh);
+            const float4 f = F4[a.F.at(vy + hh - a.wsh, vx + ww - a.wsh)];
             N += w;
-            at_ = fmaf(w, Fa[f], at_);
-            xr = fmaf(w, Fxr[f], xr);
-            xi = fmaf(w, Fxi[f], xi);
+            at_ = fmaf(w, f.x, at_);
+            xr = fmaf(w, f.y, xr);
+            xi = fmaf(w, f.z, xi);
         }
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
@@ -419,6 +553,37 @@ static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 static const int kTY[3] = {27, 18, 54};
 static const int kTX[3] = {54, 54, 54};
+
+// Block grid of the weights kernel over the four launch mappings: reference tiles covered,
+// times sub-tiles per reference tile; the one-row / one-column mappings use a single block
+// row / column.  Returns the number of blocks.
+static int nlinsar_grid(int step, int TY, int TX, const Launch* L, int* gx, int* nsx, int* nsy,
+                        int* bstart)
+{
+    int total = 0;
+    for (int i = 0; i < 4; i++) {
+        bstart[i] = total;
+        if (L[i].ny <= 0 || L[i].nx <= 0) {
+            gx[i] = nsx[i] = nsy[i] = 1;
+            continue;
+        }
+        nsy[i] = L[i].ny == 1 ? 1 : (step + TY - 1) / TY;
+        nsx[i] = L[i].nx == 1 ? 1 : (step + TX - 1) / TX;
+        const int gy = L[i].ny == 1 ? 1 : ((L[i].ny + step - 1) / step) * nsy[i];
+        gx[i] = L[i].nx == 1 ? 1 : ((L[i].nx + step - 1) / step) * nsx[i];
+        total += gx[i] * gy;
+    }
+    bstart[4] = total;
+    return total;
+}
+
+static void nlinsar_mappings(int H, int W, int eh, int ew, Launch* L)
+{
+    L[0] = Launch{0, 0, H - 1, W - 1, 0, 0};
+    L[1] = Launch{H - 1 + eh, 0, 1, W - 1, H - 1, 0};
+    L[2] = Launch{0, W - 1 + ew, H - 1, 1, 0, W - 1};
+    L[3] = Launch{H - 1 + eh, W - 1 + ew, 1, 1, H - 1, W - 1};
+}
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
 {
@@ -466,55 +631,66 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t q = off; off += (bytes + 255) / 256 * 256; return q; };
     const size_t n = (size_t)H * W;
-    plan->off_R = take(sizeof(float) * 3 * plan->R.plane);
-    plan->off_E = take(sizeof(float) * 3 * plan->E.plane);
-    plan->off_F = take(sizeof(float) * 3 * plan->F.plane);
+    // kernels address the float4 grids with 32-bit element offsets
+    if (plan->R.plane >= (1ll << 30) || plan->F.plane >= (1ll << 30)) {
+        set_error("nlinsar: image too large for one call (split it into bands)");
+        return -4;
+    }
+    plan->off_R = take(sizeof(float) * 4 * plan->R.plane);
+    plan->off_E = take(sizeof(float) * 4 * plan->E.plane);
+    plan->off_F = take(sizeof(float) * 4 * plan->F.plane);
     plan->off_est = take(sizeof(float) * 6 * n);
-    plan->off_w = take(sizeof(float) * (size_t)nfull * n);
-    plan->off_list = take(sizeof(int) * n);
-    plan->off_cnt = take(sizeof(unsigned int) * 2);
+    Launch L[4];
+    nlinsar_mappings(H, W, plan->eh, plan->ew, L);
+    int gx[4], nsx[4], nsy[4], bstart[5];
+    plan->blocks = nlinsar_grid(plan->step, plan->TY, plan->TX, L, gx, nsx, nsy, bstart);
+    const size_t slots = (size_t)plan->blocks * plan->TY * plan->TX;
+    plan->off_w = take(sizeof(float) * (size_t)nfull * slots);
+    plan->off_s = take(sizeof(float) * (size_t)nfull * slots);
+    plan->off_list = take(2 * sizeof(int) * n);
+    plan->off_cnt = take(sizeof(unsigned int) * 4);
     plan->ws_bytes = off;
     return 0;
 }
 
-template <int TY, int TX>
-static int launch_weights(const WeightArgs& base, const Launch* L, int nl, hipStream_t stream)
+template <int TY, int TX, int PSH, bool FIRST>
+static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t stream)
 {
-    const int np2 = 2 * base.psh;
-    const size_t lds = sizeof(float) * (2 * (size_t)(TY + np2) * (TX + np2) + 2 * (size_t)(TY + np2) * TX);
+    constexpr int np2 = 2 * PSH;
+    const int o = base.wsh + PSH;
+    const size_t lds = sizeof(float) * (4 * (size_t)(TY + 2 * o) * (TX + 2 * o) +
+                                        (FIRST ? 2 : 1) * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, false>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, PSH, FIRST>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    for (int i = 0; i < nl; i++) {
-        if (L[i].ny <= 0 || L[i].nx <= 0) continue;
-        WeightArgs a = base;
-        a.L = L[i];
-        // reference tiles covered, times sub-tiles per reference tile; thin launches
-        // (one row / column) use a single block row / column
-        const int nsy = (base.step + TY - 1) / TY, nsx = (base.step + TX - 1) / TX;
-        const int gy = L[i].ny == 1 ? 1 : ((L[i].ny + base.step - 1) / base.step) * nsy;
-        const int gx = L[i].nx == 1 ? 1 : ((L[i].nx + base.step - 1) / base.step) * nsx;
-        dim3 grid(gx, gy);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        const bool timed = i == 0 && timing_on();
-        if (timed) {
-            DKX_HIP_CHECK(hipEventCreate(&e0));
-            DKX_HIP_CHECK(hipEventCreate(&e1));
-            DKX_HIP_CHECK(hipEventRecord(e0, stream));
-        }
-        if (i == 0)
-            hipLaunchKernelGGL((nlinsar_weights<TY, TX, false>), grid, dim3(NTW), lds, stream, a);
-        else
-            hipLaunchKernelGGL((nlinsar_weights<TY, TX, true>), grid, dim3(NTW), lds, stream, a);
-        DKX_HIP_CHECK(hipGetLastError());
-        if (timed) {
-            DKX_HIP_CHECK(hipEventRecord(e1, stream));
-            timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
-        }
+    WeightArgs a = base;
+    for (int i = 0; i < 4; i++) a.L[i] = L[i];
+    const int total = nlinsar_grid(base.step, TY, TX, L, a.gx, a.nsx, a.nsy, a.bstart);
+    if (total <= 0) return 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timing_on();
+    if (timed) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    hipLaunchKernelGGL((nlinsar_weights<TY, TX, PSH, FIRST>), dim3(total), dim3(NTW), lds, stream, a);
+    DKX_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
     }
     return 0;
+}
+
+template <int PSH, bool FIRST>
+static int launch_weights_tile(int tile, const WeightArgs& a, const Launch* L, hipStream_t stream)
+{
+    switch (tile) {
+    case 0: return launch_weights<27, 54, PSH, FIRST>(a, L, stream);
+    case 1: return launch_weights<18, 54, PSH, FIRST>(a, L, stream);
+    default: return launch_weights<54, 54, PSH, FIRST>(a, L, stream);
+    }
 }
 
 int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* d_a1,
@@ -533,10 +709,13 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     int* list = (int*)(ws + plan.off_list);
     unsigned int* cnt = (unsigned int*)(ws + plan.off_cnt);
 
-    DKX_HIP_CHECK(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned int), stream));
+    unsigned int* flag = cnt + 2;
+    float* scache = (float*)(ws + plan.off_s);
+    hipLaunchKernelGGL(nlinsar_clear_counters, dim3(1), dim3(64), 0, stream, cnt, flag, 1);
+    DKX_HIP_CHECK(hipGetLastError());
     {
         dim3 blk(256), grd((R.w + 255) / 256, R.h);
-        hipLaunchKernelGGL(grid_fill, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R);
+        hipLaunchKernelGGL(grid_fill4, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R);
         DKX_HIP_CHECK(hipGetLastError());
         dim3 grf((F.w + 255) / 256, F.h);
         hipLaunchKernelGGL(nlinsar_filter_values, grf, blk, 0, stream, R, F);
@@ -547,10 +726,7 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     float* nxt[3] = {est, est + n, est + 2 * n};
     float* alt[3] = {est + 3 * n, est + 4 * n, est + 5 * n};
     Launch L[4];
-    L[0] = Launch{0, 0, H - 1, W - 1, 0, 0};
-    L[1] = Launch{H - 1 + plan.eh, 0, 1, W - 1, H - 1, 0};
-    L[2] = Launch{0, W - 1 + plan.ew, H - 1, 1, 0, W - 1};
-    L[3] = Launch{H - 1 + plan.eh, W - 1 + plan.ew, 1, 1, H - 1, W - 1};
+    nlinsar_mappings(H, W, plan.eh, plan.ew, L);
     int hdev = 0;
     DKX_HIP_CHECK(hipGetDevice(&hdev));
     int ncu = 256;
@@ -558,31 +734,49 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     for (int it = 0; it < pb.niter; it++) {
         {
             dim3 blk(256), grd((E.w + 255) / 256, E.h);
-            hipLaunchKernelGGL(grid_fill, grd, blk, 0, stream, (const float*)cur[0],
+            hipLaunchKernelGGL(nlinsar_fill_estimates, grd, blk, 0, stream, (const float*)cur[0],
                                (const float*)cur[1], (const float*)cur[2], H, W, E);
             DKX_HIP_CHECK(hipGetLastError());
         }
-        DKX_HIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned int), stream));
+        hipLaunchKernelGGL(nlinsar_clear_counters, dim3(1), dim3(64), 0, stream, cnt, flag, 0);
+        DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_any_estimate, dim3(std::max(1, std::min(ncu * 4, (int)((n + 255) / 256)))),
+                           dim3(256), 0, stream, (const float*)cur[0], (long long)n, flag);
+        DKX_HIP_CHECK(hipGetLastError());
         WeightArgs wa;
-        wa.R = R; wa.E = E; wa.F = F;
-        wa.wsh = plan.wsh; wa.psh = plan.psh; wa.step = plan.step;
+        wa.R4 = reinterpret_cast<const float4*>(R.base);
+        wa.E4 = reinterpret_cast<const float4*>(E.base);
+        wa.F4 = reinterpret_cast<const float4*>(F.base);
+        wa.rpitch = R.pitch;
+        wa.rorg = R.oy * R.pitch + R.ox;
+        wa.fpitch = F.pitch;
+        wa.forg = F.oy * F.pitch + F.ox;
+        wa.wsh = plan.wsh; wa.step = plan.step;
         wa.wsim = pb.ref_tile - pb.sw + 1;
-        wa.h = pb.h; wa.T = pb.T; wa.lmin = (float)pb.lmin;
+        wa.rh = 1.0f / pb.h; wa.rT = 1.0f / pb.T; wa.lmin = (float)pb.lmin;
         wa.H = H; wa.W = W;
+        wa.scache = scache;
         wa.wt = wt;
+        wa.kl_on = flag;
         wa.old_ref = cur[0]; wa.old_pha = cur[1]; wa.old_coh = cur[2];
         wa.new_ref = nxt[0]; wa.new_pha = nxt[1]; wa.new_coh = nxt[2];
         wa.list = list; wa.cnt = cnt;
         int rc;
-        switch (plan.tile) {
-        case 0: rc = launch_weights<27, 54>(wa, L, 4, stream); break;
-        case 1: rc = launch_weights<18, 54>(wa, L, 4, stream); break;
-        default: rc = launch_weights<54, 54>(wa, L, 4, stream); break;
+        if (plan.psh == 1)
+            rc = it == 0 ? launch_weights_tile<1, true>(plan.tile, wa, L, stream)
+                         : launch_weights_tile<1, false>(plan.tile, wa, L, stream);
+        else if (plan.psh == 3)
+            rc = it == 0 ? launch_weights_tile<3, true>(plan.tile, wa, L, stream)
+                         : launch_weights_tile<3, false>(plan.tile, wa, L, stream);
+        else {
+            set_error("nlinsar: only patch sizes 3 and 7 are supported");
+            rc = -4;
         }
         if (rc) return rc;
         SmoothArgs sa;
         sa.R = R; sa.F = F; sa.wsh = plan.wsh; sa.lmin = pb.lmin;
         sa.H = H; sa.W = W; sa.eh = plan.eh; sa.ew = plan.ew;
+        sa.npix = plan.TY * plan.TX;
         sa.wt = wt;
         sa.old_ref = cur[0]; sa.old_pha = cur[1]; sa.old_coh = cur[2];
         sa.new_ref = nxt[0]; sa.new_pha = nxt[1]; sa.new_coh = nxt[2];
