@@ -17,21 +17,6 @@
 
 namespace dkx {
 
-// Replicates an H x W image set (three planes) into a padded grid of float4 (the fourth
-// component is 0), clamping coordinates like the reference's tile padding
-// (src/data/sub_images.cpp:37-40).
-__global__ void grid_fill4(const float* __restrict__ p0, const float* __restrict__ p1,
-                           const float* __restrict__ p2, int H, int W, Grid G)
-{
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gy = blockIdx.y;
-    if (gx >= G.w || gy >= G.h) return;
-    const int y = min(H - 1, max(0, gy - G.oy));
-    const int x = min(W - 1, max(0, gx - G.ox));
-    const long long i = (long long)y * W + x;
-    reinterpret_cast<float4*>(G.base)[(long long)gy * G.pitch + gx] = make_float4(p0[i], p1[i], p2[i], 0.f);
-}
-
 // The estimates as the KL divergence uses them (nlinsar/sim_measures.h:42-49): per element
 // of the padded grid E, float4 (ref, coh * cos phase, coh * sin phase,
 // u = 1 / (ref * (1 - coh^2))).
@@ -49,16 +34,32 @@ __global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const floa
     reinterpret_cast<float4*>(E.base)[o] = make_float4(r, c * cosf(p), c * sinf(p), 1.0f / (r * (1.0f - c * c)));
 }
 
-// precompute_filter_values.cl:19-39: a = (a1^2 + a2^2)/2, x = a1*a2*exp(-i*phase)
+// The raw data on the padded grid R: float4 (a1, a2, cos phase, sin phase), clamped like the
+// reference's tile padding (src/data/sub_images.cpp:37-40).
+__global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __restrict__ a2,
+                                 const float* __restrict__ phi, int H, int W, Grid R)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= R.w || gy >= R.h) return;
+    const int y = min(H - 1, max(0, gy - R.oy));
+    const int x = min(W - 1, max(0, gx - R.ox));
+    const long long i = (long long)y * W + x;
+    const float p = phi[i];
+    reinterpret_cast<float4*>(R.base)[(long long)gy * R.pitch + gx] = make_float4(a1[i], a2[i], cosf(p), sinf(p));
+}
+
+// precompute_filter_values.cl:19-39: a = (a1^2 + a2^2)/2, x = a1*a2*exp(-i*phase), with
+// cos(-phase) = cos(phase) and sin(-phase) = -sin(phase)
 __global__ void nlinsar_filter_values(Grid R, Grid F)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
     if (gx >= F.w || gy >= F.h) return;
     const float4 r = reinterpret_cast<const float4*>(R.base)[R.at(gy - F.oy, gx - F.ox)];
-    const float a1 = r.x, a2 = r.y, dp = r.z;
+    const float a1 = r.x, a2 = r.y;
     reinterpret_cast<float4*>(F.base)[(long long)gy * F.pitch + gx] =
-        make_float4(0.5f * (a1 * a1 + a2 * a2), a1 * a2 * cosf(-dp), a1 * a2 * sinf(-dp), 0.f);
+        make_float4(0.5f * (a1 * a1 + a2 * a2), a1 * a2 * r.z, a1 * a2 * -r.w, 0.f);
 }
 
 // flag[0] = 1 if any reflectivity estimate is nonzero: otherwise every KL divergence is
@@ -80,16 +81,20 @@ __global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, in
     }
 }
 
-// pixel_similarity, nlinsar/sim_measures.h:27-40
-__device__ __forceinline__ float nl_pixel_similarity(float am1, float as1, float dp1, float am2,
-                                                     float as2, float dp2)
+// pixel_similarity (nlinsar/sim_measures.h:27-40) from raw records (a1, a2, cos phase,
+// sin phase): cos(dp1 - dp2) = cos1*cos2 + sin1*sin2 and pow(q, 1.5) = q * sqrt(q), each a
+// few ulps from the reference's expression (inside OpenCL's bounds for cos and pow); the
+// other operations in the reference's order.
+__device__ __forceinline__ float nl_pixel_similarity_pre(float4 r1, float4 r2)
 {
+    const float am1 = r1.x, as1 = r1.y, am2 = r2.x, as2 = r2.y;
+    const float cs = fmaf(r1.z, r2.z, r1.w * r2.w);
     const float s = am1 * am1 + as1 * as1 + am2 * am2 + as2 * as2;
     const float A = s * s;
-    const float B = 4.0f * (am1 * am1 * as1 * as1 + am2 * am2 * as2 * as2 +
-                            2.0f * am1 * as1 * am2 * as2 * cosf(dp1 - dp2));
+    const float B = 4.0f * (am1 * am1 * as1 * as1 + am2 * am2 * as2 * as2 + 2.0f * am1 * as1 * am2 * as2 * cs);
     const float C = am1 * as1 * am2 * as2;
-    const float f_el = powf(C / B, 1.5f) * ((A + B) / A * sqrtf(B / (A - B)) - asinf(sqrtf(B / A)));
+    const float q = C / B;
+    const float f_el = q * sqrtf(q) * ((A + B) / A * sqrtf(B / (A - B)) - asinf(sqrtf(B / A)));
     if ((f_el == 0) || isnan(f_el)) return 0.f;
     return logf(f_el);
 }
@@ -141,7 +146,7 @@ __device__ __forceinline__ void nl_estimate(float N, float at, float xr, float x
 }
 
 struct WeightArgs {
-    const float4* R4;     // raw data: a1, a2, phase (padded by the overlap)
+    const float4* R4;     // raw data: a1, a2, cos phase, sin phase (padded by the overlap)
     const float4* E4;     // estimates, see nlinsar_fill_estimates (same grid)
     const float4* F4;     // filter values: a, Re x, Im x (padded by the window)
     int rpitch, rorg;     // rorg: element of image position (0, 0)
@@ -300,7 +305,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                         const int s1 = sorg + y1 * EW + x1, s2 = sorg + y2 * EW + x2;
                         if (FIRST) {
                             const float4 u = stg[s1], v = stg[s2];
-                            sS[r * PX + c] = nl_pixel_similarity(u.x, u.y, u.z, v.x, v.y, v.z);
+                            sS[r * PX + c] = nl_pixel_similarity_pre(u, v);
                             if (kl) {
                                 // nonzero initial estimates: read them from the grid
                                 const unsigned int p1 = (unsigned int)(a.rorg + y1 * a.rpitch + x1);
@@ -715,7 +720,7 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     DKX_HIP_CHECK(hipGetLastError());
     {
         dim3 blk(256), grd((R.w + 255) / 256, R.h);
-        hipLaunchKernelGGL(grid_fill4, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R);
+        hipLaunchKernelGGL(nlinsar_fill_raw, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R);
         DKX_HIP_CHECK(hipGetLastError());
         dim3 grf((F.w + 255) / 256, F.h);
         hipLaunchKernelGGL(nlinsar_filter_values, grf, blk, 0, stream, R, F);
