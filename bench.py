@@ -206,6 +206,101 @@ def nlinsar_form(args):
     print(json.dumps(out), flush=True)
 
 
+def served(args):
+    """Served stream: requests 70 % NL-SAR (the parameters above) / 30 % NL-InSAR (search
+    window 21, patch 7, 5 iterations, lmin 10), sides log-uniform in [128, 4096] (aspect <= 4:1),
+    each a seeded crop of one of four 4096^2 interferograms of the same generator, resident on
+    the device.  Runs (1) all requests at once through the server (aggregate Mpixel/s),
+    (2) the same requests one after another through the plain per-call entry points, (3) an
+    open-loop Poisson stream at --rate requests/s through the server (p50 / p99 latency)."""
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    stats = D.load_nlsar_stats_collection(os.path.join(ROOT, "tests", "golden", "train32_stats.txt"))
+    n = args.requests
+    rng = np.random.Generator(np.random.PCG64(S.SEED_BASE + 4))
+    kinds = (rng.random(n) < 0.3).astype(int)
+    shapes = S.tile_shapes(n, S.SEED_BASE + 4, lo=128, hi=4096)
+    scenes = []
+    for k in range(4):
+        host = S.interferogram(4096, 4096, S.SEED_BASE + 40 + k, n_regions=16, coh_range=(0.2, 0.95))
+        scenes.append([HIP.DeviceArray.from_host(x) for x in host])
+        del host
+    reqs, bufs, px = [], [], 0
+    for i in range(n):
+        h, w = shapes[i]
+        sc = scenes[i % 4]
+        y0, x0 = int(rng.integers(0, 4096 - h + 1)), int(rng.integers(0, 4096 - w + 1))
+        inp = [HIP.DeviceArray((h, w)) for _ in range(3)]
+        for d, src in zip(inp, sc):
+            HIP.copy2d(d.ptr, w * 4, src.ptr + (y0 * 4096 + x0) * 4, 4096 * 4, w * 4, h)
+        out = [HIP.DeviceArray((h, w)) for _ in range(3)]
+        bufs.append((inp, out))
+        reqs.append((int(kinds[i]), *[d.ptr for d in inp], *[d.ptr for d in out], h, w))
+        px += h * w
+    HIP.synchronize()
+    log(f"served: {n} requests, {px / 1e6:.0f} Mpixel, {int(kinds.sum())} NL-InSAR")
+    px_sar = sum(h * w for (k, *_r, h, w) in reqs if k == 0)
+
+    srv = DV.Server(args.budget_gib << 30, SW, PATCHES, SCALES, stats, HPAR, CPAR, 21, 7, 5, 10)
+    srv.run_trace(reqs[:8], [0.0] * 8)       # warm-up
+    lat_sat, span_sat = srv.run_trace(reqs, [0.0] * n)
+    b_sat = srv.counters()
+    log(f"served: saturated {span_sat:.2f} s")
+    gaps = rng.exponential(1.0 / args.rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    lat, span = srv.run_trace(reqs, arrivals.tolist())
+    b_all = srv.counters()
+    srv.close()
+    log(f"served: Poisson stream {span:.2f} s")
+
+    stream = HIP.Stream()
+    t0 = time.perf_counter()
+    for (k, a1, a2, ph, r1, r2, r3, h, w) in reqs:
+        if k == 0:
+            DV.nlsar_insar(a1, a2, ph, r1, r2, r3, h, w, SW, PATCHES, SCALES, stats, HPAR, CPAR, stream=stream.handle)
+        else:
+            HIP.hip().hipMemsetAsync(C_void(r1), 0, C_size(h * w * 4), C_void(stream.handle))
+            HIP.hip().hipMemsetAsync(C_void(r2), 0, C_size(h * w * 4), C_void(stream.handle))
+            HIP.hip().hipMemsetAsync(C_void(r3), 0, C_size(h * w * 4), C_void(stream.handle))
+            DV.nlinsar(a1, a2, ph, r1, r2, r3, h, w, 21, 7, 5, 10, stream=stream.handle)
+        stream.synchronize()
+    serial = time.perf_counter() - t0
+    D.lib().dkx_release_workspace()
+    lat = np.array(lat)
+    out = {
+        "metric": "served mixed stream: p50 / p99 latency and aggregate Mpixel/s (1 GPU)",
+        "value": round(float(np.percentile(lat, 99)), 3), "unit": "ms (p99 latency)", "n_gpus": 1,
+        "higher_is_better": False, "dtype": "f32",
+        "data": "synthetic (seeded crops of four 4096^2 interferograms of the insarsim_simu.cpp speckle model)",
+        "config": {"workload": "served stream, 70% NL-SAR / 30% NL-InSAR (5 iterations)", "requests": n,
+                   "pixels": px, "nlsar_pixels": px_sar, "rate_req_s": args.rate,
+                   "hbm_budget_gib": args.budget_gib, "sides": [128, 4096]},
+        "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_latency_ms": round(float(lat.mean()), 3),
+        "stream_seconds": round(span, 3),
+        "saturated": {"seconds": round(span_sat, 3), "mpx_s": round(px / 1e6 / span_sat, 3),
+                      "p50_latency_ms": round(float(np.percentile(lat_sat, 50)), 3)},
+        "serial_plain": {"seconds": round(serial, 3), "mpx_s": round(px / 1e6 / serial, 3)},
+        "speedup_vs_serial": round(serial / span_sat, 3),
+        "nlsar_batches": b_all[0] - b_sat[0], "nlsar_batched_images": b_all[1] - b_sat[1],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def C_void(x):
+    import ctypes
+    return ctypes.c_void_p(x)
+
+
+def C_size(x):
+    import ctypes
+    return ctypes.c_size_t(x)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,7 +308,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
-    ap.add_argument("--form", choices=("plain", "batched", "nlinsar"), default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served"), default="plain")
+    ap.add_argument("--requests", type=int, default=2000, help="requests (--form served)")
+    ap.add_argument("--rate", type=float, default=40.0, help="Poisson arrival rate, requests/s (--form served)")
+    ap.add_argument("--budget-gib", type=int, default=150, help="server HBM budget (--form served)")
     ap.add_argument("--size", type=int, default=4096, help="image side (--form nlinsar)")
     ap.add_argument("--niter", type=int, default=10, help="iterations (--form nlinsar)")
     ap.add_argument("--tiles", type=int, default=64, help="images per batch (--form batched)")
@@ -226,6 +324,8 @@ def main():
         return batched(args)
     if args.form == "nlinsar":
         return nlinsar_form(args)
+    if args.form == "served":
+        return served(args)
 
     import despeckcl_amd as D
     from despeckcl_amd import device as DV

@@ -31,6 +31,12 @@ class DkxImage(C.Structure):
                 ("height", C.c_int), ("width", C.c_int)]
 
 
+class DkxRequest(C.Structure):
+    _fields_ = [("kind", C.c_int), ("ampl_master", C.c_void_p), ("ampl_slave", C.c_void_p),
+                ("phase", C.c_void_p), ("ref_filt", C.c_void_p), ("phase_filt", C.c_void_p),
+                ("coh_filt", C.c_void_p), ("height", C.c_int), ("width", C.c_int)]
+
+
 _P = C.c_void_p
 _I = C.c_int
 _F = C.c_float
@@ -65,6 +71,15 @@ SIGNATURES = {
     "dkx_nlsar_batch_get_tile": (_I, [_P]),
     "dkx_nlsar_batch_autotune": (_I, [_P, _P, _I, _IP, _P]),
     "dkx_nlsar_batch_ub_pixels": (_I, [_P, _ULLP]),
+    "dkx_server_create": (_I, [C.c_ulonglong, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F, _I, _I, _I, _I,
+                               C.POINTER(C.c_void_p)]),
+    "dkx_server_submit": (_I, [_P, C.POINTER(DkxRequest), C.c_double, C.POINTER(C.c_longlong)]),
+    "dkx_server_poll": (_I, [_P, C.c_longlong, C.POINTER(C.c_double)]),
+    "dkx_server_wait": (_I, [_P, C.c_longlong, C.POINTER(C.c_double)]),
+    "dkx_server_run_trace": (_I, [_P, C.POINTER(DkxRequest), C.POINTER(C.c_double), _I,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "dkx_server_counters": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "dkx_server_destroy": (None, [_P]),
 }
 
 _lib = None

@@ -22,7 +22,8 @@ BUILD_DIR = os.path.join(ROOT, "_build", "despeckcl_amd")
 ARCH = os.environ.get("DKX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["nlsar.hip", "nlinsar.hip", "capi.cpp", "batch.cpp", "stats.cpp", "despeckcl_api.cpp"] + [
+SOURCES = ["nlsar.hip", "nlinsar.hip", "server.hip", "capi.cpp", "batch.cpp", "stats.cpp",
+           "despeckcl_api.cpp"] + [
     f"nlsar_pass_{kind}_{shape}.hip" for kind in ("stat", "last") for shape in ("t32r4_w10", "t32r2_w10", "t32r1_w10", "t32r4_w20")]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

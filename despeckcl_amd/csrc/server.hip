@@ -1,0 +1,545 @@
+// Serving: a request loop around the filters.  Requests of mixed filter type and image size
+// arrive at any time; a scheduler thread admits them against a fixed HBM budget (one arena,
+// sub-allocated per request, so serving never calls hipMalloc), batches the NL-SAR requests
+// that are waiting into one launch per stage (continuous batching: a new batch forms as soon
+// as a batch lane and memory are free, while earlier work still runs), runs NL-InSAR
+// requests on streams of their own so that their work overlaps the batches, and records each
+// request's completion time from a host callback on its stream.  The weight LUTs and
+// Gaussian taps stay resident on the device for the server's lifetime.
+#include "../../include/despeckcl_amd.h"
+
+#include "dkx_common.h"
+#include "dkx_nlinsar.h"
+#include "dkx_nlsar.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace dkx {
+
+int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
+                int nS, std::vector<NlsarStatsView>& views);
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over one device allocation (offsets aligned to 256 bytes).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    std::map<size_t, size_t> free_;   // offset -> size
+    void reset(char* b, size_t c)
+    {
+        base = b;
+        cap = c;
+        free_.clear();
+        free_[0] = c;
+    }
+    bool alloc(size_t n, size_t* off)
+    {
+        n = (n + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const size_t rest = it->second - n, o = it->first;
+            free_.erase(it);
+            if (rest) free_[o + n] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = (n + 255) / 256 * 256;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Request {
+    long long id;
+    dkx_request r;
+    double t_arrive = 0, t_done = -1;
+    int status = 0;   // 0 waiting, 1 on the GPU, 2 done, < 0 failed
+    size_t off = 0, bytes = 0;
+};
+
+struct Server;
+
+// One unit of GPU work: an NL-SAR batch or one NL-InSAR request.
+struct Flight {
+    Server* srv = nullptr;
+    int kind = 0, lane = 0;
+    std::vector<Request*> reqs;
+    size_t table_off = 0, table_bytes = 0;
+    std::atomic<bool> done{false};
+    std::atomic<double> t_done{0.0};
+};
+
+constexpr int SAR_LANES = 2;
+constexpr int NLI_LANES = 4;
+constexpr int MAX_BATCH = 256;
+
+struct Server {
+    // NL-SAR parameters (owned copies) and resident constants
+    std::vector<int> P, S;
+    std::vector<std::vector<float>> lut_q, lut_c;
+    std::vector<NlsarStatsView> views;
+    NlsarParams pp{};
+    NlsarCommon cm{};
+    float* consts = nullptr;
+    // NL-InSAR parameters
+    int nl_sw = 21, nl_p = 7, nl_niter = 5, nl_lmin = 10;
+    float nl_h = 0.f, nl_T = 0.f;
+    // device memory and streams
+    Arena arena;
+    void* arena_ptr = nullptr;
+    hipStream_t sar_stream[SAR_LANES] = {};
+    hipStream_t nli_stream[NLI_LANES] = {};
+    NlsarImageDesc* pinned_table[SAR_LANES] = {};
+    Flight* sar_busy[SAR_LANES] = {};
+    Flight* nli_busy[NLI_LANES] = {};
+    // queues
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request*> incoming;
+    std::deque<Request*> wait_sar, wait_nli;
+    std::unordered_map<long long, Request*> reqs;
+    long long next_id = 1;
+    std::atomic<bool> stop{false};
+    std::atomic<long long> n_done{0};
+    std::thread sched;
+    std::string sched_error;
+    // counters
+    long long batches = 0, batched_images = 0;
+};
+
+static void host_done(void* p)
+{
+    Flight* f = (Flight*)p;
+    f->t_done.store(now_s());
+    f->done.store(true);
+    f->srv->cv.notify_all();
+}
+
+static int nlinsar_params(int p, float* h, float* T)
+{
+    float h_theo;
+    if (p == 3) h_theo = 0.58f;
+    else if (p == 7) h_theo = 0.244f;
+    else return -1;
+    const int area = p * p;
+    *h = h_theo * area;
+    *T = (float)(2.0 / *h * M_PI / 4 * area);
+    return 0;
+}
+
+static void fail_request(Server* s, Request* r, const std::string& msg)
+{
+    r->status = -1;
+    r->t_done = now_s();
+    s->sched_error = msg;
+    s->n_done.fetch_add(1);
+}
+
+// retire finished flights: free their memory, stamp their requests
+static void retire(Server* s)
+{
+    auto fin = [&](Flight*& f) {
+        if (!f || !f->done.load()) return;
+        for (Request* r : f->reqs) {
+            r->t_done = f->t_done.load();
+            r->status = 2;
+            s->arena.release(r->off, r->bytes);
+            s->n_done.fetch_add(1);
+        }
+        if (f->table_bytes) s->arena.release(f->table_off, f->table_bytes);
+        delete f;
+        f = nullptr;
+    };
+    for (int i = 0; i < SAR_LANES; i++) fin(s->sar_busy[i]);
+    for (int i = 0; i < NLI_LANES; i++) fin(s->nli_busy[i]);
+}
+
+static void launch_sar(Server* s, int lane)
+{
+    if (s->wait_sar.empty()) return;
+    Flight* f = new Flight();
+    f->srv = s;
+    f->kind = 0;
+    f->lane = lane;
+    std::vector<NlsarImageDesc> descs;
+    while (!s->wait_sar.empty() && (int)f->reqs.size() < MAX_BATCH) {
+        Request* r = s->wait_sar.front();
+        NlsarImagePlan ip;
+        if (nlsar_image_plan(s->cm, r->r.height, r->r.width, -1, -1, &ip)) {
+            s->wait_sar.pop_front();
+            fail_request(s, r, last_error());
+            continue;
+        }
+        size_t off;
+        if (!s->arena.alloc(ip.bytes, &off)) break;   // wait for memory to free up
+        s->wait_sar.pop_front();
+        r->off = off;
+        r->bytes = ip.bytes;
+        r->status = 1;
+        NlsarImageDesc d;
+        nlsar_image_desc(ip, s->arena.base + off, r->r.ampl_master, r->r.ampl_slave, r->r.phase, nullptr,
+                         r->r.ref_filt, r->r.phase_filt, r->r.coh_filt, nullptr, nullptr, &d);
+        descs.push_back(d);
+        f->reqs.push_back(r);
+    }
+    if (f->reqs.empty()) {
+        delete f;
+        return;
+    }
+    const int n = (int)descs.size();
+    int rows_c, rows_a, blocks, mcw, maw;
+    nlsar_batch_layout(descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
+    f->table_bytes = sizeof(NlsarImageDesc) * (size_t)n;
+    bool ok = s->arena.alloc(f->table_bytes, &f->table_off);
+    hipStream_t st = s->sar_stream[lane];
+    int rc = ok ? 0 : -2;
+    if (ok) {
+        memcpy(s->pinned_table[lane], descs.data(), f->table_bytes);
+        NlsarImageDesc* d_table = (NlsarImageDesc*)(s->arena.base + f->table_off);
+        if (hipMemcpyAsync(d_table, s->pinned_table[lane], f->table_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = -5;
+        if (!rc) rc = nlsar_enqueue(s->pp, s->cm, descs.data(), n, d_table, s->consts, st);
+        if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
+    } else {
+        f->table_bytes = 0;
+    }
+    if (rc) {
+        // nothing reached the GPU that still needs the memory once the stream drains
+        (void)hipStreamSynchronize(st);
+        for (Request* r : f->reqs) {
+            s->arena.release(r->off, r->bytes);
+            fail_request(s, r, ok ? last_error() : "serve: batch table does not fit");
+        }
+        if (f->table_bytes) s->arena.release(f->table_off, f->table_bytes);
+        delete f;
+        return;
+    }
+    s->batches++;
+    s->batched_images += n;
+    s->sar_busy[lane] = f;
+}
+
+__global__ void serve_zero3(float* a, float* b, float* c, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        a[i] = 0.f;
+        b[i] = 0.f;
+        c[i] = 0.f;
+    }
+}
+
+static void launch_nli(Server* s, int lane)
+{
+    if (s->wait_nli.empty()) return;
+    Request* r = s->wait_nli.front();
+    NlinsarProblem pb{r->r.height, r->r.width, s->nl_sw, s->nl_p, s->nl_niter, s->nl_lmin, s->nl_h, s->nl_T, 80, 1};
+    NlinsarPlan plan;
+    if (nlinsar_plan(pb, &plan)) {
+        s->wait_nli.pop_front();
+        fail_request(s, r, last_error());
+        return;
+    }
+    size_t off;
+    if (!s->arena.alloc(plan.ws_bytes, &off)) return;   // wait for memory to free up
+    s->wait_nli.pop_front();
+    r->off = off;
+    r->bytes = plan.ws_bytes;
+    r->status = 1;
+    Flight* f = new Flight();
+    f->srv = s;
+    f->kind = 1;
+    f->lane = lane;
+    f->reqs.push_back(r);
+    hipStream_t st = s->nli_stream[lane];
+    const long long n = (long long)r->r.height * r->r.width;
+    // the first iteration starts from zero estimates, like the Python API
+    hipLaunchKernelGGL(serve_zero3, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, st,
+                       r->r.ref_filt, r->r.phase_filt, r->r.coh_filt, n);
+    int rc = hipGetLastError() == hipSuccess ? 0 : -5;
+    if (!rc) rc = nlinsar_run(pb, plan, r->r.ampl_master, r->r.ampl_slave, r->r.phase, r->r.ref_filt,
+                              r->r.phase_filt, r->r.coh_filt, s->arena.base + off, nullptr, st);
+    if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        s->arena.release(r->off, r->bytes);
+        fail_request(s, r, last_error());
+        delete f;
+        return;
+    }
+    s->nli_busy[lane] = f;
+}
+
+static void scheduler(Server* s)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        while (!s->incoming.empty()) {
+            Request* r = s->incoming.front();
+            s->incoming.pop_front();
+            (r->r.kind == 0 ? s->wait_sar : s->wait_nli).push_back(r);
+        }
+        retire(s);
+        for (int i = 0; i < SAR_LANES; i++)
+            if (!s->sar_busy[i]) launch_sar(s, i);
+        for (int i = 0; i < NLI_LANES; i++)
+            if (!s->nli_busy[i]) launch_nli(s, i);
+        // a request larger than the whole budget can never be admitted
+        auto reject_oversized = [&](std::deque<Request*>& q, bool sar) {
+            bool idle = true;
+            for (int i = 0; i < SAR_LANES; i++) idle &= s->sar_busy[i] == nullptr;
+            for (int i = 0; i < NLI_LANES; i++) idle &= s->nli_busy[i] == nullptr;
+            if (!idle || q.empty()) return;
+            Request* r = q.front();
+            q.pop_front();
+            fail_request(s, r, sar ? "serve: NL-SAR request exceeds the HBM budget"
+                                   : "serve: NL-InSAR request exceeds the HBM budget");
+        };
+        reject_oversized(s->wait_sar, true);
+        reject_oversized(s->wait_nli, false);
+        if (s->stop.load() && s->incoming.empty() && s->wait_sar.empty() && s->wait_nli.empty()) {
+            bool busy = false;
+            for (int i = 0; i < SAR_LANES; i++) busy |= s->sar_busy[i] != nullptr;
+            for (int i = 0; i < NLI_LANES; i++) busy |= s->nli_busy[i] != nullptr;
+            if (!busy) break;
+        }
+        // completions notify without the lock, so wake up periodically as well
+        s->cv.wait_for(lk, std::chrono::microseconds(200));
+    }
+}
+
+}  // namespace dkx
+
+using namespace dkx;
+
+extern "C" {
+
+int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_size, const int* patch_sizes,
+                      int n_patch_sizes, const int* scale_sizes, int n_scale_sizes, const dkx_nlsar_stats* stats,
+                      int n_stats, float h_param, float c_param, int nl_search_window_size, int nl_patch_size,
+                      int nl_niter, int nl_lmin, dkx_server* out)
+{
+    if (!out) { set_error("serve: null handle pointer"); return DKX_EINVAL; }
+    if (!patch_sizes || !scale_sizes || n_patch_sizes < 1 || n_scale_sizes < 1) {
+        set_error("serve: empty patch or scale size list");
+        return DKX_EINVAL;
+    }
+    std::vector<NlsarStatsView> views;
+    int rc = check_stats(stats, n_stats, patch_sizes, n_patch_sizes, scale_sizes, n_scale_sizes, views);
+    if (rc) return rc;
+    float nh, nT;
+    if (nlinsar_params(nl_patch_size, &nh, &nT)) {
+        set_error("serve: NL-InSAR patch sizes 3 and 7 only");
+        return DKX_EUNSUPPORTED;
+    }
+    if (nl_niter < 0 || nl_lmin < 1 || nl_search_window_size < 3 || nl_search_window_size % 2 == 0) {
+        set_error("serve: invalid NL-InSAR parameters");
+        return DKX_EINVAL;
+    }
+    Server* s = new Server();
+    s->P.assign(patch_sizes, patch_sizes + n_patch_sizes);
+    s->S.assign(scale_sizes, scale_sizes + n_scale_sizes);
+    for (const NlsarStatsView& v : views) {
+        s->lut_q.emplace_back(v.quantilles, v.quantilles + v.lut_size);
+        s->lut_c.emplace_back(v.chi2cdf_inv, v.chi2cdf_inv + v.lut_size);
+    }
+    for (size_t k = 0; k < views.size(); k++) {
+        NlsarStatsView v = views[k];
+        v.quantilles = s->lut_q[k].data();
+        v.chi2cdf_inv = s->lut_c[k].data();
+        s->views.push_back(v);
+    }
+    s->pp = NlsarParams{0, search_window_size, s->P.data(), n_patch_sizes, s->S.data(), n_scale_sizes,
+                        s->views.data(), (int)s->views.size(), h_param, c_param, 1};
+    rc = nlsar_common(s->pp, &s->cm);
+    std::vector<float> blob;
+    if (!rc) rc = nlsar_build_constants(s->pp, s->cm, blob);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    s->nl_sw = nl_search_window_size;
+    s->nl_p = nl_patch_size;
+    s->nl_niter = nl_niter;
+    s->nl_lmin = nl_lmin;
+    s->nl_h = nh;
+    s->nl_T = nT;
+    auto cleanup = [&](const std::string& msg) {
+        set_error(msg);
+        if (s->consts) (void)hipFree(s->consts);
+        if (s->arena_ptr) (void)hipFree(s->arena_ptr);
+        for (int i = 0; i < SAR_LANES; i++) {
+            if (s->sar_stream[i]) (void)hipStreamDestroy(s->sar_stream[i]);
+            if (s->pinned_table[i]) (void)hipHostFree(s->pinned_table[i]);
+        }
+        for (int i = 0; i < NLI_LANES; i++)
+            if (s->nli_stream[i]) (void)hipStreamDestroy(s->nli_stream[i]);
+        delete s;
+        return DKX_ERESOURCE;
+    };
+    if (hipMalloc(&s->consts, blob.size() * sizeof(float)) != hipSuccess) return cleanup("serve: constants allocation failed");
+    if (hipMemcpy(s->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup("serve: constants upload failed");
+    if (hipMalloc(&s->arena_ptr, (size_t)hbm_budget_bytes) != hipSuccess)
+        return cleanup("serve: the HBM budget of " + std::to_string(hbm_budget_bytes) + " bytes cannot be allocated");
+    s->arena.reset((char*)s->arena_ptr, (size_t)hbm_budget_bytes);
+    for (int i = 0; i < SAR_LANES; i++) {
+        if (hipStreamCreateWithFlags(&s->sar_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
+        if (hipHostMalloc((void**)&s->pinned_table[i], sizeof(NlsarImageDesc) * MAX_BATCH, 0) != hipSuccess)
+            return cleanup("serve: pinned staging allocation failed");
+    }
+    for (int i = 0; i < NLI_LANES; i++)
+        if (hipStreamCreateWithFlags(&s->nli_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
+    s->sched = std::thread(scheduler, s);
+    *out = reinterpret_cast<dkx_server>(s);
+    return 0;
+}
+
+int dkx_server_submit(dkx_server h, const dkx_request* req, double arrival_time_s, long long* id)
+{
+    Server* s = reinterpret_cast<Server*>(h);
+    if (!s || !req) { set_error("serve: null argument"); return DKX_EINVAL; }
+    if ((req->kind != 0 && req->kind != 1) || req->height < 1 || req->width < 1 || !req->ampl_master ||
+        !req->ampl_slave || !req->phase || !req->ref_filt || !req->phase_filt || !req->coh_filt) {
+        set_error("serve: invalid request");
+        return DKX_EINVAL;
+    }
+    Request* r = new Request();
+    r->r = *req;
+    r->t_arrive = arrival_time_s > 0 ? arrival_time_s : now_s();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop.load()) {
+            delete r;
+            set_error("serve: server is shutting down");
+            return DKX_EINVAL;
+        }
+        r->id = s->next_id++;
+        s->reqs[r->id] = r;
+        s->incoming.push_back(r);
+    }
+    s->cv.notify_all();
+    if (id) *id = r->id;
+    return 0;
+}
+
+// 1 done (latency in ms from arrival to completion), 0 still pending, < 0 failed or unknown
+int dkx_server_poll(dkx_server h, long long id, double* latency_ms)
+{
+    Server* s = reinterpret_cast<Server*>(h);
+    if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { set_error("serve: unknown request"); return DKX_EINVAL; }
+    Request* r = it->second;
+    if (r->status == 2) {
+        if (latency_ms) *latency_ms = (r->t_done - r->t_arrive) * 1e3;
+        return 1;
+    }
+    if (r->status < 0) {
+        set_error(s->sched_error.empty() ? "serve: request failed" : s->sched_error);
+        return DKX_EHIP;
+    }
+    return 0;
+}
+
+int dkx_server_wait(dkx_server h, long long id, double* latency_ms)
+{
+    while (true) {
+        const int rc = dkx_server_poll(h, id, latency_ms);
+        if (rc != 0) return rc == 1 ? 0 : rc;
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+    }
+}
+
+int dkx_server_run_trace(dkx_server h, const dkx_request* reqs, const double* arrival_s, int n, double* latency_ms,
+                         double* makespan_s)
+{
+    Server* s = reinterpret_cast<Server*>(h);
+    if (!s || !reqs || !arrival_s || n < 0) { set_error("serve: null argument"); return DKX_EINVAL; }
+    std::vector<long long> ids(n);
+    const double t0 = now_s();
+    for (int i = 0; i < n; i++) {
+        const double ta = t0 + arrival_s[i];
+        while (true) {
+            const double dt = ta - now_s();
+            if (dt <= 0) break;
+            if (dt > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(dt - 1e-4));
+        }
+        const int rc = dkx_server_submit(h, &reqs[i], ta, &ids[i]);
+        if (rc) return rc;
+    }
+    int first_err = 0;
+    double last = t0;
+    for (int i = 0; i < n; i++) {
+        double lat = 0;
+        const int rc = dkx_server_wait(h, ids[i], &lat);
+        if (rc && !first_err) first_err = rc;
+        if (latency_ms) latency_ms[i] = rc ? -1.0 : lat;
+        last = std::max(last, t0 + arrival_s[i] + lat * 1e-3);
+    }
+    if (makespan_s) *makespan_s = last - t0;
+    return first_err;
+}
+
+int dkx_server_counters(dkx_server h, long long* batches, long long* batched_images)
+{
+    Server* s = reinterpret_cast<Server*>(h);
+    if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (batches) *batches = s->batches;
+    if (batched_images) *batched_images = s->batched_images;
+    return 0;
+}
+
+void dkx_server_destroy(dkx_server h)
+{
+    Server* s = reinterpret_cast<Server*>(h);
+    if (!s) return;
+    s->stop.store(true);
+    s->cv.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    for (int i = 0; i < SAR_LANES; i++) {
+        (void)hipStreamSynchronize(s->sar_stream[i]);
+        (void)hipStreamDestroy(s->sar_stream[i]);
+        (void)hipHostFree(s->pinned_table[i]);
+    }
+    for (int i = 0; i < NLI_LANES; i++) {
+        (void)hipStreamSynchronize(s->nli_stream[i]);
+        (void)hipStreamDestroy(s->nli_stream[i]);
+    }
+    for (auto& kv : s->reqs) delete kv.second;
+    (void)hipFree(s->arena_ptr);
+    (void)hipFree(s->consts);
+    delete s;
+}
+
+}  // extern "C"

@@ -108,3 +108,68 @@ class NlsarBatch:
         out = (C.c_ulonglong * max(1, self.n))()
         check(lib().dkx_nlsar_batch_ub_pixels(self._h, out))
         return [out[i] for i in range(self.n)]
+
+
+class Server:
+    """Request loop on the device (dkx_server_* in the C ABI): fixed HBM budget, resident
+    NL-SAR LUTs, continuous batching of NL-SAR requests, NL-InSAR requests overlapped on
+    streams of their own.  Requests carry device pointers:
+    (kind, a1, a2, phase, ref, phase_filt, coh, height, width), kind 0 NL-SAR, 1 NL-InSAR."""
+
+    def __init__(self, hbm_budget_bytes, search_window_size, patch_sizes, scale_sizes, nlsar_stats,
+                 h=15.0, c=49.0, nl_search_window_size=21, nl_patch_size=7, nl_niter=5, nl_lmin=10):
+        from . import _stats_structs
+        P, nP = int_array(patch_sizes)
+        S, nS = int_array(scale_sizes)
+        st, ns, keep = _stats_structs(nlsar_stats)
+        hdl = C.c_void_p()
+        check(lib().dkx_server_create(int(hbm_budget_bytes), int(search_window_size), P, nP, S, nS, st, ns,
+                                      float(h), float(c), int(nl_search_window_size), int(nl_patch_size),
+                                      int(nl_niter), int(nl_lmin), C.byref(hdl)))
+        self._h = hdl
+
+    @staticmethod
+    def _req(r):
+        from ._lib import DkxRequest
+        kind, a1, a2, ph, ref, phf, coh, hh, ww = r
+        return DkxRequest(int(kind), a1, a2, ph, ref, phf, coh, int(hh), int(ww))
+
+    def submit(self, request):
+        i = C.c_longlong(0)
+        check(lib().dkx_server_submit(self._h, C.byref(self._req(request)), 0.0, C.byref(i)))
+        return i.value
+
+    def wait(self, rid):
+        lat = C.c_double(0)
+        check(lib().dkx_server_wait(self._h, int(rid), C.byref(lat)))
+        return lat.value
+
+    def run_trace(self, requests, arrivals_s):
+        """Open-loop replay: request i is submitted arrivals_s[i] seconds after the start.
+        Returns (latencies in ms, time to the last completion in s)."""
+        from ._lib import DkxRequest
+        n = len(requests)
+        arr = (DkxRequest * max(1, n))()
+        for i, r in enumerate(requests):
+            arr[i] = self._req(r)
+        t = (C.c_double * max(1, n))(*[float(x) for x in arrivals_s])
+        lat = (C.c_double * max(1, n))()
+        mk = C.c_double(0)
+        check(lib().dkx_server_run_trace(self._h, arr, t, n, lat, C.byref(mk)))
+        return [lat[i] for i in range(n)], mk.value
+
+    def counters(self):
+        b, im = C.c_longlong(0), C.c_longlong(0)
+        check(lib().dkx_server_counters(self._h, C.byref(b), C.byref(im)))
+        return b.value, im.value
+
+    def close(self):
+        if self._h:
+            lib().dkx_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -94,6 +94,13 @@ class DeviceArray:
             pass
 
 
+def copy2d(dst, dst_pitch_bytes, src, src_pitch_bytes, width_bytes, height):
+    """Device-to-device copy of a height x width_bytes region between pitched buffers."""
+    _check(hip().hipMemcpy2D(C.c_void_p(dst), C.c_size_t(dst_pitch_bytes), C.c_void_p(src),
+                             C.c_size_t(src_pitch_bytes), C.c_size_t(width_bytes), C.c_size_t(height), 3),
+           "hipMemcpy2D")
+
+
 class Stream:
     def __init__(self):
         s = C.c_void_p()

@@ -174,6 +174,50 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch batch, void* stream, int reps, int*
  * parameter selection is undefined (see dkx_nlsar_insar_device). */
 int dkx_nlsar_batch_ub_pixels(dkx_nlsar_batch batch, unsigned long long* counts);
 
+/* ---- serving ------------------------------------------------------------------------------
+ * No counterpart in the reference (which has no request loop).  A server owns a fixed HBM
+ * budget (one arena, sub-allocated per request: no allocation while serving), keeps the NL-SAR
+ * weight LUTs resident, admits requests as memory frees up, batches the waiting NL-SAR
+ * requests into one launch per stage while earlier work still runs, and runs NL-InSAR
+ * requests on streams of their own, overlapping the batches.  Each request's result equals
+ * the single-call result (dkx_nlsar_insar_device with the default pass variant, or
+ * dkx_nlinsar_device from zero estimates). */
+
+typedef struct {
+    int kind;                 /* 0 NL-SAR, 1 NL-InSAR (estimates start from zero) */
+    const float* ampl_master; /* device pointers, row-major height x width */
+    const float* ampl_slave;
+    const float* phase;
+    float* ref_filt;
+    float* phase_filt;
+    float* coh_filt;
+    int height;
+    int width;
+} dkx_request;
+
+typedef struct dkx_server_s* dkx_server;
+
+/* NL-SAR parameters as dkx_nlsar_insar; NL-InSAR parameters as dkx_nlinsar. */
+int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_size,
+                      const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                      int n_scale_sizes, const dkx_nlsar_stats* stats, int n_stats, float h_param,
+                      float c_param, int nl_search_window_size, int nl_patch_size, int nl_niter,
+                      int nl_lmin, dkx_server* server);
+/* Queues a request.  arrival_time_s: its arrival on the library's steady clock (seconds),
+ * or <= 0 for now; latencies are measured from it. */
+int dkx_server_submit(dkx_server server, const dkx_request* request, double arrival_time_s,
+                      long long* id);
+/* 1 done (latency_ms from arrival to completion), 0 pending, < 0 failed. */
+int dkx_server_poll(dkx_server server, long long id, double* latency_ms);
+int dkx_server_wait(dkx_server server, long long id, double* latency_ms);
+/* Submits requests[i] at arrival_s[i] seconds after the call (open loop), waits for all and
+ * returns each latency (ms, -1 for a failed request) and the time to the last completion. */
+int dkx_server_run_trace(dkx_server server, const dkx_request* requests, const double* arrival_s,
+                         int n, double* latency_ms, double* makespan_s);
+int dkx_server_counters(dkx_server server, long long* batches, long long* batched_images);
+/* Finishes the queued work, then frees the server. */
+void dkx_server_destroy(dkx_server server);
+
 #ifdef __cplusplus
 }
 #endif
