@@ -137,7 +137,7 @@ def nlinsar_form(args):
     from despeckcl_amd import hip as HIP
     from despeckcl_amd import synthetic as S
 
-    n = args.size
+    n = args.size or 4096
     big = n >= 1024
     a1, a2, ph = S.interferogram(n, n, S.SEED_BASE + (2 if big else 0), n_regions=32 if big else 8,
                                  coh_range=(0.3, 0.9), scatterer_fraction=0.05 if big else 0.0)
@@ -203,6 +203,69 @@ def nlinsar_form(args):
                                "cores": cores, "kind": "port",
                                "sample": f"NL-InSAR, {it} iterations on the {c}x{c} top-left crop, {dt:.1f} s "
                                          f"on {cores} OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
+def covmat_form(args):
+    """NL-SAR on covariance matrices (default: seeded single-look PolSAR, dimension 3,
+    2048^2, search window 21, patch sizes 3/5/7, scale sizes 1/3), statistics trained by the
+    library on a 32x32 homogeneous area of the same dimension; the 2*dim*dim interlaced
+    planes resident in HBM."""
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    n, dim = args.size or 2048, args.dim
+    c = S.polsar(n, n, S.SEED_BASE + 3, dim, n_regions=16)
+    inter = np.empty((2 * dim * dim, n, n), np.float32)
+    inter[::2] = c.real.reshape(-1, n, n)
+    inter[1::2] = c.imag.reshape(-1, n, n)
+    del c
+    st = D.nlsar_train(S.polsar(32, 32, S.SEED_BASE + 200 + dim, dim, n_regions=0), PATCHES, SCALES)
+    stream = HIP.Stream()
+    d_in = HIP.DeviceArray.from_host(inter)
+    d_out = HIP.DeviceArray(inter.shape)
+
+    def call(want=False):
+        return DV.nlsar_covmat(d_in.ptr, d_out.ptr, n, n, dim, SW, PATCHES, SCALES, st, HPAR, CPAR,
+                               stream=stream.handle, want_ub=want)
+
+    ub = call(True)
+    for _ in range(args.warmup):
+        call()
+    stream.synchronize()
+    HIP.synchronize()
+    e0, e1 = HIP.Event(), HIP.Event()
+    e0.record(stream.handle)
+    for _ in range(args.steps):
+        call()
+    e1.record(stream.handle)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    out = {
+        "metric": f"Mpixel/s NL-SAR covariance dim {dim} {n}x{n} (1 GPU)",
+        "value": round(n * n / 1e6 / (ms / 1e3), 3), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic single-look covariances (dimension {dim}, 16 regions)",
+        "config": {"workload": f"NL-SAR covariance input, one {n}x{n} dim-{dim} image per call",
+                   "search_window": SW, "patch_sizes": PATCHES, "scale_sizes": SCALES},
+        "ub_pixels": int(ub),
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        cs = min(n, 768)
+        crop = np.ascontiguousarray(inter[:, :cs, :cs])
+        t0 = time.perf_counter()
+        O.nlsar_covmat(crop, dim, SW, PATCHES, SCALES, st, HPAR, CPAR)
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(cs * cs / 1e6 / dt, 4), "unit": "Mpixel/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"covariance NL-SAR on the {cs}x{cs} top-left crop, {dt:.1f} s on "
+                                         f"{cores} OpenMP threads"}
     print(json.dumps(out), flush=True)
 
 
@@ -314,11 +377,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream latency sample")
     ap.add_argument("--tile", type=int, default=None)
-    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served"), default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served", "covmat"), default="plain")
+    ap.add_argument("--dim", type=int, default=3, help="covariance dimension (--form covmat)")
     ap.add_argument("--requests", type=int, default=2000, help="requests (--form served)")
     ap.add_argument("--rate", type=float, default=40.0, help="Poisson arrival rate, requests/s (--form served)")
     ap.add_argument("--budget-gib", type=int, default=150, help="server HBM budget (--form served)")
-    ap.add_argument("--size", type=int, default=4096, help="image side (--form nlinsar)")
+    ap.add_argument("--size", type=int, default=None, help="image side (--form nlinsar: 4096, covmat: 2048)")
     ap.add_argument("--niter", type=int, default=10, help="iterations (--form nlinsar)")
     ap.add_argument("--tiles", type=int, default=64, help="images per batch (--form batched)")
     args = ap.parse_args()
@@ -332,6 +396,8 @@ def main():
         return nlinsar_form(args)
     if args.form == "served":
         return served(args)
+    if args.form == "covmat":
+        return covmat_form(args)
 
     import despeckcl_amd as D
     from despeckcl_amd import device as DV

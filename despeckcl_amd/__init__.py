@@ -171,8 +171,16 @@ def nlsar_insar(ampl_master, ampl_slave, phase, search_window_size, patch_sizes,
 
 def nlsar(covmat_raw, search_window_size, patch_sizes, scale_sizes, nlsar_stats, h=15.0, c=49.0,
           enabled_log_levels=_DEFAULT_LOG_LEVELS):
-    """NL-SAR on (d, d, H, W) complex covariance matrices (nlsar.i:157-210)."""
-    raise NotImplementedError("the covariance-matrix NL-SAR overload is not available yet")
+    """NL-SAR on (d, d, H, W) complex covariance matrices; returns the filtered matrices in
+    the same shape (nlsar.i:157-210)."""
+    inter, d, H, W = _interlace(np.asarray(covmat_raw))
+    out = np.zeros_like(inter)
+    P, nP = int_array(patch_sizes)
+    S, nS = int_array(scale_sizes)
+    arr, ns, keep = _stats_structs(nlsar_stats)
+    check(lib().dkx_nlsar_covmat(inter.ctypes.data, out.ctypes.data, H, W, d, int(search_window_size),
+                                 P, nP, S, nS, arr, ns, float(h), float(c)))
+    return out[::2].reshape((d, d, H, W)) + 1j * out[1::2].reshape((d, d, H, W))
 
 
 def nlinsar(ampl_master, ampl_slave, phase, search_window_size, patch_size, niter, lmin,

@@ -24,7 +24,9 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 SOURCES = ["nlsar.hip", "nlinsar.hip", "server.hip", "capi.cpp", "batch.cpp", "stats.cpp",
            "despeckcl_api.cpp"] + [
-    f"nlsar_pass_{kind}_{shape}.hip" for kind in ("stat", "last") for shape in ("t32r4_w10", "t32r2_w10", "t32r1_w10", "t32r4_w20")]
+    f"nlsar_pass_{kind}_{shape}.hip" for kind in ("stat", "last")
+    for shape in ("t32r4_w10", "t32r2_w10", "t32r1_w10", "t32r4_w20", "t32r2_w10_cov2", "t32r4_w20_cov2",
+                  "t32r2_w10_cov3", "t32r4_w20_cov3")]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -208,7 +208,7 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
     for (int i = 0; i < n; i++) {
         const dkx_nlsar_image& im = images[i];
         nlsar_image_desc(plans[i], (char*)b->ws + offs[i], im.ampl_master, im.ampl_slave, im.phase, nullptr,
-                         im.ref_filt, im.phase_filt, im.coh_filt, nullptr, nullptr, &nb.descs[i]);
+                         im.ref_filt, im.phase_filt, im.coh_filt, nullptr, nullptr, nullptr, &nb.descs[i]);
     }
     int rows_c, rows_a, blocks, mcw, maw;
     nlsar_batch_layout(nb.descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);

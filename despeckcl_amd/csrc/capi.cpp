@@ -191,25 +191,26 @@ struct NlsarCall {
     NlsarImagePlan ip;
 };
 
-static int nlsar_prepare(NlsarCall& call, int height, int width, int sw, const int* P, int nP,
-                         const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats, float h,
-                         float c)
+static int nlsar_prepare(NlsarCall& call, int dim, int height, int width, int sw, const int* P,
+                         int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
+                         float h, float c)
 {
     if (!P || !S) { set_error("nlsar: null size list"); return DKX_EINVAL; }
     if (nP < 1 || nS < 1) { set_error("nlsar: empty patch or scale size list"); return DKX_EINVAL; }
     int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views);
     if (rc) return rc;
-    call.pp = NlsarParams{0, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
+    call.pp = NlsarParams{dim, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
                           g_tile[0].load()};
     rc = nlsar_common(call.pp, &call.cm);
     if (rc) return rc;
     return nlsar_image_plan(call.cm, height, width, -1, -1, &call.ip);
 }
 
+// a1 / a2 / phi -> ref / pha / coh for InSAR input, cov -> cov_out for covariance input
 static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, const float* phi,
-                             float* ref, float* pha, float* coh, float* enl_maps,
-                             float* alpha_maps, unsigned long long* ub, hipStream_t stream,
-                             DeviceWorkspace* w)
+                             const float* cov, float* ref, float* pha, float* coh, float* cov_out,
+                             float* enl_maps, float* alpha_maps, unsigned long long* ub,
+                             hipStream_t stream, DeviceWorkspace* w)
 {
     std::vector<float> blob;
     int rc = nlsar_build_constants(call.pp, call.cm, blob);
@@ -221,7 +222,7 @@ static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, 
     rc = workspace_acquire(w, stream);
     if (rc) return rc;
     NlsarImageDesc d;
-    nlsar_image_desc(call.ip, (char*)w->ptr, a1, a2, phi, nullptr, ref, pha, coh, enl_maps, alpha_maps, &d);
+    nlsar_image_desc(call.ip, (char*)w->ptr, a1, a2, phi, cov, ref, pha, coh, cov_out, enl_maps, alpha_maps, &d);
     rc = nlsar_enqueue(call.pp, call.cm, &d, 1, nullptr, w->consts, stream);
     if (rc) return rc;
     if (ub) DKX_HIP_CHECK(hipMemcpyAsync(ub, d.ub_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
@@ -321,13 +322,13 @@ int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, f
 {
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
     NlsarCall call;
-    int rc = nlsar_prepare(call, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    int rc = nlsar_prepare(call, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
     std::lock_guard<std::mutex> lk(w->mu);
-    return nlsar_device_impl(call, a1, a2, phi, ref, pha, coh, enl_maps, alpha_maps, ub,
-                             (hipStream_t)stream, w);
+    return nlsar_device_impl(call, a1, a2, phi, nullptr, ref, pha, coh, nullptr, enl_maps,
+                             alpha_maps, ub, (hipStream_t)stream, w);
 }
 
 int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
@@ -336,7 +337,7 @@ int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
 {
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
     NlsarCall call;
-    int rc = nlsar_prepare(call, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    int rc = nlsar_prepare(call, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
@@ -350,12 +351,63 @@ int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
     DKX_HIP_CHECK(hipMemcpyAsync(d, a1, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(d + n, a2, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(d + 2 * n, phi, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlsar_device_impl(call, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, nullptr,
-                           nullptr, nullptr, w->stream, w);
+    rc = nlsar_device_impl(call, d, d + n, d + 2 * n, nullptr, d + 3 * n, d + 4 * n, d + 5 * n,
+                           nullptr, nullptr, nullptr, nullptr, w->stream, w);
     if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
     DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+int dkx_nlsar_covmat_device(const float* cov, float* cov_out, int height, int width, int dim,
+                            int sw, const int* P, int nP, const int* S, int nS,
+                            const dkx_nlsar_stats* stats, int n_stats, float h, float c,
+                            float* enl_maps, float* alpha_maps, unsigned long long* ub,
+                            void* stream)
+{
+    if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+    if (dim != 2 && dim != 3) {
+        set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
+        return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
+    }
+    NlsarCall call;
+    int rc = nlsar_prepare(call, dim, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    if (rc) return rc;
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    return nlsar_device_impl(call, nullptr, nullptr, nullptr, cov, nullptr, nullptr, nullptr,
+                             cov_out, enl_maps, alpha_maps, ub, (hipStream_t)stream, w);
+}
+
+int dkx_nlsar_covmat(const float* cov, float* cov_out, int height, int width, int dim, int sw,
+                     const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                     int n_stats, float h, float c)
+{
+    if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+    if (dim != 2 && dim != 3) {
+        set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
+        return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
+    }
+    NlsarCall call;
+    int rc = nlsar_prepare(call, dim, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    if (rc) return rc;
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    rc = ensure_stream(w);
+    if (rc) return rc;
+    const size_t n = (size_t)2 * dim * dim * height * width;
+    HostStage st;
+    DKX_HIP_CHECK(hipMalloc(&st.ptr, 2 * n * sizeof(float)));
+    float* d = (float*)st.ptr;
+    DKX_HIP_CHECK(hipMemcpyAsync(d, cov, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    rc = nlsar_device_impl(call, nullptr, nullptr, nullptr, d, nullptr, nullptr, nullptr, d + n,
+                           nullptr, nullptr, nullptr, w->stream, w);
+    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+    DKX_HIP_CHECK(hipMemcpyAsync(cov_out, d + n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
     return 0;
 }

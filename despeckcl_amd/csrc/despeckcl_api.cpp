@@ -84,12 +84,19 @@ int despeckcl::nlsar(float* ampl_master, float* ampl_slave, float* phase, float*
     return 0;
 }
 
-int despeckcl::nlsar(float*, float*, const int, const int, const int, const int,
-                     const std::vector<int>, const std::vector<int>,
-                     std::map<nlsar::params, nlsar::stats>, const float, const float,
-                     std::vector<std::string>)
+// covariance overload (src/compute/nlsar/nlsar.cpp:170-193): 2*dim*dim interlaced planes in
+// and out
+int despeckcl::nlsar(float* covmat_raw, float* covmat_filt, const int height, const int width,
+                     const int dim, const int search_window_size,
+                     const std::vector<int> patch_sizes, const std::vector<int> scale_sizes,
+                     std::map<nlsar::params, nlsar::stats> nlsar_stats, const float h_param,
+                     const float c_param, std::vector<std::string>)
 {
-    throw std::invalid_argument("nlsar: the covariance-matrix overload is not available yet");
+    std::vector<dkx_nlsar_stats> v = views(nlsar_stats);
+    check(dkx_nlsar_covmat(covmat_raw, covmat_filt, height, width, dim, search_window_size,
+                           patch_sizes.data(), (int)patch_sizes.size(), scale_sizes.data(),
+                           (int)scale_sizes.size(), v.data(), (int)v.size(), h_param, c_param));
+    return 0;
 }
 
 nlsar_stats_collection despeckcl::nlsar_training(float* ampl_master, float* ampl_slave,

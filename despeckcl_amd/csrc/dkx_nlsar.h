@@ -15,6 +15,15 @@ constexpr int NLSAR_MAX_NS = 8;
 constexpr int NLSAR_TILE = 32;         // every pass variant computes 32 x 32 output tiles
 constexpr int NLSAR_NTILES = 3;        // pass variants: 4, 2 or 1 rows per thread
 
+// Records of the grids, per mode (0: InSAR input; 2, 3: covariance input of that dimension).
+// C: the input covariance; mode 0 keeps (a1^2, a2^2, Re C01, Im C01), modes 2 / 3 every plane
+// of the 2*dim*dim (padded to float4).  A: the scale average; modes 0 / 2 keep (a00, a11,
+// Re a01, Im a01) and a separate plane of log det, mode 3 (a00, a11, a22, Re/Im a01, a02, a12,
+// log det).
+__host__ __device__ constexpr int nlsar_c_float4(int mode) { return mode == 0 ? 1 : mode == 2 ? 2 : 5; }
+__host__ __device__ constexpr int nlsar_a_float4(int mode) { return mode == 3 ? 3 : 1; }
+__host__ __device__ constexpr int nlsar_a_bytes(int mode) { return mode == 3 ? 48 : 20; }
+
 // One (patch, scale) entry of the dissimilarity statistics (include/compute/nlsar/stats.h:27-55)
 struct NlsarStatsView {
     int patch_size, scale_size, lut_size;
@@ -25,7 +34,7 @@ struct NlsarStatsView {
 
 // The filter parameters of a call, or of every image of a batch.
 struct NlsarParams {
-    int dim;          // 0: InSAR input (a1, a2, phase); 2: dim-2 covariance input
+    int dim;          // 0: InSAR input (a1, a2, phase); 2 / 3: covariance input of that dimension
     int sw;
     const int* P; int nP;
     const int* S; int nS;
@@ -36,6 +45,7 @@ struct NlsarParams {
 
 // Validated parameters and what follows from them.
 struct NlsarCommon {
+    int mode;         // as NlsarParams::dim
     int tile;
     int wsh, P, smax, overlap;
     int nparam, lut_stride;
@@ -54,6 +64,7 @@ struct NlsarImagePlan {
 struct NlsarImageDesc {
     const float* a1; const float* a2; const float* phi; const float* cov;
     float* ref; float* phase; float* coh;
+    float* cov_out;                      // covariance input: the 2*dim*dim filtered planes
     float* enl_nb; float* alpha;         // [nparam][H*W] or null
     float* C;                            // float4 per element
     float* A;                            // float4 per element, then the log det plane
@@ -81,7 +92,7 @@ int nlsar_build_constants(const NlsarParams& pp, const NlsarCommon& cm, std::vec
 // descriptor of one image whose workspace starts at ws_image
 void nlsar_image_desc(const NlsarImagePlan& ip, char* ws_image, const float* a1, const float* a2,
                       const float* phi, const float* cov, float* ref, float* phase, float* coh,
-                      float* enl_maps, float* alpha_maps, NlsarImageDesc* d);
+                      float* cov_out, float* enl_maps, float* alpha_maps, NlsarImageDesc* d);
 // numbers the rows and pass blocks of a batch; returns the totals
 void nlsar_batch_layout(NlsarImageDesc* d, int n, int* rows_c, int* rows_a, int* blocks,
                         int* max_cw, int* max_aw);

@@ -168,13 +168,39 @@ __device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, Pas
 
 // Occupancy target: blocks per CU (three fit the LDS budget of the small window class; two
 // for single-row threads, whose 16-wave blocks would otherwise leave 40 registers), which
-// bounds the registers the compiler may use.
-constexpr int nlsar_blocks_per_cu(int RPT, int WMAX) { return WMAX > NLSAR_WMAX_SMALL ? 1 : RPT == 1 ? 2 : 3; }
+// bounds the registers the compiler may use.  Covariance input holds more statistics and,
+// in the last pass, every plane's weighted sum: fewer blocks leave room for them.
+constexpr int nlsar_blocks_per_cu(int RPT, int WMAX, int MODE, bool LAST)
+{
+    return WMAX > NLSAR_WMAX_SMALL ? 1
+         : MODE == 3 && LAST     ? 1
+         : MODE != 0             ? 2
+         : RPT == 1              ? 2
+                                 : 3;
+}
 
-template <int TX, int RPT, int PMASK, bool LAST, int WMAX>
-__global__ __launch_bounds__(TX * NLSAR_TY / RPT, nlsar_blocks_per_cu(RPT, WMAX) * TX * NLSAR_TY / RPT / 256)
+// Covariance of a 3x3 Hermitian matrix (det_covmat_3x3, sim_measures.h:79-97), in the
+// reference's operation order.
+__device__ __forceinline__ float nlsar_det3(float a00, float a11, float a22, float a01r, float a01i,
+                                            float a02r, float a02i, float a12r, float a12i)
+{
+    return a00 * a11 * a22 - a00 * (a12r * a12r + a12i * a12i) - a11 * (a02r * a02r + a02i * a02i) -
+           a22 * (a01r * a01r + a01i * a01i) +
+           2 * (a01r * a12r * a02r - a01i * a12i * a02r + a01r * a12i * a02i + a01i * a12r * a02i);
+}
+
+// MODE: 0 InSAR input, 2 / 3 covariance input of that dimension (all planes filtered).
+template <int TX, int RPT, int PMASK, bool LAST, int WMAX, int MODE>
+__global__ __launch_bounds__(TX * NLSAR_TY / RPT, nlsar_blocks_per_cu(RPT, WMAX, MODE, LAST) * TX * NLSAR_TY / RPT / 256)
 void nlsar_scale_pass(const ScaleArgs a)
 {
+    constexpr int CF4 = nlsar_c_float4(MODE);    // float4 per C element
+    constexpr int AF4 = nlsar_a_float4(MODE);    // float4 per A element
+    constexpr int DIM = MODE == 0 ? 2 : MODE;
+    constexpr int NSTAT = 2 + 2 * DIM;           // sum w, sum w^2, sum w I_d, sum w I_d^2
+    constexpr int NPL = 2 * DIM * DIM;           // planes of a covariance input
+    // planes averaged by the weighted mean besides the diagonal intensities
+    constexpr int NMACC = MODE == 0 ? 2 : NPL - DIM;
     constexpr int NT = TX * NLSAR_TY / RPT;   // each thread owns RPT rows of one column
     constexpr int TY = NLSAR_TY;
     constexpr int NP = pmask_count(PMASK);
@@ -201,18 +227,19 @@ void nlsar_scale_pass(const ScaleArgs a)
     for (int k = 0; k < NP; k++)
         for (int i = tid; i < a.lut_size[k]; i += NT) sL[k * a.lut_stride + i] = a.lutw[k][i];
 
-    float acc[RPT][NP][6];
-    float macc[RPT][2];
+    float acc[RPT][NP][NSTAT];
+    float macc[RPT][LAST ? NMACC : 1];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
 #pragma unroll
         for (int k = 0; k < NP; k++)
 #pragma unroll
-            for (int j = 0; j < 6; j++) acc[q][k][j] = 0.f;
-        macc[q][0] = macc[q][1] = 0.f;
+            for (int j = 0; j < NSTAT; j++) acc[q][k][j] = 0.f;
+#pragma unroll
+        for (int j = 0; j < (LAST ? NMACC : 1); j++) macc[q][j] = 0.f;
     }
 
-    const float K = nlsar_log2_term();
+    const float K = (float)(2 * DIM) * 0.693147182464599609375f;   // 2*dim*log(2), sim_measures.h:58
     const int apitch = im.apitch, cpitch = im.cpitch;
     const DKX_GLOBAL char* A4b = im.A4b;
     const DKX_GLOBAL char* LDb = im.LDb;
@@ -246,16 +273,29 @@ void nlsar_scale_pass(const ScaleArgs a)
                 const int sstep = sr_ * DXM + sc_;
                 for (int i = tid; i < n; i += NT) {
                     const unsigned int g2 = g1 + dofs;
-                    const float4 u = gload4(A4b + 16u * g1);
-                    const float4 v = gload4(A4b + 16u * g2);
-                    const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g1);
-                    const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g2);
-                    const float s00 = u.x + v.x;
-                    const float s01r = u.z + v.z;
-                    const float s01i = u.w + v.w;
-                    const float s11 = u.y + v.y;
-                    const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
-                    float sim = -(((K + l1) + l2) - 2.0f * logf(det));
+                    float sim;
+                    if constexpr (MODE == 3) {
+                        // pixel_similarity_3x3, sim_measures.h:100-158
+                        const float4 u0 = gload4(A4b + 48u * g1), u1 = gload4(A4b + 48u * g1 + 16u);
+                        const float4 u2 = gload4(A4b + 48u * g1 + 32u);
+                        const float4 v0 = gload4(A4b + 48u * g2), v1 = gload4(A4b + 48u * g2 + 16u);
+                        const float4 v2 = gload4(A4b + 48u * g2 + 32u);
+                        const float det = nlsar_det3(u0.x + v0.x, u0.y + v0.y, u0.z + v0.z, u0.w + v0.w,
+                                                     u1.x + v1.x, u1.y + v1.y, u1.z + v1.z, u1.w + v1.w,
+                                                     u2.x + v2.x);
+                        sim = -(((K + u2.y) + v2.y) - 2.0f * logf(det));
+                    } else {
+                        const float4 u = gload4(A4b + 16u * g1);
+                        const float4 v = gload4(A4b + 16u * g2);
+                        const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g1);
+                        const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g2);
+                        const float s00 = u.x + v.x;
+                        const float s01r = u.z + v.z;
+                        const float s01i = u.w + v.w;
+                        const float s11 = u.y + v.y;
+                        const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
+                        sim = -(((K + l1) + l2) - 2.0f * logf(det));
+                    }
                     if (isnan(sim)) sim = 0.f;
                     sD[so] = sim;
                     c += sc_;
@@ -373,41 +413,85 @@ void nlsar_scale_pass(const ScaleArgs a)
                 for (int q = 0; q < RPT; q++) {
                     const unsigned int cu = cu0 + (unsigned int)(q * cpitch);
                     const unsigned int cl = cl0 + (unsigned int)(q * cpitch);
-                    float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
-                    if (LAST) {
-                        const float4 u = gload4(C4b + 16u * cu);
-                        const float4 l = gload4(C4b + 16u * cl);
-                        i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
-                        i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
-                    } else {
-                        const float2 u = gload2(C4b + 16u * cu);
-                        const float2 l = gload2(C4b + 16u * cl);
-                        i0u = u.x; i1u = u.y;
-                        i0l = l.x; i1l = l.y;
-                    }
-                    const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
-                    const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
+                    if constexpr (MODE == 0) {
+                        float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
+                        if (LAST) {
+                            const float4 u = gload4(C4b + 16u * cu);
+                            const float4 l = gload4(C4b + 16u * cl);
+                            i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
+                            i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
+                        } else {
+                            const float2 u = gload2(C4b + 16u * cu);
+                            const float2 l = gload2(C4b + 16u * cl);
+                            i0u = u.x; i1u = u.y;
+                            i0l = l.x; i1l = l.y;
+                        }
+                        const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
+                        const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
 #pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const float wu = wu0[k * WPL + q * RXM];
-                        const float wl = wl0[k * WPL + q * RXM];
-                        acc[q][k][0] += wu;
-                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
-                        acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
-                        acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
-                        acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
-                        acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
-                        acc[q][k][0] += wl;
-                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
-                        acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
-                        acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
-                        acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
-                        acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
-                        if (LAST && k == NP - 1) {
-                            macc[q][0] = fmaf(wu, cru, macc[q][0]);
-                            macc[q][1] = fmaf(wu, ciu, macc[q][1]);
-                            macc[q][0] = fmaf(wl, crl, macc[q][0]);
-                            macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                        for (int k = 0; k < NP; k++) {
+                            const float wu = wu0[k * WPL + q * RXM];
+                            const float wl = wl0[k * WPL + q * RXM];
+                            acc[q][k][0] += wu;
+                            acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+                            acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
+                            acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
+                            acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
+                            acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
+                            acc[q][k][0] += wl;
+                            acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+                            acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
+                            acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
+                            acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
+                            acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
+                            if (LAST && k == NP - 1) {
+                                macc[q][0] = fmaf(wu, cru, macc[q][0]);
+                                macc[q][1] = fmaf(wu, ciu, macc[q][1]);
+                                macc[q][0] = fmaf(wl, crl, macc[q][0]);
+                                macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                            }
+                        }
+                    } else {
+                        // covariance input: the diagonal intensities (planes (2*dim+2)*d,
+                        // compute_nl_statistics.cl:69-84) and, for the mean, every other plane
+                        float cu_[4 * CF4], cl_[4 * CF4];
+#pragma unroll
+                        for (int j = 0; j < CF4; j++) {
+                            const float4 u = gload4(C4b + (16u * CF4) * cu + 16u * j);
+                            const float4 l = gload4(C4b + (16u * CF4) * cl + 16u * j);
+                            cu_[4 * j] = u.x; cu_[4 * j + 1] = u.y; cu_[4 * j + 2] = u.z; cu_[4 * j + 3] = u.w;
+                            cl_[4 * j] = l.x; cl_[4 * j + 1] = l.y; cl_[4 * j + 2] = l.z; cl_[4 * j + 3] = l.w;
+                        }
+#pragma unroll
+                        for (int k = 0; k < NP; k++) {
+                            const float wu = wu0[k * WPL + q * RXM];
+                            const float wl = wl0[k * WPL + q * RXM];
+                            acc[q][k][0] += wu;
+                            acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) {
+                                const float I = cu_[(2 * DIM + 2) * d];
+                                acc[q][k][2 + 2 * d] = fmaf(wu, I, acc[q][k][2 + 2 * d]);
+                                acc[q][k][3 + 2 * d] = fmaf(wu, I * I, acc[q][k][3 + 2 * d]);
+                            }
+                            acc[q][k][0] += wl;
+                            acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+#pragma unroll
+                            for (int d = 0; d < DIM; d++) {
+                                const float I = cl_[(2 * DIM + 2) * d];
+                                acc[q][k][2 + 2 * d] = fmaf(wl, I, acc[q][k][2 + 2 * d]);
+                                acc[q][k][3 + 2 * d] = fmaf(wl, I * I, acc[q][k][3 + 2 * d]);
+                            }
+                            if (LAST && k == NP - 1) {
+                                int j = 0;
+#pragma unroll
+                                for (int pl = 0; pl < NPL; pl++) {
+                                    if (pl % (2 * DIM + 2) == 0) continue;   // a diagonal intensity
+                                    macc[q][j] = fmaf(wu, cu_[pl], macc[q][j]);
+                                    macc[q][j] = fmaf(wl, cl_[pl], macc[q][j]);
+                                    j++;
+                                }
+                            }
                         }
                     }
                 }
@@ -431,39 +515,48 @@ void nlsar_scale_pass(const ScaleArgs a)
     DKX_GLOBAL float* const o_coh = gptr(uni(dsc.coh));
     unsigned long long* const o_ub = uni(dsc.ub_count);
     unsigned int ubl = 0;
+    DKX_GLOBAL float* const o_cov = gptr(uni(dsc.cov_out));
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
         const int py = g * RPT + q;
         const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
         const bool valid = iy < L.ny && ix < L.nx;
-        const float4 cc = gload4(C4b + 16u * (unsigned int)(im.corg + (ty0 + py) * cpitch + (tx0 + px)));
-        const float i0c = cc.x, i1c = cc.y;
+        // the pixel's own covariance (C record) and its diagonal intensities
+        float cc[4 * CF4];
+        const unsigned int cidx = (unsigned int)(im.corg + (ty0 + py) * cpitch + (tx0 + px));
+#pragma unroll
+        for (int j = 0; j < CF4; j++) {
+            const float4 v = gload4(C4b + (16u * CF4) * cidx + 16u * j);
+            cc[4 * j] = v.x; cc[4 * j + 1] = v.y; cc[4 * j + 2] = v.z; cc[4 * j + 3] = v.w;
+        }
+        float ic[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) ic[d] = MODE == 0 ? cc[d] : cc[(2 * DIM + 2) * d];
         const long long opix = valid ? (long long)(L.oy0 + iy) * imW + (L.ox0 + ix) : 0;
         bool fail = false;
-        float alpha_mean = 0.f, s_mean = 1.f, m0_mean = 0.f, m1_mean = 0.f;
+        float alpha_mean = 0.f, s_mean = 1.f;
+        float dsum[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++) dsum[d] = 0.f;
 #pragma unroll
         for (int k = 0; k < NP; k++) {
             float* s = acc[q][k];
             s[0] += 1.0f;
             s[1] = fmaf(1.0f, 1.0f, s[1]);
-            s[2] = fmaf(1.0f, i0c, s[2]);
-            s[3] = fmaf(1.0f, i0c * i0c, s[3]);
-            s[4] = fmaf(1.0f, i1c, s[4]);
-            s[5] = fmaf(1.0f, i1c * i1c, s[5]);
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                s[2 + 2 * d] = fmaf(1.0f, ic[d], s[2 + 2 * d]);
+                s[3 + 2 * d] = fmaf(1.0f, ic[d] * ic[d], s[3 + 2 * d]);
+            }
             // compute_nl_statistics.cl:86-102, compute_alphas.cl:19-44,
             // compute_enls_nobias.cl:19-39
             const float ws = s[0];
             const float enl = ws * ws / s[1];
             float alpha = 0.0f;
-            {
-                const float inl = s[2] / ws;
-                const float wv = s[3] / ws;
-                const float var = wv - inl * inl;
-                alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
-            }
-            {
-                const float inl = s[4] / ws;
-                const float wv = s[5] / ws;
+#pragma unroll
+            for (int d = 0; d < DIM; d++) {
+                const float inl = s[2 + 2 * d] / ws;
+                const float wv = s[3 + 2 * d] / ws;
                 const float var = wv - inl * inl;
                 alpha = fmaxf(alpha, fmaxf(0.0f, (var - inl * inl / 1.0f) / var));
             }
@@ -477,11 +570,21 @@ void nlsar_scale_pass(const ScaleArgs a)
             if (LAST && k == NP - 1) {
                 alpha_mean = alpha;
                 s_mean = ws;
-                m0_mean = s[2];
-                m1_mean = s[4];
-                // centre weight 1 on the off-diagonal sums as well
-                macc[q][0] = fmaf(1.0f, cc.z, macc[q][0]);
-                macc[q][1] = fmaf(1.0f, cc.w, macc[q][1]);
+#pragma unroll
+                for (int d = 0; d < DIM; d++) dsum[d] = s[2 + 2 * d];
+                // centre weight 1 on the other planes' sums as well
+                if constexpr (MODE == 0) {
+                    macc[q][0] = fmaf(1.0f, cc[2], macc[q][0]);
+                    macc[q][1] = fmaf(1.0f, cc[3], macc[q][1]);
+                } else {
+                    int j = 0;
+#pragma unroll
+                    for (int pl = 0; pl < NPL; pl++) {
+                        if (pl % (2 * DIM + 2) == 0) continue;
+                        macc[q][j] = fmaf(1.0f, cc[pl], macc[q][j]);
+                        j++;
+                    }
+                }
             }
         }
         if (!valid) continue;
@@ -493,24 +596,42 @@ void nlsar_scale_pass(const ScaleArgs a)
             // (patch, scale) map whenever every ENL exceeds -1; pixels where the reference's
             // selection loop is undefined are counted and also take the last key.
             if (fail || (!a.first_scale && o_flag[opix])) ubl++;
-            // weighted_means.cl:62-80 and covmat_decompose.cl:19-39
+            // weighted_means.cl:62-80
             const float al = alpha_mean;
-            const float m00 = (1 - al) * m0_mean / s_mean + al * i0c;
-            const float m11 = (1 - al) * m1_mean / s_mean + al * i1c;
-            const float mr = (1 - al) * macc[q][0] / s_mean + al * cc.z;
-            const float mi = (1 - al) * macc[q][1] / s_mean + al * cc.w;
-            const float am = sqrtf(m00), as = sqrtf(m11);
-            const float t = 0.5f * (am + as);
-            o_ref[opix] = t * t;
-            o_phase[opix] = atan2f(mi, mr);
-            o_coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
+            if constexpr (MODE == 0) {
+                // and covmat_decompose.cl:19-39
+                const float m00 = (1 - al) * dsum[0] / s_mean + al * ic[0];
+                const float m11 = (1 - al) * dsum[1] / s_mean + al * ic[1];
+                const float mr = (1 - al) * macc[q][0] / s_mean + al * cc[2];
+                const float mi = (1 - al) * macc[q][1] / s_mean + al * cc[3];
+                const float am = sqrtf(m00), as = sqrtf(m11);
+                const float t = 0.5f * (am + as);
+                o_ref[opix] = t * t;
+                o_phase[opix] = atan2f(mi, mr);
+                o_coh[opix] = sqrtf(mi * mi + mr * mr) / (am * as);
+            } else {
+                // every plane of the covariance, plane-major in the output
+                const long long HW = (long long)imH * imW;
+                int j = 0;
+#pragma unroll
+                for (int pl = 0; pl < NPL; pl++) {
+                    float sum;
+                    if (pl % (2 * DIM + 2) == 0) {
+                        sum = dsum[pl / (2 * DIM + 2)];
+                    } else {
+                        sum = macc[q][j];
+                        j++;
+                    }
+                    o_cov[(long long)pl * HW + opix] = (1 - al) * sum / s_mean + al * cc[pl];
+                }
+            }
         }
     }
     if (LAST && ubl) atomicAdd(o_ub, (unsigned long long)ubl);
 }
 
 // Launch of one pass; defined in the instantiating translation units.
-template <bool LAST, int TX, int RPT, int WMAX>
+template <bool LAST, int TX, int RPT, int WMAX, int MODE>
 int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
 
 // Every patch set of one to three half widths in [0, 5].
@@ -520,7 +641,7 @@ int nlsar_pass_launch(int pmask, const ScaleArgs& a, hipStream_t stream);
     X(7) X(11) X(19) X(35) X(13) X(21) X(37) X(25) X(41) X(49) X(14) X(22) X(38) X(26)      \
     X(42) X(50) X(28) X(44) X(52) X(56)
 
-template <bool LAST, int TX, int RPT, int WMAX, int PMASK>
+template <bool LAST, int TX, int RPT, int WMAX, int MODE, int PMASK>
 int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
 {
     constexpr int NP = pmask_count(PMASK);
@@ -531,7 +652,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
     }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     static_assert(TX == NLSAR_TILE && NLSAR_TY == NLSAR_TILE, "block counts assume 32 x 32 tiles");
     const int total = a.total_blocks;
@@ -543,7 +664,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
         DKX_HIP_CHECK(hipEventCreate(&e1));
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
-    hipLaunchKernelGGL((nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX>), dim3(total), dim3(TX * NLSAR_TY / RPT),
+    hipLaunchKernelGGL((nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>), dim3(total), dim3(TX * NLSAR_TY / RPT),
                        lds, stream, a);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
@@ -553,15 +674,16 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
     return 0;
 }
 
-#define DKX_NLSAR_PASS_CASE(M) case M: return nlsar_pass_launch_mask<LAST_, TX_, RPT_, WMAX_, M>(a, stream);
-#define DKX_NLSAR_PASS_INSTANTIATE(LAST_V, TX_V, RPT_V, WMAX_V)                               \
+#define DKX_NLSAR_PASS_CASE(M) case M: return nlsar_pass_launch_mask<LAST_, TX_, RPT_, WMAX_, MODE_, M>(a, stream);
+#define DKX_NLSAR_PASS_INSTANTIATE(LAST_V, TX_V, RPT_V, WMAX_V, MODE_V)                       \
     template <>                                                                              \
-    int nlsar_pass_launch<LAST_V, TX_V, RPT_V, WMAX_V>(int pmask, const ScaleArgs& a,       \
-                                                       hipStream_t stream)                   \
+    int nlsar_pass_launch<LAST_V, TX_V, RPT_V, WMAX_V, MODE_V>(int pmask, const ScaleArgs& a, \
+                                                               hipStream_t stream)           \
     {                                                                                        \
         constexpr bool LAST_ = LAST_V;                                                       \
         constexpr int TX_ = TX_V;                                                            \
         constexpr int RPT_ = RPT_V;                                                          \
+        constexpr int MODE_ = MODE_V;                                                        \
         constexpr int WMAX_ = WMAX_V;                                                        \
         switch (pmask) {                                                                     \
             DKX_NLSAR_PMASKS(DKX_NLSAR_PASS_CASE)                                            \

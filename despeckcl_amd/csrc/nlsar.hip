@@ -52,10 +52,10 @@ __global__ void nlsar_zero_counts(const PrepArgs a)
     }
 }
 
-// Covariance of the input pixel on the padded grid C, one float4 per element:
-// InSAR input (covmat_create.cl:19-56): a1^2, a2^2, a1*a2*cos(phi), a1*a2*sin(phi);
-// dim-2 covariance input (8 interlaced planes, nlsar.i:146-153): the diagonal real parts and
-// the (0,1) element.
+// Covariance of the input pixel on the padded grid C (records of nlsar_c_float4(MODE)
+// float4): InSAR input (covmat_create.cl:19-56): a1^2, a2^2, a1*a2*cos(phi), a1*a2*sin(phi);
+// covariance input (2*dim*dim interlaced planes, nlsar.i:146-153): every plane, in order.
+template <int MODE>
 __global__ void nlsar_prep(const PrepArgs a)
 {
     const int row = blockIdx.x / a.cblocks;
@@ -67,23 +67,30 @@ __global__ void nlsar_prep(const PrepArgs a)
     const int y = min(H - 1, max(0, gy - d.coy));
     const int x = min(W - 1, max(0, gx - d.cox));
     const long long i = (long long)y * W + x;
-    float4 v;
-    if (d.cov) {
+    constexpr int CF4 = nlsar_c_float4(MODE);
+    DKX_GLOBAL float* C = gptr(d.C) + 4 * CF4 * ((long long)gy * d.cpitch + gx);
+    if constexpr (MODE == 0) {
+        const float m = gptr(d.a1)[i], sl = gptr(d.a2)[i], p = gptr(d.phi)[i];
+        gstore4(C, make_float4(m * m, sl * sl, m * sl * cosf(p), m * sl * sinf(p)));
+    } else {
+        constexpr int NPL = 2 * MODE * MODE;
         const DKX_GLOBAL float* cov = gptr(d.cov);
         const long long n = (long long)H * W;
-        v = make_float4(cov[i], cov[6 * n + i], cov[2 * n + i], cov[3 * n + i]);
-    } else {
-        const float m = gptr(d.a1)[i], sl = gptr(d.a2)[i], p = gptr(d.phi)[i];
-        v = make_float4(m * m, sl * sl, m * sl * cosf(p), m * sl * sinf(p));
+        float v[4 * CF4];
+#pragma unroll
+        for (int pl = 0; pl < 4 * CF4; pl++) v[pl] = pl < NPL ? cov[pl * n + i] : 0.f;
+#pragma unroll
+        for (int j = 0; j < CF4; j++) gstore4(C + 4 * j, make_float4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]));
     }
-    gstore4(gptr(d.C) + 4 * ((long long)gy * d.cpitch + gx), v);
 }
 
 // Gaussian s x s average of the rescaled covariance (covmat_rescale.cl:19-41 then
 // covmat_spatial_avg.cl:19-63; planes combined per pixel in the same operation order),
 // plus the log-determinant of each averaged matrix, which every pixel dissimilarity
-// involving that pixel uses (sim_measures.h:51-52).
-// A holds one float4 per element (a00, a11, Re a01, Im a01) followed by the log det plane.
+// involving that pixel uses (sim_measures.h:51-52, 100-158).  Only the planes the
+// dissimilarity reads are averaged: (0,0), (1,1), (0,1) for dim 2; all 9 real values of the
+// Hermitian matrix for dim 3.  Records as described at nlsar_a_float4.
+template <int MODE>
 __global__ void nlsar_scale_average(const PrepArgs a)
 {
     const int row = blockIdx.x / a.cblocks;
@@ -94,27 +101,44 @@ __global__ void nlsar_scale_average(const PrepArgs a)
     const int y = gy - d.aoy, x = gx - d.aox;  // image coordinates
     const int s = a.s;
     const int ssh = (s - 1) / 2;
-    const DKX_GLOBAL float* C4 = gptr(d.C);
+    constexpr int CF4 = nlsar_c_float4(MODE);
+    const DKX_GLOBAL float* C = gptr(d.C);
     const DKX_GLOBAL float* gauss = gptr(a.gauss);
-    float s00 = 0.f, s11 = 0.f, sr = 0.f, si = 0.f;
+    const float gamma = a.gamma;
+    // (plane of the C record, off-diagonal) of each averaged value
+    constexpr int NV = MODE == 3 ? 9 : 4;
+    constexpr int src[9] = {0, MODE == 0 ? 1 : MODE == 2 ? 6 : 8, MODE == 0 ? 2 : MODE == 2 ? 2 : 16,
+                            MODE == 0 ? 3 : MODE == 2 ? 3 : 2, 3, 4, 5, 10, 11};
+    constexpr bool offd[9] = {false, false, MODE != 3, true, true, true, true, true, true};
+    float acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[v] = 0.f;
     for (int kx = 0; kx < s; kx++) {
         const long long rowc = (long long)(y - ssh + kx + d.coy) * d.cpitch + (x - ssh + d.cox);
         for (int ky = 0; ky < s; ky++) {
             const float g = gauss[kx * s + ky];
-            const float4 c = gload4(C4 + 4 * (rowc + ky));
-            s00 = fmaf(g, c.x, s00);
-            s11 = fmaf(g, c.y, s11);
-            const float r = c.z * a.gamma;
-            const float im = c.w * a.gamma;
-            sr = fmaf(g, r, sr);
-            si = fmaf(g, im, si);
+            const DKX_GLOBAL float* c = C + 4 * CF4 * (rowc + ky);
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const float val = offd[v] ? c[src[v]] * gamma : c[src[v]];
+                acc[v] = fmaf(g, val, acc[v]);
+            }
         }
     }
     const long long o = (long long)gy * d.apitch + gx;
     DKX_GLOBAL float* A = gptr(d.A);
-    gstore4(A + 4 * o, make_float4(s00, s11, sr, si));
-    const float det = (s00 * s11) - (sr * sr + si * si);
-    A[4ll * d.aplane + o] = logf(det);
+    if constexpr (MODE == 3) {
+        const float det = nlsar_det3(acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], acc[6], acc[7], acc[8]);
+        DKX_GLOBAL float* r = A + 12 * o;
+        gstore4(r, make_float4(acc[0], acc[1], acc[2], acc[3]));
+        gstore4(r + 4, make_float4(acc[4], acc[5], acc[6], acc[7]));
+        gstore4(r + 8, make_float4(acc[8], logf(det), 0.f, 0.f));
+    } else {
+        const float s00 = acc[0], s11 = acc[1], sr = acc[2], si = acc[3];
+        gstore4(A + 4 * o, make_float4(s00, s11, sr, si));
+        const float det = (s00 * s11) - (sr * sr + si * si);
+        A[4ll * d.aplane + o] = logf(det);
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -176,6 +200,11 @@ int nlsar_common(const NlsarParams& pp, NlsarCommon* cm)
     for (int i = 0; i < pp.nS; i++)
         for (int j = i + 1; j < pp.nS; j++)
             if (pp.S[i] == pp.S[j]) { set_error("nlsar: duplicate scale size"); return -1; }
+    if (pp.dim != 0 && pp.dim != 2 && pp.dim != 3) {
+        set_error("nlsar: covariance dimensions 2 and 3 are supported");
+        return -1;
+    }
+    cm->mode = pp.dim;
     cm->tile = std::min(std::max(pp.tile, 0), NLSAR_NTILES - 1);
     cm->wsh = (pp.sw - 1) / 2;
     cm->P = (pmax - 1) / 2;
@@ -206,15 +235,17 @@ int nlsar_image_plan(const NlsarCommon& cm, int H, int W, int edge_h, int edge_w
     ip->C.plane = (long long)ip->C.pitch * ip->C.h;
     ip->A = Grid{nullptr, 0, round_up(Wx + 2 * ra, 64), ra, ra, Hx + 2 * ra, Wx + 2 * ra};
     ip->A.plane = (long long)ip->A.pitch * ip->A.h;
-    // kernels address grids with 32-bit byte offsets (16 bytes per element)
-    if (ip->C.plane >= (1ll << 28) || ip->A.plane >= (1ll << 28)) {
+    // kernels address grids with 32-bit byte offsets
+    const double cbytes = 16.0 * nlsar_c_float4(cm.mode) * ip->C.plane;
+    const double abytes = 16.0 * nlsar_a_float4(cm.mode) * ip->A.plane;
+    if (cbytes >= 4294967296.0 || abytes >= 4294967296.0) {
         set_error("nlsar: image too large for one call (split it into bands)");
         return -4;
     }
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) / 256 * 256; return o; };
-    ip->off_C = take(sizeof(float) * 4 * ip->C.plane);
-    ip->off_A = take(sizeof(float) * 5 * ip->A.plane);
+    ip->off_C = take((size_t)16 * nlsar_c_float4(cm.mode) * ip->C.plane);
+    ip->off_A = take((size_t)nlsar_a_bytes(cm.mode) * ip->A.plane);
     ip->off_flag = take((size_t)H * W);
     ip->off_ub = take(sizeof(unsigned long long));
     ip->bytes = off;
@@ -257,7 +288,7 @@ int nlsar_build_constants(const NlsarParams& pp, const NlsarCommon& cm, std::vec
 
 void nlsar_image_desc(const NlsarImagePlan& ip, char* ws, const float* a1, const float* a2,
                       const float* phi, const float* cov, float* ref, float* phase, float* coh,
-                      float* enl_maps, float* alpha_maps, NlsarImageDesc* d)
+                      float* cov_out, float* enl_maps, float* alpha_maps, NlsarImageDesc* d)
 {
     memset(d, 0, sizeof(*d));
     d->a1 = a1;
@@ -267,6 +298,7 @@ void nlsar_image_desc(const NlsarImagePlan& ip, char* ws, const float* a1, const
     d->ref = ref;
     d->phase = phase;
     d->coh = coh;
+    d->cov_out = cov_out;
     d->enl_nb = enl_maps;
     d->alpha = alpha_maps;
     d->C = (float*)(ws + ip.off_C);
@@ -344,10 +376,16 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
     hipLaunchKernelGGL(nlsar_zero_counts, dim3((n + 255) / 256), dim3(256), 0, stream, pa);
     DKX_HIP_CHECK(hipGetLastError());
     pa.cblocks = (max_cw + 255) / 256;
-    hipLaunchKernelGGL(nlsar_prep, dim3((unsigned)rows_c * pa.cblocks), dim3(256), 0, stream, pa);
+    const dim3 gprep((unsigned)rows_c * pa.cblocks);
+    switch (cm.mode) {
+    case 0: hipLaunchKernelGGL(nlsar_prep<0>, gprep, dim3(256), 0, stream, pa); break;
+    case 2: hipLaunchKernelGGL(nlsar_prep<2>, gprep, dim3(256), 0, stream, pa); break;
+    default: hipLaunchKernelGGL(nlsar_prep<3>, gprep, dim3(256), 0, stream, pa); break;
+    }
     DKX_HIP_CHECK(hipGetLastError());
-    // covmat_rescale.cl:24-25 with nlooks = 1 and dimension 2
-    pa.gamma = std::pow(std::min(1.0f / 2, 1.0f), 0.33333f);
+    // covmat_rescale.cl:24-25 with nlooks = 1 (the InSAR input has dimension 2)
+    const int dim = cm.mode == 0 ? 2 : cm.mode;
+    pa.gamma = std::pow(std::min(1.0f / dim, 1.0f), 0.33333f);
     pa.cblocks = (max_aw + 255) / 256;
 
     // Passes: each handles one scale and up to three patch sizes (one dissimilarity field
@@ -384,7 +422,12 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
         if (si != cur_scale) {
             pa.gauss = d_gauss + (size_t)si * 31 * 31;
             pa.s = pp.S[si];
-            hipLaunchKernelGGL(nlsar_scale_average, dim3((unsigned)rows_a * pa.cblocks), dim3(256), 0, stream, pa);
+            const dim3 gavg((unsigned)rows_a * pa.cblocks);
+            switch (cm.mode) {
+            case 0: hipLaunchKernelGGL(nlsar_scale_average<0>, gavg, dim3(256), 0, stream, pa); break;
+            case 2: hipLaunchKernelGGL(nlsar_scale_average<2>, gavg, dim3(256), 0, stream, pa); break;
+            default: hipLaunchKernelGGL(nlsar_scale_average<3>, gavg, dim3(256), 0, stream, pa); break;
+            }
             DKX_HIP_CHECK(hipGetLastError());
             cur_scale = si;
         }
@@ -421,24 +464,41 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
         sa.lut_stride = cm.lut_stride;
         sa.first_scale = oi == 0 ? 1 : 0;
         int rc;
-        if (cm.wsh <= NLSAR_WMAX_SMALL) {
-            switch (cm.tile) {
-            case 0:
-                rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream)
-                             : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_SMALL>(pmask, sa, stream);
-                break;
-            case 2:
-                rc = ps.last ? nlsar_pass_launch<true, 32, 1, NLSAR_WMAX_SMALL>(pmask, sa, stream)
-                             : nlsar_pass_launch<false, 32, 1, NLSAR_WMAX_SMALL>(pmask, sa, stream);
-                break;
-            default:
-                rc = ps.last ? nlsar_pass_launch<true, 32, 2, NLSAR_WMAX_SMALL>(pmask, sa, stream)
-                             : nlsar_pass_launch<false, 32, 2, NLSAR_WMAX_SMALL>(pmask, sa, stream);
-                break;
+        if (cm.mode == 0) {
+            if (cm.wsh <= NLSAR_WMAX_SMALL) {
+                switch (cm.tile) {
+                case 0:
+                    rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream)
+                                 : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream);
+                    break;
+                case 2:
+                    rc = ps.last ? nlsar_pass_launch<true, 32, 1, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream)
+                                 : nlsar_pass_launch<false, 32, 1, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream);
+                    break;
+                default:
+                    rc = ps.last ? nlsar_pass_launch<true, 32, 2, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream)
+                                 : nlsar_pass_launch<false, 32, 2, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream);
+                    break;
+                }
+            } else {
+                rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_LARGE, 0>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE, 0>(pmask, sa, stream);
             }
+        } else if (cm.mode == 2) {
+            // covariance input: one pass variant per window class
+            if (cm.wsh <= NLSAR_WMAX_SMALL)
+                rc = ps.last ? nlsar_pass_launch<true, 32, 2, NLSAR_WMAX_SMALL, 2>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 2, NLSAR_WMAX_SMALL, 2>(pmask, sa, stream);
+            else
+                rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_LARGE, 2>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE, 2>(pmask, sa, stream);
         } else {
-            rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_LARGE>(pmask, sa, stream)
-                         : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE>(pmask, sa, stream);
+            if (cm.wsh <= NLSAR_WMAX_SMALL)
+                rc = ps.last ? nlsar_pass_launch<true, 32, 2, NLSAR_WMAX_SMALL, 3>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 2, NLSAR_WMAX_SMALL, 3>(pmask, sa, stream);
+            else
+                rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_LARGE, 3>(pmask, sa, stream)
+                             : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_LARGE, 3>(pmask, sa, stream);
         }
         if (rc) return rc;
     }

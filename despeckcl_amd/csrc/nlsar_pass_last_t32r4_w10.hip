@@ -3,5 +3,5 @@
 #include "dkx_nlsar_pass.h"
 
 namespace dkx {
-DKX_NLSAR_PASS_INSTANTIATE(true, 32, 4, 10)
+DKX_NLSAR_PASS_INSTANTIATE(true, 32, 4, 10, 0)
 }  // namespace dkx

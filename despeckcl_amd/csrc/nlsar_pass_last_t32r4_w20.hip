@@ -3,5 +3,5 @@
 #include "dkx_nlsar_pass.h"
 
 namespace dkx {
-DKX_NLSAR_PASS_INSTANTIATE(true, 32, 4, 20)
+DKX_NLSAR_PASS_INSTANTIATE(true, 32, 4, 20, 0)
 }  // namespace dkx

@@ -3,5 +3,5 @@
 #include "dkx_nlsar_pass.h"
 
 namespace dkx {
-DKX_NLSAR_PASS_INSTANTIATE(false, 32, 2, 10)
+DKX_NLSAR_PASS_INSTANTIATE(false, 32, 2, 10, 0)
 }  // namespace dkx

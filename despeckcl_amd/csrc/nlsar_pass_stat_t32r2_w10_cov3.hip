@@ -1,0 +1,7 @@
+// Instantiation of the NL-SAR scale pass for the statistics-only passes on dimension-3 covariance
+// input: tiles of 32 x 32 with 2 rows per thread, search windows of half width up to 10.
+#include "dkx_nlsar_pass.h"
+
+namespace dkx {
+DKX_NLSAR_PASS_INSTANTIATE(false, 32, 2, 10, 3)
+}  // namespace dkx

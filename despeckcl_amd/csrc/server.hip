@@ -208,7 +208,7 @@ static void launch_sar(Server* s, int lane)
         r->status = 1;
         NlsarImageDesc d;
         nlsar_image_desc(ip, s->arena.base + off, r->r.ampl_master, r->r.ampl_slave, r->r.phase, nullptr,
-                         r->r.ref_filt, r->r.phase_filt, r->r.coh_filt, nullptr, nullptr, &d);
+                         r->r.ref_filt, r->r.phase_filt, r->r.coh_filt, nullptr, nullptr, nullptr, &d);
         descs.push_back(d);
         f->reqs.push_back(r);
     }

@@ -28,6 +28,22 @@ def nlsar_insar(a1, a2, phase, ref, phase_filt, coh, height, width, search_windo
     return ub.value if want_ub else None
 
 
+def nlsar_covmat(cov, cov_out, height, width, dim, search_window_size, patch_sizes, scale_sizes,
+                 nlsar_stats, h=15.0, c=49.0, enl_maps=0, alpha_maps=0, stream=0, want_ub=False):
+    """Enqueue NL-SAR on device covariance planes (2*dim*dim interlaced planes in and out).
+    Returns the undefined-selection pixel count when want_ub (synchronises), else None."""
+    from . import _stats_structs
+    P, nP = int_array(patch_sizes)
+    S, nS = int_array(scale_sizes)
+    arr, ns, keep = _stats_structs(nlsar_stats)
+    ub = C.c_ulonglong(0)
+    check(lib().dkx_nlsar_covmat_device(cov, cov_out, int(height), int(width), int(dim),
+                                        int(search_window_size), P, nP, S, nS, arr, ns, float(h),
+                                        float(c), enl_maps or None, alpha_maps or None,
+                                        C.byref(ub) if want_ub else None, stream or None))
+    return ub.value if want_ub else None
+
+
 def nlinsar(a1, a2, phase, ref, phase_filt, coh, height, width, search_window_size, patch_size,
             niter, lmin, stream=0, want_count=False):
     """Enqueue NL-InSAR on device buffers (ref/phase_filt/coh hold the initial estimates).

@@ -9,14 +9,16 @@ The speckle model is the reference's own InSAR simulator
 
 Reflectivity R is piecewise constant over random rectangles (log-uniform values), the
 coherence gamma is uniform per rectangle, and the true phase is a linear ramp along the
-columns plus a Gaussian bump.  Everything is float32 and drawn from numpy's PCG64 with the
+columns plus a Gaussian bump.  polsar() draws dim-channel scattering vectors
+k = chol(Sigma) n, n ~ CN(0, I), with a random Hermitian covariance Sigma per rectangle, and
+returns the single-look covariance matrices k k^H.  Everything is float32 and drawn from numpy's PCG64 with the
 given seed, so a (shape, seed) pair always yields the same image.
 """
 from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["interferogram", "homogeneous_area", "SEED_BASE"]
+__all__ = ["interferogram", "homogeneous_area", "polsar", "SEED_BASE"]
 
 SEED_BASE = 20260
 
@@ -86,3 +88,26 @@ def tile_shapes(n, seed, lo=256, hi=2048, max_aspect=4.0):
         if max(h, w) <= max_aspect * min(h, w):
             out.append((h, w))
     return out
+
+
+def polsar(height, width, seed, dim=3, n_regions=6, refl_range=(1.0, 100.0)):
+    """Return single-look covariance matrices, complex64 of shape (dim, dim, height, width),
+    the layout the covariance NL-SAR entry point takes (before interlacing)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lab = np.zeros((height, width), np.int64)
+    for r in range(1, n_regions + 1):
+        h0, h1 = np.sort(rng.integers(0, height + 1, size=2))
+        w0, w1 = np.sort(rng.integers(0, width + 1, size=2))
+        lab[h0:max(h1, h0 + 2), w0:max(w1, w0 + 2)] = r
+    lo, hi = np.log(refl_range[0]), np.log(refl_range[1])
+    chols = []
+    for _ in range(n_regions + 1):
+        g = rng.standard_normal((dim, dim)) + 1j * rng.standard_normal((dim, dim))
+        sig = g @ g.conj().T + 0.2 * np.eye(dim)
+        sig *= np.exp(rng.uniform(lo, hi)) / np.real(np.trace(sig)) * dim
+        chols.append(np.linalg.cholesky(sig))
+    n = (rng.standard_normal((dim, height, width)) + 1j * rng.standard_normal((dim, height, width))) / np.sqrt(2)
+    L = np.stack(chols)[lab]                      # (H, W, dim, dim)
+    k = np.einsum("hwij,jhw->ihw", L, n)          # (dim, H, W)
+    cov = k[:, None] * np.conj(k[None, :])        # (dim, dim, H, W)
+    return np.ascontiguousarray(cov.astype(np.complex64))

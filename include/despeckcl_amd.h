@@ -69,6 +69,26 @@ int dkx_nlsar_insar_device(const float* d_ampl_master, const float* d_ampl_slave
                            float h_param, float c_param, float* d_enl_maps, float* d_alpha_maps,
                            unsigned long long* ub_pixels, void* stream);
 
+/* despeckcl::nlsar, covariance overload (include/despeckcl.h:61-72;
+ * src/compute/nlsar/nlsar.cpp:170-193).  covmat / covmat_filt: 2*dim*dim planes of
+ * height x width, real and imaginary parts of each matrix element interlaced in row-major
+ * element order (swig/python/nlsar.i:196-201); every plane of the result is the
+ * alpha-blended weighted mean of the selected (patch, scale) pair.  dim 2 or 3 (the
+ * reference's dissimilarities exist for those two, src/compute/nlsar/sim_measures.h:37-158). */
+int dkx_nlsar_covmat(const float* covmat, float* covmat_filt, int height, int width, int dim,
+                     int search_window_size, const int* patch_sizes, int n_patch_sizes,
+                     const int* scale_sizes, int n_scale_sizes, const dkx_nlsar_stats* stats,
+                     int n_stats, float h_param, float c_param);
+
+/* Same on device buffers, enqueued on `stream`; the optional outputs as for
+ * dkx_nlsar_insar_device. */
+int dkx_nlsar_covmat_device(const float* d_covmat, float* d_covmat_filt, int height, int width,
+                            int dim, int search_window_size, const int* patch_sizes,
+                            int n_patch_sizes, const int* scale_sizes, int n_scale_sizes,
+                            const dkx_nlsar_stats* stats, int n_stats, float h_param,
+                            float c_param, float* d_enl_maps, float* d_alpha_maps,
+                            unsigned long long* ub_pixels, void* stream);
+
 /* despeckcl::nlinsar (include/despeckcl.h:97-109; src/compute/nlinsar/nlinsar.cpp:42-146).
  * ref_filt / phase_filt / coh_filt are read as the first iteration's estimates and
  * overwritten with the result, like the reference (nlinsar.cpp:59-61).  Patch sizes 3 and

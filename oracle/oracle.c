@@ -836,8 +836,10 @@ static int nlsar_insar_tile(const float* tin, float* tout, int th, int tw, void*
 static int nlsar_covmat_tile(const float* tin, float* tout, int th, int tw, void* vctx)
 {
     nlsar_ctx* cx = (nlsar_ctx*)vctx;
+    const size_t n = (size_t)th * tw;
     return nlsar_tile(tin, tout, th, tw, cx->dim, cx->sw, cx->P, cx->nP, cx->S, cx->nS, cx->st,
-                      cx->nst, cx->h, cx->c, &cx->ub, NULL);
+                      cx->nst, cx->h, cx->c, &cx->ub,
+                      cx->want_cond ? tout + (size_t)2 * cx->dim * cx->dim * n : NULL);
 }
 
 int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref_filt,
@@ -889,15 +891,39 @@ int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, in
                      int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
                      int nst, float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
+    return orc_nlsar_covmat_cond(covmat_raw, covmat_filt, NULL, height, width, dim, sw, P, nP, S,
+                                 nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
+}
+
+/* as orc_nlsar_covmat; cond (optional, height x width) as in orc_nlsar_insar_cond */
+int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* cond, int height,
+                          int width, int dim, int sw, const int* P, int nP, const int* S, int nS,
+                          const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
+                          int tile_w, long long* ub_pixels)
+{
     int rc = check_nlsar_args(height, width, dim, sw, P, nP, S, nS, st, nst);
     if (rc) return rc;
     const int overlap = nlsar_overlap(sw, P, nP, S, nS);
     if (tile_h <= 0) tile_h = height + 3 * overlap;
     if (tile_w <= 0) tile_w = width + 3 * overlap;
     if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
-    nlsar_ctx cx = {dim, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, 0};
-    rc = map_tiles(covmat_raw, covmat_filt, height, width, 2 * dim * dim, 2 * dim * dim, tile_h,
-                   tile_w, overlap, nlsar_covmat_tile, &cx);
+    const int np = 2 * dim * dim;
+    const size_t n = (size_t)height * width;
+    float* out = covmat_filt;
+    if (cond) {
+        out = (float*)calloc((size_t)(np + 1) * n, sizeof(float));
+        memcpy(out, covmat_filt, sizeof(float) * np * n);
+    }
+    nlsar_ctx cx = {dim, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, cond != NULL};
+    rc = map_tiles(covmat_raw, out, height, width, np, cond ? np + 1 : np, tile_h, tile_w,
+                   overlap, nlsar_covmat_tile, &cx);
+    if (cond) {
+        if (rc == 0) {
+            memcpy(covmat_filt, out, sizeof(float) * np * n);
+            memcpy(cond, out + (size_t)np * n, sizeof(float) * n);
+        }
+        free(out);
+    }
     if (ub_pixels) *ub_pixels = cx.ub;
     return rc;
 }

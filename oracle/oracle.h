@@ -89,6 +89,10 @@ int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, in
                      int search_window_size, const int* patch_sizes, int n_patch,
                      const int* scale_sizes, int n_scale, const orc_stats* stats, int n_stats,
                      float h_param, float c_param, int tile_h, int tile_w, long long* ub_pixels);
+int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* cond, int height,
+                          int width, int dim, int sw, const int* P, int nP, const int* S, int nS,
+                          const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
+                          int tile_w, long long* ub_pixels);
 
 /* despeckcl::nlsar_training, nlsar_training.cpp:35-96; out must hold n_patch*n_scale entries
  * (ordered patch-major like the reference's std::map); quantilles/chi2 storage is supplied

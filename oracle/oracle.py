@@ -25,9 +25,9 @@ class OrcStats(C.Structure):
 
 
 def build(force=False):
-    """Compile liboracle.so with the committed Makefile (gcc)."""
-    if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    """Compile liboracle.so with the committed Makefile (gcc; a no-op when it is newer than
+    its sources)."""
+    subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
 
 
 def lib():
@@ -137,19 +137,25 @@ def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats
 
 
 def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stats,
-                 h=15.0, c=49.0, tile=(0, 0)):
+                 h=15.0, c=49.0, tile=(0, 0), want_cond=False):
+    """covmat: (2*dim*dim, H, W) interlaced planes.  Returns (filtered planes, ub_pixels),
+    plus the conditioning map (as nlsar_insar) when want_cond."""
     cov = _f32(covmat)
     H, W = cov.shape[-2:]
     out = np.zeros_like(cov)
+    cond = np.zeros((H, W), np.float32)
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
-    rc = lib().orc_nlsar_covmat(
-        cov.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W),
+    rc = lib().orc_nlsar_covmat_cond(
+        cov.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+        cond.ctypes.data_as(C.c_void_p) if want_cond else None, C.c_int(H), C.c_int(W),
         C.c_int(dim), C.c_int(search_window_size), _ints(patch_sizes), C.c_int(len(patch_sizes)),
         _ints(scale_sizes), C.c_int(len(scale_sizes)), arr, C.c_int(len(stats)), C.c_float(h),
         C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]), C.byref(ub))
     if rc != 0:
         raise ValueError(f"oracle nlsar_covmat failed ({rc})")
+    if want_cond:
+        return out, ub.value, cond
     return out, ub.value
 
 

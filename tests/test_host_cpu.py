@@ -82,6 +82,14 @@ def test_argument_errors_without_device(D):
     bad[D.params(3, 1)].quantilles[4] = 1.5
     with pytest.raises(ValueError, match="quantile"):
         D.nlsar_insar(a, a, a, 21, [3, 5], [1], bad)
+    for d in (1, 4):
+        c = np.ones((d, d, 16, 16), np.complex64)
+        with pytest.raises(ValueError, match="dimension 2 or 3"):
+            D.nlsar(c, 21, [3, 5], [1], st)
+    with pytest.raises(ValueError, match="square"):
+        D.nlsar(np.ones((2, 3, 16, 16), np.complex64), 21, [3, 5], [1], st)
+    with pytest.raises(ValueError, match="odd"):
+        D.nlsar(np.ones((3, 3, 16, 16), np.complex64), 20, [3, 5], [1], st)
     with pytest.raises(ValueError, match="patch sizes 3 and 7"):
         D.nlinsar(a, a, a, 21, 5, 2, 10)
     with pytest.raises(ValueError, match="odd"):

@@ -489,3 +489,89 @@ def test_nlinsar_first_iteration_tile_independence(oracle):
     for k in range(3):
         d = np.abs(r80[k][:-1, :-1] - r120[k][:-1, :-1])
         assert np.quantile(d, 0.99) <= 1e-4 * (1 + np.abs(r80[k][:-1, :-1]).max())
+
+
+# ---- covariance input (nlsar.cpp:170-193), dimensions 2 and 3 ----------------------------
+
+
+def _interlace(c):
+    d = c.shape[0]
+    out = np.empty((2 * d * d,) + c.shape[2:], np.float32)
+    out[::2] = c.real.reshape((d * d,) + c.shape[2:])
+    out[1::2] = c.imag.reshape((d * d,) + c.shape[2:])
+    return out
+
+
+def _deinterlace(x, d):
+    return x[::2].reshape((d, d) + x.shape[1:]) + 1j * x[1::2].reshape((d, d) + x.shape[1:])
+
+
+def _train_pol(oracle, dim):
+    from despeckcl_amd import synthetic as S
+    c = S.polsar(32, 32, S.SEED_BASE + 200 + dim, dim, n_regions=0)
+    return oracle.nlsar_train_covmat(_interlace(c), dim, [3, 5, 7], [1, 3])
+
+
+def _normalised_error(g, o):
+    """Per pixel: max over the matrix elements of |g - o| / sqrt(o_ii o_jj) (the relative
+    error on the diagonal)."""
+    d = o.shape[0]
+    diag = np.abs(np.stack([o[i, i] for i in range(d)]))
+    scale = np.sqrt(diag[:, None] * diag[None, :])
+    return (np.abs(g - o) / np.maximum(scale, 1e-30)).max(axis=(0, 1))
+
+
+def test_nlsar_covmat_dim2_equals_insar_path(oracle):
+    """Covariances made from the interferogram (covmat_create), filtered through the
+    covariance entry and decomposed (covmat_decompose), equal the InSAR entry bit for bit."""
+    from despeckcl_amd import synthetic as S
+    st = _train(oracle)
+    h, w = 40, 36
+    a1, a2, ph = S.interferogram(h, w, 24)
+    cov = np.zeros((8, h, w), np.float32)
+    oracle.call("orc_covmat_create", a1, a2, ph, cov, h, w)
+    filt, ub = oracle.nlsar_covmat(cov, 2, 21, [3, 5, 7], [1, 3], st)
+    got = [np.zeros((h, w), np.float32) for _ in range(3)]
+    oracle.call("orc_covmat_decompose", filt, *got, h, w)
+    want = oracle.nlsar_insar(a1, a2, ph, 21, [3, 5, 7], [1, 3], st)
+    assert ub == want[3]
+    for k in range(3):
+        assert np.array_equal(got[k], want[k])
+
+
+@pytest.mark.parametrize("perm", [(1, 2, 0), (2, 1, 0)])
+def test_nlsar_covmat_dim3_permutation_equivariance(oracle, perm):
+    """Reordering the channels (C -> P C P^T) leaves every determinant, the off-diagonal
+    rescaling and the per-channel statistics unchanged, so the filtered matrices are the
+    reordered ones; a wrong plane index for any element breaks this."""
+    from despeckcl_amd import synthetic as S
+    st = _train_pol(oracle, 3)
+    c = S.polsar(36, 40, 31, 3)
+    p = list(perm)
+    cp = c[p][:, p]
+    o, _, cond = oracle.nlsar_covmat(_interlace(c), 3, 11, [3, 5, 7], [1, 3], st, want_cond=True)
+    op, _ = oracle.nlsar_covmat(_interlace(cp), 3, 11, [3, 5, 7], [1, 3], st)
+    want = _deinterlace(o, 3)[p][:, p]
+    e = _normalised_error(_deinterlace(op, 3), want)[cond >= 0.03]
+    print("perm", perm, "q99", np.quantile(e, 0.99), "above 1e-3:", int((e > 1e-3).sum()), "of", e.size)
+    assert np.quantile(e, 0.99) <= 1e-3
+    assert (e > 1e-3).sum() <= max(2, 0.002 * e.size)
+
+
+def test_nlsar_covmat_dim3_phase_rotation_equivariance(oracle):
+    """C -> D C D^H with D = diag(exp(i theta_j)) keeps every |C_ij| and determinant (the
+    cross term 2 Re{a01 a12 a20} included), so the result rotates the same way; a swapped
+    real / imaginary plane or a conjugation error breaks this."""
+    from despeckcl_amd import synthetic as S
+    st = _train_pol(oracle, 3)
+    c = S.polsar(36, 40, 32, 3)
+    th = np.array([0.0, 1.1, -2.3])
+    D = np.exp(1j * th)
+    rot = (D[:, None] * np.conj(D)[None, :])[:, :, None, None]
+    o, _, cond = oracle.nlsar_covmat(_interlace(c), 3, 11, [3, 5, 7], [1, 3], st, want_cond=True)
+    orot, _ = oracle.nlsar_covmat(_interlace(c * rot), 3, 11, [3, 5, 7], [1, 3], st)
+    want = _deinterlace(o, 3) * rot
+    e = _normalised_error(_deinterlace(orot, 3), want)[cond >= 0.03]
+    print("rotation q99", np.quantile(e, 0.99), "above 1e-3:", int((e > 1e-3).sum()), "of", e.size)
+    assert np.quantile(e, 0.99) <= 1e-3
+    assert (e > 1e-3).sum() <= max(2, 0.002 * e.size)
