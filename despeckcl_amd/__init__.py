@@ -9,6 +9,7 @@ runs in libdespeckcl_amd.so (hand-written HIP kernels for gfx950) through its C 
 from __future__ import annotations
 
 import ctypes as C
+import os
 from collections import namedtuple
 
 import numpy as np
@@ -198,40 +199,24 @@ def nlinsar(ampl_master, ampl_slave, phase, search_window_size, patch_size, nite
     return ref, phf, coh
 
 
-_STATS_MAGIC = "despeckcl_amd-nlsar-stats 1"
-
-
 def store_nlsar_stats_collection(nsc, filename):
-    """Write the statistics in the text format of the C++ store_nlsar_stats_collection."""
-    with open(filename, "w") as f:
-        f.write(_STATS_MAGIC + "\n")
-        f.write(f"{len(nsc)}\n")
-        for k in sorted(nsc.keys(), key=lambda kk: (kk[0], kk[1])):
-            s = nsc[k]
-            dmin = repr(float(np.float32(s.dissims_min)))
-            dmax = repr(float(np.float32(s.dissims_max)))
-            f.write(f"{int(k[0])} {int(k[1])} {int(s.lut_size)} {dmin} {dmax}\n")
-            f.write(" ".join(repr(float(v)) for v in np.asarray(s.quantilles, np.float32)) + "\n")
-            f.write(" ".join(repr(float(v)) for v in np.asarray(s.chi2cdf_inv, np.float32)) + "\n")
+    """Write the statistics as the reference does: a Boost text archive of
+    std::map<nlsar::params, nlsar::stats> (nlsar_training.cpp:99-104)."""
+    arr, ns, keep = _stats_structs(nsc)
+    check(lib().dkx_nlsar_stats_store(os.fsencode(filename), arr, ns))
 
 
 def load_nlsar_stats_collection(filename):
-    with open(filename) as f:
-        tokens = f.read().split()
-    if " ".join(tokens[:2]) != _STATS_MAGIC:
-        raise ValueError(f"not a statistics file: {filename}")
-    n = int(tokens[2])
-    pos = 3
-    res = nlsar_stats_collection()
-    for _ in range(n):
-        p, s_, lut = int(tokens[pos]), int(tokens[pos + 1]), int(tokens[pos + 2])
-        st = stats(lut_size=lut)
-        st.dissims_min = float(np.float32(tokens[pos + 3]))
-        st.dissims_max = float(np.float32(tokens[pos + 4]))
-        pos += 5
-        st.quantilles = np.array(tokens[pos:pos + lut], dtype=np.float32)
-        pos += lut
-        st.chi2cdf_inv = np.array(tokens[pos:pos + lut], dtype=np.float32)
-        pos += lut
-        res[params(p, s_)] = st
-    return res
+    """Read a statistics file written by the reference (Boost text archive) or by
+    store_nlsar_stats_collection (nlsar_training.cpp:106-113)."""
+    fn = os.fsencode(filename)
+    nf = C.c_longlong(0)
+    n = lib().dkx_nlsar_stats_load(fn, None, 0, None, 0, C.byref(nf))
+    if n < 0:
+        check(int(n))
+    out = (DkxStats * max(1, n))()
+    store = np.zeros(max(1, nf.value), np.float32)
+    got = lib().dkx_nlsar_stats_load(fn, out, n, store.ctypes.data, nf.value, C.byref(nf))
+    if got < 0:
+        check(int(got))
+    return _collect(out, n)

@@ -60,6 +60,9 @@ SIGNATURES = {
     "dkx_nlsar_training_covmat": (_I, [_P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _P]),
     "dkx_nlsar_stats_build": (C.c_longlong, [_P, C.c_longlong, _I, _P, _P, _P, _P]),
     "dkx_chi2_inv": (C.c_double, [C.c_double, C.c_double]),
+    "dkx_nlsar_stats_store": (_I, [C.c_char_p, _SP, _I]),
+    "dkx_nlsar_stats_load": (C.c_longlong, [C.c_char_p, _SP, _I, _P, C.c_longlong,
+                                            C.POINTER(C.c_longlong)]),
     "dkx_set_tile": (_I, [_I, _I]),
     "dkx_get_tile": (_I, [_I]),
     "dkx_set_timing": (_I, [_I]),

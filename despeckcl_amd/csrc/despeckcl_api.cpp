@@ -2,6 +2,7 @@
 #include "../../include/despeckcl.h"
 #include "../../include/despeckcl_amd.h"
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iomanip>
@@ -134,39 +135,29 @@ nlsar_stats_collection despeckcl::nlsar_training(float* covmat_raw, const int he
 // so that every value reads back bit-exactly.
 void despeckcl::store_nlsar_stats_collection(nlsar_stats_collection nsc, std::string filename)
 {
-    std::ofstream f(filename);
-    if (!f) throw std::runtime_error("cannot open " + filename);
-    f << "despeckcl_amd-nlsar-stats 1\n" << nsc.size() << "\n" << std::setprecision(9);
-    for (const auto& kv : nsc) {
-        const nlsar::stats& s = kv.second;
-        f << kv.first.patch_size << " " << kv.first.scale_size << " " << s.lut_size << " "
-          << s.dissims_min << " " << s.dissims_max << "\n";
-        for (float q : s.quantilles) f << q << " ";
-        f << "\n";
-        for (float c : s.chi2cdf_inv) f << c << " ";
-        f << "\n";
-    }
+    std::vector<dkx_nlsar_stats> v = views(nsc);
+    check(dkx_nlsar_stats_store(filename.c_str(), v.data(), (int)v.size()));
 }
 
 nlsar_stats_collection despeckcl::load_nlsar_stats_collection(std::string filename)
 {
-    std::ifstream f(filename);
-    if (!f) throw std::runtime_error("cannot open " + filename);
-    std::string magic;
-    int version = 0;
-    size_t n = 0;
-    f >> magic >> version >> n;
-    if (magic != "despeckcl_amd-nlsar-stats" || version != 1) throw std::runtime_error("not a statistics file: " + filename);
+    long long nf = 0;
+    const long long n = dkx_nlsar_stats_load(filename.c_str(), nullptr, 0, nullptr, 0, &nf);
+    if (n < 0) check((int)n);
+    std::vector<dkx_nlsar_stats> e((size_t)n);
+    std::vector<float> store((size_t)nf);
+    check((int)std::min(0ll, dkx_nlsar_stats_load(filename.c_str(), e.data(), (int)n, store.data(), nf, &nf)));
     nlsar_stats_collection nsc;
-    for (size_t i = 0; i < n; i++) {
+    for (const auto& x : e) {
         nlsar::params p;
+        p.patch_size = x.patch_size;
+        p.scale_size = x.scale_size;
         nlsar::stats s;
-        f >> p.patch_size >> p.scale_size >> s.lut_size >> s.dissims_min >> s.dissims_max;
-        s.quantilles.resize(s.lut_size);
-        s.chi2cdf_inv.resize(s.lut_size);
-        for (auto& q : s.quantilles) f >> q;
-        for (auto& c : s.chi2cdf_inv) f >> c;
-        if (!f) throw std::runtime_error("truncated statistics file: " + filename);
+        s.lut_size = (unsigned int)x.lut_size;
+        s.dissims_min = x.dissims_min;
+        s.dissims_max = x.dissims_max;
+        s.quantilles.assign(x.quantilles, x.quantilles + x.lut_size);
+        s.chi2cdf_inv.assign(x.chi2cdf_inv, x.chi2cdf_inv + x.lut_size);
         nsc.emplace(p, s);
     }
     return nsc;

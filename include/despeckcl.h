@@ -6,8 +6,8 @@
  * include/compute/nlsar/parameters.h:24-56 and include/compute/nlsar/stats.h:27-55, so
  * code written against despeckCL compiles and links against libdespeckcl_amd.so.
  * Differences:
- *  - no Boost: the statistics file format of store/load_nlsar_stats_collection is this
- *    library's own text format;
+ *  - no Boost: store/load_nlsar_stats_collection write and read the reference's Boost text
+ *    archive format with a reader / writer of their own;
  *  - errors throw std::runtime_error / std::invalid_argument instead of terminating;
  *  - stats::operator= also copies lut_size (the reference leaves it unset);
  *  - boxcar and goldstein are not part of this library.

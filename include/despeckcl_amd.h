@@ -124,6 +124,21 @@ int dkx_nlsar_training_covmat(const float* covmat, int height, int width, int di
 long long dkx_nlsar_stats_build(const float* dissims, long long n, int lut_size, float* dissims_min,
                                 float* dissims_max, float* quantilles, float* chi2cdf_inv);
 
+/* despeckcl::store_nlsar_stats_collection (include/despeckcl.h:94;
+ * src/compute/nlsar/training/nlsar_training.cpp:99-104): writes the entries in the reference's
+ * file format, a Boost text archive of std::map<nlsar::params, nlsar::stats> (library
+ * version 17), in map order. */
+int dkx_nlsar_stats_store(const char* filename, const dkx_nlsar_stats* stats, int n_stats);
+
+/* despeckcl::load_nlsar_stats_collection (include/despeckcl.h:95;
+ * nlsar_training.cpp:106-113): reads a Boost text archive as written by the reference (or
+ * by dkx_nlsar_stats_store).  Returns the number of entries, or a negative DKX_E* code.
+ * *n_floats receives the LUT floats the entries need; when out / lut_storage are non-null
+ * and max_entries / max_floats suffice, the entries are written to out in map order with
+ * their LUT arrays in lut_storage (otherwise nothing is written). */
+long long dkx_nlsar_stats_load(const char* filename, dkx_nlsar_stats* out, int max_entries,
+                               float* lut_storage, long long max_floats, long long* n_floats);
+
 /* Inverse CDF of the chi-square distribution (the reference uses GSL's gsl_cdf_chisq_Pinv,
  * src/compute/nlsar/stats.cpp:76). */
 double dkx_chi2_inv(double p, double nu);

@@ -157,6 +157,102 @@ def test_stats_file_roundtrip(D, tmp_path):
         assert np.array_equal(back[k].chi2cdf_inv, fx[k].chi2cdf_inv)
 
 
+def test_stats_file_is_a_boost_text_archive(D, tmp_path):
+    """store_nlsar_stats_collection writes the reference's format: Boost's text archive of
+    std::map<nlsar::params, nlsar::stats> (nlsar_training.cpp:99-104; members as serialized in
+    parameters.h:48-55 and stats.h:40-50)."""
+    fx = D.load_nlsar_stats_collection(os.path.join(GOLDEN, "train32_stats.txt"))
+    f = tmp_path / "nlsar_stats.txt"
+    D.store_nlsar_stats_collection(fx, str(f))
+    tok = f.read_text().split()
+    # signature, library version, map class info, count, item version, class info of the
+    # pair and of params, the first key, class info of stats, lut size
+    assert tok[:16] == ["22", "serialization::archive", "17", "0", "0", "6", "0", "0", "0", "0", "0",
+                        "3", "1", "0", "0", "256"]
+    assert float(tok[16]) == np.float32(fx[(3, 1)].dissims_min)
+    assert tok[18:20] == ["256", "0"]          # quantilles: size, item version
+    assert np.array_equal(np.array(tok[20:276], np.float32), fx[(3, 1)].quantilles)
+    assert tok[276:278] == ["256", "0"]
+    assert tok[534:536] == ["3", "3"]          # second key, no class info any more
+    assert len(tok) == 3 + 2 + 2 + 6 + 6 * (2 + 3 + 2 * (2 + 256))
+
+
+def test_load_boost_archive_layouts(D, tmp_path):
+    """Archives laid out as Boost writes them load: the current layout, 8-digit fixed-point
+    floats of older Boost versions, and library version 3 (no item versions).  No Boost is
+    available here to produce a real file, so the byte-exact output of a given Boost
+    version is parity unpinned; the layout follows the serialize() members above."""
+    e1 = "3 1 {si}3 1.5 2.25 3 {iv}0 0.5 0.75 3 {iv}1 2 100000"
+    e2 = "5 1 3 1.25 3.5 3 {iv}0 0.25 0.5 3 {iv}4 5 6"
+    cases = {
+        "v17": "22 serialization::archive 17 0 0 2 0 0 0 0 0 " + e1.format(si="0 0 ", iv="0 ") + " " + e2.format(iv="0 "),
+        "v12 fixed": "22 serialization::archive 12 0 0 2 0 0 0 0 0 " + e1.format(si="0 0 ", iv="0 ") + " "
+                     + e2.format(iv="0 ").replace("1.25", "1.2500000"),
+        "v3": "22 serialization::archive 3 0 0 2 0 0 0 0 " + e1.format(si="0 0 ", iv="") + " " + e2.format(iv=""),
+    }
+    for label, text in cases.items():
+        f = tmp_path / "a.txt"
+        f.write_text(text + "\n")
+        got = D.load_nlsar_stats_collection(str(f))
+        assert sorted(got.keys()) == [(3, 1), (5, 1)], label
+        a, b = got[(3, 1)], got[(5, 1)]
+        assert a.lut_size == 3 and a.dissims_min == 1.5 and a.dissims_max == 2.25, label
+        assert list(a.quantilles) == [0, 0.5, 0.75] and list(a.chi2cdf_inv) == [1, 2, 100000], label
+        assert b.dissims_min == 1.25 and list(b.chi2cdf_inv) == [4, 5, 6], label
+    bad = tmp_path / "bad.txt"
+    bad.write_text(cases["v17"].rsplit(" ", 1)[0] + "\n")     # one value short
+    with pytest.raises(ValueError, match="statistics file"):
+        D.load_nlsar_stats_collection(str(bad))
+    with pytest.raises(ValueError, match="cannot open"):
+        D.load_nlsar_stats_collection(str(tmp_path / "missing.txt"))
+
+
+CXX_ROUNDTRIP = r"""
+#include "despeckcl.h"
+#include <cstdio>
+#include <stdexcept>
+int main(int argc, char** argv)
+{
+    nlsar_stats_collection a = despeckcl::load_nlsar_stats_collection(argv[1]);
+    despeckcl::store_nlsar_stats_collection(a, argv[2]);
+    nlsar_stats_collection b = despeckcl::load_nlsar_stats_collection(argv[2]);
+    int same = a.size() == b.size();
+    for (const auto& kv : a) {
+        const nlsar::stats& x = kv.second;
+        const nlsar::stats& y = b.at(kv.first);
+        same &= x.lut_size == y.lut_size && x.dissims_min == y.dissims_min &&
+                x.dissims_max == y.dissims_max && x.quantilles == y.quantilles &&
+                x.chi2cdf_inv == y.chi2cdf_inv;
+    }
+    try {
+        despeckcl::load_nlsar_stats_collection(argv[3]);
+        same = 0;
+    } catch (const std::exception&) {
+    }
+    std::printf("%zu %d\n", a.size(), same);
+    return 0;
+}
+"""
+
+
+def test_cxx_api_stats_files_roundtrip(D, tmp_path):
+    """A program written against despeckcl.h stores and loads statistics through the C++ API
+    (nlsar_training.cpp:99-113), and a missing file throws."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "rt.cpp"
+    src.write_text(CXX_ROUNDTRIP)
+    exe = tmp_path / "rt"
+    libdir = os.path.dirname(D._lib.LIB_PATH)
+    subprocess.run([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src),
+                    "-L", libdir, "-ldespeckcl_amd", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe), os.path.join(GOLDEN, "train32_stats.txt"), str(tmp_path / "s.txt"),
+                          str(tmp_path / "missing.txt")], capture_output=True, text=True, check=True).stdout
+    assert out.split() == ["6", "1"]
+    assert (tmp_path / "s.txt").read_text().startswith("22 serialization::archive 17 ")
+
+
 def test_synthetic_is_seeded():
     from despeckcl_amd import synthetic as S
     a = S.interferogram(64, 48, 5)
