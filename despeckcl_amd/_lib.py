@@ -58,6 +58,7 @@ SIGNATURES = {
     "dkx_nlinsar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _ULLP, _P]),
     "dkx_nlsar_training_insar": (_I, [_P, _P, _P, _I, _I, _IP, _I, _IP, _I, _SP, _P]),
     "dkx_nlsar_training_covmat": (_I, [_P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _P]),
+    "dkx_nlsar_training_covmat_device": (_I, [_P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _P, _P]),
     "dkx_nlsar_stats_build": (C.c_longlong, [_P, C.c_longlong, _I, _P, _P, _P, _P]),
     "dkx_chi2_inv": (C.c_double, [C.c_double, C.c_double]),
     "dkx_nlsar_stats_store": (_I, [C.c_char_p, _SP, _I]),

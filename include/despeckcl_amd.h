@@ -114,10 +114,18 @@ int dkx_nlsar_training_insar(const float* ampl_master, const float* ampl_slave,
                              int n_patch_sizes, const int* scale_sizes, int n_scale_sizes,
                              dkx_nlsar_stats* out, float* lut_storage);
 
-/* despeckcl::nlsar_training, covariance overload (include/despeckcl.h:85-92). */
+/* despeckcl::nlsar_training, covariance overload (include/despeckcl.h:85-92).  Both
+ * training entry points run on the current HIP device. */
 int dkx_nlsar_training_covmat(const float* covmat, int height, int width, int dim,
                               const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
                               int n_scale_sizes, dkx_nlsar_stats* out, float* lut_storage);
+
+/* Same on device covariance planes (2*dim*dim interlaced planes of height x width), on
+ * `stream`; the statistics are written to host memory, so the call synchronises the stream. */
+int dkx_nlsar_training_covmat_device(const float* d_covmat, int height, int width, int dim,
+                                     const int* patch_sizes, int n_patch_sizes,
+                                     const int* scale_sizes, int n_scale_sizes,
+                                     dkx_nlsar_stats* out, float* lut_storage, void* stream);
 
 /* nlsar::stats::stats(dissims, lut_size) (src/compute/nlsar/stats.cpp:28-80): quantile LUT
  * and inverse chi-square(49) LUT.  Returns the number of finite dissimilarities (0: none). */
