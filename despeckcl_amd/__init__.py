@@ -18,7 +18,7 @@ from . import _lib
 from ._lib import DespeckleError, DkxStats, check, int_array, lib
 
 __all__ = ["params", "stats", "nlsar_stats_collection", "nlsar_stats_collections",
-           "nlsar_insar", "nlsar", "nlsar_train", "nlsar_train_insar", "nlinsar",
+           "nlsar_insar", "nlsar", "nlsar_train", "nlsar_train_insar", "nlinsar", "boxcar",
            "store_nlsar_stats_collection", "load_nlsar_stats_collection", "DespeckleError"]
 
 _DEFAULT_LOG_LEVELS = ["error", "warning", "fatal"]  # nlsar.i:164, nlinsar.i:43
@@ -182,6 +182,18 @@ def nlsar(covmat_raw, search_window_size, patch_sizes, scale_sizes, nlsar_stats,
     check(lib().dkx_nlsar_covmat(inter.ctypes.data, out.ctypes.data, H, W, d, int(search_window_size),
                                  P, nP, S, nS, arr, ns, float(h), float(c)))
     return out[::2].reshape((d, d, H, W)) + 1j * out[1::2].reshape((d, d, H, W))
+
+
+def boxcar(ampl_master, ampl_slave, phase, window_width, enabled_log_levels=_DEFAULT_LOG_LEVELS):
+    """Boxcar filter; returns (reflectivity, phase, coherence) (boxcar.i:35-62)."""
+    a1, a2, ph = _f32(ampl_master), _f32(ampl_slave), _f32(phase)
+    H, W = _same_shape(a1, a2, ph)
+    ref = np.zeros((H, W), np.float32)
+    phf = np.zeros((H, W), np.float32)
+    coh = np.zeros((H, W), np.float32)
+    check(lib().dkx_boxcar(a1.ctypes.data, a2.ctypes.data, ph.ctypes.data, ref.ctypes.data,
+                           phf.ctypes.data, coh.ctypes.data, H, W, int(window_width)))
+    return ref, phf, coh
 
 
 def nlinsar(ampl_master, ampl_slave, phase, search_window_size, patch_size, niter, lmin,

@@ -55,6 +55,8 @@ SIGNATURES = {
     "dkx_nlsar_covmat_device": (_I, [_P, _P, _I, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F,
                                      _P, _P, _ULLP, _P]),
     "dkx_nlinsar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I]),
+    "dkx_boxcar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I]),
+    "dkx_boxcar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P]),
     "dkx_nlinsar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _ULLP, _P]),
     "dkx_nlsar_training_insar": (_I, [_P, _P, _P, _I, _I, _IP, _I, _IP, _I, _SP, _P]),
     "dkx_nlsar_training_covmat": (_I, [_P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _P]),

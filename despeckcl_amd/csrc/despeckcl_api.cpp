@@ -70,6 +70,15 @@ nlsar::stats& nlsar::stats::operator=(const stats& other)
     return *this;
 }
 
+int despeckcl::boxcar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+                      float* phase_filt, float* coh_filt, const int height, const int width,
+                      const int window_width, std::vector<std::string>)
+{
+    check(dkx_boxcar(ampl_master, ampl_slave, phase, ref_filt, phase_filt, coh_filt, height, width,
+                     window_width));
+    return 0;
+}
+
 int despeckcl::nlsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
                      float* phase_filt, float* coh_filt, const int height, const int width,
                      const int search_window_size, const std::vector<int> patch_sizes,

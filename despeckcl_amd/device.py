@@ -55,6 +55,12 @@ def nlinsar(a1, a2, phase, ref, phase_filt, coh, height, width, search_window_si
     return n.value if want_count else None
 
 
+def boxcar(a1, a2, phase, ref, phase_filt, coh, height, width, window_width, stream=0):
+    """Enqueue the boxcar filter on device buffers."""
+    check(lib().dkx_boxcar_device(a1, a2, phase, ref, phase_filt, coh, int(height), int(width),
+                                  int(window_width), stream or None))
+
+
 def set_tile(filter_index, tile_index):
     check(lib().dkx_set_tile(int(filter_index), int(tile_index)))
 

@@ -10,7 +10,7 @@
  *    archive format with a reader / writer of their own;
  *  - errors throw std::runtime_error / std::invalid_argument instead of terminating;
  *  - stats::operator= also copies lut_size (the reference leaves it unset);
- *  - boxcar and goldstein are not part of this library.
+ *  - goldstein is not part of this library yet.
  */
 #ifndef DESPECKCL_H
 #define DESPECKCL_H
@@ -55,6 +55,10 @@ public:
 typedef std::map<nlsar::params, nlsar::stats> nlsar_stats_collection;
 
 namespace despeckcl {
+
+int boxcar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+           float* phase_filt, float* coh_filt, const int height, const int width,
+           const int window_width, std::vector<std::string> enabled_log_levels);
 
 int nlsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
           float* phase_filt, float* coh_filt, const int height, const int width,

@@ -105,6 +105,20 @@ int dkx_nlinsar_device(const float* d_ampl_master, const float* d_ampl_slave,
                        int patch_size, int niter, int lmin, unsigned long long* n_smoothed,
                        void* stream);
 
+/* despeckcl::boxcar (include/despeckcl.h:34-43; src/compute/boxcar/boxcar.cpp:35-92,
+ * cl_kernels/boxcar_kernel.cl:19-69): w x w window average of the interferogram.  The last
+ * row / column carry the reference's tile-edge values (the value at the last interior row /
+ * column of its last 512-pixel tile, computed on the edge-replicated image).  Odd window
+ * widths up to 15, the reference's range. */
+int dkx_boxcar(const float* ampl_master, const float* ampl_slave, const float* phase,
+               float* ref_filt, float* phase_filt, float* coh_filt, int height, int width,
+               int window_width);
+
+/* Same on device buffers, enqueued on `stream`. */
+int dkx_boxcar_device(const float* d_ampl_master, const float* d_ampl_slave,
+                      const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                      float* d_coh_filt, int height, int width, int window_width, void* stream);
+
 /* despeckcl::nlsar_training, InSAR overload (include/despeckcl.h:75-83;
  * src/compute/nlsar/training/nlsar_training.cpp:57-79).  Writes n_patch*n_scale entries to
  * `out` in std::map order (patch size, then scale size); their LUT arrays point into

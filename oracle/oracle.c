@@ -1399,3 +1399,74 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
     free(img);
     return rc;
 }
+
+/* ====================================================================================
+ * boxcar -- boxcar.cpp:35-92 (512 x 512 tiles, overlap (w - 1) / 2), boxcar_kernel.cl:19-69
+ * ==================================================================================== */
+typedef struct {
+    int w;
+} boxcar_ctx;
+
+/* The kernel on one tile: output (ox, oy) = (in + r) from the window rows ox - r .. ox + r
+ * in row-major order; inputs outside the tile read as 0 (the kernel's local-memory fill);
+ * outputs of the first r rows / columns are not written (they stay 0 here and are never
+ * copied back). */
+static int boxcar_tile(const float* tin, float* tout, int th, int tw, void* vctx)
+{
+    const int w = ((boxcar_ctx*)vctx)->w;
+    const int r = (w - 1) / 2;
+    const size_t n = (size_t)th * tw;
+    const float* A1 = tin;
+    const float* A2 = tin + n;
+    const float* PH = tin + 2 * n;
+    memset(tout, 0, sizeof(float) * 3 * n);
+    const int N = w * w;
+#pragma omp parallel for schedule(static)
+    for (int ox = r; ox < th; ox++)
+        for (int oy = r; oy < tw; oy++) {
+            float a = 0, xr = 0, xi = 0;
+            for (int kx = 0; kx < w; kx++)
+                for (int ky = 0; ky < w; ky++) {
+                    const int ix = ox - r + kx, iy = oy - r + ky;
+                    const int in = ix < th && iy < tw;
+                    const float m = in ? A1[(size_t)ix * tw + iy] : 0.0f;
+                    const float sl = in ? A2[(size_t)ix * tw + iy] : 0.0f;
+                    const float p = in ? PH[(size_t)ix * tw + iy] : 0.0f;
+                    /* pow(x, 2) as x * x; "+=" of a product contracted to an FMA */
+                    a += (m * m + sl * sl) / 2;
+                    const float aa = m * sl;
+                    xr = fmaf(aa, cosf(p), xr);
+                    xi = fmaf(aa, sinf(p), xi);
+                }
+            tout[(size_t)ox * tw + oy] = a / (float)N;
+            tout[n + (size_t)ox * tw + oy] = atan2f(xi, xr);
+            tout[2 * n + (size_t)ox * tw + oy] = sqrtf(xr * xr + xi * xi) / a;
+        }
+    return 0;
+}
+
+int orc_boxcar(const float* a1, const float* a2, const float* phi, float* ref_filt,
+               float* phase_filt, float* coh_filt, int height, int width, int window_width,
+               int tile)
+{
+    if (height <= 0 || width <= 0 || window_width < 1 || window_width % 2 == 0) return -1;
+    const int r = (window_width - 1) / 2;
+    if (tile <= 0) tile = 512;
+    if (tile <= 2 * r) return -1;
+    const size_t n = (size_t)height * width;
+    float* in = (float*)malloc(sizeof(float) * 3 * n);
+    float* out = (float*)calloc(3 * n, sizeof(float));
+    memcpy(in, a1, n * sizeof(float));
+    memcpy(in + n, a2, n * sizeof(float));
+    memcpy(in + 2 * n, phi, n * sizeof(float));
+    boxcar_ctx cx = {window_width};
+    const int rc = map_tiles(in, out, height, width, 3, 3, tile, tile, r, boxcar_tile, &cx);
+    if (rc == 0) {
+        memcpy(ref_filt, out, n * sizeof(float));
+        memcpy(phase_filt, out + n, n * sizeof(float));
+        memcpy(coh_filt, out + 2 * n, n * sizeof(float));
+    }
+    free(in);
+    free(out);
+    return rc;
+}

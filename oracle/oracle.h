@@ -123,6 +123,12 @@ void orc_search_window_smoothing(const float* ampl_master, float* weights, int h
                                  int width_overlap, int patch_size, int search_window_size,
                                  int lmin);
 
+/* despeckcl::boxcar, boxcar.cpp:35-92 with boxcar_kernel.cl:19-69 on 512 x 512 tiles.
+ * tile: tile side (0 = the reference's 512). */
+int orc_boxcar(const float* a1, const float* a2, const float* phi, float* ref_filt,
+               float* phase_filt, float* coh_filt, int height, int width, int window_width,
+               int tile);
+
 #ifdef __cplusplus
 }
 #endif

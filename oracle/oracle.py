@@ -159,6 +159,19 @@ def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stat
     return out, ub.value
 
 
+def boxcar(a1, a2, phi, window_width, tile=0):
+    """Returns (ref, phase, coh)."""
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    out = [np.zeros((H, W), np.float32) for _ in range(3)]
+    rc = lib().orc_boxcar(a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p),
+                          phi.ctypes.data_as(C.c_void_p), *[o.ctypes.data_as(C.c_void_p) for o in out],
+                          C.c_int(H), C.c_int(W), C.c_int(window_width), C.c_int(tile))
+    if rc != 0:
+        raise ValueError(f"oracle boxcar failed ({rc})")
+    return tuple(out)
+
+
 def _training_out(n, stats_arr, qbuf, cbuf):
     res = {}
     for i in range(n):

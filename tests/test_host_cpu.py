@@ -90,6 +90,10 @@ def test_argument_errors_without_device(D):
         D.nlsar(np.ones((2, 3, 16, 16), np.complex64), 21, [3, 5], [1], st)
     with pytest.raises(ValueError, match="odd"):
         D.nlsar(np.ones((3, 3, 16, 16), np.complex64), 20, [3, 5], [1], st)
+    with pytest.raises(ValueError, match="odd"):
+        D.boxcar(a, a, a, 4)
+    with pytest.raises(ValueError, match="above 15"):
+        D.boxcar(a, a, a, 17)
     with pytest.raises(ValueError, match="patch sizes 3 and 7"):
         D.nlinsar(a, a, a, 21, 5, 2, 10)
     with pytest.raises(ValueError, match="odd"):

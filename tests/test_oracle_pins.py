@@ -575,3 +575,77 @@ def test_nlsar_covmat_dim3_phase_rotation_equivariance(oracle):
     print("rotation q99", np.quantile(e, 0.99), "above 1e-3:", int((e > 1e-3).sum()), "of", e.size)
     assert np.quantile(e, 0.99) <= 1e-3
     assert (e > 1e-3).sum() <= max(2, 0.002 * e.size)
+
+
+# ---- boxcar (boxcar.cpp:35-92, boxcar_kernel.cl:19-69) -----------------------------------
+
+
+def _boxcar_numpy(a1, a2, ph, w, rows, cols):
+    """Mathematical boxcar in float64 on the edge-replicated input, evaluated at the given
+    (possibly virtual, beyond the edge) rows / columns."""
+    r = (w - 1) // 2
+    H, W = a1.shape
+    t = (a1.astype(np.float64) ** 2 + a2.astype(np.float64) ** 2) / 2
+    x = a1.astype(np.float64) * a2 * np.exp(1j * ph.astype(np.float64))
+    ri = np.clip(np.asarray(rows)[:, None] + np.arange(-r, r + 1)[None, :], 0, H - 1)
+    ci = np.clip(np.asarray(cols)[:, None] + np.arange(-r, r + 1)[None, :], 0, W - 1)
+    a = t[:, ci].sum(axis=2)[ri].sum(axis=1)
+    xs = x[:, ci].sum(axis=2)[ri].sum(axis=1)
+    return a / (w * w), np.angle(xs), np.abs(xs) / a
+
+
+def _virtual_last(n, w, tile=512):
+    """Image row (column) whose value the last row (column) receives: the last interior row
+    of the last tile, written through the clamped copy-back."""
+    r = (w - 1) // 2
+    step = tile - 2 * r
+    k_last = (n + r - 1) // step
+    return (k_last + 1) * step - 1
+
+
+@pytest.mark.parametrize("shape,w", [((40, 50), 5), ((23, 9), 3), ((600, 37), 7), ((5, 530), 15)])
+def test_boxcar_matches_edge_replicated_average(oracle, shape, w):
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.interferogram(*shape, 61)
+    ref, phase, coh = oracle.boxcar(a1, a2, ph, w)
+    H, W = shape
+    rows = list(range(H - 1)) + [_virtual_last(H, w)]
+    cols = list(range(W - 1)) + [_virtual_last(W, w)]
+    nr, nph, nc = _boxcar_numpy(a1, a2, ph, w, rows, cols)
+    tol = 1e-6 + 1e-7 * w * w          # float rounding of w*w sequential additions
+    assert np.allclose(ref, nr, rtol=tol, atol=0)
+    assert np.allclose(coh, nc, rtol=0, atol=tol)
+    z = np.abs(np.exp(1j * phase.astype(np.float64)) * coh - np.exp(1j * nph) * nc)
+    assert z.max() <= tol
+    # a plausible tiling mistake, the last row taken at its true position, is detected
+    if H > 1 and _virtual_last(H, w) - (w - 1) // 2 > H - 1:
+        tr = _boxcar_numpy(a1, a2, ph, w, [H - 1], cols)[0]
+        assert not np.allclose(ref[-1], tr[0], rtol=tol, atol=0)
+
+
+def test_boxcar_closed_forms(oracle):
+    a1 = np.full((20, 30), 3.0, np.float32)
+    a2 = np.full((20, 30), 4.0, np.float32)
+    ph = np.full((20, 30), 0.7, np.float32)
+    ref, phase, coh = oracle.boxcar(a1, a2, ph, 5)
+    assert np.allclose(ref, 12.5, rtol=1e-6)
+    assert np.allclose(phase, 0.7, atol=1e-6)
+    assert np.allclose(coh, 24.0 / 25.0, atol=1e-6)
+    # window 1: the pixel itself
+    from despeckcl_amd import synthetic as S
+    b1, b2, bp = S.interferogram(17, 13, 62)
+    r1, p1, c1 = oracle.boxcar(b1, b2, bp, 1)
+    assert np.array_equal(r1, (b1 * b1 + b2 * b2) / np.float32(2))
+    b1d, b2d = b1.astype(np.float64), b2.astype(np.float64)
+    assert np.allclose(c1, 2 * b1d * b2d / (b1d ** 2 + b2d ** 2), atol=1e-6)
+    assert np.allclose(np.angle(np.exp(1j * (p1 - bp))), 0, atol=1e-6)
+
+
+def test_boxcar_tile_independence(oracle):
+    """Away from the last row / column the tiling does not change the result."""
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.interferogram(150, 140, 63)
+    t512 = oracle.boxcar(a1, a2, ph, 7)
+    t40 = oracle.boxcar(a1, a2, ph, 7, tile=40)
+    for k in range(3):
+        assert np.array_equal(t512[k][:-1, :-1], t40[k][:-1, :-1])
