@@ -324,6 +324,64 @@ def boxcar_form(args):
     print(json.dumps(out), flush=True)
 
 
+def goldstein_form(args):
+    """Goldstein filter (default: seeded 8192^2 interferogram, patch 32, overlap 4, alpha 0.5,
+    the reference example's parameters) with inputs and outputs resident in HBM.  Roofline:
+    HBM, 20 algorithmic bytes per pixel (three float planes read, two written)."""
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    n, P, ov, alpha = args.size or 8192, 32, 4, 0.5
+    a1, a2, ph = S.interferogram(n, n, S.SEED_BASE + 5, n_regions=16)
+    stream = HIP.Stream()
+    d_in = [HIP.DeviceArray.from_host(x) for x in (a1, a2, ph)]
+    d_out = [HIP.DeviceArray((n, n)) for _ in range(2)]
+
+    def call():
+        DV.goldstein(*[d.ptr for d in d_in], *[d.ptr for d in d_out], n, n, P, ov, alpha, stream=stream.handle)
+
+    for _ in range(max(1, args.warmup)):
+        call()
+    stream.synchronize()
+    e0, e1 = HIP.Event(), HIP.Event()
+    e0.record(stream.handle)
+    for _ in range(args.steps):
+        call()
+    e1.record(stream.handle)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / args.steps
+    gbs = 20.0 * n * n / (ms * 1e-3) / 1e9
+    out = {
+        "metric": f"Mpixel/s Goldstein {n}x{n}, patch {P}, overlap {ov}, alpha {alpha} (1 GPU)",
+        "value": round(n * n / 1e6 / (ms / 1e3), 3), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded speckle model of insarsim_simu.cpp, 16 regions)",
+        "config": {"workload": f"Goldstein, one {n}x{n} interferogram per call", "patch_size": P,
+                   "overlap": ov, "alpha": alpha},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(gbs / 8000.0, 4), "traffic": None,
+                     "kernel": "gs_patches + gs_pack (whole call)",
+                     "note": "20 algorithmic bytes per pixel per call; the weighted patches "
+                             "(8 B x (P/(P-2ov))^2 per pixel) make one extra round trip"},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.build()
+        c = min(n, 1024)
+        t0 = time.perf_counter()
+        O.goldstein(a1[:c, :c], a2[:c, :c], ph[:c, :c], P, ov, alpha)
+        dt = time.perf_counter() - t0
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": round(c * c / 1e6 / dt, 3), "unit": "Mpixel/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"Goldstein (exact DFT) on the {c}x{c} top-left crop, {dt:.2f} s on "
+                                         f"{cores} OpenMP threads"}
+    print(json.dumps(out), flush=True)
+
+
 def served(args):
     out = served_run(args.requests, args.rate, args.budget_gib)
     print(json.dumps(out), flush=True)
@@ -432,15 +490,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the served-stream latency sample")
     ap.add_argument("--tile", type=int, default=None)
-    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served", "covmat", "boxcar"),
-                    default="plain")
+    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served", "covmat", "boxcar",
+                                       "goldstein"), default="plain")
     ap.add_argument("--window", type=int, default=5, help="window width (--form boxcar)")
     ap.add_argument("--dim", type=int, default=3, help="covariance dimension (--form covmat)")
     ap.add_argument("--requests", type=int, default=2000, help="requests (--form served)")
     ap.add_argument("--rate", type=float, default=40.0, help="Poisson arrival rate, requests/s (--form served)")
     ap.add_argument("--budget-gib", type=int, default=150, help="server HBM budget (--form served)")
     ap.add_argument("--size", type=int, default=None,
-                    help="image side (--form nlinsar: 4096, covmat: 2048, boxcar: 8192)")
+                    help="image side (--form nlinsar: 4096, covmat: 2048, boxcar / goldstein: 8192)")
     ap.add_argument("--niter", type=int, default=10, help="iterations (--form nlinsar)")
     ap.add_argument("--tiles", type=int, default=64, help="images per batch (--form batched)")
     args = ap.parse_args()
@@ -458,6 +516,8 @@ def main():
         return covmat_form(args)
     if args.form == "boxcar":
         return boxcar_form(args)
+    if args.form == "goldstein":
+        return goldstein_form(args)
 
     import despeckcl_amd as D
     from despeckcl_amd import device as DV

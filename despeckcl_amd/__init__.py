@@ -18,7 +18,7 @@ from . import _lib
 from ._lib import DespeckleError, DkxStats, check, int_array, lib
 
 __all__ = ["params", "stats", "nlsar_stats_collection", "nlsar_stats_collections",
-           "nlsar_insar", "nlsar", "nlsar_train", "nlsar_train_insar", "nlinsar", "boxcar",
+           "nlsar_insar", "nlsar", "nlsar_train", "nlsar_train_insar", "nlinsar", "boxcar", "goldstein",
            "store_nlsar_stats_collection", "load_nlsar_stats_collection", "DespeckleError"]
 
 _DEFAULT_LOG_LEVELS = ["error", "warning", "fatal"]  # nlsar.i:164, nlinsar.i:43
@@ -193,6 +193,21 @@ def boxcar(ampl_master, ampl_slave, phase, window_width, enabled_log_levels=_DEF
     coh = np.zeros((H, W), np.float32)
     check(lib().dkx_boxcar(a1.ctypes.data, a2.ctypes.data, ph.ctypes.data, ref.ctypes.data,
                            phf.ctypes.data, coh.ctypes.data, H, W, int(window_width)))
+    return ref, phf, coh
+
+
+def goldstein(ampl_master, ampl_slave, phase, patch_size, overlap, alpha,
+              enabled_log_levels=_DEFAULT_LOG_LEVELS):
+    """Goldstein filter; returns (reflectivity, phase, coherence) (goldstein.i:33-75).  As in
+    the reference, the coherence is returned as passed in (zeros)."""
+    a1, a2, ph = _f32(ampl_master), _f32(ampl_slave), _f32(phase)
+    H, W = _same_shape(a1, a2, ph)
+    ref = np.zeros((H, W), np.float32)
+    phf = np.zeros((H, W), np.float32)
+    coh = np.zeros((H, W), np.float32)
+    check(lib().dkx_goldstein(a1.ctypes.data, a2.ctypes.data, ph.ctypes.data, ref.ctypes.data,
+                              phf.ctypes.data, coh.ctypes.data, H, W, int(patch_size), int(overlap),
+                              float(alpha)))
     return ref, phf, coh
 
 

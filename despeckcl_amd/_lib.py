@@ -56,6 +56,8 @@ SIGNATURES = {
                                      _P, _P, _ULLP, _P]),
     "dkx_nlinsar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I]),
     "dkx_boxcar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I]),
+    "dkx_goldstein": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F]),
+    "dkx_goldstein_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P]),
     "dkx_boxcar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P]),
     "dkx_nlinsar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _ULLP, _P]),
     "dkx_nlsar_training_insar": (_I, [_P, _P, _P, _I, _I, _IP, _I, _IP, _I, _SP, _P]),

@@ -2,6 +2,7 @@
 #include "../../include/despeckcl_amd.h"
 
 #include "dkx_common.h"
+#include "dkx_goldstein.h"
 #include "dkx_nlinsar.h"
 #include "dkx_nlsar.h"
 
@@ -269,6 +270,30 @@ static int nlinsar_device_impl(NlinsarCall& call, const float* a1, const float* 
     return workspace_release(w, stream);
 }
 
+static int goldstein_prepare(GoldsteinParams& gp, GoldsteinPlan& pl, int height, int width,
+                             int patch_size, int overlap, float alpha)
+{
+    if (!std::isfinite(alpha)) {
+        set_error("goldstein: alpha must be finite");
+        return DKX_EINVAL;
+    }
+    gp = GoldsteinParams{height, width, patch_size, overlap, alpha};
+    return goldstein_plan(gp, &pl);
+}
+
+static int goldstein_device_impl(const GoldsteinParams& gp, const GoldsteinPlan& pl,
+                                 const float* a1, const float* a2, const float* phi, float* ref,
+                                 float* pha, hipStream_t stream, DeviceWorkspace* w)
+{
+    int rc = ensure(w, std::max<size_t>(pl.ws_bytes, 256));
+    if (rc) return rc;
+    rc = workspace_acquire(w, stream);
+    if (rc) return rc;
+    rc = goldstein_run(gp, pl, a1, a2, phi, ref, pha, w->ptr, stream);
+    if (rc) return rc;
+    return workspace_release(w, stream);
+}
+
 // host arrays: copy in, run on the library stream, copy out
 struct HostStage {
     void* ptr = nullptr;
@@ -451,6 +476,51 @@ int dkx_nlinsar(const float* a1, const float* a2, const float* phi, float* ref, 
     DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+int dkx_goldstein_device(const float* a1, const float* a2, const float* phi, float* ref,
+                         float* pha, float* coh, int height, int width, int patch_size,
+                         int overlap, float alpha, void* stream)
+{
+    (void)coh;   // not written, as in the reference
+    if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
+    GoldsteinParams gp;
+    GoldsteinPlan pl;
+    int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
+    if (rc) return rc;
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    return goldstein_device_impl(gp, pl, a1, a2, phi, ref, pha, (hipStream_t)stream, w);
+}
+
+int dkx_goldstein(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                  float* coh, int height, int width, int patch_size, int overlap, float alpha)
+{
+    (void)coh;   // not written, as in the reference
+    if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
+    GoldsteinParams gp;
+    GoldsteinPlan pl;
+    int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
+    if (rc) return rc;
+    DeviceWorkspace* w = workspace_for_current_device();
+    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+    std::lock_guard<std::mutex> lk(w->mu);
+    rc = ensure_stream(w);
+    if (rc) return rc;
+    const size_t n = (size_t)height * width;
+    HostStage st;
+    DKX_HIP_CHECK(hipMalloc(&st.ptr, 5 * n * sizeof(float)));
+    float* d = (float*)st.ptr;
+    const float* src[3] = {a1, a2, phi};
+    for (int k = 0; k < 3; k++)
+        DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+    rc = goldstein_device_impl(gp, pl, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, w->stream, w);
+    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
     DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
     return 0;
 }

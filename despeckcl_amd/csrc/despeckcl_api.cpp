@@ -79,6 +79,18 @@ int despeckcl::boxcar(float* ampl_master, float* ampl_slave, float* phase, float
     return 0;
 }
 
+int despeckcl::goldstein(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+                         float* phase_filt, float* coh_filt, const unsigned int height,
+                         const unsigned int width, const unsigned int patch_size,
+                         const unsigned int overlap, const float alpha, std::vector<std::string>)
+{
+    if (height > 0x7fffffffu || width > 0x7fffffffu || patch_size > 0x7fffffffu || overlap > 0x7fffffffu)
+        throw std::invalid_argument("goldstein: sizes out of range");
+    check(dkx_goldstein(ampl_master, ampl_slave, phase, ref_filt, phase_filt, coh_filt, (int)height,
+                        (int)width, (int)patch_size, (int)overlap, alpha));
+    return 0;
+}
+
 int despeckcl::nlsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
                      float* phase_filt, float* coh_filt, const int height, const int width,
                      const int search_window_size, const std::vector<int> patch_sizes,

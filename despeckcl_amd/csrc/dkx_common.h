@@ -35,6 +35,10 @@ __device__ __forceinline__ void gstore4(DKX_GLOBAL void* p, float4 v)
 {
     *(DKX_GLOBAL dkx_f4*)p = dkx_f4{v.x, v.y, v.z, v.w};
 }
+__device__ __forceinline__ void gstore2(DKX_GLOBAL void* p, float2 v)
+{
+    *(DKX_GLOBAL dkx_f2*)p = dkx_f2{v.x, v.y};
+}
 
 // A padded planar grid in image coordinates: element (y, x) of plane k lives at
 // base[k * plane + (y + oy) * pitch + (x + ox)], for y in [-oy, h - oy), x in [-ox, w - ox).

@@ -61,6 +61,12 @@ def boxcar(a1, a2, phase, ref, phase_filt, coh, height, width, window_width, str
                                   int(window_width), stream or None))
 
 
+def goldstein(a1, a2, phase, ref, phase_filt, height, width, patch_size, overlap, alpha, stream=0):
+    """Enqueue the Goldstein filter on device buffers (no coherence output, as the reference)."""
+    check(lib().dkx_goldstein_device(a1, a2, phase, ref, phase_filt, None, int(height), int(width),
+                                     int(patch_size), int(overlap), float(alpha), stream or None))
+
+
 def set_tile(filter_index, tile_index):
     check(lib().dkx_set_tile(int(filter_index), int(tile_index)))
 

@@ -10,7 +10,8 @@
  *    archive format with a reader / writer of their own;
  *  - errors throw std::runtime_error / std::invalid_argument instead of terminating;
  *  - stats::operator= also copies lut_size (the reference leaves it unset);
- *  - goldstein is not part of this library yet.
+ *  - goldstein reproduces the reference's results, including its quirks (see
+ *    despeckcl_amd.h).
  */
 #ifndef DESPECKCL_H
 #define DESPECKCL_H
@@ -59,6 +60,12 @@ namespace despeckcl {
 int boxcar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
            float* phase_filt, float* coh_filt, const int height, const int width,
            const int window_width, std::vector<std::string> enabled_log_levels);
+
+int goldstein(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
+              float* phase_filt, float* coh_filt, const unsigned int height,
+              const unsigned int width, const unsigned int patch_size,
+              const unsigned int overlap, const float alpha,
+              std::vector<std::string> enabled_log_levels);
 
 int nlsar(float* ampl_master, float* ampl_slave, float* phase, float* ref_filt,
           float* phase_filt, float* coh_filt, const int height, const int width,

@@ -119,6 +119,23 @@ int dkx_boxcar_device(const float* d_ampl_master, const float* d_ampl_slave,
                       const float* d_phase, float* d_ref_filt, float* d_phase_filt,
                       float* d_coh_filt, int height, int width, int window_width, void* stream);
 
+/* despeckcl::goldstein (include/despeckcl.h:111-122; src/compute/goldstein/goldstein.cpp:39-106,
+ * goldstein_filter_sub_image.cpp:73-187).  ref_filt / phase_filt receive |z| and arg z of the
+ * filtered interferogram; coh_filt is not written, as in the reference.  Reproduces the
+ * reference: the filtered patches are added onto the raw interferogram (so alpha = 0 gives
+ * |z| = a1 + a2), the tile is the reference's single-device tile, and the last row / column
+ * carry its tile-edge values.  Patch sizes 16, 32 and 64; 0 <= overlap <= patch_size / 4 (at
+ * most two patches overlap, the case the reference's packing passes handle). */
+int dkx_goldstein(const float* ampl_master, const float* ampl_slave, const float* phase,
+                  float* ref_filt, float* phase_filt, float* coh_filt, int height, int width,
+                  int patch_size, int overlap, float alpha);
+
+/* Same on device buffers, enqueued on `stream`. */
+int dkx_goldstein_device(const float* d_ampl_master, const float* d_ampl_slave,
+                         const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                         float* d_coh_filt, int height, int width, int patch_size, int overlap,
+                         float alpha, void* stream);
+
 /* despeckcl::nlsar_training, InSAR overload (include/despeckcl.h:75-83;
  * src/compute/nlsar/training/nlsar_training.cpp:57-79).  Writes n_patch*n_scale entries to
  * `out` in std::map order (patch size, then scale size); their LUT arrays point into

@@ -1470,3 +1470,202 @@ int orc_boxcar(const float* a1, const float* a2, const float* phi, float* ref_fi
     free(out);
     return rc;
 }
+
+/* ====================================================================================
+ * Goldstein -- goldstein.cpp:39-106, goldstein_filter_sub_image.cpp:73-187,
+ * tile_size.cpp:24-33 (round_up), cl_kernels/{raw_interferogram, patches_unpack,
+ * weighted_multiply, patches_pack, slc2real}.cl, cl_wrappers/patches_pack.cpp:48-72
+ * ==================================================================================== */
+
+/* patches_unpack.cl:19-42 */
+void orc_gs_unpack(const float* packed_re, const float* packed_im, float* unpacked_re,
+                   float* unpacked_im, int height_unpacked, int width_unpacked, int patch_size,
+                   int overlap)
+{
+    const int m = patch_size - 2 * overlap;
+    const int width_packed = (width_unpacked / patch_size) * m;
+    const int height_packed = (height_unpacked / patch_size) * m;
+    for (int ty = 0; ty < height_unpacked; ty++)
+        for (int tx = 0; tx < width_unpacked; tx++) {
+            const int px = tx / patch_size, py = ty / patch_size;
+            const int rx = tx % patch_size, ry = ty % patch_size;
+            const int ix = imin(width_packed - 1, imax(0, px * m + (rx - overlap)));
+            const int iy = imin(height_packed - 1, imax(0, py * m + (ry - overlap)));
+            unpacked_re[(size_t)ty * width_unpacked + tx] = packed_re[(size_t)iy * width_packed + ix];
+            unpacked_im[(size_t)ty * width_unpacked + tx] = packed_im[(size_t)iy * width_packed + ix];
+        }
+}
+
+/* patches_pack.cl:19-66 in the four passes of patches_pack.cpp:48-72, offsets (x, y) =
+ * (0, 0), (0, P), (P, 0), (P, P); "+=" into the packed buffer as it is.  Patches beyond
+ * the unpacked extent (the reference's launch covers one when a direction has a single
+ * patch, and reads out of bounds) contribute nothing. */
+void orc_gs_pack(const float* unpacked_re, const float* unpacked_im, float* packed_re,
+                 float* packed_im, int height_unpacked, int width_unpacked, int patch_size,
+                 int overlap)
+{
+    const int P = patch_size, m = patch_size - 2 * overlap;
+    const int width_packed = width_unpacked / P * m;
+    const int height_packed = height_unpacked / P * m;
+    const int npx = width_unpacked / P, npy = height_unpacked / P;
+    const int offs[4][2] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+    for (int o = 0; o < 4; o++)
+        for (int py = offs[o][1]; py < npy; py += 2)
+            for (int px = offs[o][0]; px < npx; px += 2)
+                for (int ry = 0; ry < P; ry++)
+                    for (int rx = 0; rx < P; rx++) {
+                        const int in_x = px * P + rx, in_y = py * P + ry;
+                        const int out_x = px * m + (rx - overlap), out_y = py * m + (ry - overlap);
+                        float sf = 1.0f;
+                        if (rx < 2 * overlap || rx >= P - 2 * overlap) sf *= 0.5f;
+                        if (ry < 2 * overlap || ry >= P - 2 * overlap) sf *= 0.5f;
+                        if (out_x < overlap || out_x >= width_packed - overlap) sf *= 2.0f;
+                        if (out_y < overlap || out_y >= height_packed - overlap) sf *= 2.0f;
+                        const float vr = sf * unpacked_re[(size_t)in_y * width_unpacked + in_x];
+                        const float vi = sf * unpacked_im[(size_t)in_y * width_unpacked + in_x];
+                        if (out_x >= 0 && out_x < width_packed && out_y >= 0 && out_y < height_packed) {
+                            packed_re[(size_t)out_y * width_packed + out_x] += vr;
+                            packed_im[(size_t)out_y * width_packed + out_x] += vi;
+                        }
+                    }
+}
+
+/* 1-D DFT of n points in double, in place (sign -1 forward, +1 inverse, unscaled) */
+static void dft1(double* re, double* im, int n, int sign, const double* c, const double* s)
+{
+    double tr[256], ti[256];
+    for (int k = 0; k < n; k++) {
+        double ar = 0, ai = 0;
+        for (int j = 0; j < n; j++) {
+            const int t = (int)(((long long)j * k) % n);
+            ar += re[j] * c[t] - sign * im[j] * s[t];
+            ai += im[j] * c[t] + sign * re[j] * s[t];
+        }
+        tr[k] = ar;
+        ti[k] = ai;
+    }
+    memcpy(re, tr, sizeof(double) * n);
+    memcpy(im, ti, sizeof(double) * n);
+}
+
+/* 2-D transform of every P x P patch of a planar complex image (goldstein_patch_ft.cpp:24-48
+ * with clFFT's default scales: 1 forward, 1/P^2 backward), exact in double, rounded to
+ * float */
+void orc_gs_patch_ft(float* re, float* im, int height, int width, int P, int inverse)
+{
+    double c[256], s[256];
+    for (int k = 0; k < P; k++) {
+        c[k] = cos(2.0 * M_PI * k / P);
+        s[k] = sin(2.0 * M_PI * k / P);
+    }
+    const int sign = inverse ? 1 : -1;
+    const double scale = inverse ? 1.0 / ((double)P * P) : 1.0;
+    const int npy = height / P, npx = width / P;
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < npy * npx; q++) {
+        const int py = q / npx, px = q % npx;
+        double br[256 * 256 / 4], bi[256 * 256 / 4];   /* P <= 128 */
+        double vr[256], vi[256];
+        for (int y = 0; y < P; y++)
+            for (int x = 0; x < P; x++) {
+                const size_t g = (size_t)(py * P + y) * width + px * P + x;
+                br[y * P + x] = re[g];
+                bi[y * P + x] = im[g];
+            }
+        for (int y = 0; y < P; y++) dft1(br + y * P, bi + y * P, P, sign, c, s);
+        for (int x = 0; x < P; x++) {
+            for (int y = 0; y < P; y++) {
+                vr[y] = br[y * P + x];
+                vi[y] = bi[y * P + x];
+            }
+            dft1(vr, vi, P, sign, c, s);
+            for (int y = 0; y < P; y++) {
+                br[y * P + x] = vr[y];
+                bi[y * P + x] = vi[y];
+            }
+        }
+        for (int y = 0; y < P; y++)
+            for (int x = 0; x < P; x++) {
+                const size_t g = (size_t)(py * P + y) * width + px * P + x;
+                re[g] = (float)(br[y * P + x] * scale);
+                im[g] = (float)(bi[y * P + x] * scale);
+            }
+    }
+}
+
+typedef struct {
+    int P, ov;
+    float alpha;
+} gs_ctx;
+
+/* goldstein::filter_sub_image on one T x T tile; outputs |z| and arg z */
+static int goldstein_tile(const float* tin, float* tout, int T, int Tw, void* vctx)
+{
+    const gs_ctx* cx = (const gs_ctx*)vctx;
+    const int P = cx->P, ov = cx->ov, m = P - 2 * ov;
+    const size_t n = (size_t)T * Tw;
+    const int npy = T / m, npx = Tw / m;
+    const int hu = P * npy, wu = P * npx;
+    float* zr = (float*)calloc(n, sizeof(float));
+    float* zi = (float*)calloc(n, sizeof(float));
+    float* ur = (float*)malloc(sizeof(float) * (size_t)hu * wu);
+    float* ui = (float*)malloc(sizeof(float) * (size_t)hu * wu);
+    /* raw_interferogram.cl:19-35 */
+    for (size_t i = 0; i < n; i++) {
+        const float a = 0.5f * (tin[i] + tin[n + i]);
+        zr[i] = a * cosf(tin[2 * n + i]);
+        zi[i] = a * sinf(tin[2 * n + i]);
+    }
+    orc_gs_unpack(zr, zi, ur, ui, hu, wu, P, ov);
+    orc_gs_patch_ft(ur, ui, hu, wu, P, 0);
+    /* weighted_multiply.cl:19-36: |Z|^alpha Z */
+    for (size_t i = 0; i < (size_t)hu * wu; i++) {
+        const float psd = sqrtf(ur[i] * ur[i] + ui[i] * ui[i]);
+        ur[i] = powf(psd, cx->alpha) * ur[i];
+        ui[i] = powf(psd, cx->alpha) * ui[i];
+    }
+    orc_gs_patch_ft(ur, ui, hu, wu, P, 1);
+    /* the packed buffer still holds the raw interferogram */
+    orc_gs_pack(ur, ui, zr, zi, hu, wu, P, ov);
+    /* slc2real.cl:19-35 */
+    for (size_t i = 0; i < n; i++) {
+        tout[i] = fabsf(sqrtf(zr[i] * zr[i] + zi[i] * zi[i]));
+        tout[n + i] = atan2f(zi[i], zr[i]);
+    }
+    free(zr);
+    free(zi);
+    free(ur);
+    free(ui);
+    return 0;
+}
+
+int orc_goldstein(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                  float* phase_filt, int height, int width, int patch_size, int overlap,
+                  float alpha)
+{
+    if (height <= 0 || width <= 0 || patch_size <= 0 || patch_size > 128 || overlap < 0 ||
+        patch_size - 2 * overlap <= 0)
+        return -1;
+    const int m = patch_size - 2 * overlap;
+    /* goldstein.cpp:64-67 with tile_size.cpp:30-33 (round_up adds a full multiple); the
+     * device-memory cap of tile_size.cpp:35-77 is taken as inactive */
+    const int big = imax(height + 2 * overlap, width + 2 * overlap);
+    const int T = big - big % m + m;
+    const size_t n = (size_t)height * width;
+    float* in = (float*)malloc(sizeof(float) * 3 * n);
+    float* out = (float*)calloc(2 * n, sizeof(float));
+    memcpy(in, a1, n * sizeof(float));
+    memcpy(in + n, a2, n * sizeof(float));
+    memcpy(in + 2 * n, phi, n * sizeof(float));
+    memcpy(out, ref_filt, n * sizeof(float));
+    memcpy(out + n, phase_filt, n * sizeof(float));
+    gs_ctx cx = {patch_size, overlap, alpha};
+    const int rc = map_tiles(in, out, height, width, 3, 2, T, T, overlap, goldstein_tile, &cx);
+    if (rc == 0) {
+        memcpy(ref_filt, out, n * sizeof(float));
+        memcpy(phase_filt, out + n, n * sizeof(float));
+    }
+    free(in);
+    free(out);
+    return rc;
+}

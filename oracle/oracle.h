@@ -129,6 +129,23 @@ int orc_boxcar(const float* a1, const float* a2, const float* phi, float* ref_fi
                float* phase_filt, float* coh_filt, int height, int width, int window_width,
                int tile);
 
+/* Goldstein (goldstein.cpp:39-106, goldstein_filter_sub_image.cpp:73-187, the kernels of
+ * src/compute/goldstein/cl_kernels).
+ * ref_filt / phase_filt receive |z| and arg z; coh_filt is not touched (the reference never
+ * writes it).  The FFTs are exact (double-precision DFT, rounded to float). */
+int orc_goldstein(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                  float* phase_filt, int height, int width, int patch_size, int overlap,
+                  float alpha);
+/* the single kernels, for the reference's own unit tests (patches_unpack.cl, patches_pack.cl
+ * with its four passes, goldstein_patch_ft.cpp with clFFT's default scaling) */
+void orc_gs_unpack(const float* packed_re, const float* packed_im, float* unpacked_re,
+                   float* unpacked_im, int height_unpacked, int width_unpacked, int patch_size,
+                   int overlap);
+void orc_gs_pack(const float* unpacked_re, const float* unpacked_im, float* packed_re,
+                 float* packed_im, int height_unpacked, int width_unpacked, int patch_size,
+                 int overlap);
+void orc_gs_patch_ft(float* re, float* im, int height, int width, int patch_size, int inverse);
+
 #ifdef __cplusplus
 }
 #endif

@@ -172,6 +172,21 @@ def boxcar(a1, a2, phi, window_width, tile=0):
     return tuple(out)
 
 
+def goldstein(a1, a2, phi, patch_size, overlap, alpha):
+    """Returns (ref, phase); the reference leaves coh_filt untouched."""
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    ref = np.zeros((H, W), np.float32)
+    ph = np.zeros((H, W), np.float32)
+    rc = lib().orc_goldstein(a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p),
+                             phi.ctypes.data_as(C.c_void_p), ref.ctypes.data_as(C.c_void_p),
+                             ph.ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W),
+                             C.c_int(patch_size), C.c_int(overlap), C.c_float(alpha))
+    if rc != 0:
+        raise ValueError(f"oracle goldstein failed ({rc})")
+    return ref, ph
+
+
 def _training_out(n, stats_arr, qbuf, cbuf):
     res = {}
     for i in range(n):

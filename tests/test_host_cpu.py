@@ -94,6 +94,12 @@ def test_argument_errors_without_device(D):
         D.boxcar(a, a, a, 4)
     with pytest.raises(ValueError, match="above 15"):
         D.boxcar(a, a, a, 17)
+    with pytest.raises(ValueError, match="patch sizes 16, 32 and 64"):
+        D.goldstein(a, a, a, 48, 4, 0.5)
+    with pytest.raises(ValueError, match="overlap"):
+        D.goldstein(a, a, a, 32, 16, 0.5)
+    with pytest.raises(ValueError, match="quarter"):
+        D.goldstein(a, a, a, 16, 5, 0.5)
     with pytest.raises(ValueError, match="patch sizes 3 and 7"):
         D.nlinsar(a, a, a, 21, 5, 2, 10)
     with pytest.raises(ValueError, match="odd"):

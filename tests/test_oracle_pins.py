@@ -649,3 +649,93 @@ def test_boxcar_tile_independence(oracle):
     t40 = oracle.boxcar(a1, a2, ph, 7, tile=40)
     for k in range(3):
         assert np.array_equal(t512[k][:-1, :-1], t40[k][:-1, :-1])
+
+
+# ---- Goldstein (goldstein_filter_sub_image.cpp:73-187, goldstein/cl_kernels) -------------
+
+
+def _gs_pack(oracle, ur, ui, P, ov, init=None):
+    hu, wu = ur.shape
+    m = P - 2 * ov
+    hp, wp = hu // P * m, wu // P * m
+    pr = np.zeros((hp, wp), np.float32) if init is None else init[0].copy()
+    pi = np.zeros((hp, wp), np.float32) if init is None else init[1].copy()
+    oracle.call("orc_gs_pack", f32(ur), f32(ui), pr, pi, hu, wu, P, ov)
+    return pr, pi
+
+
+def test_goldstein_pack_completeness(oracle):
+    """patches_pack_test.cpp:35-85: packing all-one patches gives one everywhere."""
+    P, ov = 32, 4
+    hp, wp = 48, 72
+    hu, wu = hp // (P - 2 * ov) * P, wp // (P - 2 * ov) * P
+    pr, pi = _gs_pack(oracle, np.ones((hu, wu)), np.ones((hu, wu)), P, ov)
+    assert pr.shape == (hp, wp)
+    assert np.all(pr == 1.0) and np.all(pi == 1.0)
+
+
+def test_goldstein_unpack_pack_roundtrip(oracle):
+    """patches_packing_test.cpp:33-110: unpack then pack returns the packed image."""
+    P, ov = 32, 4
+    hp, wp = 48, 72
+    hu, wu = hp // (P - 2 * ov) * P, wp // (P - 2 * ov) * P
+    rng = np.random.default_rng(5)
+    xr = f32(rng.integers(1, 10, (hp, wp)))
+    xi = f32(rng.integers(1, 10, (hp, wp)))
+    ur = np.zeros((hu, wu), np.float32)
+    ui = np.zeros((hu, wu), np.float32)
+    oracle.call("orc_gs_unpack", xr, xi, ur, ui, hu, wu, P, ov)
+    pr, pi = _gs_pack(oracle, ur, ui, P, ov)
+    assert np.allclose(pr, xr, atol=1e-4) and np.allclose(pi, xi, atol=1e-4)
+
+
+def test_goldstein_patch_ft_known_answers(oracle):
+    """goldstein_patch_ft_test.cpp:35-130: a constant patch transforms to P^2 at its corner;
+    the inverse (scaled 1/P^2, clFFT's default) undoes the forward; every patch agrees with
+    numpy's 2-D FFT."""
+    P = 16
+    h, w = 2 * P, 3 * P
+    re = np.ones((h, w), np.float32)
+    im = np.zeros((h, w), np.float32)
+    oracle.call("orc_gs_patch_ft", re, im, h, w, P, 0)
+    want = np.zeros((h, w))
+    want[::P, ::P] = P * P
+    assert np.allclose(re, want, atol=1e-3) and np.allclose(im, 0, atol=1e-3)
+    rng = np.random.default_rng(6)
+    xr, xi = f32(rng.standard_normal((h, w))), f32(rng.standard_normal((h, w)))
+    fr, fi = xr.copy(), xi.copy()
+    oracle.call("orc_gs_patch_ft", fr, fi, h, w, P, 0)
+    z = xr.astype(np.float64) + 1j * xi
+    for py in range(2):
+        for px in range(3):
+            s = np.s_[py * P:(py + 1) * P, px * P:(px + 1) * P]
+            ref = np.fft.fft2(z[s])
+            assert np.allclose(fr[s] + 1j * fi[s], ref, atol=1e-4)
+    oracle.call("orc_gs_patch_ft", fr, fi, h, w, P, 1)
+    assert np.allclose(fr, xr, atol=1e-5) and np.allclose(fi, xi, atol=1e-5)
+
+
+def test_goldstein_alpha0_closed_form(oracle):
+    """With alpha = 0 every patch comes back unchanged and the overlap weights sum to one, so
+    the filtered interferogram equals the raw one; the reference adds it to the raw
+    interferogram still held in the packed buffer (goldstein_filter_sub_image.cpp:125-175,
+    patches_pack.cl:58-62), so |z| = a1 + a2 and arg z = phase."""
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.interferogram(70, 90, 64)
+    ref, phase = oracle.goldstein(a1, a2, ph, 32, 4, 0.0)
+    s = a1.astype(np.float64) + a2
+    assert np.allclose(ref, s, rtol=2e-5, atol=0)
+    assert np.abs(np.angle(np.exp(1j * (phase.astype(np.float64) - ph)))).max() <= 1e-4
+
+
+def test_goldstein_constant_input_closed_form(oracle):
+    """Constant z: every patch transform is P^2 z at DC only, so the filtered value is
+    |P^2 z|^alpha z and the output is z (1 + |P^2 z|^alpha)."""
+    P, ov, alpha = 16, 2, 0.5
+    a1 = np.full((40, 37), 2.0, np.float32)
+    a2 = np.full((40, 37), 3.0, np.float32)
+    ph = np.full((40, 37), -1.2, np.float32)
+    ref, phase = oracle.goldstein(a1, a2, ph, P, ov, alpha)
+    z = 2.5
+    assert np.allclose(ref, z * (1 + (P * P * z) ** alpha), rtol=1e-5)
+    assert np.allclose(phase, -1.2, atol=1e-5)
