@@ -79,6 +79,7 @@ SIGNATURES = {
     "dkx_nlsar_batch_enqueue": (_I, [_P, _P]),
     "dkx_nlsar_batch_graph_launch": (_I, [_P, _P]),
     "dkx_nlsar_batch_set_tile": (_I, [_P, _I]),
+    "dkx_nlsar_batch_set_diagnostics": (_I, [_P, _I]),
     "dkx_nlsar_batch_get_tile": (_I, [_P]),
     "dkx_nlsar_batch_autotune": (_I, [_P, _P, _I, _IP, _P]),
     "dkx_nlsar_batch_ub_pixels": (_I, [_P, _ULLP]),

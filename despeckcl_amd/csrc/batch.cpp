@@ -30,6 +30,7 @@ struct NlsarBatch {
     NlsarCommon cm{};
     int tile = -1;               // -1: the autotuner's choice for this batch shape, else fixed
     int tile_used = 1;
+    int diagnostics = 0;         // statistics of every key (undefined-selection counts)
     // images
     std::vector<NlsarImageDesc> descs;
     size_t ws_bytes = 0;
@@ -162,7 +163,7 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         nb.views.push_back(v);
     }
     nb.pp = NlsarParams{0, sw, nb.P.data(), nP, nb.S.data(), nS, nb.views.data(), (int)nb.views.size(),
-                        hpar, cpar, 1};
+                        hpar, cpar, 1, b->diagnostics};
     rc = nlsar_common(nb.pp, &nb.cm);
     if (rc) return rc;
     std::vector<NlsarImagePlan> plans(n);
@@ -243,6 +244,19 @@ int dkx_nlsar_batch_set_tile(dkx_nlsar_batch h, int tile)
     return 0;
 }
 
+int dkx_nlsar_batch_set_diagnostics(dkx_nlsar_batch h, int on)
+{
+    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+    if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
+    int rc = batch_wait(b);
+    if (rc) return rc;
+    rc = drop_graph(b);
+    if (rc) return rc;
+    b->diagnostics = on ? 1 : 0;
+    b->pp.all_keys = b->diagnostics;
+    return 0;
+}
+
 int dkx_nlsar_batch_get_tile(dkx_nlsar_batch h)
 {
     NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
@@ -289,6 +303,11 @@ int dkx_nlsar_batch_ub_pixels(dkx_nlsar_batch h, unsigned long long* counts)
 {
     NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
     if (!b || !counts) { set_error("nlsar batch: null argument"); return DKX_EINVAL; }
+    if (!b->diagnostics) {
+        set_error("nlsar batch: undefined-selection counts need the statistics of every key "
+                  "(dkx_nlsar_batch_set_diagnostics)");
+        return DKX_EINVAL;
+    }
     int rc = batch_wait(b);
     if (rc) return rc;
     for (size_t i = 0; i < b->descs.size(); i++)

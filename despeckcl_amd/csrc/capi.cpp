@@ -192,16 +192,18 @@ struct NlsarCall {
     NlsarImagePlan ip;
 };
 
-static int nlsar_prepare(NlsarCall& call, int dim, int height, int width, int sw, const int* P,
-                         int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
-                         float h, float c)
+// all_keys: compute every key's statistics (the ENL / alpha maps and the undefined-selection
+// count need them); the plan depends on it (the wide pass variant serves the selected key only)
+static int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int width, int sw,
+                         const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                         int n_stats, float h, float c)
 {
     if (!P || !S) { set_error("nlsar: null size list"); return DKX_EINVAL; }
     if (nP < 1 || nS < 1) { set_error("nlsar: empty patch or scale size list"); return DKX_EINVAL; }
     int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views);
     if (rc) return rc;
     call.pp = NlsarParams{dim, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
-                          g_tile[0].load()};
+                          g_tile[0].load(), all_keys};
     rc = nlsar_common(call.pp, &call.cm);
     if (rc) return rc;
     return nlsar_image_plan(call.cm, height, width, -1, -1, &call.ip);
@@ -213,6 +215,10 @@ static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, 
                              float* enl_maps, float* alpha_maps, unsigned long long* ub,
                              hipStream_t stream, DeviceWorkspace* w)
 {
+    if ((enl_maps || alpha_maps || ub) && !call.pp.all_keys) {
+        set_error("nlsar: internal error, diagnostics without every key's statistics");
+        return DKX_EINVAL;
+    }
     std::vector<float> blob;
     int rc = nlsar_build_constants(call.pp, call.cm, blob);
     if (rc) return rc;
@@ -311,7 +317,7 @@ int dkx_version(void) { return 1; }
 
 int dkx_set_tile(int filter, int tile_index)
 {
-    if (filter < 0 || filter > 1 || tile_index < 0 || tile_index > 3) {
+    if (filter < 0 || filter > 1 || tile_index < 0 || tile_index > 2) {
         set_error("dkx_set_tile: invalid filter or tile index");
         return DKX_EINVAL;
     }
@@ -347,7 +353,8 @@ int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, f
 {
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
     NlsarCall call;
-    int rc = nlsar_prepare(call, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
+    int rc = nlsar_prepare(call, 0, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
@@ -362,7 +369,7 @@ int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
 {
     if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
     NlsarCall call;
-    int rc = nlsar_prepare(call, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    int rc = nlsar_prepare(call, 0, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
@@ -398,7 +405,8 @@ int dkx_nlsar_covmat_device(const float* cov, float* cov_out, int height, int wi
         return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
     }
     NlsarCall call;
-    int rc = nlsar_prepare(call, dim, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
+    int rc = nlsar_prepare(call, dim, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }
@@ -417,7 +425,7 @@ int dkx_nlsar_covmat(const float* cov, float* cov_out, int height, int width, in
         return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
     }
     NlsarCall call;
-    int rc = nlsar_prepare(call, dim, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+    int rc = nlsar_prepare(call, dim, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
     if (rc) return rc;
     DeviceWorkspace* w = workspace_for_current_device();
     if (!w) { set_error("no HIP device"); return DKX_EHIP; }

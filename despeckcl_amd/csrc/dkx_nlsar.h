@@ -41,6 +41,11 @@ struct NlsarParams {
     const NlsarStatsView* stats; int nst;
     float h, c;
     int tile;         // pass variant (0..NLSAR_NTILES-1)
+    // 1: statistics of every (patch, scale) key, for the ENL / alpha maps and the count of
+    // pixels whose best-parameter selection is undefined; 0: only the last key, the one the
+    // selection returns whenever it is defined (best_params.cpp:27-53), so the filtered
+    // output is the same either way
+    int all_keys;
 };
 
 // Validated parameters and what follows from them.

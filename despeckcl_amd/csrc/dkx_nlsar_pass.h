@@ -24,7 +24,6 @@ __host__ __device__ constexpr int pmask_bit(int m, int k, int b = 0)
     return (m & 1) ? (k == 0 ? b : pmask_bit(m >> 1, k - 1, b + 1)) : pmask_bit(m >> 1, k, b + 1);
 }
 
-constexpr int NLSAR_TY = 32;    // tile rows
 constexpr int NLSAR_BAND = 8;   // tile rows per band of the block order
 
 struct ScaleArgs {
@@ -44,18 +43,18 @@ struct ScaleArgs {
     int first_scale;                     // 1: first pass (ubflag not yet written)
 };
 
-// LDS layout (floats) shared by the kernel and its launcher, fixed at compile time for
-// search windows of half width up to WMAX so that every LDS access takes an immediate
-// offset.  XU/YU: the union region of a tile and the tile shifted by an offset, rounded up
-// to runs of 4.
+// LDS layout (floats) of a pass over TX x TX output tiles, shared by the kernel and its
+// launcher, fixed at compile time for search windows of half width up to WMAX so that every
+// LDS access takes an immediate offset.  XU/YU: the union region of a tile and the tile
+// shifted by an offset, rounded up to runs of 4.
 template <int TX, int P, int WMAX>
 struct PassLds {
     static constexpr int W2 = 4 + 2 * P;          // a run of 4 plus the patch halo
     static constexpr int W2R = (W2 + 3) & ~3;     // the same in float4 loads
     static constexpr int XU = (TX + WMAX + 3) & ~3;
-    static constexpr int YU = (NLSAR_TY + WMAX + 3) & ~3;
+    static constexpr int YU = (TX + WMAX + 3) & ~3;
     static constexpr int DXM = XU + W2R;           // dissimilarity row
-    static constexpr int DYM = NLSAR_TY + WMAX + 2 * P;
+    static constexpr int DYM = TX + WMAX + 2 * P;
     static constexpr int RXM = XU;                 // row-sum / weight row
     static constexpr int RRows = YU + 2 * P;
     static constexpr int WRows = YU;
@@ -72,6 +71,28 @@ struct PassLds {
 // Window classes: the largest search-window half width each compiled layout serves.
 constexpr int NLSAR_WMAX_SMALL = 10;   // search windows up to 21
 constexpr int NLSAR_WMAX_LARGE = 20;   // search windows up to 41
+
+// Natural log of a positive normal float: the device library's logf without its subnormal
+// branch (log2 by v_log_f32, then the product with ln 2 carried in two parts), so the result
+// is the same bits for every normal input; non-finite log2 values pass through as the
+// plain product.  Determinants of averaged covariances are never subnormal in practice
+// (a subnormal input reads as 0 here and gives -inf).
+__device__ __forceinline__ float nlsar_logf(float x)
+{
+    const float l2 = __builtin_amdgcn_logf(x);
+    const float hi = l2 * 0x1.62e42ep-1f;            // ln 2, leading part
+    float e = fmaf(l2, 0x1.62e42ep-1f, -hi);
+    e = fmaf(l2, 0x1.efa39ep-25f, e);                // ln 2, trailing part
+    return fabsf(l2) < __builtin_inff() ? hi + e : hi;
+}
+
+// q = n / d and n % d for 0 <= n < 2^16, 1 <= d <= 2^10: (n + 1/2) / d lies at least 1/(2d)
+// from an integer, far beyond the rounding of the float product, so the truncation is exact
+__device__ __forceinline__ void nlsar_divmod(int n, int d, float inv_d, int& q, int& r)
+{
+    q = (int)(((float)n + 0.5f) * inv_d);
+    r = n - q * d;
+}
 
 // 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
 __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
@@ -153,7 +174,7 @@ __device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, Pas
     const int bh = min(NLSAR_BAND, gy - band * NLSAR_BAND);
     const int bx = within / bh;
     const int by = band * NLSAR_BAND + (within - bx * bh);
-    ty0 = uni(d.L[m].vy0) + by * NLSAR_TY;
+    ty0 = uni(d.L[m].vy0) + by * TX;
     tx0 = uni(d.L[m].vx0) + bx * TX;
     v.img = img;
     v.m = m;
@@ -170,7 +191,7 @@ __device__ __forceinline__ void nlsar_block_tile(const ScaleArgs& a, int TX, Pas
 // for single-row threads, whose 16-wave blocks would otherwise leave 40 registers), which
 // bounds the registers the compiler may use.  Covariance input holds more statistics and,
 // in the last pass, every plane's weighted sum: fewer blocks leave room for them.
-constexpr int nlsar_blocks_per_cu(int RPT, int WMAX, int MODE, bool LAST)
+constexpr int nlsar_blocks_per_cu(int TX, int RPT, int WMAX, int MODE, bool LAST)
 {
     return WMAX > NLSAR_WMAX_SMALL ? 1
          : MODE == 3 && LAST     ? 1
@@ -191,7 +212,7 @@ __device__ __forceinline__ float nlsar_det3(float a00, float a11, float a22, flo
 
 // MODE: 0 InSAR input, 2 / 3 covariance input of that dimension (all planes filtered).
 template <int TX, int RPT, int PMASK, bool LAST, int WMAX, int MODE>
-__global__ __launch_bounds__(TX * NLSAR_TY / RPT, nlsar_blocks_per_cu(RPT, WMAX, MODE, LAST) * TX * NLSAR_TY / RPT / 256)
+__global__ __launch_bounds__(TX * TX / RPT, nlsar_blocks_per_cu(TX, RPT, WMAX, MODE, LAST) * TX * TX / RPT / 256)
 void nlsar_scale_pass(const ScaleArgs a)
 {
     constexpr int CF4 = nlsar_c_float4(MODE);    // float4 per C element
@@ -201,8 +222,8 @@ void nlsar_scale_pass(const ScaleArgs a)
     constexpr int NPL = 2 * DIM * DIM;           // planes of a covariance input
     // planes averaged by the weighted mean besides the diagonal intensities
     constexpr int NMACC = MODE == 0 ? 2 : NPL - DIM;
-    constexpr int NT = TX * NLSAR_TY / RPT;   // each thread owns RPT rows of one column
-    constexpr int TY = NLSAR_TY;
+    constexpr int NT = TX * TX / RPT;   // each thread owns RPT rows of one column
+    constexpr int TY = TX;              // square tiles
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
     using Lay = PassLds<TX, P, WMAX>;
@@ -262,8 +283,10 @@ void nlsar_scale_pass(const ScaleArgs a)
             //    DY x DX region: row r, column c; byte offsets advance incrementally.
             {
                 const int n = DY * DX;
-                const int sr_ = NT / DX, sc_ = NT - sr_ * DX;
-                int r = tid / DX, c = tid - r * DX;
+                const float inv = 1.0f / (float)DX;
+                int sr_, sc_, r, c;
+                nlsar_divmod(NT, DX, inv, sr_, sc_);
+                nlsar_divmod(tid, DX, inv, r, c);
                 const unsigned int gb0 = (unsigned int)(im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P));
                 unsigned int g1 = gb0 + (unsigned int)(r * apitch + c);      // element index
                 const unsigned int dofs = (unsigned int)(dy * apitch + dx);
@@ -283,7 +306,7 @@ void nlsar_scale_pass(const ScaleArgs a)
                         const float det = nlsar_det3(u0.x + v0.x, u0.y + v0.y, u0.z + v0.z, u0.w + v0.w,
                                                      u1.x + v1.x, u1.y + v1.y, u1.z + v1.z, u1.w + v1.w,
                                                      u2.x + v2.x);
-                        sim = -(((K + u2.y) + v2.y) - 2.0f * logf(det));
+                        sim = -(((K + u2.y) + v2.y) - 2.0f * nlsar_logf(det));
                     } else {
                         const float4 u = gload4(A4b + 16u * g1);
                         const float4 v = gload4(A4b + 16u * g2);
@@ -294,7 +317,7 @@ void nlsar_scale_pass(const ScaleArgs a)
                         const float s01i = u.w + v.w;
                         const float s11 = u.y + v.y;
                         const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
-                        sim = -(((K + l1) + l2) - 2.0f * logf(det));
+                        sim = -(((K + l1) + l2) - 2.0f * nlsar_logf(det));
                     }
                     if (isnan(sim)) sim = 0.f;
                     sD[so] = sim;
@@ -314,8 +337,10 @@ void nlsar_scale_pass(const ScaleArgs a)
             {
                 const int nrx = (RX + 3) >> 2;
                 const int n = DY * nrx;
-                const int qs = NT / nrx, qr = NT - qs * nrx;
-                int r = tid / nrx, xi = tid - r * nrx;
+                const float inv = 1.0f / (float)nrx;
+                int qs, qr, r, xi;
+                nlsar_divmod(NT, nrx, inv, qs, qr);
+                nlsar_divmod(tid, nrx, inv, r, xi);
                 for (int i = tid; i < n; i += NT) {
                     const int x0 = xi << 2;
                     float w[W2R];
@@ -359,8 +384,10 @@ void nlsar_scale_pass(const ScaleArgs a)
                 const int nrx = (RX + 1) >> 1;
                 const int nry = (RY + 3) >> 2;
                 const int n = nry * nrx;
-                const int qs = NT / nrx, qr = NT - qs * nrx;
-                int ry = tid / nrx, xi = tid - ry * nrx;
+                const float inv = 1.0f / (float)nrx;
+                int qs, qr, ry, xi;
+                nlsar_divmod(NT, nrx, inv, qs, qr);
+                nlsar_divmod(tid, nrx, inv, ry, xi);
                 for (int i = tid; i < n; i += NT) {
                     const int off = (ry << 2) * RXM + (xi << 1);
 #pragma unroll
@@ -654,7 +681,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
     }
     DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    static_assert(TX == NLSAR_TILE && NLSAR_TY == NLSAR_TILE, "block counts assume 32 x 32 tiles");
+    static_assert(TX == NLSAR_TILE, "block counts assume 32 x 32 tiles");
     const int total = a.total_blocks;
     if (total <= 0) return 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -664,7 +691,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
         DKX_HIP_CHECK(hipEventCreate(&e1));
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
-    hipLaunchKernelGGL((nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>), dim3(total), dim3(TX * NLSAR_TY / RPT),
+    hipLaunchKernelGGL((nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>), dim3(total), dim3(TX * TX / RPT),
                        lds, stream, a);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {

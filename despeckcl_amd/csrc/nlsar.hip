@@ -414,6 +414,9 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
     }
     // the final pass must come last
     std::stable_partition(passes.begin(), passes.end(), [](const Pass& q) { return !q.last; });
+    // without diagnostics only the selected key's statistics are needed: the last key, on its
+    // own (the other keys of its pass only feed the maps and the undefined-selection count)
+    if (!pp.all_keys) passes.assign(1, Pass{smax_i, std::vector<int>{pmax_i}, true});
 
     int cur_scale = -1;
     for (size_t oi = 0; oi < passes.size(); oi++) {

@@ -123,6 +123,10 @@ class NlsarBatch:
     def set_tile(self, tile):
         check(lib().dkx_nlsar_batch_set_tile(self._h, int(tile)))
 
+    def set_diagnostics(self, on=True):
+        """Compute every (patch, scale) key's statistics (needed by ub_pixels)."""
+        check(lib().dkx_nlsar_batch_set_diagnostics(self._h, 1 if on else 0))
+
     def tile(self):
         return lib().dkx_nlsar_batch_get_tile(self._h)
 

@@ -182,7 +182,9 @@ long long dkx_nlsar_stats_load(const char* filename, dkx_nlsar_stats* out, int m
  * src/compute/nlsar/stats.cpp:76). */
 double dkx_chi2_inv(double p, double nu);
 
-/* Tile shape used by the NL-SAR / NL-InSAR kernels (chosen by the autotuner). */
+/* Tile shape of the single-call entry points.  NL-SAR: 0, 1 (default), 2 = 32 x 32 output
+ * tiles with 4, 2 or 1 rows per thread.  NL-InSAR: 0, 1 (default), 2 = 27, 18 or 54 rows of
+ * 54 columns.  Results do not depend on the tile shape. */
 int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
 int dkx_get_tile(int filter);
 
@@ -238,6 +240,10 @@ int dkx_nlsar_batch_enqueue(dkx_nlsar_batch batch, void* stream);
 int dkx_nlsar_batch_graph_launch(dkx_nlsar_batch batch, void* stream);
 /* Pass variant: -1 = the autotuner's choice, 0/1/2 = 4/2/1 rows per thread. */
 int dkx_nlsar_batch_set_tile(dkx_nlsar_batch batch, int tile);
+/* on != 0: later enqueues compute the statistics of every (patch, scale) key, which
+ * dkx_nlsar_batch_ub_pixels needs; off (the default): only the key the output uses.  The
+ * filtered images are the same either way. */
+int dkx_nlsar_batch_set_diagnostics(dkx_nlsar_batch batch, int on);
 int dkx_nlsar_batch_get_tile(dkx_nlsar_batch batch);
 /* Times every pass variant on the batch (reps runs each, after one warm-up run), records the
  * fastest for this parameter set and total pixel count, and selects it.  ms (optional)

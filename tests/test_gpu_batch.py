@@ -85,15 +85,21 @@ def test_batch_equals_single_calls(D, stats, tile):
         b.set([im.tuple() for im in imgs], 21, PATCHES, SCALES, stats)
         b.set_tile(tile)
         assert b.tile() == tile
-        for im in imgs:
-            im.clear()
-        hip.synchronize()
-        b.enqueue(stream.handle)
-        stream.synchronize()
-        for i, im in enumerate(imgs):
-            print(f"tile {tile} image {SHAPES[i]}")
-            assert _same(im.result(), singles[i]), f"image {i} {SHAPES[i]} differs from its single call"
+        # without and with the statistics of every key (the diagnostics): same images
+        for diag in (False, True):
+            b.set_diagnostics(diag)
+            for im in imgs:
+                im.clear()
+            hip.synchronize()
+            b.enqueue(stream.handle)
+            stream.synchronize()
+            for i, im in enumerate(imgs):
+                print(f"tile {tile} diagnostics {diag} image {SHAPES[i]}")
+                assert _same(im.result(), singles[i]), f"image {i} {SHAPES[i]} differs from its single call"
         assert b.ub_pixels() == [0] * len(imgs)
+        b.set_diagnostics(False)
+        with pytest.raises(ValueError, match="diagnostics"):
+            b.ub_pixels()
         b.close()
     finally:
         DV.set_tile(0, old)

@@ -128,7 +128,7 @@ def test_nlsar_tile_shapes_bit_identical(D, stats):
     outs = []
     old = DV.get_tile(0)
     try:
-        for t in range(3):
+        for t in range(3):   # 4, 2 or 1 rows per thread
             DV.set_tile(0, t)
             outs.append(D.nlsar_insar(a1, a2, ph, 21, [3, 5, 7], [1, 3], stats))
     finally:
@@ -158,6 +158,34 @@ def test_nlsar_device_entry_equals_host_entry(D, stats):
     a = alp.download()
     assert np.all(np.isfinite(e)) and np.all(e > 0)
     assert np.all((a >= 0) & np.isfinite(a))
+
+
+@pytest.mark.parametrize("example", [False, True])
+def test_nlsar_last_key_equals_all_keys(D, stats, example):
+    """The output depends only on the key the selection returns (the last one,
+    best_params.cpp:27-53): the default entry, which computes only that key's statistics,
+    gives the same bits as a call that computes every key (requested through the
+    undefined-selection count)."""
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+    if example:
+        z = np.load(os.path.join(GOLDEN, "example_crop.npz"))
+        a1, a2, ph = z["ampl_master"], z["ampl_slave"], z["dphase"]
+        P, S_ = [3, 5, 7, 9, 11], [1, 3, 5]
+        ta = np.s_[50:75, 50:75]
+        st = D.nlsar_train_insar(a1[ta], a2[ta], ph[ta], P, S_)
+    else:
+        a1, a2, ph = S.interferogram(300, 257, 15)
+        P, S_, st = [3, 5, 7], [1, 3], stats
+    H, W = a1.shape
+    host = D.nlsar_insar(a1, a2, ph, 21, P, S_, st)
+    din = [HIP.DeviceArray.from_host(np.ascontiguousarray(x)) for x in (a1, a2, ph)]
+    dout = [HIP.DeviceArray((H, W)) for _ in range(3)]
+    ub = DV.nlsar_insar(*[d.ptr for d in din], *[d.ptr for d in dout], H, W, 21, P, S_, st, want_ub=True)
+    print("undefined-selection pixels", ub)
+    for k in range(3):
+        assert np.array_equal(dout[k].download(), host[k], equal_nan=True)
 
 
 def test_nlsar_full_size_locality(D, stats):
