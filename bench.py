@@ -29,10 +29,11 @@ H = W = 2048
 SW, PATCHES, SCALES = 21, [3, 5, 7], [1, 3]
 HPAR, CPAR = 15.0, 49.0
 PEAK_FP32_TFLOPS = 157.3
-# HBM bytes per launch of the last scale pass: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
-# this benchmark (KiB per dispatch), scaled by what the same counters read for a 256 MiB
-# device-to-device copy (FETCH x 2.000, WRITE x 1.000); profiles/nlsar_2048_scale_pass_pmc.md
-TRAFFIC_LAST_PASS_BYTES = round((5234330 * 2.000 + 49344 * 1.000) * 1024)
+# HBM-side bytes per launch of the selected key's pass: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this benchmark (KiB per dispatch), scaled by what the same counters read for
+# a 256 MiB device-to-device copy (FETCH x 2.000, WRITE x 1.000);
+# profiles/nlsar_2048_scale_pass_pmc.md
+TRAFFIC_LAST_PASS_BYTES = round((11587358 * 2.000 + 49344 * 1.000) * 1024)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),
