@@ -175,21 +175,53 @@ struct WeightArgs {
 
 constexpr int NTW = 512;
 constexpr int NLI_BAND = 4;   // block rows per band of the block order
+constexpr int NLI_RSEG = 3;   // row sums per thread (a run of adjacent outputs)
+
+// nl_pixel_kl_pre(e1, e2) with the pixel the reference passes second given by `second`:
+// r, and a1 + a2, are the same bits for either argument order (products and fmaf commute,
+// and a2 + a1 == a1 + a2), so only the zero test depends on the order.
+__device__ __forceinline__ float nl_pixel_kl_sym(float4 e1, float4 e2, float second)
+{
+    const float r = 1.0f - fmaf(e1.y, e2.y, e1.z * e2.z);
+    const float a1 = e1.x * r * e2.w;
+    const float a2 = e2.x * r * e1.w;
+    float kl = -4.0f / 3.14159274101257324219f * (a1 + a2 - 2.0f);
+    // computed on every lane: otherwise the compiler moves the loads of e2 behind the zero
+    // test and serialises two LDS round trips
+    __asm__ volatile("" : "+v"(kl));
+    return second != 0.0f ? kl : -0.0f;
+}
 
 // FIRST: evaluate the likelihood similarities and keep their patch sums (they depend on the
 // raw data only); later iterations read them back and evaluate only the KL divergences.
+//
+// Work split (TX = 54, TY a multiple of 9): every thread keeps the same positions for all
+// offsets, so per offset only the offset's displacement changes.
+//   field:    positions tid + k*NTW of the (TY+2p) x (TX+2p) patch-position grid; the record
+//             at the position itself stays in registers, the displaced one is read from LDS;
+//   row sums: runs of NLI_RSEG adjacent outputs of one row;
+//   outputs:  threads tid < 9*TX own TY/9 vertically adjacent pixels of one column, slot
+//             tid + m*9*TX in the block's similarity / weight chunk.
 template <int TY, int TX, int PSH, bool FIRST>
 __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
 {
+    static_assert(TX % NLI_RSEG == 0 && TY % 9 == 0, "tile shape");
     constexpr int NPIX = TY * TX;
-    constexpr int PPT = (NPIX + NTW - 1) / NTW;
     constexpr int NP2 = 2 * PSH;
     constexpr int PY = TY + NP2, PX = TX + NP2;
+    constexpr int NF = PY * PX;
+    constexpr int K1 = (NF + NTW - 1) / NTW;
+    constexpr int SEGS = TX / NLI_RSEG;
+    constexpr int NRS = PY * SEGS;
+    constexpr int K2 = (NRS + NTW - 1) / NTW;
+    constexpr int RPT = TY / 9;
+    constexpr int NCT = 9 * TX;
+    static_assert(NCT <= NTW, "output threads");
     extern __shared__ float4 smem4[];
     const int wsh = a.wsh;
     const int sw = 2 * wsh + 1;
     // LDS: the records of the block's whole neighbourhood (raw data in the first iteration,
-    // estimates after), staged once; then the per-offset similarity / KL fields and sums
+    // estimates after), staged once; then the per-offset similarity / KL fields and row sums
     const int o = wsh + PSH;
     const int EW = TX + 2 * o, EH = TY + 2 * o;
     float4* stg = smem4;
@@ -238,26 +270,25 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
 
     // this block's chunk of the similarity cache and of the weights
     const long long chunk = (long long)t * (sw * sw) * NPIX;
-    float acc[PPT][5];
-    long long opix[PPT];
-    bool valid[PPT];
-    unsigned int fofs[PPT];   // F grid element of the pixel (offset 0)
-    // the next offset's cached similarity sum and filter values, loaded one offset ahead so
-    // that their latency overlaps the KL divergences of the current offset
-    float sv[PPT];
-    float4 fv[PPT];
+
+    // output pixels of this thread: column oc, rows og*RPT .. og*RPT + RPT - 1.  Threads past
+    // NCT repeat the last group's reads (nothing is stored for them), so that every load
+    // issued is consumed on every path.
+    const bool owner = tid < NCT;
+    const int og = min(tid / TX, 8), oc = tid - (tid / TX) * TX;
+    const int slot0 = min(tid, NCT - 1);
+    float acc[RPT][5];
+    long long opix[RPT];
+    bool valid[RPT];
+    unsigned int fofs[RPT];   // F grid element of the pixel (offset 0)
 #pragma unroll
-    for (int q = 0; q < PPT; q++) {
+    for (int q = 0; q < RPT; q++) {
         acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = acc[q][4] = 0.f;
-        const int pq = tid + q * NTW;
-        const int py = pq / TX, px = pq - (pq / TX) * TX;
+        const int py = og * RPT + q, px = oc;
         const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
-        valid[q] = pq < NPIX && py < rows_lim && px < cols_lim && iy < L.ny && ix < L.nx;
+        valid[q] = owner && py < rows_lim && px < cols_lim && iy < L.ny && ix < L.nx;
         opix[q] = valid[q] ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
         fofs[q] = (unsigned int)(a.forg + (ty0 + py) * a.fpitch + (tx0 + px));
-        // offset plane 0: (-wsh, -wsh)
-        sv[q] = (!FIRST && valid[q]) ? a.scache[chunk + pq] : 0.f;
-        fv[q] = a.F4[fofs[q] - wsh * a.fpitch - wsh];
     }
 
     // stage the neighbourhood: rows ty0 - o .. ty0 + TY + o, columns tx0 - o .. tx0 + TX + o
@@ -269,7 +300,35 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             stg[i] = src[(unsigned int)(a.rorg + (ty0 - o + r) * a.rpitch + (tx0 - o + c))];
         }
     }
-    const int sorg = -(ty0 - o) * EW - (tx0 - o);   // staged index of image position (0, 0)
+    // field positions of this thread: staged index, grid element, inside the reference
+    // tile's similarity border (the orientation rule below), and the record at the position
+    int fs[K1];
+    unsigned int fg[K1];
+    bool finner[K1];
+    float4 fe[K1];
+#pragma unroll
+    for (int k = 0; k < K1; k++) {
+        const int i = tid + k * NTW;
+        const int r = i / PX, c = i - (i / PX) * PX;
+        fs[k] = (wsh + r) * EW + (wsh + c);
+        const int gy = ty0 - PSH + r, gx = tx0 - PSH + c;
+        fg[k] = (unsigned int)(a.rorg + gy * a.rpitch + gx);
+        const int sh = gy - Y0 + PSH, sx = gx - X0 + PSH;
+        finner[k] = sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
+    }
+    // row-sum runs of this thread: first field element and first output
+    int rsrc[K2], rdst[K2];
+#pragma unroll
+    for (int k = 0; k < K2; k++) {
+        const int j = tid + k * NTW;
+        const int r = j / SEGS, g = j - (j / SEGS) * SEGS;
+        rsrc[k] = r * PX + g * NLI_RSEG;
+        rdst[k] = r * TX + g * NLI_RSEG;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K1; k++)
+        if (tid + k * NTW < NF) fe[k] = stg[fs[k]];
 
     const int centre = wsh * sw + wsh;
     for (int hh = 0; hh < sw; hh++) {
@@ -277,73 +336,80 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
         for (int ww = 0; ww < sw; ww++) {
             const int dx = ww - wsh;
             const int plane = hh * sw + ww;
-            if (plane == centre) {
-                // centre weight 0 (nlinsar_sub_image.cpp:189-197); adding it changes no sum
+            const long long pbase = chunk + (long long)plane * NPIX;
+            // centre weight 0 (nlinsar_sub_image.cpp:189-197), accumulated like the others
+            const bool self = plane == centre;
+            // this offset's cached similarity sums and filter values, issued before the field
+            // and row-sum steps that hide their latency
+            float sv[RPT];
+            float4 fv[RPT];
+            const int fd = dy * a.fpitch + dx;
 #pragma unroll
-                for (int q = 0; q < PPT; q++)
-                    if (valid[q]) a.wt[chunk + (long long)plane * NPIX + tid + q * NTW] = 0.f;
-                continue;
+            for (int q = 0; q < RPT; q++) {
+                if (!FIRST) sv[q] = a.scache[pbase + slot0 + q * NCT];
+                fv[q] = a.F4[fofs[q] + fd];
             }
-            if (FIRST || kl) {
+            if ((FIRST || kl) && !self) {
                 __syncthreads();
                 // 1. similarity and KL at every patch position of the block.  Offsets with
                 //    dy >= 0 are evaluated centre-first everywhere; offsets with dy < 0 are the
                 //    mirrored values (the pixel at +d first) at positions inside the reference
                 //    tile's similarity border, centre-first on the border
                 //    (precompute_similarities_1st_pass.cl.in:36-42, _2nd_pass.cl:30-48).
-                {
-                    constexpr int n = PY * PX;
-                    int r = tid / PX, c = tid - r * PX;
-                    constexpr int sr_ = NTW / PX, sc_ = NTW - sr_ * PX;
-                    for (int i = tid; i < n; i += NTW) {
-                        const int gy = ty0 - PSH + r, gx = tx0 - PSH + c;
-                        const int sh = gy - Y0 + PSH, sx = gx - X0 + PSH;
-                        const bool interior = sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
-                        const bool swap = dy < 0 && interior;
-                        const int y1 = swap ? gy + dy : gy, x1 = swap ? gx + dx : gx;
-                        const int y2 = swap ? gy : gy + dy, x2 = swap ? gx : gx + dx;
-                        const int s1 = sorg + y1 * EW + x1, s2 = sorg + y2 * EW + x2;
-                        if (FIRST) {
-                            const float4 u = stg[s1], v = stg[s2];
-                            sS[r * PX + c] = nl_pixel_similarity_pre(u, v);
-                            if (kl) {
-                                // nonzero initial estimates: read them from the grid
-                                const unsigned int p1 = (unsigned int)(a.rorg + y1 * a.rpitch + x1);
-                                const unsigned int p2 = (unsigned int)(a.rorg + y2 * a.rpitch + x2);
-                                sK[r * PX + c] = nl_pixel_kl_pre(a.E4[p1], a.E4[p2]);
-                            }
-                        } else {
-                            sK[r * PX + c] = nl_pixel_kl_pre(stg[s1], stg[s2]);
+                const int doff = dy * EW + dx;
+                const unsigned int goff = (unsigned int)(dy * a.rpitch + dx);
+                // the displaced records first, all reads in flight together
+                float4 vd[K1];
+#pragma unroll
+                for (int k = 0; k < K1; k++)
+                    if (!(k * NTW + NTW > NF && tid + k * NTW >= NF)) vd[k] = stg[fs[k] + doff];
+#pragma unroll
+                for (int k = 0; k < K1; k++) {
+                    const int i = tid + k * NTW;
+                    if (k * NTW + NTW > NF && i >= NF) continue;
+                    const float4 v = vd[k];
+                    const bool swp = dy < 0 && finner[k];
+                    if (FIRST) {
+                        const float4 u1 = swp ? v : fe[k], u2 = swp ? fe[k] : v;
+                        sS[i] = nl_pixel_similarity_pre(u1, u2);
+                        if (kl) {
+                            // nonzero initial estimates: read them from the grid
+                            const unsigned int p1 = swp ? fg[k] + goff : fg[k];
+                            const unsigned int p2 = swp ? fg[k] : fg[k] + goff;
+                            sK[i] = nl_pixel_kl_pre(a.E4[p1], a.E4[p2]);
                         }
-                        r += sr_;
-                        c += sc_;
-                        if (c >= PX) { c -= PX; r++; }
+                    } else {
+                        sK[i] = nl_pixel_kl_sym(fe[k], v, swp ? fe[k].x : v.x);
                     }
                 }
                 __syncthreads();
                 // 2. row sums along x (precompute_patch_similarities.cl:45-50)
-                {
-                    constexpr int n = PY * TX;
-                    int r = tid / TX, c = tid - r * TX;
-                    constexpr int sr_ = NTW / TX, sc_ = NTW - sr_ * TX;
-                    for (int i = tid; i < n; i += NTW) {
-                        if (FIRST) {
-                            const float* s1 = sS + r * PX + c;
-                            float u = s1[0];
 #pragma unroll
-                            for (int j = 1; j <= NP2; j++) u += s1[j];
-                            rS[r * TX + c] = u;
-                        }
-                        if (kl) {
-                            const float* s2 = sK + r * PX + c;
-                            float v = s2[0];
+                for (int k = 0; k < K2; k++) {
+                    if (k * NTW + NTW > NRS && tid + k * NTW >= NRS) continue;
+                    if (FIRST) {
+                        float x[NLI_RSEG + NP2];
 #pragma unroll
-                            for (int j = 1; j <= NP2; j++) v += s2[j];
-                            rK[r * TX + c] = v;
+                        for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sS[rsrc[k] + j];
+#pragma unroll
+                        for (int g = 0; g < NLI_RSEG; g++) {
+                            float u = x[g];
+#pragma unroll
+                            for (int j = 1; j <= NP2; j++) u += x[g + j];
+                            rS[rdst[k] + g] = u;
                         }
-                        r += sr_;
-                        c += sc_;
-                        if (c >= TX) { c -= TX; r++; }
+                    }
+                    if (kl) {
+                        float x[NLI_RSEG + NP2];
+#pragma unroll
+                        for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sK[rsrc[k] + j];
+#pragma unroll
+                        for (int g = 0; g < NLI_RSEG; g++) {
+                            float v = x[g];
+#pragma unroll
+                            for (int j = 1; j <= NP2; j++) v += x[g + j];
+                            rK[rdst[k] + g] = v;
+                        }
                     }
                 }
                 __syncthreads();
@@ -351,46 +417,43 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             // 3. rows accumulated top to bottom (precompute_patch_similarities.cl:53-58),
             //    weight (compute_weights.cl:19-30), ENL sums (compute_number_of_looks.cl)
             //    and weighted means (compute_insar.cl:81-93) in plane order
-            // next plane, skipping the centre
-            int nplane = plane + 1;
-            if (nplane == centre) nplane++;
-            const bool has_next = nplane < sw * sw;
-            const int nfd = (nplane / sw - wsh) * a.fpitch + (nplane - (nplane / sw) * sw - wsh);
-#pragma unroll
-            for (int q = 0; q < PPT; q++) {
-                const int pq = tid + q * NTW;
-                if (pq >= NPIX) continue;
-                const int py = pq / TX, px = pq - (pq / TX) * TX;
-                float u;
+            {
+                float cs[RPT + NP2], ck[RPT + NP2];
                 if (FIRST) {
-                    const float* s1 = rS + py * TX + px;
-                    u = s1[0];
 #pragma unroll
-                    for (int j = 1; j <= NP2; j++) u += s1[j * TX];
-                    if (valid[q]) a.scache[chunk + (long long)plane * NPIX + pq] = u;
-                } else {
-                    u = sv[q];
+                    for (int j = 0; j < RPT + NP2; j++) cs[j] = rS[(og * RPT + j) * TX + oc];
                 }
-                float v = -0.0f;
                 if (kl) {
-                    const float* s2 = rK + py * TX + px;
-                    v = s2[0];
 #pragma unroll
-                    for (int j = 1; j <= NP2; j++) v += s2[j * TX];
+                    for (int j = 0; j < RPT + NP2; j++) ck[j] = rK[(og * RPT + j) * TX + oc];
                 }
-                float w = expf(u * a.rh + v * a.rT);
-                if (isnan(w) || isinf(w)) w = 0.0f;
-                if (valid[q]) a.wt[chunk + (long long)plane * NPIX + pq] = w;
-                const float4 f = fv[q];
-                if (has_next) {
-                    if (!FIRST) sv[q] = valid[q] ? a.scache[chunk + (long long)nplane * NPIX + pq] : 0.f;
-                    fv[q] = a.F4[fofs[q] + nfd];
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    float u;
+                    if (FIRST) {
+                        u = cs[q];
+#pragma unroll
+                        for (int j = 1; j <= NP2; j++) u += cs[q + j];
+                        if (valid[q] && !self) a.scache[pbase + tid + q * NCT] = u;
+                    } else {
+                        u = sv[q];
+                    }
+                    float v = -0.0f;
+                    if (kl) {
+                        v = ck[q];
+#pragma unroll
+                        for (int j = 1; j <= NP2; j++) v += ck[q + j];
+                    }
+                    float w = expf(u * a.rh + v * a.rT);
+                    if (isnan(w) || isinf(w) || self) w = 0.0f;
+                    if (valid[q]) a.wt[pbase + tid + q * NCT] = w;
+                    const float4 f = fv[q];
+                    acc[q][0] += w;
+                    acc[q][1] = fmaf(w, w, acc[q][1]);
+                    acc[q][2] = fmaf(w, f.x, acc[q][2]);
+                    acc[q][3] = fmaf(w, f.y, acc[q][3]);
+                    acc[q][4] = fmaf(w, f.z, acc[q][4]);
                 }
-                acc[q][0] += w;
-                acc[q][1] = fmaf(w, w, acc[q][1]);
-                acc[q][2] = fmaf(w, f.x, acc[q][2]);
-                acc[q][3] = fmaf(w, f.y, acc[q][3]);
-                acc[q][4] = fmaf(w, f.z, acc[q][4]);
             }
         }
     }
@@ -398,7 +461,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     // atomic per wave keeps a wave's pixels, which are neighbours, together in the queue)
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int q = 0; q < PPT; q++) {
+    for (int q = 0; q < RPT; q++) {
         const float ws = acc[q][0];
         const float nol = ws * ws / acc[q][1];
         const bool queued = valid[q] && nol < a.lmin;
@@ -410,7 +473,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             if (queued) {
                 const unsigned int k = base + __popcll(qmask & ((1ull << lane) - 1ull));
                 a.list[2 * k] = (int)opix[q];
-                a.list[2 * k + 1] = t * NPIX + tid + q * NTW;   // weight slot
+                a.list[2 * k + 1] = t * NPIX + tid + q * NCT;   // weight slot
             }
         }
         if (valid[q] && !queued)
