@@ -22,7 +22,7 @@ struct NlinsarPlan {
                                    // coh), filter values (a, Re x, Im x)
     int heap_block;                // threads per block of the smoothing kernel
     int blocks;                    // blocks of the weights kernel (weight / cache chunks)
-    size_t off_R, off_E, off_F, off_est, off_w, off_s, off_list, off_cnt, ws_bytes;
+    size_t off_R, off_A, off_E, off_F, off_est, off_w, off_s, off_list, off_xlist, off_cnt, ws_bytes;
 };
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);

@@ -35,9 +35,11 @@ __global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const floa
 }
 
 // The raw data on the padded grid R: float4 (a1, a2, cos phase, sin phase), clamped like the
-// reference's tile padding (src/data/sub_images.cpp:37-40).
+// reference's tile padding (src/data/sub_images.cpp:37-40); a1 alone also into A (same
+// element layout as R) for the smoothing's amplitude test.
 __global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __restrict__ a2,
-                                 const float* __restrict__ phi, int H, int W, Grid R)
+                                 const float* __restrict__ phi, int H, int W, Grid R,
+                                 float* __restrict__ A)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
@@ -47,6 +49,7 @@ __global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __re
     const long long i = (long long)y * W + x;
     const float p = phi[i];
     reinterpret_cast<float4*>(R.base)[(long long)gy * R.pitch + gx] = make_float4(a1[i], a2[i], cosf(p), sinf(p));
+    A[(long long)gy * R.pitch + gx] = a1[i];
 }
 
 // precompute_filter_values.cl:19-39: a = (a1^2 + a2^2)/2, x = a1*a2*exp(-i*phase), with
@@ -77,6 +80,7 @@ __global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, in
     if (threadIdx.x == 0) {
         cnt[0] = 0u;
         if (clear_total) cnt[1] = 0u;
+        cnt[3] = 0u;
         *flag = 0u;
     }
 }
@@ -496,122 +500,325 @@ struct SmoothArgs {
     float* new_pha;
     float* new_coh;
     const int* list;    // (output pixel, weight slot) pairs
-    unsigned int* cnt;  // [0] queued this iteration, [1] running total
+    unsigned int* cnt;  // [0] queued this iteration, [1] running total, [3] exact queue
+    int* xlist;         // entries of list that need the exact queue
+    const float* A;     // a1 on R's element layout
 };
 
 // std::priority_queue<weight_el> of smoothing.cpp:27-36 (top = smallest weight): sift
 // rules of libstdc++'s push_heap / pop_heap, which decide which of several equal weights
-// stays in the queue.
+// stays in the queue.  Entry j of a thread's heap: weight hw[j * bs], index hi[j * bs] (the
+// heaps of a block are interleaved, so a wave's accesses to one entry hit distinct banks).
 __device__ __forceinline__ bool hp_less(float wa, float wb) { return wa > wb; }
 
-__device__ void hp_push(float* hw, int* hi, int& n, float w, int idx)
+__device__ void hp_push(float* hw, int* hi, int bs, int& n, float w, int idx)
 {
     int hole = n++;
     int parent = (hole - 1) / 2;
-    while (hole > 0 && hp_less(hw[parent], w)) {
-        hw[hole] = hw[parent];
-        hi[hole] = hi[parent];
+    while (hole > 0 && hp_less(hw[parent * bs], w)) {
+        hw[hole * bs] = hw[parent * bs];
+        hi[hole * bs] = hi[parent * bs];
         hole = parent;
         parent = (hole - 1) / 2;
     }
-    hw[hole] = w;
-    hi[hole] = idx;
+    hw[hole * bs] = w;
+    hi[hole * bs] = idx;
 }
 
-__device__ void hp_pop(float* hw, int* hi, int& n)
+// pop_heap: the top moves to position n - 1, the heap shrinks to n - 1 entries
+__device__ void hp_pop(float* hw, int* hi, int bs, int& n)
 {
     const int len = n - 1;
     if (len <= 0) { n = 0; return; }
-    const float vw = hw[len];
-    const int vi = hi[len];
-    hw[len] = hw[0];
-    hi[len] = hi[0];
+    const float vw = hw[len * bs];
+    const int vi = hi[len * bs];
+    hw[len * bs] = hw[0];
+    hi[len * bs] = hi[0];
     int hole = 0, child = 0;
     while (child < (len - 1) / 2) {
         child = 2 * (child + 1);
-        if (hp_less(hw[child], hw[child - 1])) child--;
-        hw[hole] = hw[child];
-        hi[hole] = hi[child];
+        if (hp_less(hw[child * bs], hw[(child - 1) * bs])) child--;
+        hw[hole * bs] = hw[child * bs];
+        hi[hole * bs] = hi[child * bs];
         hole = child;
     }
     if ((len & 1) == 0 && child == (len - 2) / 2) {
         child = 2 * (child + 1);
-        hw[hole] = hw[child - 1];
-        hi[hole] = hi[child - 1];
+        hw[hole * bs] = hw[(child - 1) * bs];
+        hi[hole * bs] = hi[(child - 1) * bs];
         hole = child - 1;
     }
     int parent = (hole - 1) / 2;
-    while (hole > 0 && hp_less(hw[parent], vw)) {
-        hw[hole] = hw[parent];
-        hi[hole] = hi[parent];
+    while (hole > 0 && hp_less(hw[parent * bs], vw)) {
+        hw[hole * bs] = hw[parent * bs];
+        hi[hole * bs] = hi[parent * bs];
         hole = parent;
         parent = (hole - 1) / 2;
     }
-    hw[hole] = vw;
-    hi[hole] = vi;
+    hw[hole * bs] = vw;
+    hi[hole * bs] = vi;
     n = len;
 }
 
-// search_window_smoothing (smoothing.cpp:78-129) and compute_insar for the queued pixels
-__global__ void nlinsar_smooth(const SmoothArgs a)
+constexpr int NLI_SCH = 4;    // offsets whose loads the smoothing issues together
+static_assert(NLI_SCH <= 8, "candidate flags of a run are kept in one byte");
+constexpr int NLI_TOPK = 16;  // largest lmin of the smoothing's register selection
+
+struct SmoothPixel {
+    const float* A;       // a1 on R's element layout
+    const float4* F4;
+    const float* wts;     // weight of offset k at wts[k * npix]
+    unsigned int rb, fb;  // R / F element of window offset (0, 0) (grids hold < 2^30)
+    float as2;            // 2 * amplitude of the pixel
+};
+
+// The exact queue of search_window_smoothing (smoothing.cpp:88-129) followed by the weighted
+// means: the lmin largest candidate weights kept by the heap above (its tie rules decide
+// which of several equal weights stay), replaced by their mean, summed in pop order.
+__device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, float* hw, int* hi,
+                                int bs, float& N, float& at_, float& xr, float& xi)
 {
-    extern __shared__ float smem[];
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
-    const int mwords = (nfull + 31) / 32;
-    const int per = 2 * a.lmin + mwords;
-    float* hw = smem + threadIdx.x * per;
-    int* hi = (int*)(hw + a.lmin);
-    unsigned int* mask = (unsigned int*)(hi + a.lmin);
+    const int lmin = a.lmin;
+    int n = 0;
+    float top = 0.0f;   // hw[0] once the queue holds lmin entries
+    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
+        float at[NLI_SCH], wv[NLI_SCH];
+        int hh = hh0, ww = ww0;
+#pragma unroll
+        for (int j = 0; j < NLI_SCH; j++) {
+            if (k0 + j < nfull) {
+                at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
+                wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+            }
+            if (++ww == sw) { ww = 0; hh++; }
+        }
+#pragma unroll
+        for (int j = 0; j < NLI_SCH; j++) {
+            if (k0 + j < nfull && at[j] < px.as2) {
+                const float w = wv[j];
+                if (n < lmin) {
+                    hp_push(hw, hi, bs, n, w, k0 + j);
+                    top = hw[0];
+                } else if (top < w) {
+                    hp_pop(hw, hi, bs, n);
+                    hp_push(hw, hi, bs, n, w, k0 + j);
+                    top = hw[0];
+                }
+            }
+        }
+        hh0 = hh;
+        ww0 = ww;
+    }
+    // empty the queue in its order (the order of the sum); pop_heap leaves the popped
+    // entries at positions m-1 (first) .. 0 (last)
+    const int m = n;
+    float wsum = 0.0f;
+    while (n > 0) {
+        wsum += hw[0];
+        hp_pop(hw, hi, bs, n);
+    }
+    wsum /= (float)m;
+    // the selected offsets in increasing order
+    for (int i = 1; i < m; i++) {
+        const int v = hi[i * bs];
+        int j = i - 1;
+        while (j >= 0 && hi[j * bs] > v) {
+            hi[(j + 1) * bs] = hi[j * bs];
+            j--;
+        }
+        hi[(j + 1) * bs] = v;
+    }
+    int sel = 0;
+    int nxt = m > 0 ? hi[0] : nfull;
+    N = at_ = xr = xi = 0.f;
+    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
+        float wv[NLI_SCH];
+        float4 f[NLI_SCH];
+        int hh = hh0, ww = ww0;
+#pragma unroll
+        for (int j = 0; j < NLI_SCH; j++) {
+            if (k0 + j < nfull) {
+                wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
+            }
+            if (++ww == sw) { ww = 0; hh++; }
+        }
+#pragma unroll
+        for (int j = 0; j < NLI_SCH; j++) {
+            if (k0 + j < nfull) {
+                float w = wv[j];
+                if (k0 + j == nxt) {
+                    w = wsum;
+                    sel++;
+                    nxt = sel < m ? hi[sel * bs] : nfull;
+                }
+                N += w;
+                at_ = fmaf(w, f[j].x, at_);
+                xr = fmaf(w, f[j].y, xr);
+                xi = fmaf(w, f[j].z, xi);
+            }
+        }
+        hh0 = hh;
+        ww0 = ww;
+    }
+}
+
+__device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsigned int li, long long* opix)
+{
+    const int nfull = (2 * a.wsh + 1) * (2 * a.wsh + 1);
+    *opix = a.list[2 * li];
+    const int slot = a.list[2 * li + 1];
+    const int oy = (int)(*opix / a.W), ox = (int)(*opix - (long long)oy * a.W);
+    const int vy = oy == a.H - 1 ? oy + a.eh : oy;
+    const int vx = ox == a.W - 1 ? ox + a.ew : ox;
+    SmoothPixel px;
+    px.A = a.A;
+    px.F4 = reinterpret_cast<const float4*>(a.F.base);
+    px.wts = a.wt + ((long long)(slot / a.npix) * nfull * a.npix + slot % a.npix);
+    px.rb = (unsigned int)a.R.at(vy - a.wsh, vx - a.wsh);
+    px.fb = (unsigned int)a.F.at(vy - a.wsh, vx - a.wsh);
+    // at < 2.0 * (double)as in the reference: 2 * as is exact in float unless it
+    // overflows, and then both forms are true for every finite at
+    px.as2 = 2.0f * px.A[a.R.at(vy, vx)];
+    return px;
+}
+
+// search_window_smoothing (smoothing.cpp:78-129) and compute_insar for the queued pixels,
+// one pixel per thread.  Offsets are visited in the reference's row-major order, NLI_SCH at
+// a time: their loads are issued together and then used in order.
+//
+// The queue keeps the lmin largest candidate weights (candidates: amplitude < 2 * the
+// pixel's).  Let T be the smallest kept weight.  Unless some candidate that the queue
+// dropped also weighs exactly T, the kept set is exactly {candidates with weight >= T},
+// whatever the heap's tie rules, and the pop-order sum is the ascending sum of those
+// weights.  So for lmin < NLI_TOPK the largest weights are tracked in registers (one
+// median-of-three per slot and offset); the (lmin+1)-th largest tells whether such a tie
+// exists.  Pixels without one get their weighted means here; the others (and every pixel
+// when lmin >= NLI_TOPK) go to a second queue that nlinsar_smooth_exact works through with
+// the heap, so that neither kernel's waves wait on the other path.  The first pass keeps
+// the candidate flags of each run of NLI_SCH offsets as one LDS byte (byte c of a thread at
+// c * blockDim.x), which the second pass reads instead of the amplitudes.
+__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
+{
+    extern __shared__ unsigned char cflags[];
+    const int sw = 2 * a.wsh + 1;
+    const int nfull = sw * sw;
+    const int lmin = a.lmin;
+    const int lane = threadIdx.x & 63;
+    unsigned char* cf = cflags + threadIdx.x;
+    const int bs = blockDim.x;
     const unsigned int total = a.cnt[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
     for (unsigned int li = blockIdx.x * blockDim.x + threadIdx.x; li < total;
          li += gridDim.x * blockDim.x) {
-        const long long opix = a.list[2 * li];
-        const int slot = a.list[2 * li + 1];
-        // the pixel's weights: plane k at wts[k * npix]
-        const float* wts = a.wt + ((long long)(slot / a.npix) * nfull * a.npix + slot % a.npix);
-        const int oy = (int)(opix / a.W), ox = (int)(opix - (long long)oy * a.W);
-        const int vy = oy == a.H - 1 ? oy + a.eh : oy;
-        const int vx = ox == a.W - 1 ? ox + a.ew : ox;
-        const float4* R4 = reinterpret_cast<const float4*>(a.R.base);
-        const float4* F4 = reinterpret_cast<const float4*>(a.F.base);
-        const double as = R4[a.R.at(vy, vx)].x;
-        int n = 0;
-        for (int hh = 0; hh < sw; hh++)
-            for (int ww = 0; ww < sw; ww++) {
-                const float at = R4[a.R.at(vy + hh - a.wsh, vx + ww - a.wsh)].x;
-                if ((double)at < 2.0 * as) {
-                    const int idx = hh * sw + ww;
-                    const float w = wts[(long long)idx * a.npix];
-                    if (n < a.lmin) {
-                        hp_push(hw, hi, n, w, idx);
-                    } else if (hw[0] < w) {
-                        hp_pop(hw, hi, n);
-                        hp_push(hw, hi, n, w, idx);
+        long long opix;
+        const SmoothPixel px = nl_smooth_pixel(a, li, &opix);
+        bool exact = lmin >= NLI_TOPK;
+        float T = -INFINITY, wsum = 0.0f;
+        int m = 0;
+        if (!exact) {
+            // the NLI_TOPK largest candidate weights, v[0] >= v[1] >= ...
+            float v[NLI_TOPK];
+#pragma unroll
+            for (int i = 0; i < NLI_TOPK; i++) v[i] = -INFINITY;
+            int cnt = 0;
+            for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
+                float at[NLI_SCH], wv[NLI_SCH];
+                int hh = hh0, ww = ww0;
+#pragma unroll
+                for (int j = 0; j < NLI_SCH; j++) {
+                    if (k0 + j < nfull) {
+                        at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
+                        wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                    }
+                    if (++ww == sw) { ww = 0; hh++; }
+                }
+                unsigned int bits = 0u;
+#pragma unroll
+                for (int j = 0; j < NLI_SCH; j++) {
+                    if (k0 + j < nfull) {
+                        const bool cand = at[j] < px.as2;
+                        const float x = cand ? wv[j] : -INFINITY;
+                        cnt += cand;
+                        bits |= (unsigned int)cand << j;
+#pragma unroll
+                        for (int i = NLI_TOPK - 1; i > 0; i--) v[i] = __builtin_amdgcn_fmed3f(v[i], x, v[i - 1]);
+                        v[0] = fmaxf(v[0], x);
                     }
                 }
+                cf[(k0 / NLI_SCH) * bs] = (unsigned char)bits;
+                hh0 = hh;
+                ww0 = ww;
             }
-        for (int k = 0; k < mwords; k++) mask[k] = 0u;
-        int m = 0;
-        float wsum = 0.0f;
-        while (n > 0) {
-            wsum += hw[0];
-            mask[hi[0] >> 5] |= 1u << (hi[0] & 31);
-            m++;
-            hp_pop(hw, hi, n);
+            m = min(cnt, lmin);
+            float next = -INFINITY;
+#pragma unroll
+            for (int i = NLI_TOPK - 1; i >= 0; i--) {
+                if (cnt >= lmin && i == lmin - 1) T = v[i];
+                if (i == lmin) next = v[i];
+                if (i < m) wsum += v[i];
+            }
+            wsum /= (float)m;
+            exact = cnt > lmin && next == T;
         }
-        wsum /= (float)m;
+        const unsigned long long xm = __ballot(exact);
+        if (xm) {
+            unsigned int base = 0;
+            const int first = __ffsll((long long)xm) - 1;
+            if (lane == first) base = atomicAdd(a.cnt + 3, (unsigned int)__popcll(xm));
+            base = __shfl(base, first);
+            if (exact) a.xlist[base + __popcll(xm & ((1ull << lane) - 1ull))] = (int)li;
+        }
+        if (exact) continue;
         float N = 0.f, at_ = 0.f, xr = 0.f, xi = 0.f;
-        for (int k = 0; k < nfull; k++) {
-            const float w = ((mask[k >> 5] >> (k & 31)) & 1u) ? wsum : wts[(long long)k * a.npix];
-            const int hh = k / sw, ww = k - (k / sw) * sw;
-            const float4 f = F4[a.F.at(vy + hh - a.wsh, vx + ww - a.wsh)];
-            N += w;
-            at_ = fmaf(w, f.x, at_);
-            xr = fmaf(w, f.y, xr);
-            xi = fmaf(w, f.z, xi);
+        for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
+            float wv[NLI_SCH];
+            float4 f[NLI_SCH];
+            int hh = hh0, ww = ww0;
+#pragma unroll
+            for (int j = 0; j < NLI_SCH; j++) {
+                if (k0 + j < nfull) {
+                    wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                    f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
+                }
+                if (++ww == sw) { ww = 0; hh++; }
+            }
+            const unsigned int bits = cf[(k0 / NLI_SCH) * bs];
+#pragma unroll
+            for (int j = 0; j < NLI_SCH; j++) {
+                if (k0 + j < nfull) {
+                    const bool sel = ((bits >> j) & 1u) && wv[j] >= T;
+                    const float w = sel ? wsum : wv[j];
+                    N += w;
+                    at_ = fmaf(w, f[j].x, at_);
+                    xr = fmaf(w, f[j].y, xr);
+                    xi = fmaf(w, f[j].z, xi);
+                }
+            }
+            hh0 = hh;
+            ww0 = ww;
         }
+        nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
+                    a.new_coh);
+    }
+}
+
+// The pixels nlinsar_smooth queued for the exact heap (ties at the smallest kept weight).
+__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a)
+{
+    extern __shared__ float smem[];
+    const int bs = blockDim.x;
+    float* hw = smem + threadIdx.x;
+    int* hi = reinterpret_cast<int*>(smem + a.lmin * bs + threadIdx.x);
+    const unsigned int total = a.cnt[3];
+    for (unsigned int xi_ = blockIdx.x * blockDim.x + threadIdx.x; xi_ < total;
+         xi_ += gridDim.x * blockDim.x) {
+        long long opix;
+        const SmoothPixel px = nl_smooth_pixel(a, (unsigned int)a.xlist[xi_], &opix);
+        float N, at_, xr, xi;
+        nl_smooth_exact(a, px, hw, hi, bs, N, at_, xr, xi);
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
     }
@@ -692,7 +899,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->F = Grid{nullptr, 0, round_up(Wx + 2 * f, 64), f, f, Hx + 2 * f, Wx + 2 * f};
     plan->F.plane = (long long)plan->F.pitch * plan->F.h;
     const int nfull = pb.sw * pb.sw;
-    const int per = (2 * pb.lmin + (nfull + 31) / 32) * 4;
+    const int per = 2 * pb.lmin * 4;
     int hb = 256;
     while (hb > 1 && (size_t)hb * per > 48 * 1024) hb /= 2;
     plan->heap_block = hb;
@@ -705,6 +912,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
         return -4;
     }
     plan->off_R = take(sizeof(float) * 4 * plan->R.plane);
+    plan->off_A = take(sizeof(float) * plan->R.plane);
     plan->off_E = take(sizeof(float) * 4 * plan->E.plane);
     plan->off_F = take(sizeof(float) * 4 * plan->F.plane);
     plan->off_est = take(sizeof(float) * 6 * n);
@@ -716,6 +924,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->off_w = take(sizeof(float) * (size_t)nfull * slots);
     plan->off_s = take(sizeof(float) * (size_t)nfull * slots);
     plan->off_list = take(2 * sizeof(int) * n);
+    plan->off_xlist = take(sizeof(int) * n);
     plan->off_cnt = take(sizeof(unsigned int) * 4);
     plan->ws_bytes = off;
     return 0;
@@ -783,7 +992,8 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     DKX_HIP_CHECK(hipGetLastError());
     {
         dim3 blk(256), grd((R.w + 255) / 256, R.h);
-        hipLaunchKernelGGL(nlinsar_fill_raw, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R);
+        hipLaunchKernelGGL(nlinsar_fill_raw, grd, blk, 0, stream, d_a1, d_a2, d_phi, H, W, R,
+                           (float*)(ws + plan.off_A));
         DKX_HIP_CHECK(hipGetLastError());
         dim3 grf((F.w + 255) / 256, F.h);
         hipLaunchKernelGGL(nlinsar_filter_values, grf, blk, 0, stream, R, F);
@@ -849,16 +1059,23 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         sa.old_ref = cur[0]; sa.old_pha = cur[1]; sa.old_coh = cur[2];
         sa.new_ref = nxt[0]; sa.new_pha = nxt[1]; sa.new_coh = nxt[2];
         sa.list = list; sa.cnt = cnt;
-        const int nfull = pb.sw * pb.sw;
-        const size_t lds = (size_t)plan.heap_block * (2 * pb.lmin + (nfull + 31) / 32) * 4;
-        const int blocks = std::max(1, std::min((int)((n + plan.heap_block - 1) / plan.heap_block), ncu * 8));
+        sa.xlist = (int*)(ws + plan.off_xlist);
+        sa.A = (const float*)(ws + plan.off_A);
+        const size_t lds = (size_t)plan.heap_block * 2 * pb.lmin * 4;
+        const int nchunks = (pb.sw * pb.sw + NLI_SCH - 1) / NLI_SCH;   // flag bytes per thread
+        int fbs = 256;
+        while (fbs > 64 && (size_t)fbs * nchunks > 48 * 1024) fbs /= 2;
+        const int blocks = std::max(1, std::min((int)((n + fbs - 1) / fbs), ncu * 8));
+        const int xblocks = std::max(1, std::min((int)((n + plan.heap_block - 1) / plan.heap_block), ncu * 8));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing_on()) {
             DKX_HIP_CHECK(hipEventCreate(&e0));
             DKX_HIP_CHECK(hipEventCreate(&e1));
             DKX_HIP_CHECK(hipEventRecord(e0, stream));
         }
-        hipLaunchKernelGGL(nlinsar_smooth, dim3(blocks), dim3(plan.heap_block), lds, stream, sa);
+        hipLaunchKernelGGL(nlinsar_smooth, dim3(blocks), dim3(fbs), (size_t)fbs * nchunks, stream, sa);
+        DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_smooth_exact, dim3(xblocks), dim3(plan.heap_block), lds, stream, sa);
         DKX_HIP_CHECK(hipGetLastError());
         if (e0) {
             DKX_HIP_CHECK(hipEventRecord(e1, stream));
