@@ -335,6 +335,106 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
         if (tid + k * NTW < NF) fe[k] = stg[fs[k]];
 
     const int centre = wsh * sw + wsh;
+    if constexpr (!FIRST) {
+        // Later iterations: the KL field of offset p, the row sums of offset p-1 and the
+        // weights of offset p-2 are computed between the same two barriers, with the fields
+        // and row sums double-buffered (buffer b at sS + b*(NF + PY*TX): field, then rows).
+        const int npl = sw * sw;
+        float* const fbuf0 = sS;
+        float* const fbuf1 = sS + NF + PY * TX;
+        auto field = [&](int plane, float* dst) {
+            const int dy = plane / sw - wsh, dx = plane - (plane / sw) * sw - wsh;
+            const int doff = dy * EW + dx;
+            float4 vd[K1];
+#pragma unroll
+            for (int k = 0; k < K1; k++)
+                if (!(k * NTW + NTW > NF && tid + k * NTW >= NF)) vd[k] = stg[fs[k] + doff];
+#pragma unroll
+            for (int k = 0; k < K1; k++) {
+                const int i = tid + k * NTW;
+                if (k * NTW + NTW > NF && i >= NF) continue;
+                // orientation rule of the similarities, as in the first iteration's loop
+                const bool swp = dy < 0 && finner[k];
+                dst[i] = nl_pixel_kl_sym(fe[k], vd[k], swp ? fe[k].x : vd[k].x);
+            }
+        };
+        auto rowsum = [&](const float* src, float* dst) {
+#pragma unroll
+            for (int k = 0; k < K2; k++) {
+                if (k * NTW + NTW > NRS && tid + k * NTW >= NRS) continue;
+                float x[NLI_RSEG + NP2];
+#pragma unroll
+                for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = src[rsrc[k] + j];
+#pragma unroll
+                for (int g = 0; g < NLI_RSEG; g++) {
+                    float v = x[g];
+#pragma unroll
+                    for (int j = 1; j <= NP2; j++) v += x[g + j];
+                    dst[rdst[k] + g] = v;
+                }
+            }
+        };
+        // loads of the cached similarity sums and filter values of one offset
+        auto fetch = [&](int plane, float (&sv)[RPT], float4 (&fv)[RPT]) {
+            const long long pbase = chunk + (long long)plane * NPIX;
+            const int fd = (plane / sw - wsh) * a.fpitch + (plane - (plane / sw) * sw - wsh);
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                sv[q] = a.scache[pbase + slot0 + q * NCT];
+                fv[q] = a.F4[fofs[q] + fd];
+            }
+        };
+        // column sums, weight (centre weight 0), ENL sums and weighted means of one offset
+        auto accumulate = [&](int plane, const float (&sv)[RPT], const float4 (&fv)[RPT],
+                              const float* rows) {
+            const long long pbase = chunk + (long long)plane * NPIX;
+            const bool self = plane == centre;
+            float ck[RPT + NP2];
+            if (rows) {
+#pragma unroll
+                for (int j = 0; j < RPT + NP2; j++) ck[j] = rows[(og * RPT + j) * TX + oc];
+            }
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                float v = -0.0f;
+                if (rows) {
+                    v = ck[q];
+#pragma unroll
+                    for (int j = 1; j <= NP2; j++) v += ck[q + j];
+                }
+                float w = expf(sv[q] * a.rh + v * a.rT);
+                if (isnan(w) || isinf(w) || self) w = 0.0f;
+                if (valid[q]) a.wt[pbase + tid + q * NCT] = w;
+                const float4 f = fv[q];
+                acc[q][0] += w;
+                acc[q][1] = fmaf(w, w, acc[q][1]);
+                acc[q][2] = fmaf(w, f.x, acc[q][2]);
+                acc[q][3] = fmaf(w, f.y, acc[q][3]);
+                acc[q][4] = fmaf(w, f.z, acc[q][4]);
+            }
+        };
+        if (!kl) {
+            for (int plane = 0; plane < npl; plane++) {
+                float sv[RPT];
+                float4 fv[RPT];
+                fetch(plane, sv, fv);
+                accumulate(plane, sv, fv, nullptr);
+            }
+        } else {
+            for (int ph = 0; ph < npl + 2; ph++) {
+                const int p3 = ph - 2;
+                float sv[RPT];
+                float4 fv[RPT];
+                if (p3 >= 0) fetch(p3, sv, fv);
+                if (ph < npl && ph != centre) field(ph, (ph & 1) ? fbuf1 : fbuf0);
+                const int p2 = ph - 1;
+                if (p2 >= 0 && p2 < npl && p2 != centre)
+                    rowsum((p2 & 1) ? fbuf1 : fbuf0, ((p2 & 1) ? fbuf1 : fbuf0) + NF);
+                if (p3 >= 0) accumulate(p3, sv, fv, p3 != centre ? ((p3 & 1) ? fbuf1 : fbuf0) + NF : nullptr);
+                __syncthreads();
+            }
+        }
+    } else {
     for (int hh = 0; hh < sw; hh++) {
         const int dy = hh - wsh;
         for (int ww = 0; ww < sw; ww++) {
@@ -460,6 +560,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                 }
             }
         }
+    }
     }
     // pixels whose ENL is below lmin are queued for the smoothing, a wave at a time (one
     // atomic per wave keeps a wave's pixels, which are neighbours, together in the queue)
@@ -935,8 +1036,10 @@ static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t s
 {
     constexpr int np2 = 2 * PSH;
     const int o = base.wsh + PSH;
+    // staged records; then fields and row sums: similarity and KL once in the first
+    // iteration, KL double-buffered after
     const size_t lds = sizeof(float) * (4 * (size_t)(TY + 2 * o) * (TX + 2 * o) +
-                                        (FIRST ? 2 : 1) * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
+                                        2 * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
     DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, PSH, FIRST>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
