@@ -150,6 +150,8 @@ def nlinsar_form(args):
 
     n = args.size or 4096
     big = n >= 1024
+    if args.tile is not None:
+        DV.set_tile(1, args.tile)
     a1, a2, ph = S.interferogram(n, n, S.SEED_BASE + (2 if big else 0), n_regions=32 if big else 8,
                                  coh_range=(0.3, 0.9), scatterer_fraction=0.05 if big else 0.0)
     stream = HIP.Stream()

@@ -11,8 +11,12 @@ struct NlinsarProblem {
     int niter, lmin;
     float h, T;       // weight parameters (nlinsar.cpp:65-78)
     int ref_tile;     // side of the reference's fixed tiles (80, nlinsar.cpp:106)
-    int tile;         // tile shape index
+    int tile;         // tile shape index, or NLINSAR_TILE_AUTO
 };
+
+// Tile shape chosen from the image size: 54 x 54 with 1024 threads from 2^20 pixels up,
+// 18 x 54 with 512 threads below (where the larger blocks would leave CUs idle).
+constexpr int NLINSAR_TILE_AUTO = -1;
 
 struct NlinsarPlan {
     int tile, TY, TX;

@@ -177,7 +177,6 @@ struct WeightArgs {
     unsigned int* cnt;
 };
 
-constexpr int NTW = 512;
 constexpr int NLI_BAND = 4;   // block rows per band of the block order
 constexpr int NLI_RSEG = 3;   // row sums per thread (a run of adjacent outputs)
 
@@ -199,17 +198,19 @@ __device__ __forceinline__ float nl_pixel_kl_sym(float4 e1, float4 e2, float sec
 // FIRST: evaluate the likelihood similarities and keep their patch sums (they depend on the
 // raw data only); later iterations read them back and evaluate only the KL divergences.
 //
-// Work split (TX = 54, TY a multiple of 9): every thread keeps the same positions for all
-// offsets, so per offset only the offset's displacement changes.
+// Work split (TX = 54, NTW threads): every thread keeps the same positions for all offsets,
+// so per offset only the offset's displacement changes.
 //   field:    positions tid + k*NTW of the (TY+2p) x (TX+2p) patch-position grid; the record
 //             at the position itself stays in registers, the displaced one is read from LDS;
 //   row sums: runs of NLI_RSEG adjacent outputs of one row;
-//   outputs:  threads tid < 9*TX own TY/9 vertically adjacent pixels of one column, slot
-//             tid + m*9*TX in the block's similarity / weight chunk.
-template <int TY, int TX, int PSH, bool FIRST>
+//   outputs:  threads tid < NG*TX (NG row groups) own TY/NG vertically adjacent pixels of one
+//             column, slot tid + m*NG*TX in the block's similarity / weight chunk.
+template <int TY, int TX, int PSH, bool FIRST, int NTW>
 __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
 {
-    static_assert(TX % NLI_RSEG == 0 && TY % 9 == 0, "tile shape");
+    // output row groups: as many as the threads allow, dividing TY
+    constexpr int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
+    static_assert(TX % NLI_RSEG == 0 && TY % NG == 0, "tile shape");
     constexpr int NPIX = TY * TX;
     constexpr int NP2 = 2 * PSH;
     constexpr int PY = TY + NP2, PX = TX + NP2;
@@ -218,8 +219,8 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     constexpr int SEGS = TX / NLI_RSEG;
     constexpr int NRS = PY * SEGS;
     constexpr int K2 = (NRS + NTW - 1) / NTW;
-    constexpr int RPT = TY / 9;
-    constexpr int NCT = 9 * TX;
+    constexpr int RPT = TY / NG;
+    constexpr int NCT = NG * TX;
     static_assert(NCT <= NTW, "output threads");
     extern __shared__ float4 smem4[];
     const int wsh = a.wsh;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     // NCT repeat the last group's reads (nothing is stored for them), so that every load
     // issued is consumed on every path.
     const bool owner = tid < NCT;
-    const int og = min(tid / TX, 8), oc = tid - (tid / TX) * TX;
+    const int og = min(tid / TX, NG - 1), oc = tid - (tid / TX) * TX;
     const int slot0 = min(tid, NCT - 1);
     float acc[RPT][5];
     long long opix[RPT];
@@ -342,8 +343,10 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
         const int npl = sw * sw;
         float* const fbuf0 = sS;
         float* const fbuf1 = sS + NF + PY * TX;
-        auto field = [&](int plane, float* dst) {
-            const int dy = plane / sw - wsh, dx = plane - (plane / sw) * sw - wsh;
+        // Per offset only uniform values change: the staged displacement, and the base
+        // pointers of the cache, weight and filter-value planes (each lane adds its own
+        // 32-bit element offset, fixed for the whole kernel).
+        auto field = [&](int dy, int dx, float* dst) {
             const int doff = dy * EW + dx;
             float4 vd[K1];
 #pragma unroll
@@ -375,19 +378,19 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             }
         };
         // loads of the cached similarity sums and filter values of one offset
-        auto fetch = [&](int plane, float (&sv)[RPT], float4 (&fv)[RPT]) {
-            const long long pbase = chunk + (long long)plane * NPIX;
-            const int fd = (plane / sw - wsh) * a.fpitch + (plane - (plane / sw) * sw - wsh);
+        auto fetch = [&](int plane, int dy, int dx, float (&sv)[RPT], float4 (&fv)[RPT]) {
+            const float* sb = a.scache + (chunk + (long long)plane * NPIX);
+            const float4* fb = a.F4 + (dy * a.fpitch + dx);
 #pragma unroll
             for (int q = 0; q < RPT; q++) {
-                sv[q] = a.scache[pbase + slot0 + q * NCT];
-                fv[q] = a.F4[fofs[q] + fd];
+                sv[q] = sb[(unsigned int)(slot0 + q * NCT)];
+                fv[q] = fb[fofs[q]];
             }
         };
         // column sums, weight (centre weight 0), ENL sums and weighted means of one offset
         auto accumulate = [&](int plane, const float (&sv)[RPT], const float4 (&fv)[RPT],
                               const float* rows) {
-            const long long pbase = chunk + (long long)plane * NPIX;
+            float* wb = a.wt + (chunk + (long long)plane * NPIX);
             const bool self = plane == centre;
             float ck[RPT + NP2];
             if (rows) {
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                 }
                 float w = expf(sv[q] * a.rh + v * a.rT);
                 if (isnan(w) || isinf(w) || self) w = 0.0f;
-                if (valid[q]) a.wt[pbase + tid + q * NCT] = w;
+                if (valid[q]) wb[(unsigned int)(tid + q * NCT)] = w;
                 const float4 f = fv[q];
                 acc[q][0] += w;
                 acc[q][1] = fmaf(w, w, acc[q][1]);
@@ -414,23 +417,30 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             }
         };
         if (!kl) {
-            for (int plane = 0; plane < npl; plane++) {
+            for (int plane = 0, dy = -wsh, dx = -wsh; plane < npl; plane++) {
                 float sv[RPT];
                 float4 fv[RPT];
-                fetch(plane, sv, fv);
+                fetch(plane, dy, dx, sv, fv);
                 accumulate(plane, sv, fv, nullptr);
+                if (++dx > wsh) { dx = -wsh; dy++; }
             }
         } else {
+            // (fy, fx): offset of the field step; (ay, ax): offset of the weights step
+            int fy = -wsh, fx = -wsh, ay = -wsh, ax = -wsh;
             for (int ph = 0; ph < npl + 2; ph++) {
                 const int p3 = ph - 2;
                 float sv[RPT];
                 float4 fv[RPT];
-                if (p3 >= 0) fetch(p3, sv, fv);
-                if (ph < npl && ph != centre) field(ph, (ph & 1) ? fbuf1 : fbuf0);
+                if (p3 >= 0) fetch(p3, ay, ax, sv, fv);
+                if (ph < npl && ph != centre) field(fy, fx, (ph & 1) ? fbuf1 : fbuf0);
                 const int p2 = ph - 1;
                 if (p2 >= 0 && p2 < npl && p2 != centre)
                     rowsum((p2 & 1) ? fbuf1 : fbuf0, ((p2 & 1) ? fbuf1 : fbuf0) + NF);
-                if (p3 >= 0) accumulate(p3, sv, fv, p3 != centre ? ((p3 & 1) ? fbuf1 : fbuf0) + NF : nullptr);
+                if (p3 >= 0) {
+                    accumulate(p3, sv, fv, p3 != centre ? ((p3 & 1) ? fbuf1 : fbuf0) + NF : nullptr);
+                    if (++ax > wsh) { ax = -wsh; ay++; }
+                }
+                if (++fx > wsh) { fx = -wsh; fy++; }
                 __syncthreads();
             }
         }
@@ -977,7 +987,8 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->overlap = plan->wsh + plan->psh;
     plan->step = pb.ref_tile - 2 * plan->overlap;
     if (plan->step <= 0) { set_error("nlinsar: search window and patch too large for the 80-pixel tiles"); return -1; }
-    plan->tile = std::min(std::max(pb.tile, 0), 2);
+    plan->tile = pb.tile == NLINSAR_TILE_AUTO ? ((long long)pb.H * pb.W >= (1ll << 20) ? 2 : 1)
+                                              : std::min(std::max(pb.tile, 0), 2);
     plan->TY = kTY[plan->tile];
     plan->TX = kTX[plan->tile];
     // virtual position written to the last row/column by the reference's 80x80 tiling
@@ -1031,7 +1042,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     return 0;
 }
 
-template <int TY, int TX, int PSH, bool FIRST>
+template <int TY, int TX, int PSH, bool FIRST, int NT>
 static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t stream)
 {
     constexpr int np2 = 2 * PSH;
@@ -1041,7 +1052,7 @@ static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t s
     const size_t lds = sizeof(float) * (4 * (size_t)(TY + 2 * o) * (TX + 2 * o) +
                                         2 * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
-    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, PSH, FIRST>,
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, PSH, FIRST, NT>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     WeightArgs a = base;
     for (int i = 0; i < 4; i++) a.L[i] = L[i];
@@ -1054,7 +1065,7 @@ static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t s
         DKX_HIP_CHECK(hipEventCreate(&e1));
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
-    hipLaunchKernelGGL((nlinsar_weights<TY, TX, PSH, FIRST>), dim3(total), dim3(NTW), lds, stream, a);
+    hipLaunchKernelGGL((nlinsar_weights<TY, TX, PSH, FIRST, NT>), dim3(total), dim3(NT), lds, stream, a);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
         DKX_HIP_CHECK(hipEventRecord(e1, stream));
@@ -1067,9 +1078,9 @@ template <int PSH, bool FIRST>
 static int launch_weights_tile(int tile, const WeightArgs& a, const Launch* L, hipStream_t stream)
 {
     switch (tile) {
-    case 0: return launch_weights<27, 54, PSH, FIRST>(a, L, stream);
-    case 1: return launch_weights<18, 54, PSH, FIRST>(a, L, stream);
-    default: return launch_weights<54, 54, PSH, FIRST>(a, L, stream);
+    case 0: return launch_weights<27, 54, PSH, FIRST, 512>(a, L, stream);
+    case 1: return launch_weights<18, 54, PSH, FIRST, 512>(a, L, stream);
+    default: return launch_weights<54, 54, PSH, FIRST, 1024>(a, L, stream);
     }
 }
 

@@ -261,7 +261,8 @@ static void launch_nli(Server* s, int lane)
 {
     if (s->wait_nli.empty()) return;
     Request* r = s->wait_nli.front();
-    NlinsarProblem pb{r->r.height, r->r.width, s->nl_sw, s->nl_p, s->nl_niter, s->nl_lmin, s->nl_h, s->nl_T, 80, 1};
+    NlinsarProblem pb{r->r.height, r->r.width, s->nl_sw, s->nl_p, s->nl_niter, s->nl_lmin, s->nl_h, s->nl_T, 80,
+                      NLINSAR_TILE_AUTO};
     NlinsarPlan plan;
     if (nlinsar_plan(pb, &plan)) {
         s->wait_nli.pop_front();
