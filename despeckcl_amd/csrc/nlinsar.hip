@@ -671,8 +671,9 @@ __device__ void hp_pop(float* hw, int* hi, int bs, int& n)
     n = len;
 }
 
-constexpr int NLI_SCH = 4;    // offsets whose loads the smoothing issues together
+constexpr int NLI_SCH = 4;    // offsets whose loads the register selection issues together
 static_assert(NLI_SCH <= 8, "candidate flags of a run are kept in one byte");
+constexpr int XSCH = 8;       // the same for the exact queue (fewer live values)
 constexpr int NLI_TOPK = 16;  // largest lmin of the smoothing's register selection
 
 struct SmoothPixel {
@@ -694,11 +695,11 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
     const int lmin = a.lmin;
     int n = 0;
     float top = 0.0f;   // hw[0] once the queue holds lmin entries
-    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
-        float at[NLI_SCH], wv[NLI_SCH];
+    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += XSCH) {
+        float at[XSCH], wv[XSCH];
         int hh = hh0, ww = ww0;
 #pragma unroll
-        for (int j = 0; j < NLI_SCH; j++) {
+        for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull) {
                 at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
                 wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
@@ -706,7 +707,7 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
             if (++ww == sw) { ww = 0; hh++; }
         }
 #pragma unroll
-        for (int j = 0; j < NLI_SCH; j++) {
+        for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull && at[j] < px.as2) {
                 const float w = wv[j];
                 if (n < lmin) {
@@ -744,12 +745,12 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
     int sel = 0;
     int nxt = m > 0 ? hi[0] : nfull;
     N = at_ = xr = xi = 0.f;
-    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
-        float wv[NLI_SCH];
-        float4 f[NLI_SCH];
+    for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += XSCH) {
+        float wv[XSCH];
+        float4 f[XSCH];
         int hh = hh0, ww = ww0;
 #pragma unroll
-        for (int j = 0; j < NLI_SCH; j++) {
+        for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull) {
                 wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
                 f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
@@ -757,7 +758,7 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
             if (++ww == sw) { ww = 0; hh++; }
         }
 #pragma unroll
-        for (int j = 0; j < NLI_SCH; j++) {
+        for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull) {
                 float w = wv[j];
                 if (k0 + j == nxt) {
