@@ -445,45 +445,42 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
             }
         }
     } else {
-    for (int hh = 0; hh < sw; hh++) {
-        const int dy = hh - wsh;
-        for (int ww = 0; ww < sw; ww++) {
-            const int dx = ww - wsh;
-            const int plane = hh * sw + ww;
-            const long long pbase = chunk + (long long)plane * NPIX;
-            // centre weight 0 (nlinsar_sub_image.cpp:189-197), accumulated like the others
-            const bool self = plane == centre;
-            // this offset's cached similarity sums and filter values, issued before the field
-            // and row-sum steps that hide their latency
-            float sv[RPT];
-            float4 fv[RPT];
-            const int fd = dy * a.fpitch + dx;
+        // First iteration: similarities (and KL only when the initial estimates are not all
+        // zero) at every patch position, row sums, then per pixel the column sums, which are
+        // also kept in the similarity cache for the later iterations.
+        for (int hh = 0; hh < sw; hh++) {
+            const int dy = hh - wsh;
+            for (int ww = 0; ww < sw; ww++) {
+                const int dx = ww - wsh;
+                const int plane = hh * sw + ww;
+                const long long pbase = chunk + (long long)plane * NPIX;
+                // centre weight 0 (nlinsar_sub_image.cpp:189-197), accumulated like the others
+                const bool self = plane == centre;
+                // this offset's filter values, issued before the field and row-sum steps
+                float4 fv[RPT];
+                const int fd = dy * a.fpitch + dx;
 #pragma unroll
-            for (int q = 0; q < RPT; q++) {
-                if (!FIRST) sv[q] = a.scache[pbase + slot0 + q * NCT];
-                fv[q] = a.F4[fofs[q] + fd];
-            }
-            if ((FIRST || kl) && !self) {
-                __syncthreads();
-                // 1. similarity and KL at every patch position of the block.  Offsets with
-                //    dy >= 0 are evaluated centre-first everywhere; offsets with dy < 0 are the
-                //    mirrored values (the pixel at +d first) at positions inside the reference
-                //    tile's similarity border, centre-first on the border
-                //    (precompute_similarities_1st_pass.cl.in:36-42, _2nd_pass.cl:30-48).
-                const int doff = dy * EW + dx;
-                const unsigned int goff = (unsigned int)(dy * a.rpitch + dx);
-                // the displaced records first, all reads in flight together
-                float4 vd[K1];
+                for (int q = 0; q < RPT; q++) fv[q] = a.F4[fofs[q] + fd];
+                if (!self) {
+                    __syncthreads();
+                    // 1. similarity and KL at every patch position of the block.  Offsets with
+                    //    dy >= 0 are evaluated centre-first everywhere; offsets with dy < 0 are
+                    //    the mirrored values (the pixel at +d first) at positions inside the
+                    //    reference tile's similarity border, centre-first on the border
+                    //    (precompute_similarities_1st_pass.cl.in:36-42, _2nd_pass.cl:30-48).
+                    const int doff = dy * EW + dx;
+                    const unsigned int goff = (unsigned int)(dy * a.rpitch + dx);
+                    // the displaced records first, all reads in flight together
+                    float4 vd[K1];
 #pragma unroll
-                for (int k = 0; k < K1; k++)
-                    if (!(k * NTW + NTW > NF && tid + k * NTW >= NF)) vd[k] = stg[fs[k] + doff];
+                    for (int k = 0; k < K1; k++)
+                        if (!(k * NTW + NTW > NF && tid + k * NTW >= NF)) vd[k] = stg[fs[k] + doff];
 #pragma unroll
-                for (int k = 0; k < K1; k++) {
-                    const int i = tid + k * NTW;
-                    if (k * NTW + NTW > NF && i >= NF) continue;
-                    const float4 v = vd[k];
-                    const bool swp = dy < 0 && finner[k];
-                    if (FIRST) {
+                    for (int k = 0; k < K1; k++) {
+                        const int i = tid + k * NTW;
+                        if (k * NTW + NTW > NF && i >= NF) continue;
+                        const float4 v = vd[k];
+                        const bool swp = dy < 0 && finner[k];
                         const float4 u1 = swp ? v : fe[k], u2 = swp ? fe[k] : v;
                         sS[i] = nl_pixel_similarity_pre(u1, u2);
                         if (kl) {
@@ -492,66 +489,55 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                             const unsigned int p2 = swp ? fg[k] : fg[k] + goff;
                             sK[i] = nl_pixel_kl_pre(a.E4[p1], a.E4[p2]);
                         }
-                    } else {
-                        sK[i] = nl_pixel_kl_sym(fe[k], v, swp ? fe[k].x : v.x);
                     }
-                }
-                __syncthreads();
-                // 2. row sums along x (precompute_patch_similarities.cl:45-50)
+                    __syncthreads();
+                    // 2. row sums along x (precompute_patch_similarities.cl:45-50)
 #pragma unroll
-                for (int k = 0; k < K2; k++) {
-                    if (k * NTW + NTW > NRS && tid + k * NTW >= NRS) continue;
-                    if (FIRST) {
-                        float x[NLI_RSEG + NP2];
+                    for (int k = 0; k < K2; k++) {
+                        if (k * NTW + NTW > NRS && tid + k * NTW >= NRS) continue;
+                        {
+                            float x[NLI_RSEG + NP2];
 #pragma unroll
-                        for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sS[rsrc[k] + j];
+                            for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sS[rsrc[k] + j];
 #pragma unroll
-                        for (int g = 0; g < NLI_RSEG; g++) {
-                            float u = x[g];
+                            for (int g = 0; g < NLI_RSEG; g++) {
+                                float u = x[g];
 #pragma unroll
-                            for (int j = 1; j <= NP2; j++) u += x[g + j];
-                            rS[rdst[k] + g] = u;
+                                for (int j = 1; j <= NP2; j++) u += x[g + j];
+                                rS[rdst[k] + g] = u;
+                            }
+                        }
+                        if (kl) {
+                            float x[NLI_RSEG + NP2];
+#pragma unroll
+                            for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sK[rsrc[k] + j];
+#pragma unroll
+                            for (int g = 0; g < NLI_RSEG; g++) {
+                                float v = x[g];
+#pragma unroll
+                                for (int j = 1; j <= NP2; j++) v += x[g + j];
+                                rK[rdst[k] + g] = v;
+                            }
                         }
                     }
-                    if (kl) {
-                        float x[NLI_RSEG + NP2];
-#pragma unroll
-                        for (int j = 0; j < NLI_RSEG + NP2; j++) x[j] = sK[rsrc[k] + j];
-#pragma unroll
-                        for (int g = 0; g < NLI_RSEG; g++) {
-                            float v = x[g];
-#pragma unroll
-                            for (int j = 1; j <= NP2; j++) v += x[g + j];
-                            rK[rdst[k] + g] = v;
-                        }
-                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
-            // 3. rows accumulated top to bottom (precompute_patch_similarities.cl:53-58),
-            //    weight (compute_weights.cl:19-30), ENL sums (compute_number_of_looks.cl)
-            //    and weighted means (compute_insar.cl:81-93) in plane order
-            {
+                // 3. rows accumulated top to bottom (precompute_patch_similarities.cl:53-58),
+                //    weight (compute_weights.cl:19-30), ENL sums (compute_number_of_looks.cl)
+                //    and weighted means (compute_insar.cl:81-93) in plane order
                 float cs[RPT + NP2], ck[RPT + NP2];
-                if (FIRST) {
 #pragma unroll
-                    for (int j = 0; j < RPT + NP2; j++) cs[j] = rS[(og * RPT + j) * TX + oc];
-                }
+                for (int j = 0; j < RPT + NP2; j++) cs[j] = rS[(og * RPT + j) * TX + oc];
                 if (kl) {
 #pragma unroll
                     for (int j = 0; j < RPT + NP2; j++) ck[j] = rK[(og * RPT + j) * TX + oc];
                 }
 #pragma unroll
                 for (int q = 0; q < RPT; q++) {
-                    float u;
-                    if (FIRST) {
-                        u = cs[q];
+                    float u = cs[q];
 #pragma unroll
-                        for (int j = 1; j <= NP2; j++) u += cs[q + j];
-                        if (valid[q] && !self) a.scache[pbase + tid + q * NCT] = u;
-                    } else {
-                        u = sv[q];
-                    }
+                    for (int j = 1; j <= NP2; j++) u += cs[q + j];
+                    if (valid[q] && !self) a.scache[pbase + tid + q * NCT] = u;
                     float v = -0.0f;
                     if (kl) {
                         v = ck[q];
@@ -570,7 +556,6 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
                 }
             }
         }
-    }
     }
     // pixels whose ENL is below lmin are queued for the smoothing, a wave at a time (one
     // atomic per wave keeps a wave's pixels, which are neighbours, together in the queue)
