@@ -923,6 +923,16 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a)
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// LDS of one weights block: the staged records of its search neighbourhood, then the
+// similarity / KL fields and row sums (once each in the first iteration, KL double-buffered
+// after)
+static size_t nlinsar_lds_bytes(int TY, int TX, int wsh, int psh)
+{
+    const int o = wsh + psh, np2 = 2 * psh;
+    return sizeof(float) * (4 * (size_t)(TY + 2 * o) * (TX + 2 * o) +
+                            2 * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
+}
+
 static const int kTY[3] = {27, 18, 54};
 static const int kTX[3] = {54, 54, 54};
 
@@ -973,8 +983,15 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->overlap = plan->wsh + plan->psh;
     plan->step = pb.ref_tile - 2 * plan->overlap;
     if (plan->step <= 0) { set_error("nlinsar: search window and patch too large for the 80-pixel tiles"); return -1; }
-    plan->tile = pb.tile == NLINSAR_TILE_AUTO ? ((long long)pb.H * pb.W >= (1ll << 20) ? 2 : 1)
-                                              : std::min(std::max(pb.tile, 0), 2);
+    if (pb.tile == NLINSAR_TILE_AUTO) {
+        // 54 x 54 blocks when the image fills the GPU with them, a reference tile interior
+        // is a whole number of them, and their LDS fits; 18 x 54 otherwise
+        const bool big = (long long)pb.H * pb.W >= (1ll << 20) && plan->step % kTY[2] == 0 &&
+                         nlinsar_lds_bytes(kTY[2], kTX[2], plan->wsh, plan->psh) <= 160 * 1024;
+        plan->tile = big ? 2 : 1;
+    } else {
+        plan->tile = std::min(std::max(pb.tile, 0), 2);
+    }
     plan->TY = kTY[plan->tile];
     plan->TX = kTX[plan->tile];
     // virtual position written to the last row/column by the reference's 80x80 tiling
@@ -1031,12 +1048,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
 template <int TY, int TX, int PSH, bool FIRST, int NT>
 static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t stream)
 {
-    constexpr int np2 = 2 * PSH;
-    const int o = base.wsh + PSH;
-    // staged records; then fields and row sums: similarity and KL once in the first
-    // iteration, KL double-buffered after
-    const size_t lds = sizeof(float) * (4 * (size_t)(TY + 2 * o) * (TX + 2 * o) +
-                                        2 * ((size_t)(TY + np2) * (TX + np2) + (size_t)(TY + np2) * TX));
+    const size_t lds = nlinsar_lds_bytes(TY, TX, base.wsh, PSH);
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
     DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_weights<TY, TX, PSH, FIRST, NT>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

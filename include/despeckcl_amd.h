@@ -184,8 +184,9 @@ double dkx_chi2_inv(double p, double nu);
 
 /* Tile shape of the single-call entry points.  NL-SAR: 0, 1 (default), 2 = 32 x 32 output
  * tiles with 4, 2 or 1 rows per thread.  NL-InSAR: 0 = 27 x 54 and 1 = 18 x 54 (512 threads),
- * 2 = 54 x 54 (1024 threads), -1 (default) = 2 for images of 2^20 pixels or more, else 1.
- * Results do not depend on the tile shape. */
+ * 2 = 54 x 54 (1024 threads), -1 (default) = 2 for images of 2^20 pixels or more whose
+ * 80-pixel reference tiles have a 54 x 54 interior (search window + patch = 28, e.g. 21 and 7)
+ * and whose LDS fits, else 1.  Results do not depend on the tile shape. */
 int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
 int dkx_get_tile(int filter);
 

@@ -329,3 +329,27 @@ def test_nlinsar_smoothing_ties_and_large_lmin(D, oracle, lmin):
     g = D.nlinsar(a1, a2, ph, 21, 7, 3, lmin)
     o = oracle.nlinsar(a1, a2, ph, 21, 7, 3, lmin)
     assert_close(g, o[:3], q=0.95, tol_q=1e-3, flip_frac=0.03, label=f"nlinsar smoothing lmin {lmin} x3")
+
+
+@pytest.mark.parametrize("sw,p,lmin,shape", [(3, 3, 2, (40, 37)), (5, 7, 4, (33, 50)), (9, 3, 20, (45, 41))])
+def test_nlinsar_small_windows(D, oracle, sw, p, lmin, shape):
+    """Search windows of 3 to 9 (fewer offsets than one load run of the smoothing, lmin above
+    the register selection's 16 slots) against the oracle."""
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.interferogram(*shape, 40 + sw, n_regions=4, scatterer_fraction=0.05)
+    g = D.nlinsar(a1, a2, ph, sw, p, 2, lmin)
+    o = oracle.nlinsar(a1, a2, ph, sw, p, 2, lmin)
+    assert_close(g, o[:3], q=0.99, tol_q=1e-3, flip_frac=0.01, label=f"nlinsar sw {sw} p {p}")
+
+
+def test_nlinsar_large_window_large_image(D):
+    """A search window of 41 on a 1024 x 1024 image: the automatic choice must not pick the
+    54 x 54 blocks, whose LDS (and reference-tile interior of 34) do not suit this window;
+    one iteration, equal to the 18 x 54 blocks' result."""
+    from despeckcl_amd import synthetic as S
+    a1, a2, ph = S.interferogram(1024, 1024, 41, n_regions=16)
+    auto = _nlinsar_with_tile(D, -1, a1, a2, ph, 41, 7, 1, 10)
+    t1 = _nlinsar_with_tile(D, 1, a1, a2, ph, 41, 7, 1, 10)
+    for k in range(3):
+        assert np.array_equal(auto[k], t1[k], equal_nan=True), f"output {k}"
+        assert np.isfinite(auto[k][:-1, :-1]).all()
