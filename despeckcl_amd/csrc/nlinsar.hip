@@ -13,6 +13,7 @@
 #include "dkx_nlinsar.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 
 namespace dkx {
@@ -657,8 +658,9 @@ __device__ void hp_pop(float* hw, int* hi, int bs, int& n)
 }
 
 constexpr int NLI_SCH = 4;    // offsets whose loads the register selection issues together
-static_assert(NLI_SCH <= 8, "candidate flags of a run are kept in one byte");
 constexpr int XSCH = 8;       // the same for the exact queue (fewer live values)
+// Queues shorter than NLI_QWAVE pixels run one wave per pixel (nlinsar_smooth_wave).
+constexpr unsigned int NLI_QWAVE = 1u << 15;
 constexpr int NLI_TOPK = 16;  // largest lmin of the smoothing's register selection
 
 struct SmoothPixel {
@@ -672,6 +674,7 @@ struct SmoothPixel {
 // The exact queue of search_window_smoothing (smoothing.cpp:88-129) followed by the weighted
 // means: the lmin largest candidate weights kept by the heap above (its tie rules decide
 // which of several equal weights stay), replaced by their mean, summed in pop order.
+template <int XSCH>
 __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, float* hw, int* hi,
                                 int bs, float& N, float& at_, float& xr, float& xi)
 {
@@ -784,7 +787,8 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 
 // search_window_smoothing (smoothing.cpp:78-129) and compute_insar for the queued pixels,
 // one pixel per thread.  Offsets are visited in the reference's row-major order, NLI_SCH at
-// a time: their loads are issued together and then used in order.
+// a time (CH; NLI_SCH_SMALL for short queues): their loads are issued together and then used
+// in order.  Each variant serves the queue lengths in [qlo, qhi) and returns otherwise.
 //
 // The queue keeps the lmin largest candidate weights (candidates: amplitude < 2 * the
 // pixel's).  Let T be the smallest kept weight.  Unless some candidate that the queue
@@ -795,18 +799,22 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 // exists.  Pixels without one get their weighted means here; the others (and every pixel
 // when lmin >= NLI_TOPK) go to a second queue that nlinsar_smooth_exact works through with
 // the heap, so that neither kernel's waves wait on the other path.  The first pass keeps
-// the candidate flags of each run of NLI_SCH offsets as one LDS byte (byte c of a thread at
+// the candidate flags of each run of CH offsets in LDS (entry c of a thread at
 // c * blockDim.x), which the second pass reads instead of the amplitudes.
-__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
+template <int CH>
+__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a, unsigned int qlo, unsigned int qhi)
 {
-    extern __shared__ unsigned char cflags[];
+    using Flags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
+    static_assert(CH <= 32, "candidate flags of a run are kept in one 32-bit word");
+    extern __shared__ float4 smem_flags[];
+    const unsigned int total = a.cnt[0];
+    if (total < qlo || total >= qhi) return;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
     const int lmin = a.lmin;
     const int lane = threadIdx.x & 63;
-    unsigned char* cf = cflags + threadIdx.x;
+    Flags* cf = reinterpret_cast<Flags*>(smem_flags) + threadIdx.x;
     const int bs = blockDim.x;
-    const unsigned int total = a.cnt[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
     for (unsigned int li = blockIdx.x * blockDim.x + threadIdx.x; li < total;
          li += gridDim.x * blockDim.x) {
@@ -821,11 +829,11 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
 #pragma unroll
             for (int i = 0; i < NLI_TOPK; i++) v[i] = -INFINITY;
             int cnt = 0;
-            for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
-                float at[NLI_SCH], wv[NLI_SCH];
+            for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += CH) {
+                float at[CH], wv[CH];
                 int hh = hh0, ww = ww0;
 #pragma unroll
-                for (int j = 0; j < NLI_SCH; j++) {
+                for (int j = 0; j < CH; j++) {
                     if (k0 + j < nfull) {
                         at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
                         wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
@@ -834,7 +842,7 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
                 }
                 unsigned int bits = 0u;
 #pragma unroll
-                for (int j = 0; j < NLI_SCH; j++) {
+                for (int j = 0; j < CH; j++) {
                     if (k0 + j < nfull) {
                         const bool cand = at[j] < px.as2;
                         const float x = cand ? wv[j] : -INFINITY;
@@ -845,7 +853,7 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
                         v[0] = fmaxf(v[0], x);
                     }
                 }
-                cf[(k0 / NLI_SCH) * bs] = (unsigned char)bits;
+                cf[(k0 / CH) * bs] = (Flags)bits;
                 hh0 = hh;
                 ww0 = ww;
             }
@@ -870,21 +878,21 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
         }
         if (exact) continue;
         float N = 0.f, at_ = 0.f, xr = 0.f, xi = 0.f;
-        for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += NLI_SCH) {
-            float wv[NLI_SCH];
-            float4 f[NLI_SCH];
+        for (int k0 = 0, hh0 = 0, ww0 = 0; k0 < nfull; k0 += CH) {
+            float wv[CH];
+            float4 f[CH];
             int hh = hh0, ww = ww0;
 #pragma unroll
-            for (int j = 0; j < NLI_SCH; j++) {
+            for (int j = 0; j < CH; j++) {
                 if (k0 + j < nfull) {
                     wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
                     f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
                 }
                 if (++ww == sw) { ww = 0; hh++; }
             }
-            const unsigned int bits = cf[(k0 / NLI_SCH) * bs];
+            const unsigned int bits = cf[(k0 / CH) * bs];
 #pragma unroll
-            for (int j = 0; j < NLI_SCH; j++) {
+            for (int j = 0; j < CH; j++) {
                 if (k0 + j < nfull) {
                     const bool sel = ((bits >> j) & 1u) && wv[j] >= T;
                     const float w = sel ? wsum : wv[j];
@@ -902,22 +910,145 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a)
     }
 }
 
-// The pixels nlinsar_smooth queued for the exact heap (ties at the smallest kept weight).
-__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a)
+// The pixels nlinsar_smooth queued for the exact heap (ties at the smallest kept weight);
+// the variant with runs of CH offsets serves exact-queue lengths in [qlo, qhi).
+template <int CH>
+__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a, unsigned int qlo, unsigned int qhi)
 {
     extern __shared__ float smem[];
+    const unsigned int total = a.cnt[3];
+    if (total < qlo || total >= qhi) return;
     const int bs = blockDim.x;
     float* hw = smem + threadIdx.x;
     int* hi = reinterpret_cast<int*>(smem + a.lmin * bs + threadIdx.x);
-    const unsigned int total = a.cnt[3];
     for (unsigned int xi_ = blockIdx.x * blockDim.x + threadIdx.x; xi_ < total;
          xi_ += gridDim.x * blockDim.x) {
         long long opix;
         const SmoothPixel px = nl_smooth_pixel(a, (unsigned int)a.xlist[xi_], &opix);
         float N, at_, xr, xi;
-        nl_smooth_exact(a, px, hw, hi, bs, N, at_, xr, xi);
+        nl_smooth_exact<CH>(a, px, hw, hi, bs, N, at_, xr, xi);
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
+    }
+}
+
+// Short queues, one wave per pixel.  Per thread, the chain of dependent load runs of the
+// kernels above sets a floor of about half a millisecond, and the heap replays of a wave's
+// tied pixels add one more, however few pixels are queued.  Here the wave loads the pixel's
+// window (candidate flags, weights, filter values) into LDS at once, and its first lane runs
+// the reference's queue and the weighted sums in plane order from there, with the
+// arithmetic of nl_smooth_exact.  One block is one wave.
+__global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothArgs a, unsigned int qhi)
+{
+    extern __shared__ float4 smem_wave[];
+    const unsigned int total = a.cnt[0];
+    if (total >= qhi) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
+    const int sw = 2 * a.wsh + 1;
+    const int nfull = sw * sw;
+    const int np4 = (nfull + 3) & ~3;                 // arrays padded to whole float4
+    const int nwords = (nfull + 63) / 64 * 2;
+    const int lane = threadIdx.x;
+    float* W = reinterpret_cast<float*>(smem_wave);   // weight of offset k
+    float* Fx = W + np4;                              // filter values a, Re x, Im x
+    float* Fy = Fx + np4;
+    float* Fz = Fy + np4;
+    unsigned int* C = reinterpret_cast<unsigned int*>(Fz + np4);   // candidate flags
+    float* hw = reinterpret_cast<float*>(C + nwords);
+    int* hi = reinterpret_cast<int*>(hw + a.lmin);
+    for (unsigned int li = blockIdx.x; li < total; li += gridDim.x) {
+        long long opix;
+        const SmoothPixel px = nl_smooth_pixel(a, li, &opix);
+        for (int k0 = 0; k0 < nfull; k0 += 64) {
+            const int k = k0 + lane;
+            bool cand = false;
+            if (k < nfull) {
+                const int hh = k / sw, ww = k - (k / sw) * sw;
+                cand = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)] < px.as2;
+                W[k] = px.wts[(unsigned int)(k * a.npix)];
+                const float4 f = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
+                Fx[k] = f.x;
+                Fy[k] = f.y;
+                Fz[k] = f.z;
+            }
+            const unsigned long long b = __ballot(cand);
+            if (lane == 0) {
+                C[k0 / 32] = (unsigned int)b;
+                C[k0 / 32 + 1] = (unsigned int)(b >> 32);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) {
+            const int lmin = a.lmin;
+            int n = 0;
+            float top = 0.0f;
+            // four offsets per LDS read; runs of four without a candidate are skipped
+            for (int k0 = 0; k0 < nfull; k0 += 4) {
+                const unsigned int c4 = (C[k0 >> 5] >> (k0 & 31)) & 15u;
+                if (!c4) continue;
+                const float4 w4 = reinterpret_cast<const float4*>(W)[k0 >> 2];
+                const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (!((c4 >> j) & 1u)) continue;
+                    const float w = wj[j];
+                    if (n < lmin) {
+                        hp_push(hw, hi, 1, n, w, k0 + j);
+                        top = hw[0];
+                    } else if (top < w) {
+                        hp_pop(hw, hi, 1, n);
+                        hp_push(hw, hi, 1, n, w, k0 + j);
+                        top = hw[0];
+                    }
+                }
+            }
+            const int m = n;
+            float wsum = 0.0f;
+            while (n > 0) {
+                wsum += hw[0];
+                hp_pop(hw, hi, 1, n);
+            }
+            wsum /= (float)m;
+            for (int i = 1; i < m; i++) {
+                const int v = hi[i];
+                int j = i - 1;
+                while (j >= 0 && hi[j] > v) {
+                    hi[j + 1] = hi[j];
+                    j--;
+                }
+                hi[j + 1] = v;
+            }
+            int sel = 0;
+            int nxt = m > 0 ? hi[0] : nfull;
+            float N = 0.f, at_ = 0.f, xr = 0.f, xi = 0.f;
+            for (int k0 = 0; k0 < nfull; k0 += 4) {
+                const float4 w4 = reinterpret_cast<const float4*>(W)[k0 >> 2];
+                const float4 x4 = reinterpret_cast<const float4*>(Fx)[k0 >> 2];
+                const float4 y4 = reinterpret_cast<const float4*>(Fy)[k0 >> 2];
+                const float4 z4 = reinterpret_cast<const float4*>(Fz)[k0 >> 2];
+                const float wj[4] = {w4.x, w4.y, w4.z, w4.w};
+                const float xj[4] = {x4.x, x4.y, x4.z, x4.w};
+                const float yj[4] = {y4.x, y4.y, y4.z, y4.w};
+                const float zj[4] = {z4.x, z4.y, z4.z, z4.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (k0 + j >= nfull) break;
+                    float w = wj[j];
+                    if (k0 + j == nxt) {
+                        w = wsum;
+                        sel++;
+                        nxt = sel < m ? hi[sel] : nfull;
+                    }
+                    N += w;
+                    at_ = fmaf(w, xj[j], at_);
+                    xr = fmaf(w, yj[j], xr);
+                    xi = fmaf(w, zj[j], xi);
+                }
+            }
+            nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
+                        a.new_coh);
+        }
+        __syncthreads();
     }
 }
 
@@ -1174,20 +1305,33 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         sa.xlist = (int*)(ws + plan.off_xlist);
         sa.A = (const float*)(ws + plan.off_A);
         const size_t lds = (size_t)plan.heap_block * 2 * pb.lmin * 4;
-        const int nchunks = (pb.sw * pb.sw + NLI_SCH - 1) / NLI_SCH;   // flag bytes per thread
+        const int nfull = pb.sw * pb.sw;
+        // candidate flags: one byte per run of NLI_SCH offsets; blocks are narrowed until
+        // they fit 48 KB
+        const size_t fl = (size_t)(nfull + NLI_SCH - 1) / NLI_SCH;
         int fbs = 256;
-        while (fbs > 64 && (size_t)fbs * nchunks > 48 * 1024) fbs /= 2;
+        while (fbs > 64 && fbs * fl > 48 * 1024) fbs /= 2;
         const int blocks = std::max(1, std::min((int)((n + fbs - 1) / fbs), ncu * 8));
         const int xblocks = std::max(1, std::min((int)((n + plan.heap_block - 1) / plan.heap_block), ncu * 8));
+        // one wave per pixel for short queues: window in LDS, then the heap
+        const size_t wlds = 4 * (4 * (size_t)((nfull + 3) & ~3) + (size_t)(nfull + 63) / 64 * 2 + 2 * (size_t)pb.lmin);
+        const int wblocks = (int)std::min<size_t>(n, NLI_QWAVE);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing_on()) {
             DKX_HIP_CHECK(hipEventCreate(&e0));
             DKX_HIP_CHECK(hipEventCreate(&e1));
             DKX_HIP_CHECK(hipEventRecord(e0, stream));
         }
-        hipLaunchKernelGGL(nlinsar_smooth, dim3(blocks), dim3(fbs), (size_t)fbs * nchunks, stream, sa);
+        // the variants split the queue lengths between them: each returns at once outside
+        // its range (the lengths are known on the device only)
+        if (wlds > 64 * 1024) { set_error("nlinsar: search window too large for the smoothing"); return -2; }
+        hipLaunchKernelGGL(nlinsar_smooth_wave, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
         DKX_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(nlinsar_smooth_exact, dim3(xblocks), dim3(plan.heap_block), lds, stream, sa);
+        hipLaunchKernelGGL(nlinsar_smooth<NLI_SCH>, dim3(blocks), dim3(fbs), fbs * fl, stream, sa,
+                           NLI_QWAVE, 0xffffffffu);
+        DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_smooth_exact<XSCH>, dim3(xblocks), dim3(plan.heap_block), lds, stream, sa,
+                           0u, 0xffffffffu);
         DKX_HIP_CHECK(hipGetLastError());
         if (e0) {
             DKX_HIP_CHECK(hipEventRecord(e1, stream));
