@@ -937,13 +937,15 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a, 
 // tied pixels add one more, however few pixels are queued.  Here the wave loads the pixel's
 // window (candidate flags, weights, filter values) into LDS at once, and its first lane runs
 // the reference's queue and the weighted sums in plane order from there, with the
-// arithmetic of nl_smooth_exact.  One block is one wave.
+// arithmetic of nl_smooth_exact.  One block is one wave.  TIES: the pixels of the exact
+// queue that nlinsar_smooth left (when that queue is short), instead of the whole queue.
+template <bool TIES>
 __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothArgs a, unsigned int qhi)
 {
     extern __shared__ float4 smem_wave[];
-    const unsigned int total = a.cnt[0];
+    const unsigned int total = TIES ? a.cnt[3] : a.cnt[0];
     if (total >= qhi) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
+    if (!TIES && blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
     const int np4 = (nfull + 3) & ~3;                 // arrays padded to whole float4
@@ -956,7 +958,8 @@ __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothArgs a, un
     unsigned int* C = reinterpret_cast<unsigned int*>(Fz + np4);   // candidate flags
     float* hw = reinterpret_cast<float*>(C + nwords);
     int* hi = reinterpret_cast<int*>(hw + a.lmin);
-    for (unsigned int li = blockIdx.x; li < total; li += gridDim.x) {
+    for (unsigned int qi = blockIdx.x; qi < total; qi += gridDim.x) {
+        const unsigned int li = TIES ? (unsigned int)a.xlist[qi] : qi;
         long long opix;
         const SmoothPixel px = nl_smooth_pixel(a, li, &opix);
         for (int k0 = 0; k0 < nfull; k0 += 64) {
@@ -1325,13 +1328,17 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         // the variants split the queue lengths between them: each returns at once outside
         // its range (the lengths are known on the device only)
         if (wlds > 64 * 1024) { set_error("nlinsar: search window too large for the smoothing"); return -2; }
-        hipLaunchKernelGGL(nlinsar_smooth_wave, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
+        // short queue: one wave per pixel; else the register selection, whose tied pixels go
+        // to the waves again when they are few and to the per-thread heap otherwise
+        hipLaunchKernelGGL(nlinsar_smooth_wave<false>, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
         DKX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(nlinsar_smooth<NLI_SCH>, dim3(blocks), dim3(fbs), fbs * fl, stream, sa,
                            NLI_QWAVE, 0xffffffffu);
         DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_smooth_wave<true>, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
+        DKX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(nlinsar_smooth_exact<XSCH>, dim3(xblocks), dim3(plan.heap_block), lds, stream, sa,
-                           0u, 0xffffffffu);
+                           NLI_QWAVE, 0xffffffffu);
         DKX_HIP_CHECK(hipGetLastError());
         if (e0) {
             DKX_HIP_CHECK(hipEventRecord(e1, stream));
