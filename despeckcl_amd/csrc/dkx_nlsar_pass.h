@@ -229,6 +229,7 @@ void nlsar_scale_pass(const ScaleArgs a)
     using Lay = PassLds<TX, P, WMAX>;
     constexpr int W2 = Lay::W2, W2R = Lay::W2R;
     constexpr int DXM = Lay::DXM, RXM = Lay::RXM, RPL = Lay::RPL, WPL = Lay::WPL;
+    constexpr int DYM_ = Lay::DYM;
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     const int wsh = a.wsh;
@@ -284,17 +285,21 @@ void nlsar_scale_pass(const ScaleArgs a)
             {
                 const int n = DY * DX;
                 const float inv = 1.0f / (float)DX;
-                int sr_, sc_, r, c;
-                nlsar_divmod(NT, DX, inv, sr_, sc_);
-                nlsar_divmod(tid, DX, inv, r, c);
                 const unsigned int gb0 = (unsigned int)(im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P));
-                unsigned int g1 = gb0 + (unsigned int)(r * apitch + c);      // element index
                 const unsigned int dofs = (unsigned int)(dy * apitch + dx);
-                const unsigned int gstep = (unsigned int)(sr_ * apitch + sc_);
-                const unsigned int gwrap = (unsigned int)(apitch - DX);
-                int so = r * DXM + c;
-                const int sstep = sr_ * DXM + sc_;
-                for (int i = tid; i < n; i += NT) {
+                // elements tid + k*NT, at most KMAX of them, unrolled: row and column by the
+                // exact float quotient (see nlsar_divmod), addresses by 24-bit products
+                // (rows < 64, pitches < 2^17)
+                constexpr int KMAX = (DYM_ * (TX + WMAX + 2 * P) + NT - 1) / NT;
+                const float ft = (float)tid + 0.5f;
+#pragma unroll
+                for (int k = 0; k < KMAX; k++) {
+                    const int i = tid + k * NT;
+                    if (i >= n) break;
+                    const unsigned int r = (unsigned int)((ft + (float)(k * NT)) * inv);
+                    const unsigned int c = (unsigned int)i - __umul24(r, (unsigned int)DX);
+                    const unsigned int g1 = gb0 + __umul24(r, (unsigned int)apitch) + c;   // element index
+                    const unsigned int so = __umul24(r, (unsigned int)DXM) + c;
                     const unsigned int g2 = g1 + dofs;
                     float sim;
                     if constexpr (MODE == 3) {
@@ -321,14 +326,6 @@ void nlsar_scale_pass(const ScaleArgs a)
                     }
                     if (isnan(sim)) sim = 0.f;
                     sD[so] = sim;
-                    c += sc_;
-                    g1 += gstep;
-                    so += sstep;
-                    if (c >= DX) {
-                        c -= DX;
-                        g1 += gwrap;
-                        so += DXM - DX;
-                    }
                 }
             }
             __syncthreads();
