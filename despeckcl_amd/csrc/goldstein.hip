@@ -154,17 +154,25 @@ __global__ __launch_bounds__(gs_threads(P)) void gs_patches(const GsArgs a)
     // patch origin of each slot: tile-local packed row / column of its first pixel and the
     // tile origin in the image (-1 row origin: no patch)
     __shared__ int s_org[NPB][4];
+    // output of each slot: first float2 of the patch in U, and the U row pitch
+    __shared__ long long s_ubase[NPB];
+    __shared__ int s_upitch[NPB];
     if (tid < NPB) {
         const int job = blockIdx.x * NPB + tid;
         if (job < a.jobs) {
             const GsTileDev& td = a.tile[gs_tile_of_job(a, job)];
             const int local = job - td.job0;
-            s_org[tid][0] = (td.py0 + local / td.npx) * m - ov;
-            s_org[tid][1] = (td.px0 + local % td.npx) * m - ov;
+            const int qy = local / td.npx, qx = local - (local / td.npx) * td.npx;
+            s_org[tid][0] = (td.py0 + qy) * m - ov;
+            s_org[tid][1] = (td.px0 + qx) * m - ov;
             s_org[tid][2] = td.h_low;
             s_org[tid][3] = td.w_low;
+            s_upitch[tid] = td.npx * P;
+            s_ubase[tid] = td.u_off + (long long)qy * P * (td.npx * P) + (long long)qx * P;
         } else {
-            s_org[tid][0] = -1 << 30;
+            s_org[tid][0] = -(1 << 30);
+            s_upitch[tid] = 0;
+            s_ubase[tid] = -1;
         }
     }
     __syncthreads();
@@ -234,16 +242,12 @@ __global__ __launch_bounds__(gs_threads(P)) void gs_patches(const GsArgs a)
     }
     __syncthreads();
     // 5. coalesced store of the weighted patches
+#pragma unroll 4
     for (int e = tid; e < NPB * P * P; e += NT) {
         const int s = e / (P * P), r = (e / P) % P, c = e % P;
-        const int jb = blockIdx.x * NPB + s;
-        if (jb >= a.jobs) continue;
-        const int t = gs_tile_of_job(a, jb);
-        const GsTileDev& td = a.tile[t];
-        const int q = jb - td.job0;
-        const int qy = q / td.npx, qx = q - qy * td.npx;
-        const long long wu = (long long)td.npx * P;
-        gstore2(gptr(a.U) + td.u_off + ((long long)qy * P + r) * wu + qx * P + c, sp[(s * P + r) * LP + c]);
+        const long long ub = s_ubase[s];
+        if (ub < 0) continue;
+        gstore2(gptr(a.U) + ub + (long long)r * s_upitch[s] + c, sp[(s * P + r) * LP + c]);
     }
 }
 
