@@ -86,13 +86,11 @@ __device__ __forceinline__ float nlsar_logf(float x)
     return fabsf(l2) < __builtin_inff() ? hi + e : hi;
 }
 
-// q = n / d and n % d for 0 <= n < 2^16, 1 <= d <= 2^10: (n + 1/2) / d lies at least 1/(2d)
-// from an integer, far beyond the rounding of the float product, so the truncation is exact
-__device__ __forceinline__ void nlsar_divmod(int n, int d, float inv_d, int& q, int& r)
-{
-    q = (int)(((float)n + 0.5f) * inv_d);
-    r = n - q * d;
-}
+// Reciprocal of a small positive integer d by v_rcp_f32 (1 ulp).  Index quotients are
+// formed as q = (unsigned)((n + 1/2) * rcp(d)) for 0 <= n < 2^16: the exact (n + 1/2) / d lies
+// at least 1/(2d) from an integer, while the float product errs by at most
+// (n + 1/2) / d * 1.5 * 2^-23, below 1/(2d) for every n < 2^21, so the truncation is exact.
+__device__ __forceinline__ float nlsar_rcp(int d) { return __builtin_amdgcn_rcpf((float)d); }
 
 // 2*dim*log(2) for dim 2, the constant term of pixel_similarity_2x2 (sim_measures.h:58)
 __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.693147182464599609375f; }
@@ -105,7 +103,9 @@ __device__ __forceinline__ float nlsar_log2_term() { return 4.0f * 0.69314718246
 __device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, float rng,
                                             float rcp, float scl, const float* lut)
 {
-    const float d = fminf(fmaxf(s, dmin), dmax);
+    // the clamp as one median; a NaN s gives dmin (or NaN) and index 0 either way, as the
+    // min/max form does
+    const float d = __builtin_amdgcn_fmed3f(s, dmin, dmax);
     const float a = d - dmin;
     const float q0 = a * rcp;
     const float e = fmaf(-q0, rng, a);
@@ -241,6 +241,8 @@ void nlsar_scale_pass(const ScaleArgs a)
 
     const int tid = threadIdx.x;
     const int px = tid % TX, g = tid / TX;  // column, group of RPT rows
+    const float ft = (float)tid + 0.5f;     // numerator of the index quotients
+    const unsigned int tid4 = 4u * tid, tid8 = 8u * tid, tid16 = 16u * tid;
     PassImage im;
     int ty0, tx0;
     nlsar_block_tile(a, TX, im, ty0, tx0);
@@ -281,42 +283,45 @@ void nlsar_scale_pass(const ScaleArgs a)
             const int DY = RY + 2 * P, DX = RX + 2 * P;
             // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
             //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65).  Element i of the
-            //    DY x DX region: row r, column c; byte offsets advance incrementally.
+            //    DY x DX region: row r, column c.  Grid and LDS offsets are taken from
+            //    wave-uniform bases of this offset: element i lies r * (pitch - DX) + i
+            //    elements past the region's first element, in the grid as in LDS.
             {
                 const int n = DY * DX;
-                const float inv = 1.0f / (float)DX;
-                const unsigned int gb0 = (unsigned int)(im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P));
-                const unsigned int dofs = (unsigned int)(dy * apitch + dx);
-                // elements tid + k*NT, at most KMAX of them, unrolled: row and column by the
-                // exact float quotient (see nlsar_divmod), addresses by 24-bit products
-                // (rows < 64, pitches < 2^17)
+                const float inv = nlsar_rcp(DX);
+                const int gb0 = im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P);
+                const int dofs = dy * apitch + dx;
+                const DKX_GLOBAL char* const Au = A4b + (long long)(16 * AF4) * gb0;
+                const DKX_GLOBAL char* const Av = Au + (long long)(16 * AF4) * dofs;
+                const DKX_GLOBAL char* const Lu = LDb + 4ll * gb0;
+                const DKX_GLOBAL char* const Lv = Lu + 4ll * dofs;
+                const unsigned int gstep = (unsigned int)(apitch - DX);       // grid pitch >= DX
+                const unsigned int sstep = (unsigned int)(4 * (DXM - DX));    // bytes
+                // elements tid + k*NT, at most KMAX of them, unrolled
                 constexpr int KMAX = (DYM_ * (TX + WMAX + 2 * P) + NT - 1) / NT;
-                const float ft = (float)tid + 0.5f;
 #pragma unroll
                 for (int k = 0; k < KMAX; k++) {
                     const int i = tid + k * NT;
                     if (i >= n) break;
                     const unsigned int r = (unsigned int)((ft + (float)(k * NT)) * inv);
-                    const unsigned int c = (unsigned int)i - __umul24(r, (unsigned int)DX);
-                    const unsigned int g1 = gb0 + __umul24(r, (unsigned int)apitch) + c;   // element index
-                    const unsigned int so = __umul24(r, (unsigned int)DXM) + c;
-                    const unsigned int g2 = g1 + dofs;
+                    const unsigned int rel = __umul24(r, gstep) + (unsigned int)i;
                     float sim;
                     if constexpr (MODE == 3) {
                         // pixel_similarity_3x3, sim_measures.h:100-158
-                        const float4 u0 = gload4(A4b + 48u * g1), u1 = gload4(A4b + 48u * g1 + 16u);
-                        const float4 u2 = gload4(A4b + 48u * g1 + 32u);
-                        const float4 v0 = gload4(A4b + 48u * g2), v1 = gload4(A4b + 48u * g2 + 16u);
-                        const float4 v2 = gload4(A4b + 48u * g2 + 32u);
+                        const unsigned int o = 48u * rel;
+                        const float4 u0 = gload4(Au + o), u1 = gload4(Au + o + 16u);
+                        const float4 u2 = gload4(Au + o + 32u);
+                        const float4 v0 = gload4(Av + o), v1 = gload4(Av + o + 16u);
+                        const float4 v2 = gload4(Av + o + 32u);
                         const float det = nlsar_det3(u0.x + v0.x, u0.y + v0.y, u0.z + v0.z, u0.w + v0.w,
                                                      u1.x + v1.x, u1.y + v1.y, u1.z + v1.z, u1.w + v1.w,
                                                      u2.x + v2.x);
                         sim = -(((K + u2.y) + v2.y) - 2.0f * nlsar_logf(det));
                     } else {
-                        const float4 u = gload4(A4b + 16u * g1);
-                        const float4 v = gload4(A4b + 16u * g2);
-                        const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g1);
-                        const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(LDb + 4u * g2);
+                        const float4 u = gload4(Au + 16u * rel);
+                        const float4 v = gload4(Av + 16u * rel);
+                        const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(Lu + 4u * rel);
+                        const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(Lv + 4u * rel);
                         const float s00 = u.x + v.x;
                         const float s01r = u.z + v.z;
                         const float s01i = u.w + v.w;
@@ -325,23 +330,29 @@ void nlsar_scale_pass(const ScaleArgs a)
                         sim = -(((K + l1) + l2) - 2.0f * nlsar_logf(det));
                     }
                     if (isnan(sim)) sim = 0.f;
-                    sD[so] = sim;
+                    *reinterpret_cast<float*>(reinterpret_cast<char*>(sD) + (__umul24(r, sstep) + tid4) +
+                                              k * NT * 4) = sim;
                 }
             }
             __syncthreads();
             // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
-            //    patch size, for runs of 4 columns held in registers
+            //    patch size, for runs of 4 columns held in registers.  Element i: row r, run
+            //    xi = i - r * nrx; its LDS byte offsets are r * (row bytes - 16 nrx) + 16 i.
             {
                 const int nrx = (RX + 3) >> 2;
                 const int n = DY * nrx;
-                const float inv = 1.0f / (float)nrx;
-                int qs, qr, r, xi;
-                nlsar_divmod(NT, nrx, inv, qs, qr);
-                nlsar_divmod(tid, nrx, inv, r, xi);
-                for (int i = tid; i < n; i += NT) {
-                    const int x0 = xi << 2;
+                const float inv = nlsar_rcp(nrx);
+                const unsigned int sstep = (unsigned int)(4 * DXM - 16 * nrx);
+                const unsigned int rstep = (unsigned int)(4 * RXM - 16 * nrx);
+                constexpr int KR = (DYM_ * ((TX + WMAX + 3) / 4) + NT - 1) / NT;
+#pragma unroll
+                for (int kk = 0; kk < KR; kk++) {
+                    const int i = tid + kk * NT;
+                    if (i >= n) break;
+                    const unsigned int r = (unsigned int)((ft + (float)(kk * NT)) * inv);
                     float w[W2R];
-                    const float4* src = reinterpret_cast<const float4*>(sD + r * DXM + x0);
+                    const float4* src = reinterpret_cast<const float4*>(
+                        reinterpret_cast<const char*>(sD) + (__umul24(r, sstep) + tid16) + kk * NT * 16);
 #pragma unroll
                     for (int t = 0; t < W2R / 4; t++) {
                         const float4 v = src[t];
@@ -350,7 +361,8 @@ void nlsar_scale_pass(const ScaleArgs a)
                         w[4 * t + 2] = v.z;
                         w[4 * t + 3] = v.w;
                     }
-                    float* dst = sR + r * RXM + x0;
+                    float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sR) + (__umul24(r, rstep) + tid16) +
+                                                          kk * NT * 16);
 #pragma unroll
                     for (int k = 0; k < NP; k++) {
                         const int psh = pmask_bit(PMASK, k);
@@ -365,32 +377,30 @@ void nlsar_scale_pass(const ScaleArgs a)
                         }
                         *reinterpret_cast<float4*>(dst + k * RPL) = make_float4(o[0], o[1], o[2], o[3]);
                     }
-                    xi += qr;
-                    r += qs;
-                    if (xi >= nrx) {
-                        xi -= nrx;
-                        r++;
-                    }
                 }
             }
             __syncthreads();
             // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
             //    runs of 4 rows in pairs of columns, and the LUT weight
-            //    (compute_weights.cl:19-56)
+            //    (compute_weights.cl:19-56).  Element i: run ry, column pair xi = i - ry * nrx,
+            //    at LDS byte offset ry * (16 row bytes - 8 nrx) + 8 i.
             {
                 const int nrx = (RX + 1) >> 1;
                 const int nry = (RY + 3) >> 2;
                 const int n = nry * nrx;
-                const float inv = 1.0f / (float)nrx;
-                int qs, qr, ry, xi;
-                nlsar_divmod(NT, nrx, inv, qs, qr);
-                nlsar_divmod(tid, nrx, inv, ry, xi);
-                for (int i = tid; i < n; i += NT) {
-                    const int off = (ry << 2) * RXM + (xi << 1);
+                const float inv = nlsar_rcp(nrx);
+                const unsigned int ostep = (unsigned int)(16 * RXM - 8 * nrx);
+                constexpr int KC = (((TX + WMAX + 3) / 4) * ((TX + WMAX + 1) / 2) + NT - 1) / NT;
+#pragma unroll
+                for (int kk = 0; kk < KC; kk++) {
+                    const int i = tid + kk * NT;
+                    if (i >= n) break;
+                    const unsigned int ry = (unsigned int)((ft + (float)(kk * NT)) * inv);
+                    const unsigned int offb = __umul24(ry, ostep) + tid8 + (unsigned int)(kk * NT * 8);
 #pragma unroll
                     for (int k = 0; k < NP; k++) {
                         const int psh = pmask_bit(PMASK, k);
-                        const float* src = sR + k * RPL + off;
+                        const float* src = reinterpret_cast<const float*>(reinterpret_cast<const char*>(sR + k * RPL) + offb);
                         float w0[W2], w1[W2];
 #pragma unroll
                         for (int t = 0; t < W2; t++) {
@@ -399,7 +409,7 @@ void nlsar_scale_pass(const ScaleArgs a)
                             w1[t] = v.y;
                         }
                         const float* lut = sL + k * a.lut_stride;
-                        float* dst = sW + k * WPL + off;
+                        float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sW + k * WPL) + offb);
 #pragma unroll
                         for (int m = 0; m < 4; m++) {
                             float s0 = w0[m + P - psh], s1 = w1[m + P - psh];
@@ -413,12 +423,6 @@ void nlsar_scale_pass(const ScaleArgs a)
                             const float x1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
                             *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(x0, x1);
                         }
-                    }
-                    xi += qr;
-                    ry += qs;
-                    if (xi >= nrx) {
-                        xi -= nrx;
-                        ry++;
                     }
                 }
             }
