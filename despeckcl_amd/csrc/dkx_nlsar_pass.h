@@ -68,6 +68,8 @@ struct PassLds {
     static constexpr int total(int np, int lut_stride) { return off_L(np) + np * lut_stride; }
 };
 
+constexpr size_t NLSAR_LDS_PER_CU = 160 * 1024;   // gfx950
+
 // Window classes: the largest search-window half width each compiled layout serves.
 constexpr int NLSAR_WMAX_SMALL = 10;   // search windows up to 21
 constexpr int NLSAR_WMAX_LARGE = 20;   // search windows up to 41
@@ -675,10 +677,19 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
     using Lay = PassLds<TX, P, WMAX>;
-    const size_t lds = sizeof(float) * (size_t)Lay::total(NP, a.lut_stride);
-    if (a.wsh > WMAX || lds > 160 * 1024) {
+    size_t lds = sizeof(float) * (size_t)Lay::total(NP, a.lut_stride);
+    if (a.wsh > WMAX || lds > NLSAR_LDS_PER_CU) {
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
+    }
+    // Residency cap of the InSAR two-row variant: where its registers and LDS would let
+    // more than nlsar_blocks_per_cu blocks share a CU, the blocks resident on one XCD read
+    // more A / C grid than its 4 MB L2 holds and the offset loop refetches it (a single
+    // patch size at 2048 x 2048: 26 % L2 misses and 5.1 ms at 4 blocks per CU, 4.9 ms at
+    // 3); an LDS request just above a (blocks + 1)-th of the CU's LDS holds it there.
+    if (MODE == 0 && RPT == 2) {
+        constexpr size_t lds_floor = NLSAR_LDS_PER_CU / (nlsar_blocks_per_cu(TX, RPT, WMAX, MODE, LAST) + 1) + 1024;
+        if (lds < lds_floor) lds = lds_floor;
     }
     DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlsar_scale_pass<TX, RPT, PMASK, LAST, WMAX, MODE>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
