@@ -86,10 +86,16 @@ __global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, in
     }
 }
 
+// Quotient and square root at the hardware's accuracy: v_rcp_f32 then one product (within
+// 2 ulp), v_sqrt_f32 (1 ulp); OpenCL 1.2 allows 2.5 and 3 ulp for the reference's '/' and sqrt.
+__device__ __forceinline__ float nl_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float nl_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // pixel_similarity (nlinsar/sim_measures.h:27-40) from raw records (a1, a2, cos phase,
 // sin phase): cos(dp1 - dp2) = cos1*cos2 + sin1*sin2 and pow(q, 1.5) = q * sqrt(q), each a
 // few ulps from the reference's expression (inside OpenCL's bounds for cos and pow); the
-// other operations in the reference's order.
+// quotients and square roots at the hardware's accuracy (nl_div, nl_sqrt); the other
+// operations in the reference's order.
 __device__ __forceinline__ float nl_pixel_similarity_pre(float4 r1, float4 r2)
 {
     const float am1 = r1.x, as1 = r1.y, am2 = r2.x, as2 = r2.y;
@@ -98,8 +104,8 @@ __device__ __forceinline__ float nl_pixel_similarity_pre(float4 r1, float4 r2)
     const float A = s * s;
     const float B = 4.0f * (am1 * am1 * as1 * as1 + am2 * am2 * as2 * as2 + 2.0f * am1 * as1 * am2 * as2 * cs);
     const float C = am1 * as1 * am2 * as2;
-    const float q = C / B;
-    const float f_el = q * sqrtf(q) * ((A + B) / A * sqrtf(B / (A - B)) - asinf(sqrtf(B / A)));
+    const float q = nl_div(C, B);
+    const float f_el = q * nl_sqrt(q) * (nl_div(A + B, A) * nl_sqrt(nl_div(B, A - B)) - asinf(nl_sqrt(nl_div(B, A))));
     if ((f_el == 0) || isnan(f_el)) return 0.f;
     return logf(f_el);
 }
