@@ -60,15 +60,20 @@ struct PassLds {
     static constexpr int WRows = YU;
     static constexpr int RPL = RRows * RXM;        // row-sum plane
     static constexpr int WPL = WRows * RXM;        // weight plane
-    // the dissimilarities (steps 1-2) and the weights (steps 3-4) share one region
+    // the dissimilarities (steps 1-2) and the weights (steps 3-4) share one region; one
+    // such region and one row-sum region per offset slot of a barrier phase
     static constexpr int shared_floats(int np) { return DYM * DXM > np * WPL ? DYM * DXM : np * WPL; }
-    static constexpr int off_W = 0;
-    static constexpr int off_R(int np) { return shared_floats(np); }
-    static constexpr int off_L(int np) { return off_R(np) + np * RPL; }
-    static constexpr int total(int np, int lut_stride) { return off_L(np) + np * lut_stride; }
+    static constexpr int off_D(int np, int slot) { return slot * shared_floats(np); }
+    static constexpr int off_R(int np, int opb, int slot) { return opb * shared_floats(np) + slot * np * RPL; }
+    static constexpr int off_L(int np, int opb) { return off_R(np, opb, opb); }
+    static constexpr int total(int np, int opb, int lut_stride) { return off_L(np, opb) + np * lut_stride; }
 };
 
 constexpr size_t NLSAR_LDS_PER_CU = 160 * 1024;   // gfx950
+
+// Search-window offsets per barrier phase of the scale pass: two for one patch size (more
+// independent work between barriers, in LDS a single patch size leaves free), else one.
+__host__ __device__ constexpr int nlsar_offsets_per_phase(int np) { return np == 1 ? 2 : 1; }
 
 // Window classes: the largest search-window half width each compiled layout serves.
 constexpr int NLSAR_WMAX_SMALL = 10;   // search windows up to 21
@@ -235,11 +240,8 @@ void nlsar_scale_pass(const ScaleArgs a)
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     const int wsh = a.wsh;
-    const int sw = 2 * wsh + 1;
-    float* sD = smem;
-    float* sW = smem + Lay::off_W;
-    float* sR = smem + Lay::off_R(NP);
-    float* sL = smem + Lay::off_L(NP);
+    constexpr int OPB = nlsar_offsets_per_phase(NP);
+    float* sL = smem + Lay::off_L(NP, OPB);
 
     const int tid = threadIdx.x;
     const int px = tid % TX, g = tid / TX;  // column, group of RPT rows
@@ -271,264 +273,306 @@ void nlsar_scale_pass(const ScaleArgs a)
     const DKX_GLOBAL char* LDb = im.LDb;
     const DKX_GLOBAL char* C4b = im.C4b;
 
-    // half-window offsets d = (dy, dx) in plane order: rows above the centre, then the
-    // left half of the centre row (compute_pixel_similarities_2x2.cpp:29-47)
-    for (int hh = 0; hh <= wsh; hh++) {
-        const int dy = hh - wsh;                  // <= 0
-        const int wwe = (hh == wsh) ? wsh : sw;
-        for (int ww = 0; ww < wwe; ww++) {
-            const int dx = ww - wsh;
-            const int adx = dx < 0 ? -dx : dx;
-            // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
-            const int RY = TY - dy, RX = TX + adx;
-            const int rxo = dx > 0 ? -dx : 0;      // region x origin relative to tx0
-            const int DY = RY + 2 * P, DX = RX + 2 * P;
-            // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
-            //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65).  Element i of the
-            //    DY x DX region: row r, column c.  Grid and LDS offsets are taken from
-            //    wave-uniform bases of this offset: element i lies r * (pitch - DX) + i
-            //    elements past the region's first element, in the grid as in LDS.
-            {
-                const int n = DY * DX;
-                const float inv = nlsar_rcp(DX);
-                const int gb0 = im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P);
-                const int dofs = dy * apitch + dx;
-                const DKX_GLOBAL char* const Au = A4b + (long long)(16 * AF4) * gb0;
-                const DKX_GLOBAL char* const Av = Au + (long long)(16 * AF4) * dofs;
-                const DKX_GLOBAL char* const Lu = LDb + 4ll * gb0;
-                const DKX_GLOBAL char* const Lv = Lu + 4ll * dofs;
-                const unsigned int gstep = (unsigned int)(apitch - DX);       // grid pitch >= DX
-                const unsigned int sstep = (unsigned int)(4 * (DXM - DX));    // bytes
-                // elements tid + k*NT, at most KMAX of them, unrolled
-                constexpr int KMAX = (DYM_ * (TX + WMAX + 2 * P) + NT - 1) / NT;
+    // The four steps of one half-window offset d = (dy, dx), each on its own LDS buffers.
+    auto step1 = [&](const int dy, const int dx, float* sD) {
+        [[maybe_unused]] const int adx = dx < 0 ? -dx : dx;
+        // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+        [[maybe_unused]] const int RY = TY - dy, RX = TX + adx;
+        [[maybe_unused]] const int rxo = dx > 0 ? -dx : 0;   // region x origin relative to tx0
+        [[maybe_unused]] const int DY = RY + 2 * P, DX = RX + 2 * P;
+        // 1. pixel dissimilarities delta(g, g + d) of the later pixel g against the
+        //    earlier one (pixel_similarity_2x2, sim_measures.h:37-65).  Element i of the
+        //    DY x DX region: row r, column c.  Grid and LDS offsets are taken from
+        //    wave-uniform bases of this offset: element i lies r * (pitch - DX) + i
+        //    elements past the region's first element, in the grid as in LDS.
+        {
+            const int n = DY * DX;
+            const float inv = nlsar_rcp(DX);
+            const int gb0 = im.aorg + (ty0 - P) * apitch + (tx0 + rxo - P);
+            const int dofs = dy * apitch + dx;
+            const DKX_GLOBAL char* const Au = A4b + (long long)(16 * AF4) * gb0;
+            const DKX_GLOBAL char* const Av = Au + (long long)(16 * AF4) * dofs;
+            const DKX_GLOBAL char* const Lu = LDb + 4ll * gb0;
+            const DKX_GLOBAL char* const Lv = Lu + 4ll * dofs;
+            const unsigned int gstep = (unsigned int)(apitch - DX);       // grid pitch >= DX
+            const unsigned int sstep = (unsigned int)(4 * (DXM - DX));    // bytes
+            // elements tid + k*NT, at most KMAX of them, unrolled
+            constexpr int KMAX = (DYM_ * (TX + WMAX + 2 * P) + NT - 1) / NT;
 #pragma unroll
-                for (int k = 0; k < KMAX; k++) {
-                    const int i = tid + k * NT;
-                    if (i >= n) break;
-                    const unsigned int r = (unsigned int)((ft + (float)(k * NT)) * inv);
-                    const unsigned int rel = __umul24(r, gstep) + (unsigned int)i;
-                    float sim;
-                    if constexpr (MODE == 3) {
-                        // pixel_similarity_3x3, sim_measures.h:100-158
-                        const unsigned int o = 48u * rel;
-                        const float4 u0 = gload4(Au + o), u1 = gload4(Au + o + 16u);
-                        const float4 u2 = gload4(Au + o + 32u);
-                        const float4 v0 = gload4(Av + o), v1 = gload4(Av + o + 16u);
-                        const float4 v2 = gload4(Av + o + 32u);
-                        const float det = nlsar_det3(u0.x + v0.x, u0.y + v0.y, u0.z + v0.z, u0.w + v0.w,
-                                                     u1.x + v1.x, u1.y + v1.y, u1.z + v1.z, u1.w + v1.w,
-                                                     u2.x + v2.x);
-                        sim = -(((K + u2.y) + v2.y) - 2.0f * nlsar_logf(det));
-                    } else {
-                        const float4 u = gload4(Au + 16u * rel);
-                        const float4 v = gload4(Av + 16u * rel);
-                        const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(Lu + 4u * rel);
-                        const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(Lv + 4u * rel);
-                        const float s00 = u.x + v.x;
-                        const float s01r = u.z + v.z;
-                        const float s01i = u.w + v.w;
-                        const float s11 = u.y + v.y;
-                        const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
-                        sim = -(((K + l1) + l2) - 2.0f * nlsar_logf(det));
-                    }
-                    if (isnan(sim)) sim = 0.f;
-                    *reinterpret_cast<float*>(reinterpret_cast<char*>(sD) + (__umul24(r, sstep) + tid4) +
-                                              k * NT * 4) = sim;
+            for (int k = 0; k < KMAX; k++) {
+                const int i = tid + k * NT;
+                if (i >= n) break;
+                const unsigned int r = (unsigned int)((ft + (float)(k * NT)) * inv);
+                const unsigned int rel = __umul24(r, gstep) + (unsigned int)i;
+                float sim;
+                if constexpr (MODE == 3) {
+                    // pixel_similarity_3x3, sim_measures.h:100-158
+                    const unsigned int o = 48u * rel;
+                    const float4 u0 = gload4(Au + o), u1 = gload4(Au + o + 16u);
+                    const float4 u2 = gload4(Au + o + 32u);
+                    const float4 v0 = gload4(Av + o), v1 = gload4(Av + o + 16u);
+                    const float4 v2 = gload4(Av + o + 32u);
+                    const float det = nlsar_det3(u0.x + v0.x, u0.y + v0.y, u0.z + v0.z, u0.w + v0.w,
+                                                 u1.x + v1.x, u1.y + v1.y, u1.z + v1.z, u1.w + v1.w,
+                                                 u2.x + v2.x);
+                    sim = -(((K + u2.y) + v2.y) - 2.0f * nlsar_logf(det));
+                } else {
+                    const float4 u = gload4(Au + 16u * rel);
+                    const float4 v = gload4(Av + 16u * rel);
+                    const float l1 = *reinterpret_cast<const DKX_GLOBAL float*>(Lu + 4u * rel);
+                    const float l2 = *reinterpret_cast<const DKX_GLOBAL float*>(Lv + 4u * rel);
+                    const float s00 = u.x + v.x;
+                    const float s01r = u.z + v.z;
+                    const float s01i = u.w + v.w;
+                    const float s11 = u.y + v.y;
+                    const float det = (s00 * s11) - (s01r * s01r + s01i * s01i);
+                    sim = -(((K + l1) + l2) - 2.0f * nlsar_logf(det));
                 }
+                if (isnan(sim)) sim = 0.f;
+                *reinterpret_cast<float*>(reinterpret_cast<char*>(sD) + (__umul24(r, sstep) + tid4) +
+                                          k * NT * 4) = sim;
             }
-            __syncthreads();
-            // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
-            //    patch size, for runs of 4 columns held in registers.  Element i: row r, run
-            //    xi = i - r * nrx; its LDS byte offsets are r * (row bytes - 16 nrx) + 16 i.
-            {
-                const int nrx = (RX + 3) >> 2;
-                const int n = DY * nrx;
-                const float inv = nlsar_rcp(nrx);
-                const unsigned int sstep = (unsigned int)(4 * DXM - 16 * nrx);
-                const unsigned int rstep = (unsigned int)(4 * RXM - 16 * nrx);
-                constexpr int KR = (DYM_ * ((TX + WMAX + 3) / 4) + NT - 1) / NT;
-#pragma unroll
-                for (int kk = 0; kk < KR; kk++) {
-                    const int i = tid + kk * NT;
-                    if (i >= n) break;
-                    const unsigned int r = (unsigned int)((ft + (float)(kk * NT)) * inv);
-                    float w[W2R];
-                    const float4* src = reinterpret_cast<const float4*>(
-                        reinterpret_cast<const char*>(sD) + (__umul24(r, sstep) + tid16) + kk * NT * 16);
-#pragma unroll
-                    for (int t = 0; t < W2R / 4; t++) {
-                        const float4 v = src[t];
-                        w[4 * t] = v.x;
-                        w[4 * t + 1] = v.y;
-                        w[4 * t + 2] = v.z;
-                        w[4 * t + 3] = v.w;
-                    }
-                    float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sR) + (__umul24(r, rstep) + tid16) +
-                                                          kk * NT * 16);
-#pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const int psh = pmask_bit(PMASK, k);
-                        float o[4];
-#pragma unroll
-                        for (int m = 0; m < 4; m++) {
-                            float sum = w[m + P - psh];
-#pragma unroll
-                            for (int jj = -P + 1; jj <= P; jj++)
-                                if (jj > -psh && jj <= psh) sum = sum + w[m + P + jj];
-                            o[m] = sum;
-                        }
-                        *reinterpret_cast<float4*>(dst + k * RPL) = make_float4(o[0], o[1], o[2], o[3]);
-                    }
-                }
-            }
-            __syncthreads();
-            // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
-            //    runs of 4 rows in pairs of columns, and the LUT weight
-            //    (compute_weights.cl:19-56).  Element i: run ry, column pair xi = i - ry * nrx,
-            //    at LDS byte offset ry * (16 row bytes - 8 nrx) + 8 i.
-            {
-                const int nrx = (RX + 1) >> 1;
-                const int nry = (RY + 3) >> 2;
-                const int n = nry * nrx;
-                const float inv = nlsar_rcp(nrx);
-                const unsigned int ostep = (unsigned int)(16 * RXM - 8 * nrx);
-                constexpr int KC = (((TX + WMAX + 3) / 4) * ((TX + WMAX + 1) / 2) + NT - 1) / NT;
-#pragma unroll
-                for (int kk = 0; kk < KC; kk++) {
-                    const int i = tid + kk * NT;
-                    if (i >= n) break;
-                    const unsigned int ry = (unsigned int)((ft + (float)(kk * NT)) * inv);
-                    const unsigned int offb = __umul24(ry, ostep) + tid8 + (unsigned int)(kk * NT * 8);
-#pragma unroll
-                    for (int k = 0; k < NP; k++) {
-                        const int psh = pmask_bit(PMASK, k);
-                        const float* src = reinterpret_cast<const float*>(reinterpret_cast<const char*>(sR + k * RPL) + offb);
-                        float w0[W2], w1[W2];
-#pragma unroll
-                        for (int t = 0; t < W2; t++) {
-                            const float2 v = *reinterpret_cast<const float2*>(src + t * RXM);
-                            w0[t] = v.x;
-                            w1[t] = v.y;
-                        }
-                        const float* lut = sL + k * a.lut_stride;
-                        float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sW + k * WPL) + offb);
-#pragma unroll
-                        for (int m = 0; m < 4; m++) {
-                            float s0 = w0[m + P - psh], s1 = w1[m + P - psh];
-#pragma unroll
-                            for (int jj = -P + 1; jj <= P; jj++)
-                                if (jj > -psh && jj <= psh) {
-                                    s0 = s0 + w0[m + P + jj];
-                                    s1 = s1 + w1[m + P + jj];
-                                }
-                            const float x0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
-                            const float x1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
-                            *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(x0, x1);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
-            //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
-            //    (compute_nl_statistics.cl:69-84) of every patch size
-            {
-                const int py0 = g * RPT;
-                const int cy = ty0 + py0, cx = tx0 + px;
-                const unsigned int cu0 = (unsigned int)(im.corg + (cy + dy) * cpitch + (cx + dx));
-                const unsigned int cl0 = (unsigned int)(im.corg + (cy - dy) * cpitch + (cx - dx));
-                const float* wu0 = sW + py0 * RXM + (px - rxo);
-                const float* wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
-#pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    const unsigned int cu = cu0 + (unsigned int)(q * cpitch);
-                    const unsigned int cl = cl0 + (unsigned int)(q * cpitch);
-                    if constexpr (MODE == 0) {
-                        float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
-                        if (LAST) {
-                            const float4 u = gload4(C4b + 16u * cu);
-                            const float4 l = gload4(C4b + 16u * cl);
-                            i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
-                            i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
-                        } else {
-                            const float2 u = gload2(C4b + 16u * cu);
-                            const float2 l = gload2(C4b + 16u * cl);
-                            i0u = u.x; i1u = u.y;
-                            i0l = l.x; i1l = l.y;
-                        }
-                        const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
-                        const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
-#pragma unroll
-                        for (int k = 0; k < NP; k++) {
-                            const float wu = wu0[k * WPL + q * RXM];
-                            const float wl = wl0[k * WPL + q * RXM];
-                            acc[q][k][0] += wu;
-                            acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
-                            acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
-                            acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
-                            acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
-                            acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
-                            acc[q][k][0] += wl;
-                            acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
-                            acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
-                            acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
-                            acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
-                            acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
-                            if (LAST && k == NP - 1) {
-                                macc[q][0] = fmaf(wu, cru, macc[q][0]);
-                                macc[q][1] = fmaf(wu, ciu, macc[q][1]);
-                                macc[q][0] = fmaf(wl, crl, macc[q][0]);
-                                macc[q][1] = fmaf(wl, cil, macc[q][1]);
-                            }
-                        }
-                    } else {
-                        // covariance input: the diagonal intensities (planes (2*dim+2)*d,
-                        // compute_nl_statistics.cl:69-84) and, for the mean, every other plane
-                        float cu_[4 * CF4], cl_[4 * CF4];
-#pragma unroll
-                        for (int j = 0; j < CF4; j++) {
-                            const float4 u = gload4(C4b + (16u * CF4) * cu + 16u * j);
-                            const float4 l = gload4(C4b + (16u * CF4) * cl + 16u * j);
-                            cu_[4 * j] = u.x; cu_[4 * j + 1] = u.y; cu_[4 * j + 2] = u.z; cu_[4 * j + 3] = u.w;
-                            cl_[4 * j] = l.x; cl_[4 * j + 1] = l.y; cl_[4 * j + 2] = l.z; cl_[4 * j + 3] = l.w;
-                        }
-#pragma unroll
-                        for (int k = 0; k < NP; k++) {
-                            const float wu = wu0[k * WPL + q * RXM];
-                            const float wl = wl0[k * WPL + q * RXM];
-                            acc[q][k][0] += wu;
-                            acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
-#pragma unroll
-                            for (int d = 0; d < DIM; d++) {
-                                const float I = cu_[(2 * DIM + 2) * d];
-                                acc[q][k][2 + 2 * d] = fmaf(wu, I, acc[q][k][2 + 2 * d]);
-                                acc[q][k][3 + 2 * d] = fmaf(wu, I * I, acc[q][k][3 + 2 * d]);
-                            }
-                            acc[q][k][0] += wl;
-                            acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
-#pragma unroll
-                            for (int d = 0; d < DIM; d++) {
-                                const float I = cl_[(2 * DIM + 2) * d];
-                                acc[q][k][2 + 2 * d] = fmaf(wl, I, acc[q][k][2 + 2 * d]);
-                                acc[q][k][3 + 2 * d] = fmaf(wl, I * I, acc[q][k][3 + 2 * d]);
-                            }
-                            if (LAST && k == NP - 1) {
-                                int j = 0;
-#pragma unroll
-                                for (int pl = 0; pl < NPL; pl++) {
-                                    if (pl % (2 * DIM + 2) == 0) continue;   // a diagonal intensity
-                                    macc[q][j] = fmaf(wu, cu_[pl], macc[q][j]);
-                                    macc[q][j] = fmaf(wl, cl_[pl], macc[q][j]);
-                                    j++;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            // the next offset's step 1 overwrites this step's weights
-            __syncthreads();
         }
+    };
+    auto step2 = [&](const int dy, const int dx, float* sD, float* sR) {
+        [[maybe_unused]] const int adx = dx < 0 ? -dx : dx;
+        // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+        [[maybe_unused]] const int RY = TY - dy, RX = TX + adx;
+        [[maybe_unused]] const int rxo = dx > 0 ? -dx : 0;   // region x origin relative to tx0
+        [[maybe_unused]] const int DY = RY + 2 * P, DX = RX + 2 * P;
+        // 2. row sums (patch_similarities_row_pass.cl:52-60), left to right, of every
+        //    patch size, for runs of 4 columns held in registers.  Element i: row r, run
+        //    xi = i - r * nrx; its LDS byte offsets are r * (row bytes - 16 nrx) + 16 i.
+        {
+            const int nrx = (RX + 3) >> 2;
+            const int n = DY * nrx;
+            const float inv = nlsar_rcp(nrx);
+            const unsigned int sstep = (unsigned int)(4 * DXM - 16 * nrx);
+            const unsigned int rstep = (unsigned int)(4 * RXM - 16 * nrx);
+            constexpr int KR = (DYM_ * ((TX + WMAX + 3) / 4) + NT - 1) / NT;
+#pragma unroll
+            for (int kk = 0; kk < KR; kk++) {
+                const int i = tid + kk * NT;
+                if (i >= n) break;
+                const unsigned int r = (unsigned int)((ft + (float)(kk * NT)) * inv);
+                float w[W2R];
+                const float4* src = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const char*>(sD) + (__umul24(r, sstep) + tid16) + kk * NT * 16);
+#pragma unroll
+                for (int t = 0; t < W2R / 4; t++) {
+                    const float4 v = src[t];
+                    w[4 * t] = v.x;
+                    w[4 * t + 1] = v.y;
+                    w[4 * t + 2] = v.z;
+                    w[4 * t + 3] = v.w;
+                }
+                float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sR) + (__umul24(r, rstep) + tid16) +
+                                                      kk * NT * 16);
+#pragma unroll
+                for (int k = 0; k < NP; k++) {
+                    const int psh = pmask_bit(PMASK, k);
+                    float o[4];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        float sum = w[m + P - psh];
+#pragma unroll
+                        for (int jj = -P + 1; jj <= P; jj++)
+                            if (jj > -psh && jj <= psh) sum = sum + w[m + P + jj];
+                        o[m] = sum;
+                    }
+                    *reinterpret_cast<float4*>(dst + k * RPL) = make_float4(o[0], o[1], o[2], o[3]);
+                }
+            }
+        }
+    };
+    auto step3 = [&](const int dy, const int dx, float* sR, float* sW) {
+        [[maybe_unused]] const int adx = dx < 0 ? -dx : dx;
+        // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+        [[maybe_unused]] const int RY = TY - dy, RX = TX + adx;
+        [[maybe_unused]] const int rxo = dx > 0 ? -dx : 0;   // region x origin relative to tx0
+        [[maybe_unused]] const int DY = RY + 2 * P, DX = RX + 2 * P;
+        // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
+        //    runs of 4 rows in pairs of columns, and the LUT weight
+        //    (compute_weights.cl:19-56).  Element i: run ry, column pair xi = i - ry * nrx,
+        //    at LDS byte offset ry * (16 row bytes - 8 nrx) + 8 i.
+        {
+            const int nrx = (RX + 1) >> 1;
+            const int nry = (RY + 3) >> 2;
+            const int n = nry * nrx;
+            const float inv = nlsar_rcp(nrx);
+            const unsigned int ostep = (unsigned int)(16 * RXM - 8 * nrx);
+            constexpr int KC = (((TX + WMAX + 3) / 4) * ((TX + WMAX + 1) / 2) + NT - 1) / NT;
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++) {
+                const int i = tid + kk * NT;
+                if (i >= n) break;
+                const unsigned int ry = (unsigned int)((ft + (float)(kk * NT)) * inv);
+                const unsigned int offb = __umul24(ry, ostep) + tid8 + (unsigned int)(kk * NT * 8);
+#pragma unroll
+                for (int k = 0; k < NP; k++) {
+                    const int psh = pmask_bit(PMASK, k);
+                    const float* src = reinterpret_cast<const float*>(reinterpret_cast<const char*>(sR + k * RPL) + offb);
+                    float w0[W2], w1[W2];
+#pragma unroll
+                    for (int t = 0; t < W2; t++) {
+                        const float2 v = *reinterpret_cast<const float2*>(src + t * RXM);
+                        w0[t] = v.x;
+                        w1[t] = v.y;
+                    }
+                    const float* lut = sL + k * a.lut_stride;
+                    float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sW + k * WPL) + offb);
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        float s0 = w0[m + P - psh], s1 = w1[m + P - psh];
+#pragma unroll
+                        for (int jj = -P + 1; jj <= P; jj++)
+                            if (jj > -psh && jj <= psh) {
+                                s0 = s0 + w0[m + P + jj];
+                                s1 = s1 + w1[m + P + jj];
+                            }
+                        const float x0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                        const float x1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                        *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(x0, x1);
+                    }
+                }
+            }
+        }
+    };
+    auto step4 = [&](const int dy, const int dx, float* sW) {
+        [[maybe_unused]] const int adx = dx < 0 ? -dx : dx;
+        // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
+        [[maybe_unused]] const int RY = TY - dy, RX = TX + adx;
+        [[maybe_unused]] const int rxo = dx > 0 ? -dx : 0;   // region x origin relative to tx0
+        [[maybe_unused]] const int DY = RY + 2 * P, DX = RX + 2 * P;
+        // 4. accumulate w(c, d) (upper half) and w(c, -d) = w_half(c - d, d) (lower
+        //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
+        //    (compute_nl_statistics.cl:69-84) of every patch size
+        {
+            const int py0 = g * RPT;
+            const int cy = ty0 + py0, cx = tx0 + px;
+            const unsigned int cu0 = (unsigned int)(im.corg + (cy + dy) * cpitch + (cx + dx));
+            const unsigned int cl0 = (unsigned int)(im.corg + (cy - dy) * cpitch + (cx - dx));
+            const float* wu0 = sW + py0 * RXM + (px - rxo);
+            const float* wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
+#pragma unroll
+            for (int q = 0; q < RPT; q++) {
+                const unsigned int cu = cu0 + (unsigned int)(q * cpitch);
+                const unsigned int cl = cl0 + (unsigned int)(q * cpitch);
+                if constexpr (MODE == 0) {
+                    float i0u, i1u, i0l, i1l, cru = 0.f, ciu = 0.f, crl = 0.f, cil = 0.f;
+                    if (LAST) {
+                        const float4 u = gload4(C4b + 16u * cu);
+                        const float4 l = gload4(C4b + 16u * cl);
+                        i0u = u.x; i1u = u.y; cru = u.z; ciu = u.w;
+                        i0l = l.x; i1l = l.y; crl = l.z; cil = l.w;
+                    } else {
+                        const float2 u = gload2(C4b + 16u * cu);
+                        const float2 l = gload2(C4b + 16u * cl);
+                        i0u = u.x; i1u = u.y;
+                        i0l = l.x; i1l = l.y;
+                    }
+                    const float i0u2 = i0u * i0u, i1u2 = i1u * i1u;
+                    const float i0l2 = i0l * i0l, i1l2 = i1l * i1l;
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const float wu = wu0[k * WPL + q * RXM];
+                        const float wl = wl0[k * WPL + q * RXM];
+                        acc[q][k][0] += wu;
+                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wu, i0u, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wu, i0u2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wu, i1u, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wu, i1u2, acc[q][k][5]);
+                        acc[q][k][0] += wl;
+                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+                        acc[q][k][2] = fmaf(wl, i0l, acc[q][k][2]);
+                        acc[q][k][3] = fmaf(wl, i0l2, acc[q][k][3]);
+                        acc[q][k][4] = fmaf(wl, i1l, acc[q][k][4]);
+                        acc[q][k][5] = fmaf(wl, i1l2, acc[q][k][5]);
+                        if (LAST && k == NP - 1) {
+                            macc[q][0] = fmaf(wu, cru, macc[q][0]);
+                            macc[q][1] = fmaf(wu, ciu, macc[q][1]);
+                            macc[q][0] = fmaf(wl, crl, macc[q][0]);
+                            macc[q][1] = fmaf(wl, cil, macc[q][1]);
+                        }
+                    }
+                } else {
+                    // covariance input: the diagonal intensities (planes (2*dim+2)*d,
+                    // compute_nl_statistics.cl:69-84) and, for the mean, every other plane
+                    float cu_[4 * CF4], cl_[4 * CF4];
+#pragma unroll
+                    for (int j = 0; j < CF4; j++) {
+                        const float4 u = gload4(C4b + (16u * CF4) * cu + 16u * j);
+                        const float4 l = gload4(C4b + (16u * CF4) * cl + 16u * j);
+                        cu_[4 * j] = u.x; cu_[4 * j + 1] = u.y; cu_[4 * j + 2] = u.z; cu_[4 * j + 3] = u.w;
+                        cl_[4 * j] = l.x; cl_[4 * j + 1] = l.y; cl_[4 * j + 2] = l.z; cl_[4 * j + 3] = l.w;
+                    }
+#pragma unroll
+                    for (int k = 0; k < NP; k++) {
+                        const float wu = wu0[k * WPL + q * RXM];
+                        const float wl = wl0[k * WPL + q * RXM];
+                        acc[q][k][0] += wu;
+                        acc[q][k][1] = fmaf(wu, wu, acc[q][k][1]);
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) {
+                            const float I = cu_[(2 * DIM + 2) * d];
+                            acc[q][k][2 + 2 * d] = fmaf(wu, I, acc[q][k][2 + 2 * d]);
+                            acc[q][k][3 + 2 * d] = fmaf(wu, I * I, acc[q][k][3 + 2 * d]);
+                        }
+                        acc[q][k][0] += wl;
+                        acc[q][k][1] = fmaf(wl, wl, acc[q][k][1]);
+#pragma unroll
+                        for (int d = 0; d < DIM; d++) {
+                            const float I = cl_[(2 * DIM + 2) * d];
+                            acc[q][k][2 + 2 * d] = fmaf(wl, I, acc[q][k][2 + 2 * d]);
+                            acc[q][k][3 + 2 * d] = fmaf(wl, I * I, acc[q][k][3 + 2 * d]);
+                        }
+                        if (LAST && k == NP - 1) {
+                            int j = 0;
+#pragma unroll
+                            for (int pl = 0; pl < NPL; pl++) {
+                                if (pl % (2 * DIM + 2) == 0) continue;   // a diagonal intensity
+                                macc[q][j] = fmaf(wu, cu_[pl], macc[q][j]);
+                                macc[q][j] = fmaf(wl, cl_[pl], macc[q][j]);
+                                j++;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    };
+
+    // Half-window offsets in plane order: rows above the centre, then the left half of the
+    // centre row (compute_pixel_similarities_2x2.cpp:29-47), 2 wsh (wsh + 1) of them, an even
+    // count.  OPB consecutive offsets share each barrier phase, in separate LDS slots; their
+    // statistics are accumulated in plane order, as one offset per phase would.
+    const int nofs = 2 * wsh * (wsh + 1);
+    int ny = -wsh, nx = -wsh;
+    for (int t = 0; t < nofs; t += OPB) {
+        int oy[OPB], ox[OPB];
+#pragma unroll
+        for (int o = 0; o < OPB; o++) {
+            oy[o] = ny;
+            ox[o] = nx;
+            if (++nx > wsh) {
+                nx = -wsh;
+                ny++;
+            }
+        }
+#pragma unroll
+        for (int o = 0; o < OPB; o++) step1(oy[o], ox[o], smem + Lay::off_D(NP, o));
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OPB; o++) step2(oy[o], ox[o], smem + Lay::off_D(NP, o), smem + Lay::off_R(NP, OPB, o));
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OPB; o++) step3(oy[o], ox[o], smem + Lay::off_R(NP, OPB, o), smem + Lay::off_D(NP, o));
+        __syncthreads();
+#pragma unroll
+        for (int o = 0; o < OPB; o++) step4(oy[o], ox[o], smem + Lay::off_D(NP, o));
+        // the next phase's step 1 overwrites these weights
+        __syncthreads();
     }
 
     // centre weight 1 (routines.cpp:151-157), then the per-pixel statistics
@@ -677,7 +721,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
     using Lay = PassLds<TX, P, WMAX>;
-    size_t lds = sizeof(float) * (size_t)Lay::total(NP, a.lut_stride);
+    size_t lds = sizeof(float) * (size_t)Lay::total(NP, nlsar_offsets_per_phase(NP), a.lut_stride);
     if (a.wsh > WMAX || lds > NLSAR_LDS_PER_CU) {
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
