@@ -113,8 +113,8 @@ static int ensure_stream(DeviceWorkspace* w)
     return 0;
 }
 
-// NL-SAR: 2 rows per thread; NL-InSAR: chosen per image (NLINSAR_TILE_AUTO)
-static std::atomic<int> g_tile[2] = {{1}, {NLINSAR_TILE_AUTO}};
+// NL-SAR: chosen per pass (NLSAR_TILE_AUTO); NL-InSAR: chosen per image (NLINSAR_TILE_AUTO)
+static std::atomic<int> g_tile[2] = {{NLSAR_TILE_AUTO}, {NLINSAR_TILE_AUTO}};
 
 static std::atomic<bool> g_timing{false};
 static std::mutex g_timing_mu;
@@ -318,7 +318,7 @@ int dkx_version(void) { return 1; }
 
 int dkx_set_tile(int filter, int tile_index)
 {
-    if (filter < 0 || filter > 1 || tile_index < (filter == 1 ? NLINSAR_TILE_AUTO : 0) || tile_index > 2) {
+    if (filter < 0 || filter > 1 || tile_index < -1 || tile_index > 2) {
         set_error("dkx_set_tile: invalid filter or tile index");
         return DKX_EINVAL;
     }

@@ -14,6 +14,7 @@ constexpr int NLSAR_MAX_PATCHES = 16;  // patch sizes per call
 constexpr int NLSAR_MAX_NS = 8;
 constexpr int NLSAR_TILE = 32;         // every pass variant computes 32 x 32 output tiles
 constexpr int NLSAR_NTILES = 3;        // pass variants: 4, 2 or 1 rows per thread
+constexpr int NLSAR_TILE_AUTO = -1;    // per pass: 1 row per thread for one patch size, else 2
 
 // Records of the grids, per mode (0: InSAR input; 2, 3: covariance input of that dimension).
 // C: the input covariance; mode 0 keeps (a1^2, a2^2, Re C01, Im C01), modes 2 / 3 every plane
@@ -40,7 +41,7 @@ struct NlsarParams {
     const int* S; int nS;
     const NlsarStatsView* stats; int nst;
     float h, c;
-    int tile;         // pass variant (0..NLSAR_NTILES-1)
+    int tile;         // pass variant (0..NLSAR_NTILES-1), or NLSAR_TILE_AUTO
     // 1: statistics of every (patch, scale) key, for the ENL / alpha maps and the count of
     // pixels whose best-parameter selection is undefined; 0: only the last key, the one the
     // selection returns whenever it is defined (best_params.cpp:27-53), so the filtered

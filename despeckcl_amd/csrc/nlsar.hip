@@ -205,7 +205,7 @@ int nlsar_common(const NlsarParams& pp, NlsarCommon* cm)
         return -1;
     }
     cm->mode = pp.dim;
-    cm->tile = std::min(std::max(pp.tile, 0), NLSAR_NTILES - 1);
+    cm->tile = pp.tile < 0 ? NLSAR_TILE_AUTO : std::min(pp.tile, NLSAR_NTILES - 1);
     cm->wsh = (pp.sw - 1) / 2;
     cm->P = (pmax - 1) / 2;
     cm->smax = smax;
@@ -469,7 +469,10 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
         int rc;
         if (cm.mode == 0) {
             if (cm.wsh <= NLSAR_WMAX_SMALL) {
-                switch (cm.tile) {
+                // automatic: one row per thread for a single patch size (its two offsets per
+                // barrier phase fill 1024-thread blocks best), two rows otherwise
+                const int tile = cm.tile != NLSAR_TILE_AUTO ? cm.tile : pmask_count(pmask) == 1 ? 2 : 1;
+                switch (tile) {
                 case 0:
                     rc = ps.last ? nlsar_pass_launch<true, 32, 4, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream)
                                  : nlsar_pass_launch<false, 32, 4, NLSAR_WMAX_SMALL, 0>(pmask, sa, stream);

@@ -182,8 +182,10 @@ long long dkx_nlsar_stats_load(const char* filename, dkx_nlsar_stats* out, int m
  * src/compute/nlsar/stats.cpp:76). */
 double dkx_chi2_inv(double p, double nu);
 
-/* Tile shape of the single-call entry points.  NL-SAR: 0, 1 (default), 2 = 32 x 32 output
- * tiles with 4, 2 or 1 rows per thread.  NL-InSAR: 0 = 27 x 54 and 1 = 18 x 54 (512 threads),
+/* Tile shape of the single-call entry points.  NL-SAR: 0, 1, 2 = 32 x 32 output tiles with
+ * 4, 2 or 1 rows per thread; -1 (default) = per pass, 2 for a pass of one patch size (the
+ * default selected-key pass) and 1 for several (search windows up to 21; larger ones use the
+ * 4-row variant whatever the setting).  NL-InSAR: 0 = 27 x 54 and 1 = 18 x 54 (512 threads),
  * 2 = 54 x 54 (1024 threads), -1 (default) = 2 for images of 2^20 pixels or more whose
  * 80-pixel reference tiles have a 54 x 54 interior (search window + patch = 28, e.g. 21 and 7)
  * and whose LDS fits, else 1.  Results do not depend on the tile shape. */

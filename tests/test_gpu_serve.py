@@ -74,7 +74,7 @@ def test_served_results_equal_alone(D, stats):
     from despeckcl_amd import device as DV
     from despeckcl_amd import hip
     from despeckcl_amd import synthetic as S
-    assert DV.get_tile(0) == 1, "the server batches NL-SAR with the default pass variant"
+    assert DV.get_tile(0) == -1, "the per-call entry points run the automatic pass variant"
     stream = hip.Stream()
     reqs = [Req(hip, S, k, h, w, 500 + i) for i, (k, h, w) in enumerate(SHAPES)]
     alone = []

@@ -109,18 +109,19 @@ def test_argument_errors_without_device(D):
 
 
 def test_tile_settings(D):
-    """Tile shapes: NL-InSAR defaults to the automatic choice (-1) and accepts -1..2; NL-SAR
-    accepts 0..2 only; the setting round-trips and invalid values are rejected."""
+    """Tile shapes: both filters default to the automatic choice (-1) and accept -1..2; the
+    setting round-trips and invalid values are rejected."""
     from despeckcl_amd import device as DV
     assert DV.get_tile(1) == -1
-    assert DV.get_tile(0) == 1
-    for t in (2, 0, 1, -1):
-        DV.set_tile(1, t)
-        assert DV.get_tile(1) == t
-    for f, t in ((0, -1), (1, -2), (1, 3), (2, 0)):
+    assert DV.get_tile(0) == -1
+    for f in (0, 1):
+        for t in (2, 0, 1, -1):
+            DV.set_tile(f, t)
+            assert DV.get_tile(f) == t
+    for f, t in ((0, -2), (0, 3), (1, -2), (1, 3), (2, 0)):
         with pytest.raises(ValueError, match="tile"):
             DV.set_tile(f, t)
-    assert DV.get_tile(0) == 1 and DV.get_tile(1) == -1
+    assert DV.get_tile(0) == -1 and DV.get_tile(1) == -1
 
 
 def test_training_needs_a_device(D):
