@@ -29,7 +29,7 @@ struct NlsarBatch {
     NlsarParams pp{};
     NlsarCommon cm{};
     int tile = -1;               // -1: the autotuner's choice for this batch shape, else fixed
-    int tile_used = 1;
+    int tile_used = NLSAR_TILE_AUTO;
     int diagnostics = 0;         // statistics of every key (undefined-selection counts)
     // images
     std::vector<NlsarImageDesc> descs;
@@ -79,7 +79,7 @@ static int choose_tile(NlsarBatch* b)
 {
     int t = b->tile;
     if (t < 0) {
-        t = 1;
+        t = NLSAR_TILE_AUTO;   // untuned: per pass, as the single-call entry points
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find(std::make_tuple(b->pp.sw, b->P, b->S, pixel_bucket(*b)));
         if (it != g_tune.end()) t = it->second;

@@ -242,7 +242,9 @@ int dkx_nlsar_batch_enqueue(dkx_nlsar_batch batch, void* stream);
 /* Captures the batch's stage sequence into a HIP graph on first use (after each _set), then
  * replays it on `stream` (not the null stream). */
 int dkx_nlsar_batch_graph_launch(dkx_nlsar_batch batch, void* stream);
-/* Pass variant: -1 = the autotuner's choice, 0/1/2 = 4/2/1 rows per thread. */
+/* Pass variant: -1 = the autotuner's choice for this batch shape, else the automatic per-pass
+ * choice of dkx_set_tile; 0/1/2 = 4/2/1 rows per thread.  _get_tile returns the variant in
+ * use, -1 for the automatic one. */
 int dkx_nlsar_batch_set_tile(dkx_nlsar_batch batch, int tile);
 /* on != 0: later enqueues compute the statistics of every (patch, scale) key, which
  * dkx_nlsar_batch_ub_pixels needs; off (the default): only the key the output uses.  The
