@@ -128,7 +128,7 @@ def test_nlsar_tile_shapes_bit_identical(D, stats):
     outs = []
     old = DV.get_tile(0)
     try:
-        for t in range(3):   # 4, 2 or 1 rows per thread
+        for t in (0, 1, 2, -1):   # 4, 2 or 1 rows per thread, automatic per pass
             DV.set_tile(0, t)
             outs.append(D.nlsar_insar(a1, a2, ph, 21, [3, 5, 7], [1, 3], stats))
     finally:
