@@ -73,7 +73,11 @@ constexpr size_t NLSAR_LDS_PER_CU = 160 * 1024;   // gfx950
 
 // Search-window offsets per barrier phase of the scale pass: two for one patch size (more
 // independent work between barriers, in LDS a single patch size leaves free), else one.
-__host__ __device__ constexpr int nlsar_offsets_per_phase(int np) { return np == 1 ? 2 : 1; }
+// Dimension-3 covariance input keeps one: its register-heavy pass ran 4 % slower with two.
+__host__ __device__ constexpr int nlsar_offsets_per_phase(int np, int mode)
+{
+    return np == 1 && mode != 3 ? 2 : 1;
+}
 
 // Window classes: the largest search-window half width each compiled layout serves.
 constexpr int NLSAR_WMAX_SMALL = 10;   // search windows up to 21
@@ -240,7 +244,7 @@ void nlsar_scale_pass(const ScaleArgs a)
     extern __shared__ float4 smem4[];
     float* smem = reinterpret_cast<float*>(smem4);
     const int wsh = a.wsh;
-    constexpr int OPB = nlsar_offsets_per_phase(NP);
+    constexpr int OPB = nlsar_offsets_per_phase(NP, MODE);
     float* sL = smem + Lay::off_L(NP, OPB);
 
     const int tid = threadIdx.x;
@@ -721,7 +725,7 @@ int nlsar_pass_launch_mask(const ScaleArgs& a, hipStream_t stream)
     constexpr int NP = pmask_count(PMASK);
     constexpr int P = pmask_top(PMASK);
     using Lay = PassLds<TX, P, WMAX>;
-    size_t lds = sizeof(float) * (size_t)Lay::total(NP, nlsar_offsets_per_phase(NP), a.lut_stride);
+    size_t lds = sizeof(float) * (size_t)Lay::total(NP, nlsar_offsets_per_phase(NP, MODE), a.lut_stride);
     if (a.wsh > WMAX || lds > NLSAR_LDS_PER_CU) {
         set_error("nlsar: search window / patch sizes exceed the LDS budget");
         return -2;
