@@ -272,6 +272,9 @@ void nlsar_scale_pass(const ScaleArgs a)
     }
 
     const float K = (float)(2 * DIM) * 0.693147182464599609375f;   // 2*dim*log(2), sim_measures.h:58
+    // C-grid element and weight-plane position of the thread's first pixel
+    const unsigned int c_own = (unsigned int)(im.corg + (ty0 + g * RPT) * im.cpitch + (tx0 + px));
+    const int w_own = g * RPT * RXM + px;
     const int apitch = im.apitch, cpitch = im.cpitch;
     const DKX_GLOBAL char* A4b = im.A4b;
     const DKX_GLOBAL char* LDb = im.LDb;
@@ -452,12 +455,24 @@ void nlsar_scale_pass(const ScaleArgs a)
         //    half, copy_symm_weights.cl:53-74) into the nonlocal statistics
         //    (compute_nl_statistics.cl:69-84) of every patch size
         {
-            const int py0 = g * RPT;
-            const int cy = ty0 + py0, cx = tx0 + px;
-            const unsigned int cu0 = (unsigned int)(im.corg + (cy + dy) * cpitch + (cx + dx));
-            const unsigned int cl0 = (unsigned int)(im.corg + (cy - dy) * cpitch + (cx - dx));
-            const float* wu0 = sW + py0 * RXM + (px - rxo);
-            const float* wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
+            unsigned int cu0, cl0;
+            const float *wu0, *wl0;
+            if constexpr (NP == 1) {
+                // the thread's own pixel plus a wave-uniform offset: no per-offset vector
+                // multiply (with several patch sizes this schedules worse, 0.7 %)
+                const int cofs = dy * cpitch + dx;
+                cu0 = c_own + (unsigned int)cofs;
+                cl0 = c_own - (unsigned int)cofs;
+                wu0 = sW + (w_own - rxo);
+                wl0 = sW + (w_own - dy * RXM - dx - rxo);
+            } else {
+                const int py0 = g * RPT;
+                const int cy = ty0 + py0, cx = tx0 + px;
+                cu0 = (unsigned int)(im.corg + (cy + dy) * cpitch + (cx + dx));
+                cl0 = (unsigned int)(im.corg + (cy - dy) * cpitch + (cx - dx));
+                wu0 = sW + py0 * RXM + (px - rxo);
+                wl0 = sW + (py0 - dy) * RXM + (px - dx - rxo);
+            }
 #pragma unroll
             for (int q = 0; q < RPT; q++) {
                 const unsigned int cu = cu0 + (unsigned int)(q * cpitch);
