@@ -33,7 +33,7 @@ PEAK_FP32_TFLOPS = 157.3
 # WRITE_SIZE on this benchmark (KiB per dispatch), scaled by what the same counters read for
 # a 256 MiB device-to-device copy (FETCH x 2.000, WRITE x 1.000);
 # profiles/nlsar_2048_scale_pass_pmc.md
-TRAFFIC_LAST_PASS_BYTES = round((461287 * 2.000 + 49344 * 1.000) * 1024)
+TRAFFIC_LAST_PASS_BYTES = round((462552 * 2.000 + 49344 * 1.000) * 1024)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),
