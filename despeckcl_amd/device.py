@@ -68,6 +68,10 @@ def goldstein(a1, a2, phase, ref, phase_filt, height, width, patch_size, overlap
 
 
 def set_tile(filter_index, tile_index):
+    """Block shape of the single-call entry points (dkx_set_tile).  filter 0 (NL-SAR): 0, 1, 2
+    = 4, 2, 1 rows per thread; -1 (default) = per pass, 1 row for one patch size, else 2.
+    filter 1 (NL-InSAR): 0 = 27 x 54, 1 = 18 x 54, 2 = 54 x 54; -1 (default) = by image size.
+    Results do not depend on the setting."""
     check(lib().dkx_set_tile(int(filter_index), int(tile_index)))
 
 
