@@ -631,9 +631,12 @@ static int params_less(int p1, int s1, int p2, int s2)
 /* covmat_ori: 2*dim*dim planes of th x tw; covmat_filt receives the interior */
 /* cond (optional, th x tw, interior written): min over the diagonal of var / I_nl^2 of the
  * selected parameter; alpha = 1 - I_nl^2 / var is ill-conditioned where it is small */
+/* maps (optional, 2*nP*nS planes of th x tw, interior written): every key's ENL_nobias, then
+ * every key's alpha, key index scale_index*nP + patch_index */
 static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int tw, int dim,
                       int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
-                      int nst, float hpar, float cpar, long long* ub_pixels, float* cond)
+                      int nst, float hpar, float cpar, long long* ub_pixels, float* cond,
+                      float* maps)
 {
     int pmax = 0, smax = 0;
     for (int i = 0; i < nP; i++) pmax = imax(pmax, P[i]);
@@ -720,6 +723,21 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
                                               keys[order[j - 1]]); j--) {
             const int t = order[j]; order[j] = order[j - 1]; order[j - 1] = t;
         }
+    /* The enls_nobias map has an entry, zero-filled, for every key of the statistics
+     * collection (nlsar_filter_sub_image.cpp:224-228), so a collection whose last key is not
+     * filtered with makes every pixel select that key (0 > -1).  params2idx[key] and
+     * best_alpha_copy's mapptr[key] then read default entries: index 0 (the first key of the
+     * scale-major loop) and the zero-filled alphas (nlsar_filter_sub_image.cpp:345-353,
+     * best_alpha_copy.cpp:26-35). */
+    int lastp = st[0].patch_size, lasts = st[0].scale_size;
+    for (int k = 1; k < nst; k++)
+        if (params_less(lastp, lasts, st[k].patch_size, st[k].scale_size)) {
+            lastp = st[k].patch_size;
+            lasts = st[k].scale_size;
+        }
+    int last_in_set = 0;
+    for (int k = 0; k < nparam; k++)
+        if (keyp[k] == lastp && keys[k] == lasts) last_in_set = 1;
     int* best = (int*)malloc(sizeof(int) * n_ori);
     long long ub = 0;
     for (size_t i = 0; i < n_ori; i++) {
@@ -729,7 +747,7 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
             if (e > -1.0f) sel = order[k]; else all_true = 0;
         }
         if (!all_true) ub++;
-        best[i] = sel >= 0 ? sel : order[nparam - 1];
+        best[i] = !last_in_set ? 0 : sel >= 0 ? sel : order[nparam - 1];
     }
     if (ub_pixels) *ub_pixels += ub;
 
@@ -737,7 +755,7 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
     float* best_alpha = (float*)malloc(sizeof(float) * n_ori);
     float* best_w = (float*)malloc(sizeof(float) * (size_t)nfull * n_ori);
     for (size_t i = 0; i < n_ori; i++) {
-        best_alpha[i] = alphas[(size_t)best[i] * n_ori + i];
+        best_alpha[i] = last_in_set ? alphas[(size_t)best[i] * n_ori + i] : 0.0f;
         for (int k = 0; k < nfull; k++)
             best_w[(size_t)k * n_ori + i] = all_weights[((size_t)best[i] * nfull + k) * n_ori + i];
     }
@@ -750,6 +768,17 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
                 const size_t i = (size_t)y * w_ori + x;
                 cond[(size_t)(y + o) * tw + x + o] = conds[(size_t)best[i] * n_ori + i];
             }
+    }
+    if (maps) {
+        const int o = wsh + psh + (smax - 1) / 2;
+        for (int k = 0; k < nparam; k++)
+            for (int y = 0; y < h_ori; y++)
+                for (int x = 0; x < w_ori; x++) {
+                    const size_t i = (size_t)y * w_ori + x;
+                    const size_t t = (size_t)(y + o) * tw + x + o;
+                    maps[(size_t)k * tplane + t] = enls_nobias[(size_t)k * n_ori + i];
+                    maps[(size_t)(nparam + k) * tplane + t] = alphas[(size_t)k * n_ori + i];
+                }
     }
 
     free(best_w); free(best_alpha); free(best); free(order);
@@ -768,7 +797,12 @@ static int check_nlsar_args(int height, int width, int dim, int sw, const int* P
     if (sw < 3 || sw % 2 == 0 || nP <= 0 || nS <= 0) return -1;
     for (int i = 0; i < nP; i++) if (P[i] < 1 || P[i] % 2 == 0) return -1;
     for (int i = 0; i < nS; i++) if (S[i] < 1 || S[i] % 2 == 0 || S[i] > 63) return -1;
-    if (nst != nP * nS) return -3;
+    /* every (patch, scale) pair needs an entry; further entries are allowed (the reference
+     * keeps them in its maps, see nlsar_tile) but no key may appear twice */
+    if (!st || nst < nP * nS) return -3;
+    for (int i = 0; i < nst; i++)
+        for (int j = 0; j < i; j++)
+            if (st[i].patch_size == st[j].patch_size && st[i].scale_size == st[j].scale_size) return -3;
     for (int i = 0; i < nP; i++)
         for (int j = 0; j < nS; j++) {
             const orc_stats* s = find_stats(st, nst, P[i], S[j]);
@@ -813,8 +847,17 @@ typedef struct {
     const orc_stats* st;
     float h, c;
     long long ub;
-    int want_cond;
+    int want_cond, want_maps;
 } nlsar_ctx;
+
+/* tile output planes: the filter's (3 for InSAR input, 2*dim*dim for covariance input), then
+ * the conditioning plane and the 2*nP*nS map planes when requested */
+static float* extra_planes(nlsar_ctx* cx, float* tout, size_t n, int nbase, float** maps)
+{
+    float* cond = cx->want_cond ? tout + (size_t)nbase * n : NULL;
+    *maps = cx->want_maps ? tout + (size_t)(nbase + (cx->want_cond ? 1 : 0)) * n : NULL;
+    return cond;
+}
 
 static int nlsar_insar_tile(const float* tin, float* tout, int th, int tw, void* vctx)
 {
@@ -822,10 +865,12 @@ static int nlsar_insar_tile(const float* tin, float* tout, int th, int tw, void*
     const size_t n = (size_t)th * tw;
     float* cov = (float*)malloc(sizeof(float) * 8 * n);
     float* filt = (float*)calloc(8 * n, sizeof(float));
+    float* maps;
+    float* cond = extra_planes(cx, tout, n, 3, &maps);
     /* data_to_covmat, conversion.cpp:22-62 */
     orc_covmat_create(tin, tin + n, tin + 2 * n, cov, th, tw);
     int rc = nlsar_tile(cov, filt, th, tw, 2, cx->sw, cx->P, cx->nP, cx->S, cx->nS, cx->st,
-                        cx->nst, cx->h, cx->c, &cx->ub, cx->want_cond ? tout + 3 * n : NULL);
+                        cx->nst, cx->h, cx->c, &cx->ub, cond, maps);
     /* covmat_to_data, conversion.cpp:88-142 */
     if (rc == 0) orc_covmat_decompose(filt, tout, tout + n, tout + 2 * n, th, tw);
     free(filt);
@@ -837,9 +882,47 @@ static int nlsar_covmat_tile(const float* tin, float* tout, int th, int tw, void
 {
     nlsar_ctx* cx = (nlsar_ctx*)vctx;
     const size_t n = (size_t)th * tw;
+    float* maps;
+    float* cond = extra_planes(cx, tout, n, 2 * cx->dim * cx->dim, &maps);
     return nlsar_tile(tin, tout, th, tw, cx->dim, cx->sw, cx->P, cx->nP, cx->S, cx->nS, cx->st,
-                      cx->nst, cx->h, cx->c, &cx->ub,
-                      cx->want_cond ? tout + (size_t)2 * cx->dim * cx->dim * n : NULL);
+                      cx->nst, cx->h, cx->c, &cx->ub, cond, maps);
+}
+
+/* the filter over the image, for InSAR input (dim 0: in = a1, a2, phi; filt = ref, phase,
+ * coh) or covariance input (dim 2 / 3: 2*dim*dim planes each) */
+static int nlsar_image(const float* in, float* filt, float* cond, float* enl_maps,
+                       float* alpha_maps, int height, int width, int dim, int sw, const int* P,
+                       int nP, const int* S, int nS, const orc_stats* st, int nst, float hpar,
+                       float cpar, int tile_h, int tile_w, long long* ub_pixels)
+{
+    int rc = check_nlsar_args(height, width, dim ? dim : 2, sw, P, nP, S, nS, st, nst);
+    if (rc) return rc;
+    const int overlap = nlsar_overlap(sw, P, nP, S, nS);
+    if (tile_h <= 0) tile_h = height + 3 * overlap;
+    if (tile_w <= 0) tile_w = width + 3 * overlap;
+    if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
+    const int nin = dim ? 2 * dim * dim : 3, nbase = nin;
+    const int want_maps = enl_maps || alpha_maps;
+    const int nparam = nP * nS;
+    const int nout = nbase + (cond ? 1 : 0) + (want_maps ? 2 * nparam : 0);
+    const size_t n = (size_t)height * width;
+    float* out = (float*)calloc((size_t)nout * n, sizeof(float));
+    memcpy(out, filt, sizeof(float) * nbase * n);
+    nlsar_ctx cx = {dim ? dim : 2, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, cond != NULL, want_maps};
+    rc = map_tiles(in, out, height, width, nin, nout, tile_h, tile_w, overlap,
+                   dim ? nlsar_covmat_tile : nlsar_insar_tile, &cx);
+    if (rc == 0) {
+        memcpy(filt, out, sizeof(float) * nbase * n);
+        const float* extra = out + (size_t)nbase * n;
+        if (cond) { memcpy(cond, extra, sizeof(float) * n); extra += n; }
+        if (want_maps) {
+            if (enl_maps) memcpy(enl_maps, extra, sizeof(float) * nparam * n);
+            if (alpha_maps) memcpy(alpha_maps, extra + (size_t)nparam * n, sizeof(float) * nparam * n);
+        }
+    }
+    if (ub_pixels) *ub_pixels = cx.ub;
+    free(out);
+    return rc;
 }
 
 int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref_filt,
@@ -847,8 +930,9 @@ int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
                     const int* P, int nP, const int* S, int nS, const orc_stats* st, int nst,
                     float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
-    return orc_nlsar_insar_cond(a1, a2, phi, ref_filt, phase_filt, coh_filt, NULL, height, width,
-                                sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
+    return orc_nlsar_insar_maps(a1, a2, phi, ref_filt, phase_filt, coh_filt, NULL, NULL, NULL,
+                                height, width, sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h,
+                                tile_w, ub_pixels);
 }
 
 int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, float* ref_filt,
@@ -857,33 +941,36 @@ int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, flo
                          int nst, float hpar, float cpar, int tile_h, int tile_w,
                          long long* ub_pixels)
 {
-    int rc = check_nlsar_args(height, width, 2, sw, P, nP, S, nS, st, nst);
-    if (rc) return rc;
-    const int overlap = nlsar_overlap(sw, P, nP, S, nS);
-    if (tile_h <= 0) tile_h = height + 3 * overlap;
-    if (tile_w <= 0) tile_w = width + 3 * overlap;
-    if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
+    return orc_nlsar_insar_maps(a1, a2, phi, ref_filt, phase_filt, coh_filt, cond, NULL, NULL,
+                                height, width, sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h,
+                                tile_w, ub_pixels);
+}
+
+int orc_nlsar_insar_maps(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                         float* phase_filt, float* coh_filt, float* cond, float* enl_maps,
+                         float* alpha_maps, int height, int width, int sw, const int* P, int nP,
+                         const int* S, int nS, const orc_stats* st, int nst, float hpar,
+                         float cpar, int tile_h, int tile_w, long long* ub_pixels)
+{
+    if (height <= 0 || width <= 0) return -1;
     const size_t n = (size_t)height * width;
     float* in = (float*)malloc(sizeof(float) * 3 * n);
-    float* out = (float*)calloc(4 * n, sizeof(float));
+    float* filt = (float*)malloc(sizeof(float) * 3 * n);
     memcpy(in, a1, n * sizeof(float));
     memcpy(in + n, a2, n * sizeof(float));
     memcpy(in + 2 * n, phi, n * sizeof(float));
-    memcpy(out, ref_filt, n * sizeof(float));
-    memcpy(out + n, phase_filt, n * sizeof(float));
-    memcpy(out + 2 * n, coh_filt, n * sizeof(float));
-    nlsar_ctx cx = {2, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, cond != NULL};
-    rc = map_tiles(in, out, height, width, 3, cond ? 4 : 3, tile_h, tile_w, overlap,
-                   nlsar_insar_tile, &cx);
+    memcpy(filt, ref_filt, n * sizeof(float));
+    memcpy(filt + n, phase_filt, n * sizeof(float));
+    memcpy(filt + 2 * n, coh_filt, n * sizeof(float));
+    int rc = nlsar_image(in, filt, cond, enl_maps, alpha_maps, height, width, 0, sw, P, nP, S, nS,
+                         st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
     if (rc == 0) {
-        memcpy(ref_filt, out, n * sizeof(float));
-        memcpy(phase_filt, out + n, n * sizeof(float));
-        memcpy(coh_filt, out + 2 * n, n * sizeof(float));
-        if (cond) memcpy(cond, out + 3 * n, n * sizeof(float));
+        memcpy(ref_filt, filt, n * sizeof(float));
+        memcpy(phase_filt, filt + n, n * sizeof(float));
+        memcpy(coh_filt, filt + 2 * n, n * sizeof(float));
     }
-    if (ub_pixels) *ub_pixels = cx.ub;
+    free(filt);
     free(in);
-    free(out);
     return rc;
 }
 
@@ -891,8 +978,8 @@ int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, in
                      int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
                      int nst, float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
-    return orc_nlsar_covmat_cond(covmat_raw, covmat_filt, NULL, height, width, dim, sw, P, nP, S,
-                                 nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
+    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, NULL, NULL, NULL, height, width, dim,
+                                 sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
 }
 
 /* as orc_nlsar_covmat; cond (optional, height x width) as in orc_nlsar_insar_cond */
@@ -901,31 +988,19 @@ int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* co
                           const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
                           int tile_w, long long* ub_pixels)
 {
-    int rc = check_nlsar_args(height, width, dim, sw, P, nP, S, nS, st, nst);
-    if (rc) return rc;
-    const int overlap = nlsar_overlap(sw, P, nP, S, nS);
-    if (tile_h <= 0) tile_h = height + 3 * overlap;
-    if (tile_w <= 0) tile_w = width + 3 * overlap;
-    if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
-    const int np = 2 * dim * dim;
-    const size_t n = (size_t)height * width;
-    float* out = covmat_filt;
-    if (cond) {
-        out = (float*)calloc((size_t)(np + 1) * n, sizeof(float));
-        memcpy(out, covmat_filt, sizeof(float) * np * n);
-    }
-    nlsar_ctx cx = {dim, sw, nP, nS, nst, P, S, st, hpar, cpar, 0, cond != NULL};
-    rc = map_tiles(covmat_raw, out, height, width, np, cond ? np + 1 : np, tile_h, tile_w,
-                   overlap, nlsar_covmat_tile, &cx);
-    if (cond) {
-        if (rc == 0) {
-            memcpy(covmat_filt, out, sizeof(float) * np * n);
-            memcpy(cond, out + (size_t)np * n, sizeof(float) * n);
-        }
-        free(out);
-    }
-    if (ub_pixels) *ub_pixels = cx.ub;
-    return rc;
+    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, cond, NULL, NULL, height, width, dim,
+                                 sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
+}
+
+int orc_nlsar_covmat_maps(const float* covmat_raw, float* covmat_filt, float* cond,
+                          float* enl_maps, float* alpha_maps, int height, int width, int dim,
+                          int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                          int nst, float hpar, float cpar, int tile_h, int tile_w,
+                          long long* ub_pixels)
+{
+    if (dim != 2 && dim != 3) return -1;
+    return nlsar_image(covmat_raw, covmat_filt, cond, enl_maps, alpha_maps, height, width, dim,
+                       sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
 }
 
 /* ====================================================================================

@@ -84,6 +84,16 @@ int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, flo
                          int nst, float hpar, float cpar, int tile_h, int tile_w,
                          long long* ub_pixels);
 
+/* same, also writing (each optional) the conditioning map and every (patch, scale) key's
+ * bias-corrected ENL and alpha (enls_nobias / alphas of nlsar_filter_sub_image.cpp:224-228,
+ * 325-327): [nP*nS][H*W] floats, key index scale_index*nP + patch_index.  The statistics
+ * must cover P x S and may hold further keys (see the selection in oracle.c). */
+int orc_nlsar_insar_maps(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                         float* phase_filt, float* coh_filt, float* cond, float* enl_maps,
+                         float* alpha_maps, int height, int width, int sw, const int* P, int nP,
+                         const int* S, int nS, const orc_stats* st, int nst, float hpar,
+                         float cpar, int tile_h, int tile_w, long long* ub_pixels);
+
 /* despeckcl::nlsar (covariance overload), nlsar.cpp:171-194; covmats are 2*dim*dim planes. */
 int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, int width, int dim,
                      int search_window_size, const int* patch_sizes, int n_patch,
@@ -93,6 +103,26 @@ int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* co
                           int width, int dim, int sw, const int* P, int nP, const int* S, int nS,
                           const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
                           int tile_w, long long* ub_pixels);
+
+int orc_nlsar_covmat_maps(const float* covmat_raw, float* covmat_filt, float* cond,
+                          float* enl_maps, float* alpha_maps, int height, int width, int dim,
+                          int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                          int nst, float hpar, float cpar, int tile_h, int tile_w,
+                          long long* ub_pixels);
+
+/* ---- brute force (brute.c): each output pixel straight from the definitions, the check of
+ * the composition above.  flip != 0 reverses every pair's argument order (tests only). ---- */
+int brute_nlsar_covmat(const float* cov, float* out, int H, int W, int dim, int sw, const int* P,
+                       int nP, const int* S, int nS, const orc_stats* st, int nst, float h,
+                       float c, float* enl_maps, float* alpha_maps, int flip);
+int brute_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                      float* coh, int H, int W, int sw, const int* P, int nP, const int* S, int nS,
+                      const orc_stats* st, int nst, float h, float c, float* enl_maps,
+                      float* alpha_maps, int flip);
+int brute_nlinsar_iteration(const float* a1, const float* a2, const float* phi,
+                            const float* est_ref, const float* est_pha, const float* est_coh,
+                            float* out_ref, float* out_pha, float* out_coh, int H, int W, int sw,
+                            int p, int lmin, int tile, long long* n_smoothed, int flip);
 
 /* despeckcl::nlsar_training, nlsar_training.cpp:35-96; out must hold n_patch*n_scale entries
  * (ordered patch-major like the reference's std::map); quantilles/chi2 storage is supplied

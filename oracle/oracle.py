@@ -111,52 +111,130 @@ def stats_build(dissims, lut_size=256):
     return _Stats(lut_size, dmin.value, dmax.value, q, c), m
 
 
+def _vp(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
 def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats,
-                h=15.0, c=49.0, tile=(0, 0), init=None, want_cond=False):
+                h=15.0, c=49.0, tile=(0, 0), init=None, want_cond=False, want_maps=False):
     """Returns (ref, phase, coh, ub_pixels), plus the conditioning map when want_cond
-    (per pixel min over the diagonal of var / I_nl^2 of the selected parameter)."""
+    (per pixel min over the diagonal of var / I_nl^2 of the selected parameter), plus
+    (enl_maps, alpha_maps) of shape (n_patch*n_scale, H, W), key index
+    scale_index*n_patch + patch_index, when want_maps."""
     a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
     H, W = a1.shape
     out = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
-    cond = np.zeros((H, W), np.float32)
+    cond = np.zeros((H, W), np.float32) if want_cond else None
+    nk = len(patch_sizes) * len(scale_sizes)
+    enl = np.zeros((nk, H, W), np.float32) if want_maps else None
+    alp = np.zeros((nk, H, W), np.float32) if want_maps else None
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
-    rc = lib().orc_nlsar_insar_cond(
-        a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
-        out[0].ctypes.data_as(C.c_void_p), out[1].ctypes.data_as(C.c_void_p),
-        out[2].ctypes.data_as(C.c_void_p), cond.ctypes.data_as(C.c_void_p) if want_cond else None,
-        C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+    rc = lib().orc_nlsar_insar_maps(
+        _vp(a1), _vp(a2), _vp(phi), _vp(out[0]), _vp(out[1]), _vp(out[2]), _vp(cond), _vp(enl),
+        _vp(alp), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
         _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)),
         arr, C.c_int(len(stats)), C.c_float(h), C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]),
         C.byref(ub))
     if rc != 0:
         raise ValueError(f"oracle nlsar_insar failed ({rc})")
+    res = [out[0], out[1], out[2], ub.value]
     if want_cond:
-        return out[0], out[1], out[2], ub.value, cond
-    return out[0], out[1], out[2], ub.value
+        res.append(cond)
+    if want_maps:
+        res += [enl, alp]
+    return tuple(res)
 
 
 def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stats,
-                 h=15.0, c=49.0, tile=(0, 0), want_cond=False):
+                 h=15.0, c=49.0, tile=(0, 0), want_cond=False, want_maps=False):
     """covmat: (2*dim*dim, H, W) interlaced planes.  Returns (filtered planes, ub_pixels),
-    plus the conditioning map (as nlsar_insar) when want_cond."""
+    plus the conditioning map (as nlsar_insar) when want_cond, plus (enl_maps, alpha_maps)
+    when want_maps."""
     cov = _f32(covmat)
     H, W = cov.shape[-2:]
     out = np.zeros_like(cov)
-    cond = np.zeros((H, W), np.float32)
+    cond = np.zeros((H, W), np.float32) if want_cond else None
+    nk = len(patch_sizes) * len(scale_sizes)
+    enl = np.zeros((nk, H, W), np.float32) if want_maps else None
+    alp = np.zeros((nk, H, W), np.float32) if want_maps else None
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
-    rc = lib().orc_nlsar_covmat_cond(
-        cov.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
-        cond.ctypes.data_as(C.c_void_p) if want_cond else None, C.c_int(H), C.c_int(W),
+    rc = lib().orc_nlsar_covmat_maps(
+        _vp(cov), _vp(out), _vp(cond), _vp(enl), _vp(alp), C.c_int(H), C.c_int(W),
         C.c_int(dim), C.c_int(search_window_size), _ints(patch_sizes), C.c_int(len(patch_sizes)),
         _ints(scale_sizes), C.c_int(len(scale_sizes)), arr, C.c_int(len(stats)), C.c_float(h),
         C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]), C.byref(ub))
     if rc != 0:
         raise ValueError(f"oracle nlsar_covmat failed ({rc})")
+    res = [out, ub.value]
     if want_cond:
-        return out, ub.value, cond
-    return out, ub.value
+        res.append(cond)
+    if want_maps:
+        res += [enl, alp]
+    return tuple(res)
+
+
+def brute_nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats,
+                      h=15.0, c=49.0, flip=False):
+    """Per-pixel brute force (brute.c).  Returns (ref, phase, coh, enl_maps, alpha_maps)."""
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    out = [np.zeros((H, W), np.float32) for _ in range(3)]
+    nk = len(patch_sizes) * len(scale_sizes)
+    enl = np.zeros((nk, H, W), np.float32)
+    alp = np.zeros((nk, H, W), np.float32)
+    arr, keep = _stats_array(stats)
+    rc = lib().brute_nlsar_insar(
+        _vp(a1), _vp(a2), _vp(phi), _vp(out[0]), _vp(out[1]), _vp(out[2]), C.c_int(H), C.c_int(W),
+        C.c_int(search_window_size), _ints(patch_sizes), C.c_int(len(patch_sizes)),
+        _ints(scale_sizes), C.c_int(len(scale_sizes)), arr, C.c_int(len(stats)), C.c_float(h),
+        C.c_float(c), _vp(enl), _vp(alp), C.c_int(1 if flip else 0))
+    if rc != 0:
+        raise ValueError(f"brute nlsar_insar failed ({rc})")
+    return out[0], out[1], out[2], enl, alp
+
+
+def brute_nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stats,
+                       h=15.0, c=49.0, flip=False):
+    """Per-pixel brute force (brute.c).  Returns (filtered planes, enl_maps, alpha_maps)."""
+    cov = _f32(covmat)
+    H, W = cov.shape[-2:]
+    out = np.zeros_like(cov)
+    nk = len(patch_sizes) * len(scale_sizes)
+    enl = np.zeros((nk, H, W), np.float32)
+    alp = np.zeros((nk, H, W), np.float32)
+    arr, keep = _stats_array(stats)
+    rc = lib().brute_nlsar_covmat(
+        _vp(cov), _vp(out), C.c_int(H), C.c_int(W), C.c_int(dim), C.c_int(search_window_size),
+        _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)),
+        arr, C.c_int(len(stats)), C.c_float(h), C.c_float(c), _vp(enl), _vp(alp),
+        C.c_int(1 if flip else 0))
+    if rc != 0:
+        raise ValueError(f"brute nlsar_covmat failed ({rc})")
+    return out, enl, alp
+
+
+def brute_nlinsar(a1, a2, phi, search_window_size, patch_size, niter, lmin, init=None, tile=80,
+                  flip=False):
+    """NL-InSAR by the per-pixel brute force (brute.c), niter iterations.  Returns
+    (ref, phase, coh, n_smoothed)."""
+    a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
+    H, W = a1.shape
+    est = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
+    nsm_total = 0
+    for _ in range(niter):
+        new = [np.zeros((H, W), np.float32) for _ in range(3)]
+        ns = C.c_longlong(0)
+        rc = lib().brute_nlinsar_iteration(
+            _vp(a1), _vp(a2), _vp(phi), _vp(est[0]), _vp(est[1]), _vp(est[2]), _vp(new[0]),
+            _vp(new[1]), _vp(new[2]), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+            C.c_int(patch_size), C.c_int(lmin), C.c_int(tile), C.byref(ns), C.c_int(1 if flip else 0))
+        if rc != 0:
+            raise ValueError(f"brute nlinsar failed ({rc})")
+        est = new
+        nsm_total += ns.value
+    return est[0], est[1], est[2], nsm_total
 
 
 def boxcar(a1, a2, phi, window_width, tile=0):
