@@ -18,9 +18,6 @@
 
 namespace dkx {
 
-int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
-                int nS, std::vector<NlsarStatsView>& views);
-
 struct NlsarBatch {
     // parameters (owned copies)
     std::vector<int> P, S;
@@ -114,147 +111,158 @@ extern "C" {
 
 int dkx_nlsar_batch_create(dkx_nlsar_batch* out)
 {
-    if (!out) { set_error("nlsar batch: null handle pointer"); return DKX_EINVAL; }
-    *out = reinterpret_cast<dkx_nlsar_batch>(new NlsarBatch());
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        if (!out) { set_error("nlsar batch: null handle pointer"); return DKX_EINVAL; }
+        *out = reinterpret_cast<dkx_nlsar_batch>(new NlsarBatch());
+        return 0;
+    });
 }
 
 void dkx_nlsar_batch_destroy(dkx_nlsar_batch h)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b) return;
-    (void)batch_wait(b);
-    (void)drop_graph(b);
-    if (b->done) (void)hipEventDestroy(b->done);
-    if (b->ws) (void)hipFree(b->ws);
-    if (b->consts) (void)hipFree(b->consts);
-    delete b;
+    dkx::guarded_void([&] {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b) return;
+        (void)batch_wait(b);
+        (void)drop_graph(b);
+        if (b->done) (void)hipEventDestroy(b->done);
+        if (b->ws) (void)hipFree(b->ws);
+        if (b->consts) (void)hipFree(b->consts);
+        delete b;
+    });
 }
 
 int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n, int sw, const int* P,
                         int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
                         float hpar, float cpar)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b || !images || n < 1) { set_error("nlsar batch: no images"); return DKX_EINVAL; }
-    if (!P || !S || nP < 1 || nS < 1) { set_error("nlsar batch: empty patch or scale size list"); return DKX_EINVAL; }
-    for (int i = 0; i < n; i++) {
-        const dkx_nlsar_image& im = images[i];
-        if (!im.ampl_master || !im.ampl_slave || !im.phase || !im.ref_filt || !im.phase_filt || !im.coh_filt) {
-            set_error("nlsar batch: null array in image " + std::to_string(i));
-            return DKX_EINVAL;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b || !images || n < 1) { set_error("nlsar batch: no images"); return DKX_EINVAL; }
+        if (!P || !S || nP < 1 || nS < 1) { set_error("nlsar batch: empty patch or scale size list"); return DKX_EINVAL; }
+        for (int i = 0; i < n; i++) {
+            const dkx_nlsar_image& im = images[i];
+            if (!im.ampl_master || !im.ampl_slave || !im.phase || !im.ref_filt || !im.phase_filt || !im.coh_filt) {
+                set_error("nlsar batch: null array in image " + std::to_string(i));
+                return DKX_EINVAL;
+            }
         }
-    }
-    std::vector<NlsarStatsView> views;
-    int rc = check_stats(stats, n_stats, P, nP, S, nS, views);
-    if (rc) return rc;
-    // everything that can fail without the device is checked before touching the batch
-    NlsarBatch nb;
-    nb.P.assign(P, P + nP);
-    nb.S.assign(S, S + nS);
-    for (const NlsarStatsView& v : views) {
-        nb.lut_q.emplace_back(v.quantilles, v.quantilles + v.lut_size);
-        nb.lut_c.emplace_back(v.chi2cdf_inv, v.chi2cdf_inv + v.lut_size);
-    }
-    for (size_t k = 0; k < views.size(); k++) {
-        NlsarStatsView v = views[k];
-        v.quantilles = nb.lut_q[k].data();
-        v.chi2cdf_inv = nb.lut_c[k].data();
-        nb.views.push_back(v);
-    }
-    nb.pp = NlsarParams{0, sw, nb.P.data(), nP, nb.S.data(), nS, nb.views.data(), (int)nb.views.size(),
-                        hpar, cpar, 1, b->diagnostics};
-    rc = nlsar_common(nb.pp, &nb.cm);
-    if (rc) return rc;
-    std::vector<NlsarImagePlan> plans(n);
-    size_t off = (sizeof(NlsarImageDesc) * (size_t)n + 255) / 256 * 256;   // the table first
-    std::vector<size_t> offs(n);
-    for (int i = 0; i < n; i++) {
-        rc = nlsar_image_plan(nb.cm, images[i].height, images[i].width, -1, -1, &plans[i]);
+        std::vector<NlsarStatsView> views;
+        int last_unused = 0;
+        int rc = check_stats(stats, n_stats, P, nP, S, nS, views, &last_unused);
         if (rc) return rc;
-        offs[i] = off;
-        off += plans[i].bytes;
-    }
-    std::vector<float> blob;
-    rc = nlsar_build_constants(nb.pp, nb.cm, blob);
-    if (rc) return rc;
-
-    // device side: wait for the batch's last run, grow the workspace, upload
-    rc = batch_wait(b);
-    if (rc) return rc;
-    rc = drop_graph(b);
-    if (rc) return rc;
-    if (b->ws_cap < off) {
-        if (b->ws) DKX_HIP_CHECK(hipFree(b->ws));
-        b->ws = nullptr;
-        b->ws_cap = 0;
-        hipError_t e = hipMalloc(&b->ws, off);
-        if (e != hipSuccess) {
-            b->ws = nullptr;
-            set_error("nlsar batch: workspace allocation of " + std::to_string(off) + " bytes failed: " +
-                      hipGetErrorString(e));
-            return DKX_ERESOURCE;
+        // everything that can fail without the device is checked before touching the batch
+        NlsarBatch nb;
+        nb.P.assign(P, P + nP);
+        nb.S.assign(S, S + nS);
+        for (const NlsarStatsView& v : views) {
+            nb.lut_q.emplace_back(v.quantilles, v.quantilles + v.lut_size);
+            nb.lut_c.emplace_back(v.chi2cdf_inv, v.chi2cdf_inv + v.lut_size);
         }
-        b->ws_cap = off;
-    }
-    if (b->consts_cap < blob.size()) {
-        if (b->consts) DKX_HIP_CHECK(hipFree(b->consts));
-        b->consts = nullptr;
-        b->consts_cap = 0;
-        DKX_HIP_CHECK(hipMalloc(&b->consts, blob.size() * sizeof(float)));
-        b->consts_cap = blob.size();
-    }
-    DKX_HIP_CHECK(hipMemcpy(b->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice));
-    nb.descs.resize(n);
-    for (int i = 0; i < n; i++) {
-        const dkx_nlsar_image& im = images[i];
-        nlsar_image_desc(plans[i], (char*)b->ws + offs[i], im.ampl_master, im.ampl_slave, im.phase, nullptr,
-                         im.ref_filt, im.phase_filt, im.coh_filt, nullptr, nullptr, nullptr, &nb.descs[i]);
-    }
-    int rows_c, rows_a, blocks, mcw, maw;
-    nlsar_batch_layout(nb.descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
-    DKX_HIP_CHECK(hipMemcpy(b->ws, nb.descs.data(), sizeof(NlsarImageDesc) * (size_t)n, hipMemcpyHostToDevice));
+        for (size_t k = 0; k < views.size(); k++) {
+            NlsarStatsView v = views[k];
+            v.quantilles = nb.lut_q[k].data();
+            v.chi2cdf_inv = nb.lut_c[k].data();
+            nb.views.push_back(v);
+        }
+        nb.pp = NlsarParams{0, sw, nb.P.data(), nP, nb.S.data(), nS, nb.views.data(), (int)nb.views.size(),
+                            hpar, cpar, 1, b->diagnostics, last_unused};
+        rc = nlsar_common(nb.pp, &nb.cm);
+        if (rc) return rc;
+        std::vector<NlsarImagePlan> plans(n);
+        size_t off = (sizeof(NlsarImageDesc) * (size_t)n + 255) / 256 * 256;   // the table first
+        std::vector<size_t> offs(n);
+        for (int i = 0; i < n; i++) {
+            rc = nlsar_image_plan(nb.cm, images[i].height, images[i].width, -1, -1, &plans[i]);
+            if (rc) return rc;
+            offs[i] = off;
+            off += plans[i].bytes;
+        }
+        std::vector<float> blob;
+        rc = nlsar_build_constants(nb.pp, nb.cm, blob);
+        if (rc) return rc;
 
-    b->P = std::move(nb.P);
-    b->S = std::move(nb.S);
-    b->lut_q = std::move(nb.lut_q);
-    b->lut_c = std::move(nb.lut_c);
-    b->views = std::move(nb.views);
-    b->pp = nb.pp;
-    b->pp.P = b->P.data();
-    b->pp.S = b->S.data();
-    b->pp.stats = b->views.data();
-    b->cm = nb.cm;
-    b->descs = std::move(nb.descs);
-    b->ws_bytes = off;
-    b->table = (NlsarImageDesc*)b->ws;
-    return choose_tile(b);
+        // device side: wait for the batch's last run, grow the workspace, upload
+        rc = batch_wait(b);
+        if (rc) return rc;
+        rc = drop_graph(b);
+        if (rc) return rc;
+        if (b->ws_cap < off) {
+            if (b->ws) DKX_HIP_CHECK(hipFree(b->ws));
+            b->ws = nullptr;
+            b->ws_cap = 0;
+            hipError_t e = hipMalloc(&b->ws, off);
+            if (e != hipSuccess) {
+                b->ws = nullptr;
+                set_error("nlsar batch: workspace allocation of " + std::to_string(off) + " bytes failed: " +
+                          hipGetErrorString(e));
+                return DKX_ERESOURCE;
+            }
+            b->ws_cap = off;
+        }
+        if (b->consts_cap < blob.size()) {
+            if (b->consts) DKX_HIP_CHECK(hipFree(b->consts));
+            b->consts = nullptr;
+            b->consts_cap = 0;
+            DKX_HIP_CHECK(hipMalloc(&b->consts, blob.size() * sizeof(float)));
+            b->consts_cap = blob.size();
+        }
+        DKX_HIP_CHECK(hipMemcpy(b->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice));
+        nb.descs.resize(n);
+        for (int i = 0; i < n; i++) {
+            const dkx_nlsar_image& im = images[i];
+            nlsar_image_desc(plans[i], (char*)b->ws + offs[i], im.ampl_master, im.ampl_slave, im.phase, nullptr,
+                             im.ref_filt, im.phase_filt, im.coh_filt, nullptr, nullptr, nullptr, &nb.descs[i]);
+        }
+        int rows_c, rows_a, blocks, mcw, maw;
+        nlsar_batch_layout(nb.descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
+        DKX_HIP_CHECK(hipMemcpy(b->ws, nb.descs.data(), sizeof(NlsarImageDesc) * (size_t)n, hipMemcpyHostToDevice));
+
+        b->P = std::move(nb.P);
+        b->S = std::move(nb.S);
+        b->lut_q = std::move(nb.lut_q);
+        b->lut_c = std::move(nb.lut_c);
+        b->views = std::move(nb.views);
+        b->pp = nb.pp;
+        b->pp.P = b->P.data();
+        b->pp.S = b->S.data();
+        b->pp.stats = b->views.data();
+        b->cm = nb.cm;
+        b->descs = std::move(nb.descs);
+        b->ws_bytes = off;
+        b->table = (NlsarImageDesc*)b->ws;
+        return choose_tile(b);
+    });
 }
 
 int dkx_nlsar_batch_set_tile(dkx_nlsar_batch h, int tile)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b || tile < -1 || tile >= NLSAR_NTILES) { set_error("nlsar batch: invalid tile"); return DKX_EINVAL; }
-    int rc = batch_wait(b);
-    if (rc) return rc;
-    rc = drop_graph(b);
-    if (rc) return rc;
-    b->tile = tile;
-    if (!b->descs.empty()) return choose_tile(b);
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b || tile < -1 || tile >= NLSAR_NTILES) { set_error("nlsar batch: invalid tile"); return DKX_EINVAL; }
+        int rc = batch_wait(b);
+        if (rc) return rc;
+        rc = drop_graph(b);
+        if (rc) return rc;
+        b->tile = tile;
+        if (!b->descs.empty()) return choose_tile(b);
+        return 0;
+    });
 }
 
 int dkx_nlsar_batch_set_diagnostics(dkx_nlsar_batch h, int on)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
-    int rc = batch_wait(b);
-    if (rc) return rc;
-    rc = drop_graph(b);
-    if (rc) return rc;
-    b->diagnostics = on ? 1 : 0;
-    b->pp.all_keys = b->diagnostics;
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
+        int rc = batch_wait(b);
+        if (rc) return rc;
+        rc = drop_graph(b);
+        if (rc) return rc;
+        b->diagnostics = on ? 1 : 0;
+        b->pp.all_keys = b->diagnostics;
+        return 0;
+    });
 }
 
 int dkx_nlsar_batch_get_tile(dkx_nlsar_batch h)
@@ -265,100 +273,108 @@ int dkx_nlsar_batch_get_tile(dkx_nlsar_batch h)
 
 int dkx_nlsar_batch_enqueue(dkx_nlsar_batch h, void* stream)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
-    int rc = batch_enqueue(b, (hipStream_t)stream);
-    if (rc) return rc;
-    return batch_mark(b, (hipStream_t)stream);
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
+        int rc = batch_enqueue(b, (hipStream_t)stream);
+        if (rc) return rc;
+        return batch_mark(b, (hipStream_t)stream);
+    });
 }
 
 int dkx_nlsar_batch_graph_launch(dkx_nlsar_batch h, void* stream_)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
-    hipStream_t stream = (hipStream_t)stream_;
-    if (!stream) { set_error("nlsar batch: graph launches need a created stream"); return DKX_EINVAL; }
-    if (!b->exec) {
-        DKX_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = batch_enqueue(b, stream);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(stream, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b) { set_error("nlsar batch: null handle"); return DKX_EINVAL; }
+        hipStream_t stream = (hipStream_t)stream_;
+        if (!stream) { set_error("nlsar batch: graph launches need a created stream"); return DKX_EINVAL; }
+        if (!b->exec) {
+            DKX_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            int rc = batch_enqueue(b, stream);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error(std::string("nlsar batch: graph capture failed: ") + hipGetErrorString(e));
+                return DKX_EHIP;
+            }
+            b->graph = g;
+            DKX_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_stream = stream;
         }
-        if (e != hipSuccess) {
-            set_error(std::string("nlsar batch: graph capture failed: ") + hipGetErrorString(e));
-            return DKX_EHIP;
-        }
-        b->graph = g;
-        DKX_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-        b->graph_stream = stream;
-    }
-    DKX_HIP_CHECK(hipGraphLaunch(b->exec, stream));
-    return batch_mark(b, stream);
+        DKX_HIP_CHECK(hipGraphLaunch(b->exec, stream));
+        return batch_mark(b, stream);
+    });
 }
 
 int dkx_nlsar_batch_ub_pixels(dkx_nlsar_batch h, unsigned long long* counts)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b || !counts) { set_error("nlsar batch: null argument"); return DKX_EINVAL; }
-    if (!b->diagnostics) {
-        set_error("nlsar batch: undefined-selection counts need the statistics of every key "
-                  "(dkx_nlsar_batch_set_diagnostics)");
-        return DKX_EINVAL;
-    }
-    int rc = batch_wait(b);
-    if (rc) return rc;
-    for (size_t i = 0; i < b->descs.size(); i++)
-        DKX_HIP_CHECK(hipMemcpy(&counts[i], b->descs[i].ub_count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b || !counts) { set_error("nlsar batch: null argument"); return DKX_EINVAL; }
+        if (!b->diagnostics) {
+            set_error("nlsar batch: undefined-selection counts need the statistics of every key "
+                      "(dkx_nlsar_batch_set_diagnostics)");
+            return DKX_EINVAL;
+        }
+        int rc = batch_wait(b);
+        if (rc) return rc;
+        for (size_t i = 0; i < b->descs.size(); i++)
+            DKX_HIP_CHECK(hipMemcpy(&counts[i], b->descs[i].ub_count, sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 int dkx_nlsar_batch_autotune(dkx_nlsar_batch h, void* stream_, int reps, int* tile_out, float* ms_out)
 {
-    NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-    if (!b || b->descs.empty()) { set_error("nlsar batch: no images set"); return DKX_EINVAL; }
-    hipStream_t stream = (hipStream_t)stream_;
-    reps = std::max(reps, 1);
-    int rc = batch_wait(b);
-    if (rc) return rc;
-    rc = drop_graph(b);
-    if (rc) return rc;
-    hipEvent_t e0, e1;
-    DKX_HIP_CHECK(hipEventCreate(&e0));
-    DKX_HIP_CHECK(hipEventCreate(&e1));
-    int best = 1;
-    float best_ms = 1e30f;
-    for (int t = 0; t < NLSAR_NTILES; t++) {
-        b->pp.tile = t;
-        b->cm.tile = t;
-        rc = batch_enqueue(b, stream);   // warm-up
-        if (rc) break;
-        DKX_HIP_CHECK(hipEventRecord(e0, stream));
-        for (int r = 0; r < reps && !rc; r++) rc = batch_enqueue(b, stream);
-        if (rc) break;
-        DKX_HIP_CHECK(hipEventRecord(e1, stream));
-        DKX_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        DKX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        if (ms_out) ms_out[t] = ms;
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
+    return dkx::guarded<int>([&]() -> int {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b || b->descs.empty()) { set_error("nlsar batch: no images set"); return DKX_EINVAL; }
+        hipStream_t stream = (hipStream_t)stream_;
+        reps = std::max(reps, 1);
+        int rc = batch_wait(b);
+        if (rc) return rc;
+        rc = drop_graph(b);
+        if (rc) return rc;
+        hipEvent_t e0, e1;
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        int best = 1;
+        float best_ms = 1e30f;
+        for (int t = 0; t < NLSAR_NTILES; t++) {
+            b->pp.tile = t;
+            b->cm.tile = t;
+            rc = batch_enqueue(b, stream);   // warm-up
+            if (rc) break;
+            DKX_HIP_CHECK(hipEventRecord(e0, stream));
+            for (int r = 0; r < reps && !rc; r++) rc = batch_enqueue(b, stream);
+            if (rc) break;
+            DKX_HIP_CHECK(hipEventRecord(e1, stream));
+            DKX_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            DKX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms /= reps;
+            if (ms_out) ms_out[t] = ms;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
         }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune[std::make_tuple(b->pp.sw, b->P, b->S, pixel_bucket(*b))] = best;
-    }
-    if (tile_out) *tile_out = best;
-    b->tile = -1;
-    return choose_tile(b);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[std::make_tuple(b->pp.sw, b->P, b->S, pixel_bucket(*b))] = best;
+        }
+        if (tile_out) *tile_out = best;
+        b->tile = -1;
+        return choose_tile(b);
+    });
 }
 
 }  // extern "C"

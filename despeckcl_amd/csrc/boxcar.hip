@@ -228,34 +228,38 @@ extern "C" {
 int dkx_boxcar_device(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                       float* coh, int height, int width, int window_width, void* stream)
 {
-    int rc = boxcar_check(height, width, window_width);
-    if (rc) return rc;
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("boxcar: null array argument"); return DKX_EINVAL; }
-    return boxcar_enqueue(a1, a2, phi, ref, pha, coh, height, width, window_width, (hipStream_t)stream);
+    return dkx::guarded<int>([&]() -> int {
+        int rc = boxcar_check(height, width, window_width);
+        if (rc) return rc;
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("boxcar: null array argument"); return DKX_EINVAL; }
+        return boxcar_enqueue(a1, a2, phi, ref, pha, coh, height, width, window_width, (hipStream_t)stream);
+    });
 }
 
 int dkx_boxcar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                float* coh, int height, int width, int window_width)
 {
-    int rc = boxcar_check(height, width, window_width);
-    if (rc) return rc;
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("boxcar: null array argument"); return DKX_EINVAL; }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        set_error("boxcar: no HIP device");
-        return DKX_EHIP;
-    }
-    const size_t n = (size_t)height * width;
-    BoxStage st;
-    DKX_HIP_CHECK(hipMalloc(&st.p, 6 * n * sizeof(float)));
-    float* d = (float*)st.p;
-    const float* src[3] = {a1, a2, phi};
-    for (int k = 0; k < 3; k++) DKX_HIP_CHECK(hipMemcpy(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice));
-    rc = boxcar_enqueue(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width, window_width, nullptr);
-    if (rc) return rc;
-    float* dst[3] = {ref, pha, coh};
-    for (int k = 0; k < 3; k++) DKX_HIP_CHECK(hipMemcpy(dst[k], d + (3 + k) * n, n * sizeof(float), hipMemcpyDeviceToHost));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        int rc = boxcar_check(height, width, window_width);
+        if (rc) return rc;
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("boxcar: null array argument"); return DKX_EINVAL; }
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+            set_error("boxcar: no HIP device");
+            return DKX_EHIP;
+        }
+        const size_t n = (size_t)height * width;
+        BoxStage st;
+        DKX_HIP_CHECK(hipMalloc(&st.p, 6 * n * sizeof(float)));
+        float* d = (float*)st.p;
+        const float* src[3] = {a1, a2, phi};
+        for (int k = 0; k < 3; k++) DKX_HIP_CHECK(hipMemcpy(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice));
+        rc = boxcar_enqueue(d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, height, width, window_width, nullptr);
+        if (rc) return rc;
+        float* dst[3] = {ref, pha, coh};
+        for (int k = 0; k < 3; k++) DKX_HIP_CHECK(hipMemcpy(dst[k], d + (3 + k) * n, n * sizeof(float), hipMemcpyDeviceToHost));
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -22,15 +22,24 @@ const char* last_error() { return g_error.c_str(); }
 // Grow-only device workspace per device, reused across calls so that repeated calls (and
 // the benchmark's timed loop) do not allocate.  Calls on one device are serialised by the
 // mutex of that device's entry.
+// Device copies of NL-SAR constant blobs (weight LUTs, Gaussian taps), keyed by content, so
+// that calls alternating between parameter sets do not re-upload them.
+struct ConstSlot {
+    float* ptr = nullptr;
+    size_t cap = 0;
+    std::vector<float> host;        // content, and the source of the last upload
+    hipEvent_t uploaded = nullptr;  // completion of the last upload from `host`
+    unsigned long long used = 0;    // call counter at the last use (LRU)
+};
+constexpr int CONST_SLOTS = 4;
+
 struct DeviceWorkspace {
     std::mutex mu;
     void* ptr = nullptr;
     size_t bytes = 0;
     hipStream_t stream = nullptr;       // the library's own stream (host-array entry points)
-    // device constants (weight LUTs, Gaussian taps) and the host image of their content
-    float* consts = nullptr;
-    size_t consts_cap = 0;
-    std::vector<float> consts_host;
+    ConstSlot consts[CONST_SLOTS];
+    unsigned long long calls = 0;
     // completion of the last call that used this workspace, and its stream: a call on
     // another stream waits for it, so calls sharing the workspace never overlap on the GPU
     hipEvent_t done = nullptr;
@@ -54,21 +63,47 @@ static int workspace_release(DeviceWorkspace* w, hipStream_t stream)
     return 0;
 }
 
-// Upload the constants only when they differ from what the device holds; a change waits
-// for all work in flight (which may still read the previous constants).
-static int set_constants(DeviceWorkspace* w, const std::vector<float>& blob)
+// Device copy of a constants blob for a call on `stream`, which must already wait for the
+// workspace's previous user (workspace_acquire).  Every earlier call on this device is
+// ordered before that point, so a slot holding other content may be overwritten by an upload
+// enqueued on `stream`: no kernel still reads it, and nothing waits on the host for the device.
+static int set_constants(DeviceWorkspace* w, const std::vector<float>& blob, hipStream_t stream,
+                         const float** dev)
 {
-    if (w->consts && w->consts_host == blob) return 0;
-    DKX_HIP_CHECK(hipDeviceSynchronize());
-    if (w->consts_cap < blob.size()) {
-        if (w->consts) DKX_HIP_CHECK(hipFree(w->consts));
-        w->consts = nullptr;
-        w->consts_cap = 0;
-        DKX_HIP_CHECK(hipMalloc(&w->consts, std::max<size_t>(blob.size(), 1) * sizeof(float)));
-        w->consts_cap = blob.size();
+    w->calls++;
+    ConstSlot* pick = nullptr;
+    for (ConstSlot& c : w->consts)
+        if (c.ptr && c.host == blob) pick = &c;
+    if (pick) {
+        pick->used = w->calls;
+        *dev = pick->ptr;
+        return 0;
     }
-    DKX_HIP_CHECK(hipMemcpy(w->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice));
-    w->consts_host = blob;
+    pick = &w->consts[0];
+    for (ConstSlot& c : w->consts)
+        if (c.used < pick->used) pick = &c;
+    // the slot's host copy may still be the source of an upload in flight
+    if (pick->uploaded) DKX_HIP_CHECK(hipEventSynchronize(pick->uploaded));
+    else DKX_HIP_CHECK(hipEventCreateWithFlags(&pick->uploaded, hipEventDisableTiming));
+    const size_t need = std::max<size_t>(blob.size(), 1);
+    if (pick->cap < need) {
+        // an allocation change: the previous buffer may still be read by the last call
+        if (pick->ptr) {
+            if (w->have_done) DKX_HIP_CHECK(hipEventSynchronize(w->done));
+            DKX_HIP_CHECK(hipFree(pick->ptr));
+        }
+        pick->ptr = nullptr;
+        pick->cap = 0;
+        pick->host.clear();
+        DKX_HIP_CHECK(hipMalloc(&pick->ptr, need * sizeof(float)));
+        pick->cap = need;
+    }
+    pick->host = blob;
+    pick->used = w->calls;
+    DKX_HIP_CHECK(hipMemcpyAsync(pick->ptr, pick->host.data(), blob.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, stream));
+    DKX_HIP_CHECK(hipEventRecord(pick->uploaded, stream));
+    *dev = pick->ptr;
     return 0;
 }
 
@@ -91,7 +126,8 @@ static int ensure(DeviceWorkspace* w, size_t bytes)
 {
     if (w->bytes >= bytes) return 0;
     if (w->ptr) {
-        DKX_HIP_CHECK(hipDeviceSynchronize());
+        // every call that used the old buffer is ordered before the last one's event
+        if (w->have_done) DKX_HIP_CHECK(hipEventSynchronize(w->done));
         DKX_HIP_CHECK(hipFree(w->ptr));
         w->ptr = nullptr;
         w->bytes = 0;
@@ -146,42 +182,59 @@ static void timing_collect()
 }
 
 int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
-                int nS, std::vector<NlsarStatsView>& views)
+                int nS, std::vector<NlsarStatsView>& views, int* last_unused)
 {
-    // The reference looks statistics up with operator[] and builds ENL maps for every key
-    // (nlsar_filter_sub_image.cpp:224-228, 278): keys outside the patch x scale set, or
-    // missing keys, silently use empty statistics there.  Here they are rejected.
-    if (!stats || n_stats != nP * nS) {
-        set_error("nlsar: the statistics must hold one entry per (patch size, scale size) pair");
+    // The reference looks statistics up with operator[] (nlsar_filter_sub_image.cpp:278):
+    // a missing (patch, scale) entry would silently filter with empty statistics, so it is
+    // rejected here, as is a key given twice (a std::map holds each key once).  Entries for
+    // other keys are accepted: the reference builds ENL maps for them too (:224-228), which
+    // matters for the selection only when the collection's last key is one of them.
+    if (!stats || n_stats < nP * nS || n_stats > 1 << 20) {
+        set_error("nlsar: the statistics must hold an entry for every (patch size, scale size) pair");
         return DKX_ESTATS;
     }
     views.clear();
+    int lastp = 0, lasts = 0;
     for (int i = 0; i < n_stats; i++) {
         const dkx_nlsar_stats& s = stats[i];
-        bool known = false;
-        for (int a = 0; a < nP; a++)
-            for (int b = 0; b < nS; b++)
-                if (P[a] == s.patch_size && S[b] == s.scale_size) known = true;
         for (int j = 0; j < i; j++)
-            if (stats[j].patch_size == s.patch_size && stats[j].scale_size == s.scale_size) known = false;
-        if (!known) {
-            set_error("nlsar: statistics key (" + std::to_string(s.patch_size) + ", " +
-                      std::to_string(s.scale_size) + ") is not in the patch x scale set");
-            return DKX_ESTATS;
-        }
-        if (s.lut_size < 2 || s.lut_size > 65536 || !s.quantilles || !s.chi2cdf_inv ||
-            !(s.dissims_max >= s.dissims_min)) {
-            set_error("nlsar: invalid statistics entry");
-            return DKX_ESTATS;
-        }
-        for (int k = 0; k < s.lut_size; k++)
-            if (!(s.quantilles[k] >= 0.0f && s.quantilles[k] < 1.0f)) {
-                set_error("nlsar: quantile LUT values must lie in [0, 1)");
+            if (stats[j].patch_size == s.patch_size && stats[j].scale_size == s.scale_size) {
+                set_error("nlsar: statistics key (" + std::to_string(s.patch_size) + ", " +
+                          std::to_string(s.scale_size) + ") appears twice");
                 return DKX_ESTATS;
             }
-        views.push_back(NlsarStatsView{s.patch_size, s.scale_size, s.lut_size, s.dissims_min,
-                                       s.dissims_max, s.quantilles, s.chi2cdf_inv});
+        // nlsar::params::operator<, parameters.h:37-40
+        if (i == 0 || s.patch_size > lastp || (s.patch_size == lastp && s.scale_size > lasts)) {
+            lastp = s.patch_size;
+            lasts = s.scale_size;
+        }
     }
+    bool last_in = false;
+    for (int a = 0; a < nP; a++)
+        for (int b = 0; b < nS; b++) {
+            if (P[a] == lastp && S[b] == lasts) last_in = true;
+            const dkx_nlsar_stats* e = nullptr;
+            for (int i = 0; i < n_stats; i++)
+                if (stats[i].patch_size == P[a] && stats[i].scale_size == S[b]) e = &stats[i];
+            if (!e) {
+                set_error("nlsar: no statistics for (patch " + std::to_string(P[a]) + ", scale " +
+                          std::to_string(S[b]) + ")");
+                return DKX_ESTATS;
+            }
+            if (e->lut_size < 2 || e->lut_size > 65536 || !e->quantilles || !e->chi2cdf_inv ||
+                !(e->dissims_max >= e->dissims_min)) {
+                set_error("nlsar: invalid statistics entry");
+                return DKX_ESTATS;
+            }
+            for (int k = 0; k < e->lut_size; k++)
+                if (!(e->quantilles[k] >= 0.0f && e->quantilles[k] < 1.0f)) {
+                    set_error("nlsar: quantile LUT values must lie in [0, 1)");
+                    return DKX_ESTATS;
+                }
+            views.push_back(NlsarStatsView{e->patch_size, e->scale_size, e->lut_size, e->dissims_min,
+                                           e->dissims_max, e->quantilles, e->chi2cdf_inv});
+        }
+    if (last_unused) *last_unused = last_in ? 0 : 1;
     return 0;
 }
 
@@ -201,10 +254,11 @@ static int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int
 {
     if (!P || !S) { set_error("nlsar: null size list"); return DKX_EINVAL; }
     if (nP < 1 || nS < 1) { set_error("nlsar: empty patch or scale size list"); return DKX_EINVAL; }
-    int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views);
+    int last_unused = 0;
+    int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views, &last_unused);
     if (rc) return rc;
     call.pp = NlsarParams{dim, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
-                          g_tile[0].load(), all_keys};
+                          g_tile[0].load(), all_keys, last_unused};
     rc = nlsar_common(call.pp, &call.cm);
     if (rc) return rc;
     return nlsar_image_plan(call.cm, height, width, -1, -1, &call.ip);
@@ -225,13 +279,14 @@ static int nlsar_device_impl(NlsarCall& call, const float* a1, const float* a2, 
     if (rc) return rc;
     rc = ensure(w, call.ip.bytes);
     if (rc) return rc;
-    rc = set_constants(w, blob);
-    if (rc) return rc;
     rc = workspace_acquire(w, stream);
+    if (rc) return rc;
+    const float* d_const = nullptr;
+    rc = set_constants(w, blob, stream, &d_const);
     if (rc) return rc;
     NlsarImageDesc d;
     nlsar_image_desc(call.ip, (char*)w->ptr, a1, a2, phi, cov, ref, pha, coh, cov_out, enl_maps, alpha_maps, &d);
-    rc = nlsar_enqueue(call.pp, call.cm, &d, 1, nullptr, w->consts, stream);
+    rc = nlsar_enqueue(call.pp, call.cm, &d, 1, nullptr, d_const, stream);
     if (rc) return rc;
     if (ub) DKX_HIP_CHECK(hipMemcpyAsync(ub, d.ub_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     rc = workspace_release(w, stream);
@@ -318,12 +373,14 @@ int dkx_version(void) { return 1; }
 
 int dkx_set_tile(int filter, int tile_index)
 {
-    if (filter < 0 || filter > 1 || tile_index < -1 || tile_index > 2) {
-        set_error("dkx_set_tile: invalid filter or tile index");
-        return DKX_EINVAL;
-    }
-    g_tile[filter].store(tile_index);
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        if (filter < 0 || filter > 1 || tile_index < -1 || tile_index > 2) {
+            set_error("dkx_set_tile: invalid filter or tile index");
+            return DKX_EINVAL;
+        }
+        g_tile[filter].store(tile_index);
+        return 0;
+    });
 }
 
 int dkx_get_tile(int filter) { return (filter >= 0 && filter <= 1) ? g_tile[filter].load() : -1; }
@@ -336,14 +393,16 @@ int dkx_set_timing(int enabled)
 
 int dkx_stage_times(int* kinds, float* ms, int max_entries)
 {
-    timing_collect();
-    std::lock_guard<std::mutex> lk(g_timing_mu);
-    const int n = (int)g_times.size();
-    for (int i = 0; i < n && i < max_entries; i++) {
-        if (kinds) kinds[i] = g_times[i].first;
-        if (ms) ms[i] = g_times[i].second;
-    }
-    return n;
+    return dkx::guarded<int>([&]() -> int {
+        timing_collect();
+        std::lock_guard<std::mutex> lk(g_timing_mu);
+        const int n = (int)g_times.size();
+        for (int i = 0; i < n && i < max_entries; i++) {
+            if (kinds) kinds[i] = g_times[i].first;
+            if (ms) ms[i] = g_times[i].second;
+        }
+        return n;
+    });
 }
 
 int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, float* ref,
@@ -352,46 +411,50 @@ int dkx_nlsar_insar_device(const float* a1, const float* a2, const float* phi, f
                            float h, float c, float* enl_maps, float* alpha_maps,
                            unsigned long long* ub, void* stream)
 {
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
-    NlsarCall call;
-    const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
-    int rc = nlsar_prepare(call, 0, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    return nlsar_device_impl(call, a1, a2, phi, nullptr, ref, pha, coh, nullptr, enl_maps,
-                             alpha_maps, ub, (hipStream_t)stream, w);
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+        NlsarCall call;
+        const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
+        int rc = nlsar_prepare(call, 0, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        return nlsar_device_impl(call, a1, a2, phi, nullptr, ref, pha, coh, nullptr, enl_maps,
+                                 alpha_maps, ub, (hipStream_t)stream, w);
+    });
 }
 
 int dkx_nlsar_insar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                     float* coh, int height, int width, int sw, const int* P, int nP, const int* S,
                     int nS, const dkx_nlsar_stats* stats, int n_stats, float h, float c)
 {
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
-    NlsarCall call;
-    int rc = nlsar_prepare(call, 0, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    rc = ensure_stream(w);
-    if (rc) return rc;
-    const size_t n = (size_t)height * width;
-    HostStage st;
-    DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
-    float* d = (float*)st.ptr;
-    DKX_HIP_CHECK(hipMemcpyAsync(d, a1, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(d + n, a2, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(d + 2 * n, phi, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlsar_device_impl(call, d, d + n, d + 2 * n, nullptr, d + 3 * n, d + 4 * n, d + 5 * n,
-                           nullptr, nullptr, nullptr, nullptr, w->stream, w);
-    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
-    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+        NlsarCall call;
+        int rc = nlsar_prepare(call, 0, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        rc = ensure_stream(w);
+        if (rc) return rc;
+        const size_t n = (size_t)height * width;
+        HostStage st;
+        DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
+        float* d = (float*)st.ptr;
+        DKX_HIP_CHECK(hipMemcpyAsync(d, a1, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(d + n, a2, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(d + 2 * n, phi, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        rc = nlsar_device_impl(call, d, d + n, d + 2 * n, nullptr, d + 3 * n, d + 4 * n, d + 5 * n,
+                               nullptr, nullptr, nullptr, nullptr, w->stream, w);
+        if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+        DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+        return 0;
+    });
 }
 
 int dkx_nlsar_covmat_device(const float* cov, float* cov_out, int height, int width, int dim,
@@ -400,151 +463,165 @@ int dkx_nlsar_covmat_device(const float* cov, float* cov_out, int height, int wi
                             float* enl_maps, float* alpha_maps, unsigned long long* ub,
                             void* stream)
 {
-    if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
-    if (dim != 2 && dim != 3) {
-        set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
-        return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
-    }
-    NlsarCall call;
-    const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
-    int rc = nlsar_prepare(call, dim, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    return nlsar_device_impl(call, nullptr, nullptr, nullptr, cov, nullptr, nullptr, nullptr,
-                             cov_out, enl_maps, alpha_maps, ub, (hipStream_t)stream, w);
+    return dkx::guarded<int>([&]() -> int {
+        if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+        if (dim != 2 && dim != 3) {
+            set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
+            return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
+        }
+        NlsarCall call;
+        const int diag = (enl_maps || alpha_maps || ub) ? 1 : 0;
+        int rc = nlsar_prepare(call, dim, diag, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        return nlsar_device_impl(call, nullptr, nullptr, nullptr, cov, nullptr, nullptr, nullptr,
+                                 cov_out, enl_maps, alpha_maps, ub, (hipStream_t)stream, w);
+    });
 }
 
 int dkx_nlsar_covmat(const float* cov, float* cov_out, int height, int width, int dim, int sw,
                      const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
                      int n_stats, float h, float c)
 {
-    if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
-    if (dim != 2 && dim != 3) {
-        set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
-        return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
-    }
-    NlsarCall call;
-    int rc = nlsar_prepare(call, dim, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    rc = ensure_stream(w);
-    if (rc) return rc;
-    const size_t n = (size_t)2 * dim * dim * height * width;
-    HostStage st;
-    DKX_HIP_CHECK(hipMalloc(&st.ptr, 2 * n * sizeof(float)));
-    float* d = (float*)st.ptr;
-    DKX_HIP_CHECK(hipMemcpyAsync(d, cov, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlsar_device_impl(call, nullptr, nullptr, nullptr, d, nullptr, nullptr, nullptr, d + n,
-                           nullptr, nullptr, nullptr, w->stream, w);
-    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
-    DKX_HIP_CHECK(hipMemcpyAsync(cov_out, d + n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        if (!cov || !cov_out) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+        if (dim != 2 && dim != 3) {
+            set_error("nlsar: covariance matrices of dimension 2 or 3 are supported");
+            return dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
+        }
+        NlsarCall call;
+        int rc = nlsar_prepare(call, dim, 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        rc = ensure_stream(w);
+        if (rc) return rc;
+        const size_t n = (size_t)2 * dim * dim * height * width;
+        HostStage st;
+        DKX_HIP_CHECK(hipMalloc(&st.ptr, 2 * n * sizeof(float)));
+        float* d = (float*)st.ptr;
+        DKX_HIP_CHECK(hipMemcpyAsync(d, cov, n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        rc = nlsar_device_impl(call, nullptr, nullptr, nullptr, d, nullptr, nullptr, nullptr, d + n,
+                               nullptr, nullptr, nullptr, w->stream, w);
+        if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+        DKX_HIP_CHECK(hipMemcpyAsync(cov_out, d + n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+        return 0;
+    });
 }
 
 int dkx_nlinsar_device(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                        float* coh, int height, int width, int sw, int p, int niter, int lmin,
                        unsigned long long* nsm, void* stream)
 {
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
-    NlinsarCall call;
-    int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    return nlinsar_device_impl(call, a1, a2, phi, ref, pha, coh, nsm, (hipStream_t)stream, w);
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+        NlinsarCall call;
+        int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        return nlinsar_device_impl(call, a1, a2, phi, ref, pha, coh, nsm, (hipStream_t)stream, w);
+    });
 }
 
 int dkx_nlinsar(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                 float* coh, int height, int width, int sw, int p, int niter, int lmin)
 {
-    if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
-    NlinsarCall call;
-    int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    rc = ensure_stream(w);
-    if (rc) return rc;
-    const size_t n = (size_t)height * width;
-    HostStage st;
-    DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
-    float* d = (float*)st.ptr;
-    const float* src[6] = {a1, a2, phi, ref, pha, coh};
-    for (int k = 0; k < 6; k++)
-        DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = nlinsar_device_impl(call, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, nullptr,
-                             w->stream, w);
-    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
-    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+        NlinsarCall call;
+        int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        rc = ensure_stream(w);
+        if (rc) return rc;
+        const size_t n = (size_t)height * width;
+        HostStage st;
+        DKX_HIP_CHECK(hipMalloc(&st.ptr, 6 * n * sizeof(float)));
+        float* d = (float*)st.ptr;
+        const float* src[6] = {a1, a2, phi, ref, pha, coh};
+        for (int k = 0; k < 6; k++)
+            DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        rc = nlinsar_device_impl(call, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, d + 5 * n, nullptr,
+                                 w->stream, w);
+        if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+        DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(coh, d + 5 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+        return 0;
+    });
 }
 
 int dkx_goldstein_device(const float* a1, const float* a2, const float* phi, float* ref,
                          float* pha, float* coh, int height, int width, int patch_size,
                          int overlap, float alpha, void* stream)
 {
-    (void)coh;   // not written, as in the reference
-    if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
-    GoldsteinParams gp;
-    GoldsteinPlan pl;
-    int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    return goldstein_device_impl(gp, pl, a1, a2, phi, ref, pha, (hipStream_t)stream, w);
+    return dkx::guarded<int>([&]() -> int {
+        (void)coh;   // not written, as in the reference
+        if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
+        GoldsteinParams gp;
+        GoldsteinPlan pl;
+        int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        return goldstein_device_impl(gp, pl, a1, a2, phi, ref, pha, (hipStream_t)stream, w);
+    });
 }
 
 int dkx_goldstein(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
                   float* coh, int height, int width, int patch_size, int overlap, float alpha)
 {
-    (void)coh;   // not written, as in the reference
-    if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
-    GoldsteinParams gp;
-    GoldsteinPlan pl;
-    int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
-    if (rc) return rc;
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) { set_error("no HIP device"); return DKX_EHIP; }
-    std::lock_guard<std::mutex> lk(w->mu);
-    rc = ensure_stream(w);
-    if (rc) return rc;
-    const size_t n = (size_t)height * width;
-    HostStage st;
-    DKX_HIP_CHECK(hipMalloc(&st.ptr, 5 * n * sizeof(float)));
-    float* d = (float*)st.ptr;
-    const float* src[3] = {a1, a2, phi};
-    for (int k = 0; k < 3; k++)
-        DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
-    rc = goldstein_device_impl(gp, pl, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, w->stream, w);
-    if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
-    DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
-    DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        (void)coh;   // not written, as in the reference
+        if (!a1 || !a2 || !phi || !ref || !pha) { set_error("goldstein: null array argument"); return DKX_EINVAL; }
+        GoldsteinParams gp;
+        GoldsteinPlan pl;
+        int rc = goldstein_prepare(gp, pl, height, width, patch_size, overlap, alpha);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        std::lock_guard<std::mutex> lk(w->mu);
+        rc = ensure_stream(w);
+        if (rc) return rc;
+        const size_t n = (size_t)height * width;
+        HostStage st;
+        DKX_HIP_CHECK(hipMalloc(&st.ptr, 5 * n * sizeof(float)));
+        float* d = (float*)st.ptr;
+        const float* src[3] = {a1, a2, phi};
+        for (int k = 0; k < 3; k++)
+            DKX_HIP_CHECK(hipMemcpyAsync(d + k * n, src[k], n * sizeof(float), hipMemcpyHostToDevice, w->stream));
+        rc = goldstein_device_impl(gp, pl, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, w->stream, w);
+        if (rc) { (void)hipStreamSynchronize(w->stream); return rc; }
+        DKX_HIP_CHECK(hipMemcpyAsync(ref, d + 3 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipMemcpyAsync(pha, d + 4 * n, n * sizeof(float), hipMemcpyDeviceToHost, w->stream));
+        DKX_HIP_CHECK(hipStreamSynchronize(w->stream));
+        return 0;
+    });
 }
 
 void dkx_release_workspace(void)
 {
-    DeviceWorkspace* w = workspace_for_current_device();
-    if (!w) return;
-    std::lock_guard<std::mutex> lk(w->mu);
-    if (w->ptr) {
-        (void)hipDeviceSynchronize();
-        (void)hipFree(w->ptr);
-    }
-    w->ptr = nullptr;
-    w->bytes = 0;
+    dkx::guarded_void([&] {
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) return;
+        std::lock_guard<std::mutex> lk(w->mu);
+        if (w->ptr) {
+            if (w->have_done) (void)hipEventSynchronize(w->done);
+            (void)hipFree(w->ptr);
+        }
+        w->ptr = nullptr;
+        w->bytes = 0;
+    });
 }
 
 }  // extern "C"

@@ -5,6 +5,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 
 namespace dkx {
@@ -69,6 +71,35 @@ struct Launch {
 
 void set_error(const std::string& msg);
 const char* last_error();
+
+// C++ exceptions (host allocation failures, length errors of a malformed file) must not cross
+// the C ABI: every entry point runs its body through guarded(), which maps them to
+// DKX_ERESOURCE (-2) / DKX_EINVAL (-1) with a message.
+template <class R, class F>
+R guarded(F&& f) noexcept
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return (R)-2;
+    } catch (const std::exception& e) {
+        set_error(std::string("internal error: ") + e.what());
+        return (R)-1;
+    } catch (...) {
+        set_error("internal error");
+        return (R)-1;
+    }
+}
+template <class F>
+void guarded_void(F&& f) noexcept
+{
+    try {
+        f();
+    } catch (...) {
+        set_error("internal error");
+    }
+}
 
 // Stage timing (off by default): when enabled, the main launch of each fused stage is
 // bracketed by HIP events on its stream; dkx_stage_times() reports the durations.

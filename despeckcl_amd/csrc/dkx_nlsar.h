@@ -4,6 +4,7 @@
 #pragma once
 
 #include "dkx_common.h"
+#include "../../include/despeckcl_amd.h"
 
 #include <vector>
 
@@ -47,6 +48,11 @@ struct NlsarParams {
     // selection returns whenever it is defined (best_params.cpp:27-53), so the filtered
     // output is the same either way
     int all_keys;
+    // 1: the statistics collection holds keys beyond patch x scale and its last key is one
+    // of them.  The reference then selects that unused key everywhere and reads defaults for
+    // it: the weights of parameter index 0 (first scale, first patch) and alpha 0
+    // (nlsar_filter_sub_image.cpp:224-228, 345-353; best_alpha_copy.cpp:26-35).
+    int last_unused;
 };
 
 // Validated parameters and what follows from them.
@@ -83,6 +89,13 @@ struct NlsarImageDesc {
     int gx[4], gy[4], bstart[5];         // pass blocks of each mapping, [4] = image total
     int row0_c, row0_a, block0;          // first row / pass block of this image in a batch
 };
+
+// Validates a statistics collection for the patch x scale set: every pair needs exactly
+// one entry; entries for other keys are allowed, as in the reference, which keeps them in its
+// maps.  views receives the entries of the set; *last_unused (see NlsarParams) whether the
+// collection's last key lies outside it.
+int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
+                int nS, std::vector<NlsarStatsView>& views, int* last_unused);
 
 std::vector<float> gen_gauss(int scale_size);
 std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c);

@@ -41,6 +41,7 @@ struct ScaleArgs {
     int lut_size[NLSAR_MAX_NP];
     int lut_stride;                      // LDS floats per LUT
     int first_scale;                     // 1: first pass (ubflag not yet written)
+    int alpha_zero;                      // last pass: blend with alpha 0 (NlsarParams::last_unused)
 };
 
 // LDS layout (floats) of a pass over TX x TX output tiles, shared by the kernel and its
@@ -690,7 +691,7 @@ void nlsar_scale_pass(const ScaleArgs a)
             // selection loop is undefined are counted and also take the last key.
             if (fail || (!a.first_scale && o_flag[opix])) ubl++;
             // weighted_means.cl:62-80
-            const float al = alpha_mean;
+            const float al = a.alpha_zero ? 0.0f : alpha_mean;
             if constexpr (MODE == 0) {
                 // and covmat_decompose.cl:19-39
                 const float m00 = (1 - al) * dsum[0] / s_mean + al * ic[0];

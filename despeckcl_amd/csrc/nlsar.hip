@@ -416,7 +416,15 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
     std::stable_partition(passes.begin(), passes.end(), [](const Pass& q) { return !q.last; });
     // without diagnostics only the selected key's statistics are needed: the last key, on its
     // own (the other keys of its pass only feed the maps and the undefined-selection count)
-    if (!pp.all_keys) passes.assign(1, Pass{smax_i, std::vector<int>{pmax_i}, true});
+    if (pp.last_unused) {
+        // the selection lands on a key outside patch x scale: the mean uses parameter index 0
+        // (first scale, first patch) with alpha 0, after every key's statistics if wanted
+        if (!pp.all_keys) passes.clear();
+        for (Pass& q : passes) q.last = false;
+        passes.push_back(Pass{0, std::vector<int>{0}, true});
+    } else if (!pp.all_keys) {
+        passes.assign(1, Pass{smax_i, std::vector<int>{pmax_i}, true});
+    }
 
     int cur_scale = -1;
     for (size_t oi = 0; oi < passes.size(); oi++) {
@@ -460,7 +468,8 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
             sa.scl[k] = (float)(pst[idx]->lut_size - 1);
             sa.lut_size[k] = pst[idx]->lut_size;
         }
-        if (ps.last && pis.back() != pmax_i) {
+        sa.alpha_zero = ps.last && pp.last_unused ? 1 : 0;
+        if (ps.last && !pp.last_unused && pis.back() != pmax_i) {
             set_error("nlsar: internal error, the last pass must end with the largest patch");
             return -1;
         }

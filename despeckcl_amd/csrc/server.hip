@@ -27,9 +27,6 @@
 
 namespace dkx {
 
-int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
-                int nS, std::vector<NlsarStatsView>& views);
-
 static double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -94,7 +91,6 @@ struct Flight {
     Server* srv = nullptr;
     int kind = 0, lane = 0;
     std::vector<Request*> reqs;
-    size_t table_off = 0, table_bytes = 0;
     std::atomic<bool> done{false};
     std::atomic<double> t_done{0.0};
 };
@@ -119,7 +115,8 @@ struct Server {
     void* arena_ptr = nullptr;
     hipStream_t sar_stream[SAR_LANES] = {};
     hipStream_t nli_stream[NLI_LANES] = {};
-    NlsarImageDesc* pinned_table[SAR_LANES] = {};
+    NlsarImageDesc* pinned_table[SAR_LANES] = {};   // staging of a lane's batch table
+    NlsarImageDesc* dev_table[SAR_LANES] = {};      // the table itself (MAX_BATCH entries)
     Flight* sar_busy[SAR_LANES] = {};
     Flight* nli_busy[NLI_LANES] = {};
     // queues
@@ -137,12 +134,15 @@ struct Server {
     long long batches = 0, batched_images = 0;
 };
 
+// Completion callback on a flight's stream.  Once `done` is set the scheduler may retire and
+// delete the flight, so nothing of it is touched afterwards (the server outlives its streams).
 static void host_done(void* p)
 {
     Flight* f = (Flight*)p;
+    Server* s = f->srv;
     f->t_done.store(now_s());
     f->done.store(true);
-    f->srv->cv.notify_all();
+    s->cv.notify_all();
 }
 
 static int nlinsar_params(int p, float* h, float* T)
@@ -176,7 +176,6 @@ static void retire(Server* s)
             s->arena.release(r->off, r->bytes);
             s->n_done.fetch_add(1);
         }
-        if (f->table_bytes) s->arena.release(f->table_off, f->table_bytes);
         delete f;
         f = nullptr;
     };
@@ -219,27 +218,23 @@ static void launch_sar(Server* s, int lane)
     const int n = (int)descs.size();
     int rows_c, rows_a, blocks, mcw, maw;
     nlsar_batch_layout(descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
-    f->table_bytes = sizeof(NlsarImageDesc) * (size_t)n;
-    bool ok = s->arena.alloc(f->table_bytes, &f->table_off);
+    // the lane's table was reserved at creation; its previous batch has finished (the lane
+    // was free) and the stream orders the copy after it
+    const size_t table_bytes = sizeof(NlsarImageDesc) * (size_t)n;
     hipStream_t st = s->sar_stream[lane];
-    int rc = ok ? 0 : -2;
-    if (ok) {
-        memcpy(s->pinned_table[lane], descs.data(), f->table_bytes);
-        NlsarImageDesc* d_table = (NlsarImageDesc*)(s->arena.base + f->table_off);
-        if (hipMemcpyAsync(d_table, s->pinned_table[lane], f->table_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = -5;
-        if (!rc) rc = nlsar_enqueue(s->pp, s->cm, descs.data(), n, d_table, s->consts, st);
-        if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
-    } else {
-        f->table_bytes = 0;
-    }
+    int rc = 0;
+    memcpy(s->pinned_table[lane], descs.data(), table_bytes);
+    NlsarImageDesc* d_table = s->dev_table[lane];
+    if (hipMemcpyAsync(d_table, s->pinned_table[lane], table_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = -5;
+    if (!rc) rc = nlsar_enqueue(s->pp, s->cm, descs.data(), n, d_table, s->consts, st);
+    if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
     if (rc) {
         // nothing reached the GPU that still needs the memory once the stream drains
         (void)hipStreamSynchronize(st);
         for (Request* r : f->reqs) {
             s->arena.release(r->off, r->bytes);
-            fail_request(s, r, ok ? last_error() : "serve: batch table does not fit");
+            fail_request(s, r, last_error());
         }
-        if (f->table_bytes) s->arena.release(f->table_off, f->table_bytes);
         delete f;
         return;
     }
@@ -348,199 +343,219 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
                       int n_stats, float h_param, float c_param, int nl_search_window_size, int nl_patch_size,
                       int nl_niter, int nl_lmin, dkx_server* out)
 {
-    if (!out) { set_error("serve: null handle pointer"); return DKX_EINVAL; }
-    if (!patch_sizes || !scale_sizes || n_patch_sizes < 1 || n_scale_sizes < 1) {
-        set_error("serve: empty patch or scale size list");
-        return DKX_EINVAL;
-    }
-    std::vector<NlsarStatsView> views;
-    int rc = check_stats(stats, n_stats, patch_sizes, n_patch_sizes, scale_sizes, n_scale_sizes, views);
-    if (rc) return rc;
-    float nh, nT;
-    if (nlinsar_params(nl_patch_size, &nh, &nT)) {
-        set_error("serve: NL-InSAR patch sizes 3 and 7 only");
-        return DKX_EUNSUPPORTED;
-    }
-    if (nl_niter < 0 || nl_lmin < 1 || nl_search_window_size < 3 || nl_search_window_size % 2 == 0) {
-        set_error("serve: invalid NL-InSAR parameters");
-        return DKX_EINVAL;
-    }
-    Server* s = new Server();
-    s->P.assign(patch_sizes, patch_sizes + n_patch_sizes);
-    s->S.assign(scale_sizes, scale_sizes + n_scale_sizes);
-    for (const NlsarStatsView& v : views) {
-        s->lut_q.emplace_back(v.quantilles, v.quantilles + v.lut_size);
-        s->lut_c.emplace_back(v.chi2cdf_inv, v.chi2cdf_inv + v.lut_size);
-    }
-    for (size_t k = 0; k < views.size(); k++) {
-        NlsarStatsView v = views[k];
-        v.quantilles = s->lut_q[k].data();
-        v.chi2cdf_inv = s->lut_c[k].data();
-        s->views.push_back(v);
-    }
-    s->pp = NlsarParams{0, search_window_size, s->P.data(), n_patch_sizes, s->S.data(), n_scale_sizes,
-                        s->views.data(), (int)s->views.size(), h_param, c_param, 1};
-    rc = nlsar_common(s->pp, &s->cm);
-    std::vector<float> blob;
-    if (!rc) rc = nlsar_build_constants(s->pp, s->cm, blob);
-    if (rc) {
-        delete s;
-        return rc;
-    }
-    s->nl_sw = nl_search_window_size;
-    s->nl_p = nl_patch_size;
-    s->nl_niter = nl_niter;
-    s->nl_lmin = nl_lmin;
-    s->nl_h = nh;
-    s->nl_T = nT;
-    auto cleanup = [&](const std::string& msg) {
-        set_error(msg);
-        if (s->consts) (void)hipFree(s->consts);
-        if (s->arena_ptr) (void)hipFree(s->arena_ptr);
+    return dkx::guarded<int>([&]() -> int {
+        if (!out) { set_error("serve: null handle pointer"); return DKX_EINVAL; }
+        if (!patch_sizes || !scale_sizes || n_patch_sizes < 1 || n_scale_sizes < 1) {
+            set_error("serve: empty patch or scale size list");
+            return DKX_EINVAL;
+        }
+        std::vector<NlsarStatsView> views;
+        int last_unused = 0;
+        int rc = check_stats(stats, n_stats, patch_sizes, n_patch_sizes, scale_sizes, n_scale_sizes, views,
+                             &last_unused);
+        if (rc) return rc;
+        float nh, nT;
+        if (nlinsar_params(nl_patch_size, &nh, &nT)) {
+            set_error("serve: NL-InSAR patch sizes 3 and 7 only");
+            return DKX_EUNSUPPORTED;
+        }
+        if (nl_niter < 0 || nl_lmin < 1 || nl_search_window_size < 3 || nl_search_window_size % 2 == 0) {
+            set_error("serve: invalid NL-InSAR parameters");
+            return DKX_EINVAL;
+        }
+        Server* s = new Server();
+        s->P.assign(patch_sizes, patch_sizes + n_patch_sizes);
+        s->S.assign(scale_sizes, scale_sizes + n_scale_sizes);
+        for (const NlsarStatsView& v : views) {
+            s->lut_q.emplace_back(v.quantilles, v.quantilles + v.lut_size);
+            s->lut_c.emplace_back(v.chi2cdf_inv, v.chi2cdf_inv + v.lut_size);
+        }
+        for (size_t k = 0; k < views.size(); k++) {
+            NlsarStatsView v = views[k];
+            v.quantilles = s->lut_q[k].data();
+            v.chi2cdf_inv = s->lut_c[k].data();
+            s->views.push_back(v);
+        }
+        s->pp = NlsarParams{0, search_window_size, s->P.data(), n_patch_sizes, s->S.data(), n_scale_sizes,
+                            s->views.data(), (int)s->views.size(), h_param, c_param, 1, 0, last_unused};
+        rc = nlsar_common(s->pp, &s->cm);
+        std::vector<float> blob;
+        if (!rc) rc = nlsar_build_constants(s->pp, s->cm, blob);
+        if (rc) {
+            delete s;
+            return rc;
+        }
+        s->nl_sw = nl_search_window_size;
+        s->nl_p = nl_patch_size;
+        s->nl_niter = nl_niter;
+        s->nl_lmin = nl_lmin;
+        s->nl_h = nh;
+        s->nl_T = nT;
+        auto cleanup = [&](const std::string& msg) {
+            set_error(msg);
+            if (s->consts) (void)hipFree(s->consts);
+            if (s->arena_ptr) (void)hipFree(s->arena_ptr);
+            for (int i = 0; i < SAR_LANES; i++) {
+                if (s->sar_stream[i]) (void)hipStreamDestroy(s->sar_stream[i]);
+                if (s->pinned_table[i]) (void)hipHostFree(s->pinned_table[i]);
+                if (s->dev_table[i]) (void)hipFree(s->dev_table[i]);
+            }
+            for (int i = 0; i < NLI_LANES; i++)
+                if (s->nli_stream[i]) (void)hipStreamDestroy(s->nli_stream[i]);
+            delete s;
+            return DKX_ERESOURCE;
+        };
+        if (hipMalloc(&s->consts, blob.size() * sizeof(float)) != hipSuccess) return cleanup("serve: constants allocation failed");
+        if (hipMemcpy(s->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup("serve: constants upload failed");
+        if (hipMalloc(&s->arena_ptr, (size_t)hbm_budget_bytes) != hipSuccess)
+            return cleanup("serve: the HBM budget of " + std::to_string(hbm_budget_bytes) + " bytes cannot be allocated");
+        s->arena.reset((char*)s->arena_ptr, (size_t)hbm_budget_bytes);
         for (int i = 0; i < SAR_LANES; i++) {
-            if (s->sar_stream[i]) (void)hipStreamDestroy(s->sar_stream[i]);
-            if (s->pinned_table[i]) (void)hipHostFree(s->pinned_table[i]);
+            if (hipStreamCreateWithFlags(&s->sar_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
+            if (hipHostMalloc((void**)&s->pinned_table[i], sizeof(NlsarImageDesc) * MAX_BATCH, 0) != hipSuccess)
+                return cleanup("serve: pinned staging allocation failed");
+            if (hipMalloc((void**)&s->dev_table[i], sizeof(NlsarImageDesc) * MAX_BATCH) != hipSuccess)
+                return cleanup("serve: batch table allocation failed");
         }
         for (int i = 0; i < NLI_LANES; i++)
-            if (s->nli_stream[i]) (void)hipStreamDestroy(s->nli_stream[i]);
-        delete s;
-        return DKX_ERESOURCE;
-    };
-    if (hipMalloc(&s->consts, blob.size() * sizeof(float)) != hipSuccess) return cleanup("serve: constants allocation failed");
-    if (hipMemcpy(s->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup("serve: constants upload failed");
-    if (hipMalloc(&s->arena_ptr, (size_t)hbm_budget_bytes) != hipSuccess)
-        return cleanup("serve: the HBM budget of " + std::to_string(hbm_budget_bytes) + " bytes cannot be allocated");
-    s->arena.reset((char*)s->arena_ptr, (size_t)hbm_budget_bytes);
-    for (int i = 0; i < SAR_LANES; i++) {
-        if (hipStreamCreateWithFlags(&s->sar_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
-        if (hipHostMalloc((void**)&s->pinned_table[i], sizeof(NlsarImageDesc) * MAX_BATCH, 0) != hipSuccess)
-            return cleanup("serve: pinned staging allocation failed");
-    }
-    for (int i = 0; i < NLI_LANES; i++)
-        if (hipStreamCreateWithFlags(&s->nli_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
-    s->sched = std::thread(scheduler, s);
-    *out = reinterpret_cast<dkx_server>(s);
-    return 0;
+            if (hipStreamCreateWithFlags(&s->nli_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
+        s->sched = std::thread(scheduler, s);
+        *out = reinterpret_cast<dkx_server>(s);
+        return 0;
+    });
 }
 
 int dkx_server_submit(dkx_server h, const dkx_request* req, double arrival_time_s, long long* id)
 {
-    Server* s = reinterpret_cast<Server*>(h);
-    if (!s || !req) { set_error("serve: null argument"); return DKX_EINVAL; }
-    if ((req->kind != 0 && req->kind != 1) || req->height < 1 || req->width < 1 || !req->ampl_master ||
-        !req->ampl_slave || !req->phase || !req->ref_filt || !req->phase_filt || !req->coh_filt) {
-        set_error("serve: invalid request");
-        return DKX_EINVAL;
-    }
-    Request* r = new Request();
-    r->r = *req;
-    r->t_arrive = arrival_time_s > 0 ? arrival_time_s : now_s();
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop.load()) {
-            delete r;
-            set_error("serve: server is shutting down");
+    return dkx::guarded<int>([&]() -> int {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s || !req) { set_error("serve: null argument"); return DKX_EINVAL; }
+        if ((req->kind != 0 && req->kind != 1) || req->height < 1 || req->width < 1 || !req->ampl_master ||
+            !req->ampl_slave || !req->phase || !req->ref_filt || !req->phase_filt || !req->coh_filt) {
+            set_error("serve: invalid request");
             return DKX_EINVAL;
         }
-        r->id = s->next_id++;
-        s->reqs[r->id] = r;
-        s->incoming.push_back(r);
-    }
-    s->cv.notify_all();
-    if (id) *id = r->id;
-    return 0;
+        Request* r = new Request();
+        r->r = *req;
+        r->t_arrive = arrival_time_s > 0 ? arrival_time_s : now_s();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->stop.load()) {
+                delete r;
+                set_error("serve: server is shutting down");
+                return DKX_EINVAL;
+            }
+            r->id = s->next_id++;
+            s->reqs[r->id] = r;
+            s->incoming.push_back(r);
+        }
+        s->cv.notify_all();
+        if (id) *id = r->id;
+        return 0;
+    });
 }
 
 // 1 done (latency in ms from arrival to completion), 0 still pending, < 0 failed or unknown
 int dkx_server_poll(dkx_server h, long long id, double* latency_ms)
 {
-    Server* s = reinterpret_cast<Server*>(h);
-    if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) { set_error("serve: unknown request"); return DKX_EINVAL; }
-    Request* r = it->second;
-    if (r->status == 2) {
-        if (latency_ms) *latency_ms = (r->t_done - r->t_arrive) * 1e3;
-        return 1;
-    }
-    if (r->status < 0) {
-        set_error(s->sched_error.empty() ? "serve: request failed" : s->sched_error);
-        return DKX_EHIP;
-    }
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) { set_error("serve: unknown request"); return DKX_EINVAL; }
+        Request* r = it->second;
+        if (r->status == 2) {
+            if (latency_ms) *latency_ms = (r->t_done - r->t_arrive) * 1e3;
+            return 1;
+        }
+        if (r->status < 0) {
+            set_error(s->sched_error.empty() ? "serve: request failed" : s->sched_error);
+            return DKX_EHIP;
+        }
+        return 0;
+    });
 }
 
 int dkx_server_wait(dkx_server h, long long id, double* latency_ms)
 {
-    while (true) {
-        const int rc = dkx_server_poll(h, id, latency_ms);
-        if (rc != 0) return rc == 1 ? 0 : rc;
-        std::this_thread::sleep_for(std::chrono::microseconds(100));
-    }
+    return dkx::guarded<int>([&]() -> int {
+        while (true) {
+            const int rc = dkx_server_poll(h, id, latency_ms);
+            if (rc != 0) return rc == 1 ? 0 : rc;
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+    });
 }
 
 int dkx_server_run_trace(dkx_server h, const dkx_request* reqs, const double* arrival_s, int n, double* latency_ms,
                          double* makespan_s)
 {
-    Server* s = reinterpret_cast<Server*>(h);
-    if (!s || !reqs || !arrival_s || n < 0) { set_error("serve: null argument"); return DKX_EINVAL; }
-    std::vector<long long> ids(n);
-    const double t0 = now_s();
-    for (int i = 0; i < n; i++) {
-        const double ta = t0 + arrival_s[i];
-        while (true) {
-            const double dt = ta - now_s();
-            if (dt <= 0) break;
-            if (dt > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(dt - 1e-4));
+    return dkx::guarded<int>([&]() -> int {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s || !reqs || !arrival_s || n < 0) { set_error("serve: null argument"); return DKX_EINVAL; }
+        std::vector<long long> ids(n);
+        const double t0 = now_s();
+        for (int i = 0; i < n; i++) {
+            const double ta = t0 + arrival_s[i];
+            while (true) {
+                const double dt = ta - now_s();
+                if (dt <= 0) break;
+                if (dt > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(dt - 1e-4));
+            }
+            const int rc = dkx_server_submit(h, &reqs[i], ta, &ids[i]);
+            if (rc) return rc;
         }
-        const int rc = dkx_server_submit(h, &reqs[i], ta, &ids[i]);
-        if (rc) return rc;
-    }
-    int first_err = 0;
-    double last = t0;
-    for (int i = 0; i < n; i++) {
-        double lat = 0;
-        const int rc = dkx_server_wait(h, ids[i], &lat);
-        if (rc && !first_err) first_err = rc;
-        if (latency_ms) latency_ms[i] = rc ? -1.0 : lat;
-        last = std::max(last, t0 + arrival_s[i] + lat * 1e-3);
-    }
-    if (makespan_s) *makespan_s = last - t0;
-    return first_err;
+        int first_err = 0;
+        double last = t0;
+        for (int i = 0; i < n; i++) {
+            double lat = 0;
+            const int rc = dkx_server_wait(h, ids[i], &lat);
+            if (rc && !first_err) first_err = rc;
+            if (latency_ms) latency_ms[i] = rc ? -1.0 : lat;
+            last = std::max(last, t0 + arrival_s[i] + lat * 1e-3);
+        }
+        if (makespan_s) *makespan_s = last - t0;
+        return first_err;
+    });
 }
 
 int dkx_server_counters(dkx_server h, long long* batches, long long* batched_images)
 {
-    Server* s = reinterpret_cast<Server*>(h);
-    if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (batches) *batches = s->batches;
-    if (batched_images) *batched_images = s->batched_images;
-    return 0;
+    return dkx::guarded<int>([&]() -> int {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (batches) *batches = s->batches;
+        if (batched_images) *batched_images = s->batched_images;
+        return 0;
+    });
 }
 
 void dkx_server_destroy(dkx_server h)
 {
-    Server* s = reinterpret_cast<Server*>(h);
-    if (!s) return;
-    s->stop.store(true);
-    s->cv.notify_all();
-    if (s->sched.joinable()) s->sched.join();
-    for (int i = 0; i < SAR_LANES; i++) {
-        (void)hipStreamSynchronize(s->sar_stream[i]);
-        (void)hipStreamDestroy(s->sar_stream[i]);
-        (void)hipHostFree(s->pinned_table[i]);
-    }
-    for (int i = 0; i < NLI_LANES; i++) {
-        (void)hipStreamSynchronize(s->nli_stream[i]);
-        (void)hipStreamDestroy(s->nli_stream[i]);
-    }
-    for (auto& kv : s->reqs) delete kv.second;
-    (void)hipFree(s->arena_ptr);
-    (void)hipFree(s->consts);
-    delete s;
+    dkx::guarded_void([&] {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s) return;
+        s->stop.store(true);
+        s->cv.notify_all();
+        if (s->sched.joinable()) s->sched.join();
+        for (int i = 0; i < SAR_LANES; i++) {
+            (void)hipStreamSynchronize(s->sar_stream[i]);
+            (void)hipStreamDestroy(s->sar_stream[i]);
+            (void)hipHostFree(s->pinned_table[i]);
+            (void)hipFree(s->dev_table[i]);
+        }
+        for (int i = 0; i < NLI_LANES; i++) {
+            (void)hipStreamSynchronize(s->nli_stream[i]);
+            (void)hipStreamDestroy(s->nli_stream[i]);
+        }
+        for (auto& kv : s->reqs) delete kv.second;
+        (void)hipFree(s->arena_ptr);
+        (void)hipFree(s->consts);
+        delete s;
+    });
 }
 
 }  // extern "C"

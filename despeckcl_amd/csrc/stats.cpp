@@ -128,11 +128,13 @@ double dkx_chi2_inv(double p, double nu) { return dkx::chi2_inv(p, nu); }
 long long dkx_nlsar_stats_build(const float* dissims, long long n, int lut_size, float* dmin,
                                 float* dmax, float* quantilles, float* chi2cdf_inv)
 {
-    if (!dissims || n < 0 || lut_size < 2 || !dmin || !dmax || !quantilles || !chi2cdf_inv) {
-        dkx::set_error("nlsar_stats_build: invalid arguments");
-        return DKX_EINVAL;
-    }
-    return dkx::stats_build(dissims, n, lut_size, dmin, dmax, quantilles, chi2cdf_inv);
+    return dkx::guarded<long long>([&]() -> long long {
+        if (!dissims || n < 0 || lut_size < 2 || !dmin || !dmax || !quantilles || !chi2cdf_inv) {
+            dkx::set_error("nlsar_stats_build: invalid arguments");
+            return DKX_EINVAL;
+        }
+        return dkx::stats_build(dissims, n, lut_size, dmin, dmax, quantilles, chi2cdf_inv);
+    });
 }
 
 }  // extern "C"

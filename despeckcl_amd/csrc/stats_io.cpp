@@ -170,7 +170,7 @@ static int read_entries(const char* filename, std::vector<StatsEntry>& out)
             StatsEntry e;
             unsigned long long lut;
             if (p + 5 > tok.size() || !parse_int(tok[p], &e.patch_size) || !parse_int(tok[p + 1], &e.scale_size) ||
-                !parse_u64(tok[p + 2], &lut) || lut == 0 || !parse_float(tok[p + 3], &e.dmin) ||
+                !parse_u64(tok[p + 2], &lut) || lut == 0 || lut > (1ull << 20) || !parse_float(tok[p + 3], &e.dmin) ||
                 !parse_float(tok[p + 4], &e.dmax))
                 goto bad;
             p += 5;
@@ -223,82 +223,86 @@ extern "C" {
 
 int dkx_nlsar_stats_store(const char* filename, const dkx_nlsar_stats* stats, int n_stats)
 {
-    if (!filename || n_stats < 0 || (n_stats > 0 && !stats)) {
-        set_error("store_nlsar_stats_collection: invalid arguments");
-        return DKX_EINVAL;
-    }
-    std::vector<const dkx_nlsar_stats*> e(n_stats);
-    for (int i = 0; i < n_stats; i++) {
-        if (stats[i].lut_size < 1 || !stats[i].quantilles || !stats[i].chi2cdf_inv) {
-            set_error("store_nlsar_stats_collection: invalid statistics entry");
+    return dkx::guarded<int>([&]() -> int {
+        if (!filename || n_stats < 0 || (n_stats > 0 && !stats)) {
+            set_error("store_nlsar_stats_collection: invalid arguments");
             return DKX_EINVAL;
         }
-        e[i] = &stats[i];
-    }
-    // std::map order: patch size, then scale size (parameters.h:37-40)
-    std::sort(e.begin(), e.end(), [](const dkx_nlsar_stats* a, const dkx_nlsar_stats* b) {
-        return a->patch_size < b->patch_size || (a->patch_size == b->patch_size && a->scale_size < b->scale_size);
+        std::vector<const dkx_nlsar_stats*> e(n_stats);
+        for (int i = 0; i < n_stats; i++) {
+            if (stats[i].lut_size < 1 || !stats[i].quantilles || !stats[i].chi2cdf_inv) {
+                set_error("store_nlsar_stats_collection: invalid statistics entry");
+                return DKX_EINVAL;
+            }
+            e[i] = &stats[i];
+        }
+        // std::map order: patch size, then scale size (parameters.h:37-40)
+        std::sort(e.begin(), e.end(), [](const dkx_nlsar_stats* a, const dkx_nlsar_stats* b) {
+            return a->patch_size < b->patch_size || (a->patch_size == b->patch_size && a->scale_size < b->scale_size);
+        });
+        for (int i = 1; i < n_stats; i++)
+            if (e[i]->patch_size == e[i - 1]->patch_size && e[i]->scale_size == e[i - 1]->scale_size) {
+                set_error("store_nlsar_stats_collection: duplicate (patch size, scale size) key");
+                return DKX_EINVAL;
+            }
+        std::string s = "22 serialization::archive " + std::to_string(kBoostArchiveVersion);
+        s += " 0 0 " + std::to_string(n_stats) + " 0";
+        for (int i = 0; i < n_stats; i++) {
+            const dkx_nlsar_stats& st = *e[i];
+            if (i == 0) s += " 0 0 0 0";   // class info of the pair and of params
+            s += " " + std::to_string(st.patch_size) + " " + std::to_string(st.scale_size);
+            if (i == 0) s += " 0 0";       // class info of stats
+            s += " " + std::to_string((unsigned int)st.lut_size);
+            put_float(s, st.dissims_min);
+            put_float(s, st.dissims_max);
+            for (const float* v : {st.quantilles, st.chi2cdf_inv}) {
+                s += " " + std::to_string((unsigned int)st.lut_size) + " 0";
+                for (int k = 0; k < st.lut_size; k++) put_float(s, v[k]);
+            }
+        }
+        s += "\n";
+        std::ofstream f(filename, std::ios::binary);
+        if (!f) {
+            set_error(std::string("cannot open ") + filename + " for writing");
+            return DKX_EINVAL;
+        }
+        f << s;
+        if (!f) {
+            set_error(std::string("cannot write ") + filename);
+            return DKX_EINVAL;
+        }
+        return 0;
     });
-    for (int i = 1; i < n_stats; i++)
-        if (e[i]->patch_size == e[i - 1]->patch_size && e[i]->scale_size == e[i - 1]->scale_size) {
-            set_error("store_nlsar_stats_collection: duplicate (patch size, scale size) key");
-            return DKX_EINVAL;
-        }
-    std::string s = "22 serialization::archive " + std::to_string(kBoostArchiveVersion);
-    s += " 0 0 " + std::to_string(n_stats) + " 0";
-    for (int i = 0; i < n_stats; i++) {
-        const dkx_nlsar_stats& st = *e[i];
-        if (i == 0) s += " 0 0 0 0";   // class info of the pair and of params
-        s += " " + std::to_string(st.patch_size) + " " + std::to_string(st.scale_size);
-        if (i == 0) s += " 0 0";       // class info of stats
-        s += " " + std::to_string((unsigned int)st.lut_size);
-        put_float(s, st.dissims_min);
-        put_float(s, st.dissims_max);
-        for (const float* v : {st.quantilles, st.chi2cdf_inv}) {
-            s += " " + std::to_string((unsigned int)st.lut_size) + " 0";
-            for (int k = 0; k < st.lut_size; k++) put_float(s, v[k]);
-        }
-    }
-    s += "\n";
-    std::ofstream f(filename, std::ios::binary);
-    if (!f) {
-        set_error(std::string("cannot open ") + filename + " for writing");
-        return DKX_EINVAL;
-    }
-    f << s;
-    if (!f) {
-        set_error(std::string("cannot write ") + filename);
-        return DKX_EINVAL;
-    }
-    return 0;
 }
 
 long long dkx_nlsar_stats_load(const char* filename, dkx_nlsar_stats* out, int max_entries,
                                float* lut_storage, long long max_floats, long long* n_floats)
 {
-    if (!filename) {
-        set_error("load_nlsar_stats_collection: null file name");
-        return DKX_EINVAL;
-    }
-    std::vector<StatsEntry> e;
-    const int rc = read_entries(filename, e);
-    if (rc) return rc;
-    std::sort(e.begin(), e.end(), [](const StatsEntry& a, const StatsEntry& b) {
-        return a.patch_size < b.patch_size || (a.patch_size == b.patch_size && a.scale_size < b.scale_size);
+    return dkx::guarded<long long>([&]() -> long long {
+        if (!filename) {
+            set_error("load_nlsar_stats_collection: null file name");
+            return DKX_EINVAL;
+        }
+        std::vector<StatsEntry> e;
+        const int rc = read_entries(filename, e);
+        if (rc) return rc;
+        std::sort(e.begin(), e.end(), [](const StatsEntry& a, const StatsEntry& b) {
+            return a.patch_size < b.patch_size || (a.patch_size == b.patch_size && a.scale_size < b.scale_size);
+        });
+        long long need = 0;
+        for (const auto& x : e) need += 2ll * x.lut_size;
+        if (n_floats) *n_floats = need;
+        if (!out || !lut_storage || max_entries < (int)e.size() || max_floats < need) return (long long)e.size();
+        float* p = lut_storage;
+        for (size_t i = 0; i < e.size(); i++) {
+            const StatsEntry& x = e[i];
+            std::memcpy(p, x.q.data(), sizeof(float) * x.lut_size);
+            std::memcpy(p + x.lut_size, x.c.data(), sizeof(float) * x.lut_size);
+            out[i] = dkx_nlsar_stats{x.patch_size, x.scale_size, (int)x.lut_size, x.dmin, x.dmax, p, p + x.lut_size};
+            p += 2 * x.lut_size;
+        }
+        return (long long)e.size();
     });
-    long long need = 0;
-    for (const auto& x : e) need += 2ll * x.lut_size;
-    if (n_floats) *n_floats = need;
-    if (!out || !lut_storage || max_entries < (int)e.size() || max_floats < need) return (long long)e.size();
-    float* p = lut_storage;
-    for (size_t i = 0; i < e.size(); i++) {
-        const StatsEntry& x = e[i];
-        std::memcpy(p, x.q.data(), sizeof(float) * x.lut_size);
-        std::memcpy(p + x.lut_size, x.c.data(), sizeof(float) * x.lut_size);
-        out[i] = dkx_nlsar_stats{x.patch_size, x.scale_size, (int)x.lut_size, x.dmin, x.dmax, p, p + x.lut_size};
-        p += 2 * x.lut_size;
-    }
-    return (long long)e.size();
 }
 
 }  // extern "C"

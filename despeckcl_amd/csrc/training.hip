@@ -410,51 +410,57 @@ int dkx_nlsar_training_covmat_device(const float* d_covmat, int height, int widt
                                      const int* P, int nP, const int* S, int nS,
                                      dkx_nlsar_stats* out, float* lut_storage, void* stream)
 {
-    int rc = train_check(height, width, dim, P, nP, S, nS, out, lut_storage);
-    if (rc) return rc;
-    if (!d_covmat) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
-    rc = have_device();
-    if (rc) return rc;
-    return train_device(d_covmat, height, width, dim, P, nP, S, nS, out, lut_storage, (hipStream_t)stream);
+    return dkx::guarded<int>([&]() -> int {
+        int rc = train_check(height, width, dim, P, nP, S, nS, out, lut_storage);
+        if (rc) return rc;
+        if (!d_covmat) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
+        rc = have_device();
+        if (rc) return rc;
+        return train_device(d_covmat, height, width, dim, P, nP, S, nS, out, lut_storage, (hipStream_t)stream);
+    });
 }
 
 int dkx_nlsar_training_covmat(const float* covmat, int height, int width, int dim, const int* P,
                               int nP, const int* S, int nS, dkx_nlsar_stats* out, float* lut_storage)
 {
-    int rc = train_check(height, width, dim, P, nP, S, nS, out, lut_storage);
-    if (rc) return rc;
-    if (!covmat) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
-    rc = have_device();
-    if (rc) return rc;
-    return train_host_planes(covmat, height, width, dim, P, nP, S, nS, out, lut_storage);
+    return dkx::guarded<int>([&]() -> int {
+        int rc = train_check(height, width, dim, P, nP, S, nS, out, lut_storage);
+        if (rc) return rc;
+        if (!covmat) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
+        rc = have_device();
+        if (rc) return rc;
+        return train_host_planes(covmat, height, width, dim, P, nP, S, nS, out, lut_storage);
+    });
 }
 
 int dkx_nlsar_training_insar(const float* a1, const float* a2, const float* phi, int height,
                              int width, const int* P, int nP, const int* S, int nS,
                              dkx_nlsar_stats* out, float* lut_storage)
 {
-    int rc = train_check(height, width, 2, P, nP, S, nS, out, lut_storage);
-    if (rc) return rc;
-    if (!a1 || !a2 || !phi) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
-    rc = have_device();
-    if (rc) return rc;
-    // covmat_data(insar_data), src/data/data.cpp:60-89: the reference forms the training
-    // covariance on the host with float cos / sin, and so does this entry point
-    const size_t n = (size_t)height * width;
-    std::vector<float> cov(8 * n);
-    for (size_t i = 0; i < n; i++) {
-        const float re = a1[i] * a2[i] * std::cos(phi[i]);
-        const float im = a1[i] * a2[i] * std::sin(phi[i]);
-        cov[i] = a1[i] * a1[i];
-        cov[n + i] = 0.f;
-        cov[2 * n + i] = re;
-        cov[3 * n + i] = im;
-        cov[4 * n + i] = re;
-        cov[5 * n + i] = -im;
-        cov[6 * n + i] = a2[i] * a2[i];
-        cov[7 * n + i] = 0.f;
-    }
-    return train_host_planes(cov.data(), height, width, 2, P, nP, S, nS, out, lut_storage);
+    return dkx::guarded<int>([&]() -> int {
+        int rc = train_check(height, width, 2, P, nP, S, nS, out, lut_storage);
+        if (rc) return rc;
+        if (!a1 || !a2 || !phi) { set_error("nlsar_training: null array argument"); return DKX_EINVAL; }
+        rc = have_device();
+        if (rc) return rc;
+        // covmat_data(insar_data), src/data/data.cpp:60-89: the reference forms the training
+        // covariance on the host with float cos / sin, and so does this entry point
+        const size_t n = (size_t)height * width;
+        std::vector<float> cov(8 * n);
+        for (size_t i = 0; i < n; i++) {
+            const float re = a1[i] * a2[i] * std::cos(phi[i]);
+            const float im = a1[i] * a2[i] * std::sin(phi[i]);
+            cov[i] = a1[i] * a1[i];
+            cov[n + i] = 0.f;
+            cov[2 * n + i] = re;
+            cov[3 * n + i] = im;
+            cov[4 * n + i] = re;
+            cov[5 * n + i] = -im;
+            cov[6 * n + i] = a2[i] * a2[i];
+            cov[7 * n + i] = 0.f;
+        }
+        return train_host_planes(cov.data(), height, width, 2, P, nP, S, nS, out, lut_storage);
+    });
 }
 
 }  // extern "C"

@@ -67,9 +67,9 @@ def test_argument_errors_without_device(D):
     st = _stats_for(D, [(3, 1), (5, 1)])
     with pytest.raises(ValueError, match="odd"):
         D.nlsar_insar(a, a, a, 20, [3, 5], [1], st)
-    with pytest.raises(ValueError, match="patch x scale|one entry"):
+    with pytest.raises(ValueError, match="every .patch size, scale size. pair"):
         D.nlsar_insar(a, a, a, 21, [3, 5, 7], [1], st)
-    with pytest.raises(ValueError, match="not in the patch x scale set"):
+    with pytest.raises(ValueError, match=r"no statistics for \(patch 7, scale 1\)"):
         D.nlsar_insar(a, a, a, 21, [3, 7], [1], st)
     with pytest.raises(ValueError, match="odd"):
         D.nlsar_insar(a, a, a, 21, [3, 5], [2], _stats_for(D, [(3, 2), (5, 2)]))
@@ -106,6 +106,39 @@ def test_argument_errors_without_device(D):
         D.nlinsar(a, a, a, 22, 7, 2, 10)
     with pytest.raises(ValueError, match="lmin"):
         D.nlinsar(a, a, a, 21, 7, 2, 0)
+
+
+def test_statistics_collections_with_extra_keys(D):
+    """A collection trained on more (patch, scale) keys than the call filters with passes the
+    host checks, as in the reference (only the device step fails here, without a GPU); a
+    missing key or a key given twice is rejected before any device call."""
+    from despeckcl_amd import _lib as L
+    a = np.ones((16, 16), np.float32)
+    sup = _stats_for(D, [(3, 1), (5, 1), (7, 1), (3, 3)])
+    with pytest.raises(Exception) as e:
+        D.nlsar_insar(a, a, a, 21, [3, 5], [1], sup)
+    assert not isinstance(e.value, ValueError), e.value      # not a statistics error
+    arr, n, keep = D._stats_structs(_stats_for(D, [(3, 1), (5, 1)]))
+    dup = (L.DkxStats * 3)(arr[0], arr[1], arr[1])
+    P, nP = L.int_array([3, 5])
+    S_, nS = L.int_array([1])
+    out = [np.zeros((16, 16), np.float32) for _ in range(3)]
+    rc = L.lib().dkx_nlsar_insar(a.ctypes.data, a.ctypes.data, a.ctypes.data, out[0].ctypes.data,
+                                 out[1].ctypes.data, out[2].ctypes.data, 16, 16, 21, P, nP, S_, nS,
+                                 dup, 3, 15.0, 49.0)
+    assert rc == L.DKX_ESTATS and b"twice" in L.lib().dkx_last_error()
+
+
+def test_stats_file_oversized_lut_is_rejected(D, tmp_path):
+    """A malformed statistics file whose LUT size token would overflow the length check is
+    an ordinary error, not an abort (no C++ exception crosses the C ABI)."""
+    f = tmp_path / "huge.txt"
+    f.write_text("despeckcl_amd-nlsar-stats 1 1 3 1 18446744073709551615 1.0 2.0 0.5\n")
+    with pytest.raises(ValueError, match="not a readable"):
+        D.load_nlsar_stats_collection(str(f))
+    f.write_text("despeckcl_amd-nlsar-stats 1 1 3 1 4000000000 1.0 2.0 0.5\n")
+    with pytest.raises(ValueError, match="not a readable"):
+        D.load_nlsar_stats_collection(str(f))
 
 
 def test_tile_settings(D):
