@@ -414,19 +414,17 @@ def served_run(n_requests, rate, budget_gib):
 
     stats = D.load_nlsar_stats_collection(os.path.join(ROOT, "tests", "golden", "train32_stats.txt"))
     n = n_requests
-    rng = np.random.Generator(np.random.PCG64(S.SEED_BASE + 4))
-    kinds = (rng.random(n) < 0.3).astype(int)
-    shapes = S.tile_shapes(n, S.SEED_BASE + 4, lo=128, hi=4096)
+    kinds, shapes, origins, arrivals = S.served_stream(n, rate)
     scenes = []
     for k in range(4):
-        host = S.interferogram(4096, 4096, S.SEED_BASE + 40 + k, n_regions=16, coh_range=(0.2, 0.95))
+        host = S.served_scene(k)
         scenes.append([HIP.DeviceArray.from_host(x) for x in host])
         del host
     reqs, bufs, px = [], [], 0
     for i in range(n):
         h, w = shapes[i]
         sc = scenes[i % 4]
-        y0, x0 = int(rng.integers(0, 4096 - h + 1)), int(rng.integers(0, 4096 - w + 1))
+        y0, x0 = origins[i]
         inp = [HIP.DeviceArray((h, w)) for _ in range(3)]
         for d, src in zip(inp, sc):
             HIP.copy2d(d.ptr, w * 4, src.ptr + (y0 * 4096 + x0) * 4, 4096 * 4, w * 4, h)
@@ -443,8 +441,6 @@ def served_run(n_requests, rate, budget_gib):
     lat_sat, span_sat = srv.run_trace(reqs, [0.0] * n)
     b_sat = srv.counters()
     log(f"served: saturated {span_sat:.2f} s")
-    gaps = rng.exponential(1.0 / rate, n)
-    arrivals = np.cumsum(gaps) - gaps[0]
     lat, span = srv.run_trace(reqs, arrivals.tolist())
     b_all = srv.counters()
     srv.close()

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["interferogram", "homogeneous_area", "polsar", "SEED_BASE"]
+__all__ = ["interferogram", "homogeneous_area", "polsar", "tile_shapes", "served_stream", "SEED_BASE"]
 
 SEED_BASE = 20260
 
@@ -111,3 +111,27 @@ def polsar(height, width, seed, dim=3, n_regions=6, refl_range=(1.0, 100.0)):
     k = np.einsum("hwij,jhw->ihw", L, n)          # (dim, H, W)
     cov = k[:, None] * np.conj(k[None, :])        # (dim, dim, H, W)
     return np.ascontiguousarray(cov.astype(np.complex64))
+
+
+def served_stream(n, rate=40.0, scene_side=4096, n_scenes=4, lo=128, hi=4096):
+    """The seeded mixed request stream: n requests, 30 % NL-InSAR (kind 1) and 70 % NL-SAR
+    (kind 0), sides log-uniform in [lo, hi] (aspect <= 4:1), each a crop of one of n_scenes
+    scene_side^2 interferograms (request i uses scene i % n_scenes, made by
+    interferogram(scene_side, scene_side, SEED_BASE + 40 + k, n_regions=16,
+    coh_range=(0.2, 0.95))), and Poisson arrival times at `rate` requests/s.  Returns
+    (kinds, shapes, origins, arrivals_s)."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 4))
+    kinds = (rng.random(n) < 0.3).astype(int)
+    shapes = tile_shapes(n, SEED_BASE + 4, lo=lo, hi=hi)
+    origins = []
+    for h, w in shapes:
+        y0, x0 = int(rng.integers(0, scene_side - h + 1)), int(rng.integers(0, scene_side - w + 1))
+        origins.append((y0, x0))
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    return kinds, shapes, origins, arrivals
+
+
+def served_scene(k, scene_side=4096):
+    """Scene k of served_stream."""
+    return interferogram(scene_side, scene_side, SEED_BASE + 40 + k, n_regions=16, coh_range=(0.2, 0.95))

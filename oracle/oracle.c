@@ -631,8 +631,9 @@ static int params_less(int p1, int s1, int p2, int s2)
 /* covmat_ori: 2*dim*dim planes of th x tw; covmat_filt receives the interior */
 /* cond (optional, th x tw, interior written): min over the diagonal of var / I_nl^2 of the
  * selected parameter; alpha = 1 - I_nl^2 / var is ill-conditioned where it is small */
-/* maps (optional, 2*nP*nS planes of th x tw, interior written): every key's ENL_nobias, then
- * every key's alpha, key index scale_index*nP + patch_index */
+/* maps (optional, 3*nP*nS planes of th x tw, interior written): every key's ENL_nobias, then
+ * every key's alpha, then every key's conditioning (as cond), key index
+ * scale_index*nP + patch_index */
 static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int tw, int dim,
                       int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
                       int nst, float hpar, float cpar, long long* ub_pixels, float* cond,
@@ -778,6 +779,7 @@ static int nlsar_tile(const float* covmat_ori, float* covmat_filt, int th, int t
                     const size_t t = (size_t)(y + o) * tw + x + o;
                     maps[(size_t)k * tplane + t] = enls_nobias[(size_t)k * n_ori + i];
                     maps[(size_t)(nparam + k) * tplane + t] = alphas[(size_t)k * n_ori + i];
+                    maps[(size_t)(2 * nparam + k) * tplane + t] = conds[(size_t)k * n_ori + i];
                 }
     }
 
@@ -851,7 +853,7 @@ typedef struct {
 } nlsar_ctx;
 
 /* tile output planes: the filter's (3 for InSAR input, 2*dim*dim for covariance input), then
- * the conditioning plane and the 2*nP*nS map planes when requested */
+ * the conditioning plane and the 3*nP*nS map planes when requested */
 static float* extra_planes(nlsar_ctx* cx, float* tout, size_t n, int nbase, float** maps)
 {
     float* cond = cx->want_cond ? tout + (size_t)nbase * n : NULL;
@@ -891,7 +893,7 @@ static int nlsar_covmat_tile(const float* tin, float* tout, int th, int tw, void
 /* the filter over the image, for InSAR input (dim 0: in = a1, a2, phi; filt = ref, phase,
  * coh) or covariance input (dim 2 / 3: 2*dim*dim planes each) */
 static int nlsar_image(const float* in, float* filt, float* cond, float* enl_maps,
-                       float* alpha_maps, int height, int width, int dim, int sw, const int* P,
+                       float* alpha_maps, float* cond_maps, int height, int width, int dim, int sw, const int* P,
                        int nP, const int* S, int nS, const orc_stats* st, int nst, float hpar,
                        float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
@@ -902,9 +904,9 @@ static int nlsar_image(const float* in, float* filt, float* cond, float* enl_map
     if (tile_w <= 0) tile_w = width + 3 * overlap;
     if (tile_h <= 2 * overlap || tile_w <= 2 * overlap) return -1;
     const int nin = dim ? 2 * dim * dim : 3, nbase = nin;
-    const int want_maps = enl_maps || alpha_maps;
+    const int want_maps = enl_maps || alpha_maps || cond_maps;
     const int nparam = nP * nS;
-    const int nout = nbase + (cond ? 1 : 0) + (want_maps ? 2 * nparam : 0);
+    const int nout = nbase + (cond ? 1 : 0) + (want_maps ? 3 * nparam : 0);
     const size_t n = (size_t)height * width;
     float* out = (float*)calloc((size_t)nout * n, sizeof(float));
     memcpy(out, filt, sizeof(float) * nbase * n);
@@ -918,6 +920,7 @@ static int nlsar_image(const float* in, float* filt, float* cond, float* enl_map
         if (want_maps) {
             if (enl_maps) memcpy(enl_maps, extra, sizeof(float) * nparam * n);
             if (alpha_maps) memcpy(alpha_maps, extra + (size_t)nparam * n, sizeof(float) * nparam * n);
+            if (cond_maps) memcpy(cond_maps, extra + (size_t)2 * nparam * n, sizeof(float) * nparam * n);
         }
     }
     if (ub_pixels) *ub_pixels = cx.ub;
@@ -931,8 +934,8 @@ int orc_nlsar_insar(const float* a1, const float* a2, const float* phi, float* r
                     float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
     return orc_nlsar_insar_maps(a1, a2, phi, ref_filt, phase_filt, coh_filt, NULL, NULL, NULL,
-                                height, width, sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h,
-                                tile_w, ub_pixels);
+                                NULL, height, width, sw, P, nP, S, nS, st, nst, hpar, cpar,
+                                tile_h, tile_w, ub_pixels);
 }
 
 int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, float* ref_filt,
@@ -942,15 +945,16 @@ int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, flo
                          long long* ub_pixels)
 {
     return orc_nlsar_insar_maps(a1, a2, phi, ref_filt, phase_filt, coh_filt, cond, NULL, NULL,
-                                height, width, sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h,
-                                tile_w, ub_pixels);
+                                NULL, height, width, sw, P, nP, S, nS, st, nst, hpar, cpar,
+                                tile_h, tile_w, ub_pixels);
 }
 
 int orc_nlsar_insar_maps(const float* a1, const float* a2, const float* phi, float* ref_filt,
                          float* phase_filt, float* coh_filt, float* cond, float* enl_maps,
-                         float* alpha_maps, int height, int width, int sw, const int* P, int nP,
-                         const int* S, int nS, const orc_stats* st, int nst, float hpar,
-                         float cpar, int tile_h, int tile_w, long long* ub_pixels)
+                         float* alpha_maps, float* cond_maps, int height, int width, int sw,
+                         const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                         int nst, float hpar, float cpar, int tile_h, int tile_w,
+                         long long* ub_pixels)
 {
     if (height <= 0 || width <= 0) return -1;
     const size_t n = (size_t)height * width;
@@ -962,7 +966,7 @@ int orc_nlsar_insar_maps(const float* a1, const float* a2, const float* phi, flo
     memcpy(filt, ref_filt, n * sizeof(float));
     memcpy(filt + n, phase_filt, n * sizeof(float));
     memcpy(filt + 2 * n, coh_filt, n * sizeof(float));
-    int rc = nlsar_image(in, filt, cond, enl_maps, alpha_maps, height, width, 0, sw, P, nP, S, nS,
+    int rc = nlsar_image(in, filt, cond, enl_maps, alpha_maps, cond_maps, height, width, 0, sw, P, nP, S, nS,
                          st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
     if (rc == 0) {
         memcpy(ref_filt, filt, n * sizeof(float));
@@ -978,7 +982,7 @@ int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, in
                      int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
                      int nst, float hpar, float cpar, int tile_h, int tile_w, long long* ub_pixels)
 {
-    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, NULL, NULL, NULL, height, width, dim,
+    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, NULL, NULL, NULL, NULL, height, width, dim,
                                  sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
 }
 
@@ -988,18 +992,18 @@ int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* co
                           const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
                           int tile_w, long long* ub_pixels)
 {
-    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, cond, NULL, NULL, height, width, dim,
+    return orc_nlsar_covmat_maps(covmat_raw, covmat_filt, cond, NULL, NULL, NULL, height, width, dim,
                                  sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
 }
 
 int orc_nlsar_covmat_maps(const float* covmat_raw, float* covmat_filt, float* cond,
-                          float* enl_maps, float* alpha_maps, int height, int width, int dim,
-                          int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
-                          int nst, float hpar, float cpar, int tile_h, int tile_w,
-                          long long* ub_pixels)
+                          float* enl_maps, float* alpha_maps, float* cond_maps, int height,
+                          int width, int dim, int sw, const int* P, int nP, const int* S, int nS,
+                          const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
+                          int tile_w, long long* ub_pixels)
 {
     if (dim != 2 && dim != 3) return -1;
-    return nlsar_image(covmat_raw, covmat_filt, cond, enl_maps, alpha_maps, height, width, dim,
+    return nlsar_image(covmat_raw, covmat_filt, cond, enl_maps, alpha_maps, cond_maps, height, width, dim,
                        sw, P, nP, S, nS, st, nst, hpar, cpar, tile_h, tile_w, ub_pixels);
 }
 
@@ -1251,15 +1255,131 @@ typedef struct {
     int sw, p, lmin;
     float h, T;
     long long nsmooth;
+    int want_frag;
+    int alt;   /* 1: the equivalent forms below; 2: double precision (sensitivity check only) */
 } nlinsar_ctx;
+
+/* Algebraically equivalent forms of the similarity and KL terms (sim_measures.h:27-49):
+ * cos(dp1 - dp2) = cos1 cos2 + sin1 sin2, pow(q, 1.5) = q sqrt(q), quotients as products with
+ * a reciprocal, the KL's per-pixel factors 1 / (ref (1 - coh^2)).  Each differs from the
+ * reference's expression by a few ulps (OpenCL 1.2 itself allows up to 4 ulp for cos, 16 for
+ * pow, 2.5 for division), so where they move an output noticeably the reference's output is
+ * ill-conditioned: its own builtins may move it as much. */
+static float alt_similarity(float am1, float as1, float dp1, float am2, float as2, float dp2)
+{
+    const float cs = fmaf(cosf(dp1), cosf(dp2), sinf(dp1) * sinf(dp2));
+    const float s = am1 * am1 + as1 * as1 + am2 * am2 + as2 * as2;
+    const float A = s * s;
+    const float B = 4.0f * (am1 * am1 * as1 * as1 + am2 * am2 * as2 * as2 + 2.0f * am1 * as1 * am2 * as2 * cs);
+    const float C = am1 * as1 * am2 * as2;
+    const float q = C * (1.0f / B);
+    const float f_el = q * sqrtf(q) * ((A + B) * (1.0f / A) * sqrtf(B * (1.0f / (A - B))) -
+                                       asinf(sqrtf(B * (1.0f / A))));
+    if ((f_el == 0) || isnan(f_el)) return 0;
+    return logf(f_el);
+}
+
+/* the same terms evaluated in double precision, rounded to float */
+static float dbl_similarity(float am1_, float as1_, float dp1, float am2_, float as2_, float dp2)
+{
+    const double am1 = am1_, as1 = as1_, am2 = am2_, as2 = as2_;
+    const double s = am1 * am1 + as1 * as1 + am2 * am2 + as2 * as2;
+    const double A = s * s;
+    const double B = 4.0 * (am1 * am1 * as1 * as1 + am2 * am2 * as2 * as2 +
+                            2.0 * am1 * as1 * am2 * as2 * cos((double)dp1 - (double)dp2));
+    const double C = am1 * as1 * am2 * as2;
+    const double f_el = pow(C / B, 1.5) * ((A + B) / A * sqrt(B / (A - B)) - asin(sqrt(B / A)));
+    if ((f_el == 0) || isnan(f_el)) return 0;
+    return (float)log(f_el);
+}
+
+static float dbl_kl(float ref1, float dp1, float coh1, float ref2, float dp2, float coh2)
+{
+    const double cs = cos((double)dp1 - (double)dp2);
+    const double a1 = (ref2 != 0.0f) ? (double)ref1 / ref2 * (1.0 - (double)coh1 * coh2 * cs) / (1.0 - (double)coh2 * coh2) : 1.0;
+    const double a2 = (ref2 != 0.0f) ? (double)ref2 / ref1 * (1.0 - (double)coh1 * coh2 * cs) / (1.0 - (double)coh1 * coh1) : 1.0;
+    return (float)(-4.0 / M_PI * (a1 + a2 - 2.0));
+}
+
+static float alt_kl(float ref1, float dp1, float coh1, float ref2, float dp2, float coh2)
+{
+    const float r = 1.0f - fmaf(coh1 * cosf(dp1), coh2 * cosf(dp2), (coh1 * sinf(dp1)) * (coh2 * sinf(dp2)));
+    const float u1 = 1.0f / (ref1 * (1.0f - coh1 * coh1)), u2 = 1.0f / (ref2 * (1.0f - coh2 * coh2));
+    const float a1 = ref1 * r * u2, a2 = ref2 * r * u1;
+    return ref2 != 0.0f ? -4.0f / (float)M_PI * (a1 + a2 - 2.0f) : -0.0f;
+}
+
+/* Where a last-bit difference of the weights can change the lmin smoothing's kept set
+ * (smoothing.cpp:78-129): the lmin-th and (lmin+1)-th largest candidate weights tied or
+ * within a relative 1e-4 (2), or a tiny weight (below 1e-30, a denormal or nearly so, with
+ * few significant bits) at the kept threshold (4).  Candidates as in the smoothing. */
+static int smoothing_fragility(const float* ampl_master, const float* weights, int h, int w,
+                               int width_overlap, int patch_size, int sw, int lmin)
+{
+    const int psh = (patch_size - 1) / 2, wsh = (sw - 1) / 2;
+    const int overlap = wsh + psh;
+    float c[1024];
+    int n = 0, tiny = 0;
+    const double as = ampl_master[(size_t)(h + overlap) * width_overlap + (w + overlap)];
+    for (int hh = 0; hh < sw; hh++)
+        for (int ww = 0; ww < sw; ww++) {
+            const float at = ampl_master[(size_t)(h + overlap + hh - wsh) * width_overlap +
+                                         (w + overlap + ww - wsh)];
+            if (at < 2.0 * as) {
+                c[n++] = weights[hh * sw + ww];
+                if (c[n - 1] > 0 && c[n - 1] < 1e-30f) tiny = 1;
+            }
+        }
+    if (n <= lmin) return 0;
+    qsort(c, (size_t)n, sizeof(float), cmp_float);   /* ascending */
+    const float T = c[n - lmin], U = c[n - lmin - 1];
+    int flag = 0;
+    if ((T == U && T > 0) || (T != U && T - U <= 1e-4f * T)) flag |= 2;
+    if (T < 1e-30f && tiny) flag |= 4;
+    return flag;
+}
 
 static int is_valid(float x) { return !(isnan(x) || isinf(x) || x == 0); }
 
 /* nlinsar_sub_image, nlinsar_sub_image.cpp:22-262, on one tile of 6 planes
  * (a1, a2, phi, ref, phase, coh); writes the updated estimate planes */
+static int nlinsar_tile_impl(const float* tin, float* tout, int th, int tw, nlinsar_ctx* cx);
+
+/* the tile, plus (want_frag) the sensitivity plane (5th): how far the equivalent forms, or
+ * the same terms in double precision, move an output (relative for reflectivity and
+ * coherence, radians for the phase), and flag 8 where that exceeds 1e-4 */
 static int nlinsar_tile(const float* tin, float* tout, int th, int tw, void* vctx)
 {
     nlinsar_ctx* cx = (nlinsar_ctx*)vctx;
+    int rc = nlinsar_tile_impl(tin, tout, th, tw, cx);
+    if (rc || !cx->want_frag) return rc;
+    const size_t n = (size_t)th * tw;
+    float* alt = (float*)calloc(3 * n, sizeof(float));
+    for (int mode = 1; mode <= 2 && rc == 0; mode++) {
+    nlinsar_ctx ac = *cx;
+    ac.want_frag = 0;
+    ac.alt = mode;
+    rc = nlinsar_tile_impl(tin, alt, th, tw, &ac);
+    const int overlap = (cx->sw - 1) / 2 + (cx->p - 1) / 2;
+    for (int h = overlap; rc == 0 && h < th - overlap; h++)
+        for (int w = overlap; w < tw - overlap; w++) {
+            const size_t i = (size_t)h * tw + w;
+            const double r0 = tout[i], r1 = alt[i], c0 = tout[2 * n + i], c1 = alt[2 * n + i];
+            double dph = fabs((double)tout[n + i] - (double)alt[n + i]);
+            if (dph > M_PI) dph = 2 * M_PI - dph;
+            const double er = fabs(r1 - r0) / fabs(r0), ec = fabs(c1 - c0) / fabs(c0);
+            double sens = er > ec ? er : ec;
+            if (dph > sens) sens = dph;
+            if (!(sens <= 1e-4)) tout[3 * n + i] = (float)((int)tout[3 * n + i] | 8);
+            if (!(sens <= tout[4 * n + i])) tout[4 * n + i] = isnan(sens) ? INFINITY : (float)sens;
+        }
+    }
+    free(alt);
+    return rc;
+}
+
+static int nlinsar_tile_impl(const float* tin, float* tout, int th, int tw, nlinsar_ctx* cx)
+{
     const int sw = cx->sw, p = cx->p;
     const int psh = (p - 1) / 2, wsh = (sw - 1) / 2, overlap = wsh + psh;
     const size_t n = (size_t)th * tw;
@@ -1284,8 +1404,12 @@ static int nlinsar_tile(const float* tin, float* tout, int th, int tw, void* vct
                     const size_t ci = (size_t)(h + wsh) * tw + w + wsh;
                     const size_t wi = (size_t)(h + hh) * tw + w + ww;
                     const size_t o = (size_t)(hh * sw + ww) * splane + (size_t)h * w_sim + w;
-                    sims[o] = orc_nlinsar_pixel_similarity(a1[ci], a2[ci], dp[ci], a1[wi], a2[wi], dp[wi]);
-                    kls[o] = orc_nlinsar_pixel_kl(ref[ci], pha[ci], coh[ci], ref[wi], pha[wi], coh[wi]);
+                    sims[o] = cx->alt == 1 ? alt_similarity(a1[ci], a2[ci], dp[ci], a1[wi], a2[wi], dp[wi])
+                            : cx->alt == 2 ? dbl_similarity(a1[ci], a2[ci], dp[ci], a1[wi], a2[wi], dp[wi])
+                                           : orc_nlinsar_pixel_similarity(a1[ci], a2[ci], dp[ci], a1[wi], a2[wi], dp[wi]);
+                    kls[o] = cx->alt == 1 ? alt_kl(ref[ci], pha[ci], coh[ci], ref[wi], pha[wi], coh[wi])
+                           : cx->alt == 2 ? dbl_kl(ref[ci], pha[ci], coh[ci], ref[wi], pha[wi], coh[wi])
+                                          : orc_nlinsar_pixel_kl(ref[ci], pha[ci], coh[ci], ref[wi], pha[wi], coh[wi]);
                 }
         }
     /* 2nd pass, precompute_similarities_2nd_pass.cl:19-50 (launched only when the tile is
@@ -1323,7 +1447,8 @@ static int nlinsar_tile(const float* tin, float* tout, int th, int tw, void* vct
     /* weights, nlinsar compute_weights.cl:19-30; centre weight 0 (nlinsar_sub_image.cpp:189-197) */
     float* wts = (float*)malloc(sizeof(float) * nfull * oplane);
     for (size_t i = 0; i < (size_t)nfull * oplane; i++) {
-        float weight = expf(psim[i] / cx->h + pkl[i] / cx->T);
+        float weight = cx->alt == 1 ? expf(psim[i] * (1.0f / cx->h) + pkl[i] * (1.0f / cx->T))
+                                    : expf(psim[i] / cx->h + pkl[i] / cx->T);
         if (isnan(weight) || isinf(weight)) weight = 0.0f;
         wts[i] = weight;
     }
@@ -1343,6 +1468,33 @@ static int nlinsar_tile(const float* tin, float* tout, int th, int tw, void* vct
     float* wt = (float*)malloc(sizeof(float) * nfull * oplane);
     for (size_t i = 0; i < oplane; i++)
         for (int k = 0; k < nfull; k++) wt[i * nfull + k] = wts[(size_t)k * oplane + i];
+    /* fragility flags (optional 4th output plane): ENL within a relative 1e-4 of lmin (1),
+     * the smoothing's kept set (2, 4; smoothing_fragility), weights near underflow (16: the
+     * float ENL, finite, more than 1e-3 from the ENL of the same weights in double, or on the
+     * other side of lmin, as happens when the squares of tiny weights lose their bits; or
+     * every weight subnormal) */
+    if (cx->want_frag)
+        for (int h = 0; h < h_ori; h++)
+            for (int w = 0; w < w_ori; w++) {
+                const float e = nols[(size_t)h * w_ori + w];
+                int flag = fabsf(e - (float)cx->lmin) <= 1e-4f * (float)cx->lmin ? 1 : 0;
+                const float* wr = wt + ((size_t)h * w_ori + w) * nfull;
+                double dws = 0.0, dws2 = 0.0;
+                float wmax = 0.0f;
+                for (int k = 0; k < nfull; k++) {
+                    dws += wr[k];
+                    dws2 += (double)wr[k] * wr[k];
+                    wmax = fmaxf(wmax, wr[k]);
+                }
+                const double denl = dws * dws / dws2;
+                if ((isfinite(e) && !(fabs(e - denl) <= 1e-3 * denl)) || ((e < cx->lmin) != (denl < cx->lmin)) ||
+                    (wmax > 0.0f && wmax < 1e-36f))
+                    flag |= 16;
+                if (e < cx->lmin)
+                    flag |= smoothing_fragility(a1, wt + ((size_t)h * w_ori + w) * nfull, h, w,
+                                                w_ori + 2 * overlap, p, sw, cx->lmin);
+                tout[3 * n + (size_t)(h + overlap) * tw + (w + overlap)] = (float)flag;
+            }
     /* smoothing::run, smoothing.cpp:39-76 */
     for (int h = 0; h < h_ori; h++)
         for (int w = 0; w < w_ori; w++)
@@ -1407,10 +1559,20 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
                 float* phase_filt, float* coh_filt, int height, int width, int sw, int p,
                 int niter, int lmin, int tile, long long* n_smoothed)
 {
+    return orc_nlinsar_frag(a1, a2, phi, ref_filt, phase_filt, coh_filt, NULL, NULL, height, width,
+                            sw, p, niter, lmin, tile, n_smoothed);
+}
+
+int orc_nlinsar_frag(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                     float* phase_filt, float* coh_filt, float* frag, float* sens, int height,
+                     int width, int sw, int p, int niter, int lmin, int tile,
+                     long long* n_smoothed)
+{
     if (height <= 0 || width <= 0 || sw < 3 || sw % 2 == 0 || p < 1 || p % 2 == 0 || niter < 0)
         return -1;
     if (sw * sw > 1024) return -1;
-    nlinsar_ctx cx = {sw, p, lmin, 0, 0, 0};
+    nlinsar_ctx cx = {sw, p, lmin, 0, 0, 0, frag != NULL || sens != NULL, 0};
+    const int nplanes = cx.want_frag ? 5 : 3;
     if (orc_nlinsar_params(p, &cx.h, &cx.T)) return -4;
     if (tile <= 0) tile = 80; /* nlinsar.cpp:106 */
     const int overlap = (p - 1) / 2 + (sw - 1) / 2;
@@ -1418,7 +1580,7 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
     const size_t n = (size_t)height * width;
     /* total_image (6 planes) and the per-iteration deep copy, nlinsar.cpp:59-133 */
     float* img = (float*)malloc(sizeof(float) * 6 * n);
-    float* tmp = (float*)malloc(sizeof(float) * 3 * n);
+    float* tmp = (float*)calloc((size_t)5 * n, sizeof(float));
     memcpy(img, a1, n * sizeof(float));
     memcpy(img + n, a2, n * sizeof(float));
     memcpy(img + 2 * n, phi, n * sizeof(float));
@@ -1443,7 +1605,7 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
             const int h_low = -overlap + (t / ntw) * step;
             const int w_low = -overlap + (t % ntw) * step;
             float* tin = (float*)malloc(sizeof(float) * 6 * (size_t)tile * tile);
-            outs[t] = (float*)malloc(sizeof(float) * 3 * (size_t)tile * tile);
+            outs[t] = (float*)calloc((size_t)nplanes * tile * tile, sizeof(float));
             get_sub_image(img, height, width, 6, h_low, w_low, tile, tile, tin);
             nlinsar_ctx lc = cx;
             lc.nsmooth = 0;
@@ -1457,7 +1619,7 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
             if (rc == 0) {
                 const int h_low = -overlap + (t / ntw) * step;
                 const int w_low = -overlap + (t % ntw) * step;
-                write_interior(tmp, height, width, 3, outs[t], h_low, w_low, tile, tile, overlap);
+                write_interior(tmp, height, width, nplanes, outs[t], h_low, w_low, tile, tile, overlap);
             }
             free(outs[t]);
         }
@@ -1468,6 +1630,8 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi, float* ref_f
         memcpy(ref_filt, img + 3 * n, n * sizeof(float));
         memcpy(phase_filt, img + 4 * n, n * sizeof(float));
         memcpy(coh_filt, img + 5 * n, n * sizeof(float));
+        if (frag) memcpy(frag, tmp + 3 * n, n * sizeof(float));
+        if (sens) memcpy(sens, tmp + 4 * n, n * sizeof(float));
     }
     if (n_smoothed) *n_smoothed = cx.nsmooth;
     free(tmp);

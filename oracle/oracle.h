@@ -85,14 +85,15 @@ int orc_nlsar_insar_cond(const float* a1, const float* a2, const float* phi, flo
                          long long* ub_pixels);
 
 /* same, also writing (each optional) the conditioning map and every (patch, scale) key's
- * bias-corrected ENL and alpha (enls_nobias / alphas of nlsar_filter_sub_image.cpp:224-228,
- * 325-327): [nP*nS][H*W] floats, key index scale_index*nP + patch_index.  The statistics
- * must cover P x S and may hold further keys (see the selection in oracle.c). */
+ * bias-corrected ENL, alpha (enls_nobias / alphas of nlsar_filter_sub_image.cpp:224-228,
+ * 325-327) and conditioning: [nP*nS][H*W] floats, key index scale_index*nP + patch_index.
+ * The statistics must cover P x S and may hold further keys (see the selection in oracle.c). */
 int orc_nlsar_insar_maps(const float* a1, const float* a2, const float* phi, float* ref_filt,
                          float* phase_filt, float* coh_filt, float* cond, float* enl_maps,
-                         float* alpha_maps, int height, int width, int sw, const int* P, int nP,
-                         const int* S, int nS, const orc_stats* st, int nst, float hpar,
-                         float cpar, int tile_h, int tile_w, long long* ub_pixels);
+                         float* alpha_maps, float* cond_maps, int height, int width, int sw,
+                         const int* P, int nP, const int* S, int nS, const orc_stats* st,
+                         int nst, float hpar, float cpar, int tile_h, int tile_w,
+                         long long* ub_pixels);
 
 /* despeckcl::nlsar (covariance overload), nlsar.cpp:171-194; covmats are 2*dim*dim planes. */
 int orc_nlsar_covmat(const float* covmat_raw, float* covmat_filt, int height, int width, int dim,
@@ -105,10 +106,10 @@ int orc_nlsar_covmat_cond(const float* covmat_raw, float* covmat_filt, float* co
                           int tile_w, long long* ub_pixels);
 
 int orc_nlsar_covmat_maps(const float* covmat_raw, float* covmat_filt, float* cond,
-                          float* enl_maps, float* alpha_maps, int height, int width, int dim,
-                          int sw, const int* P, int nP, const int* S, int nS, const orc_stats* st,
-                          int nst, float hpar, float cpar, int tile_h, int tile_w,
-                          long long* ub_pixels);
+                          float* enl_maps, float* alpha_maps, float* cond_maps, int height,
+                          int width, int dim, int sw, const int* P, int nP, const int* S, int nS,
+                          const orc_stats* st, int nst, float hpar, float cpar, int tile_h,
+                          int tile_w, long long* ub_pixels);
 
 /* ---- brute force (brute.c): each output pixel straight from the definitions, the check of
  * the composition above.  flip != 0 reverses every pair's argument order (tests only). ---- */
@@ -146,6 +147,22 @@ int orc_nlinsar(const float* a1, const float* a2, const float* phi,
                 float* ref_filt, float* phase_filt, float* coh_filt,
                 int height, int width, int search_window_size, int patch_size, int niter,
                 int lmin, int tile, long long* n_smoothed);
+/* same; frag (optional, H x W) receives the last iteration's fragility flags of each pixel:
+ * 1 = its ENL lies within a relative 1e-4 of lmin, 2 = the lmin smoothing's kept threshold
+ * is tied or within a relative 1e-4 of the next candidate weight, 4 = that threshold is a
+ * tiny weight (< 1e-30) among tiny ones (there a last-bit difference of the weights changes
+ * which weights the pixel uses); 8 = evaluating the similarity and KL terms in
+ * algebraically equivalent forms (cos of a difference as products, pow(q, 1.5) as q sqrt q,
+ * quotients through reciprocals; a few ulps each), or in double precision, moves one of its
+ * outputs by more than 1e-4 (the pixel is ill-conditioned: near-identical pixel pairs make
+ * the similarity's A - B and asin(sqrt(B / A)) cancel); 16 = weights near underflow (its
+ * finite float ENL more than 1e-3 from the ENL of the same weights in double, or on the other
+ * side of lmin, or every weight subnormal).  sens (optional, H x W): the largest such move of an output
+ * (relative for reflectivity and coherence, radians for the phase). */
+int orc_nlinsar_frag(const float* a1, const float* a2, const float* phi, float* ref_filt,
+                     float* phase_filt, float* coh_filt, float* frag, float* sens, int height,
+                     int width, int sw, int p, int niter, int lmin, int tile,
+                     long long* n_smoothed);
 /* h and T used by the NL-InSAR weights, nlinsar.cpp:65-78 (patch 3 and 7 only) */
 int orc_nlinsar_params(int patch_size, float* h_para, float* T_para);
 /* search_window_smoothing on one window, smoothing.cpp:78-129 */

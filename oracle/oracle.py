@@ -119,7 +119,7 @@ def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats
                 h=15.0, c=49.0, tile=(0, 0), init=None, want_cond=False, want_maps=False):
     """Returns (ref, phase, coh, ub_pixels), plus the conditioning map when want_cond
     (per pixel min over the diagonal of var / I_nl^2 of the selected parameter), plus
-    (enl_maps, alpha_maps) of shape (n_patch*n_scale, H, W), key index
+    (enl_maps, alpha_maps, cond_maps) of shape (n_patch*n_scale, H, W), key index
     scale_index*n_patch + patch_index, when want_maps."""
     a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
     H, W = a1.shape
@@ -128,11 +128,12 @@ def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats
     nk = len(patch_sizes) * len(scale_sizes)
     enl = np.zeros((nk, H, W), np.float32) if want_maps else None
     alp = np.zeros((nk, H, W), np.float32) if want_maps else None
+    cnd = np.zeros((nk, H, W), np.float32) if want_maps else None
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
     rc = lib().orc_nlsar_insar_maps(
         _vp(a1), _vp(a2), _vp(phi), _vp(out[0]), _vp(out[1]), _vp(out[2]), _vp(cond), _vp(enl),
-        _vp(alp), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+        _vp(alp), _vp(cnd), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
         _ints(patch_sizes), C.c_int(len(patch_sizes)), _ints(scale_sizes), C.c_int(len(scale_sizes)),
         arr, C.c_int(len(stats)), C.c_float(h), C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]),
         C.byref(ub))
@@ -142,15 +143,15 @@ def nlsar_insar(a1, a2, phi, search_window_size, patch_sizes, scale_sizes, stats
     if want_cond:
         res.append(cond)
     if want_maps:
-        res += [enl, alp]
+        res += [enl, alp, cnd]
     return tuple(res)
 
 
 def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stats,
                  h=15.0, c=49.0, tile=(0, 0), want_cond=False, want_maps=False):
     """covmat: (2*dim*dim, H, W) interlaced planes.  Returns (filtered planes, ub_pixels),
-    plus the conditioning map (as nlsar_insar) when want_cond, plus (enl_maps, alpha_maps)
-    when want_maps."""
+    plus the conditioning map (as nlsar_insar) when want_cond, plus (enl_maps, alpha_maps,
+    cond_maps) when want_maps."""
     cov = _f32(covmat)
     H, W = cov.shape[-2:]
     out = np.zeros_like(cov)
@@ -158,10 +159,11 @@ def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stat
     nk = len(patch_sizes) * len(scale_sizes)
     enl = np.zeros((nk, H, W), np.float32) if want_maps else None
     alp = np.zeros((nk, H, W), np.float32) if want_maps else None
+    cnd = np.zeros((nk, H, W), np.float32) if want_maps else None
     arr, keep = _stats_array(stats)
     ub = C.c_longlong(0)
     rc = lib().orc_nlsar_covmat_maps(
-        _vp(cov), _vp(out), _vp(cond), _vp(enl), _vp(alp), C.c_int(H), C.c_int(W),
+        _vp(cov), _vp(out), _vp(cond), _vp(enl), _vp(alp), _vp(cnd), C.c_int(H), C.c_int(W),
         C.c_int(dim), C.c_int(search_window_size), _ints(patch_sizes), C.c_int(len(patch_sizes)),
         _ints(scale_sizes), C.c_int(len(scale_sizes)), arr, C.c_int(len(stats)), C.c_float(h),
         C.c_float(c), C.c_int(tile[0]), C.c_int(tile[1]), C.byref(ub))
@@ -171,7 +173,7 @@ def nlsar_covmat(covmat, dim, search_window_size, patch_sizes, scale_sizes, stat
     if want_cond:
         res.append(cond)
     if want_maps:
-        res += [enl, alp]
+        res += [enl, alp, cnd]
     return tuple(res)
 
 
@@ -319,18 +321,24 @@ def training_dissims(covmat, dim, patch_size, scale_size):
     return out[:n]
 
 
-def nlinsar(a1, a2, phi, search_window_size, patch_size, niter, lmin, init=None, tile=80):
+def nlinsar(a1, a2, phi, search_window_size, patch_size, niter, lmin, init=None, tile=80,
+            want_frag=False):
+    """Returns (ref, phase, coh, n_smoothed), plus the last iteration's fragility flags
+    (uint8) and output sensitivity (see orc_nlinsar_frag) when want_frag."""
     a1, a2, phi = _f32(a1), _f32(a2), _f32(phi)
     H, W = a1.shape
     out = [np.zeros((H, W), np.float32) for _ in range(3)] if init is None else [_f32(x).copy() for x in init]
+    frag = np.zeros((H, W), np.float32) if want_frag else None
+    sens = np.zeros((H, W), np.float32) if want_frag else None
     ns = C.c_longlong(0)
-    rc = lib().orc_nlinsar(
-        a1.ctypes.data_as(C.c_void_p), a2.ctypes.data_as(C.c_void_p), phi.ctypes.data_as(C.c_void_p),
-        out[0].ctypes.data_as(C.c_void_p), out[1].ctypes.data_as(C.c_void_p),
-        out[2].ctypes.data_as(C.c_void_p), C.c_int(H), C.c_int(W), C.c_int(search_window_size),
+    rc = lib().orc_nlinsar_frag(
+        _vp(a1), _vp(a2), _vp(phi), _vp(out[0]), _vp(out[1]), _vp(out[2]), _vp(frag), _vp(sens),
+        C.c_int(H), C.c_int(W), C.c_int(search_window_size),
         C.c_int(patch_size), C.c_int(niter), C.c_int(lmin), C.c_int(tile), C.byref(ns))
     if rc != 0:
         raise ValueError(f"oracle nlinsar failed ({rc})")
+    if want_frag:
+        return out[0], out[1], out[2], ns.value, frag.astype(np.uint8), sens
     return out[0], out[1], out[2], ns.value
 
 
