@@ -51,6 +51,8 @@ SIGNATURES = {
     "dkx_nlsar_insar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F]),
     "dkx_nlsar_insar_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I,
                                     _F, _F, _P, _P, _ULLP, _P]),
+    "dkx_nlsar_insar_device_noalloc": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _IP, _I, _IP, _I, _SP,
+                                            _I, _F, _F, _P, _P, _ULLP, _P]),
     "dkx_nlsar_covmat": (_I, [_P, _P, _I, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F]),
     "dkx_nlsar_covmat_device": (_I, [_P, _P, _I, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F,
                                      _P, _P, _ULLP, _P]),

@@ -239,16 +239,9 @@ int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP,
 }
 
 // Everything that can be checked without a device: argument checks, statistics, plan.
-struct NlsarCall {
-    std::vector<NlsarStatsView> views;
-    NlsarParams pp;
-    NlsarCommon cm;
-    NlsarImagePlan ip;
-};
-
 // all_keys: compute every key's statistics (the ENL / alpha maps and the undefined-selection
 // count need them); the plan depends on it (the wide pass variant serves the selected key only)
-static int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int width, int sw,
+int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int width, int sw,
                          const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
                          int n_stats, float h, float c)
 {

@@ -97,6 +97,18 @@ struct NlsarImageDesc {
 int check_stats(const dkx_nlsar_stats* stats, int n_stats, const int* P, int nP, const int* S,
                 int nS, std::vector<NlsarStatsView>& views, int* last_unused);
 
+// Everything about one NL-SAR call that can be checked and planned without a device.
+struct NlsarCall {
+    std::vector<NlsarStatsView> views;
+    NlsarParams pp;
+    NlsarCommon cm;
+    NlsarImagePlan ip;
+};
+// dim 0: InSAR input, 2 / 3: covariance input.  all_keys as NlsarParams::all_keys.
+int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int width, int sw,
+                  const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                  int n_stats, float h, float c);
+
 std::vector<float> gen_gauss(int scale_size);
 std::vector<float> weight_lut(const NlsarStatsView& st, float h, float c);
 

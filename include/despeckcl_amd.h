@@ -69,6 +69,24 @@ int dkx_nlsar_insar_device(const float* d_ampl_master, const float* d_ampl_slave
                            float h_param, float c_param, float* d_enl_maps, float* d_alpha_maps,
                            unsigned long long* ub_pixels, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): the same filter
+ * and outputs as dkx_nlsar_insar_device, with every scratch buffer (padded covariance and
+ * scale-average grids, flags, the undefined-selection count, the weight LUTs) in one
+ * 64 MiB device array of the library, one per device, held until the process ends; the call
+ * allocates nothing.  Images up to about 1200 x 1200 pixels fit (search window 21, patch
+ * sizes up to 7, scale sizes up to 3); larger ones return DKX_ERESOURCE.  The array keeps the
+ * last call's contents and is shared by every call on the device: calls that overlap on one
+ * device race, so the caller orders them (one stream, or events). */
+int dkx_nlsar_insar_device_noalloc(const float* d_ampl_master, const float* d_ampl_slave,
+                                   const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                                   float* d_coh_filt, int height, int width,
+                                   int search_window_size, const int* patch_sizes,
+                                   int n_patch_sizes, const int* scale_sizes, int n_scale_sizes,
+                                   const dkx_nlsar_stats* stats, int n_stats, float h_param,
+                                   float c_param, float* d_enl_maps, float* d_alpha_maps,
+                                   unsigned long long* ub_pixels, void* stream);
+
 /* despeckcl::nlsar, covariance overload (include/despeckcl.h:61-72;
  * src/compute/nlsar/nlsar.cpp:170-193).  covmat / covmat_filt: 2*dim*dim planes of
  * height x width, real and imaginary parts of each matrix element interlaced in row-major
