@@ -12,6 +12,8 @@
 #include "dkx_common.h"
 #include "dkx_nlsar.h"
 
+#include <type_traits>
+
 namespace dkx {
 
 // Patch sets are compile-time bit masks of patch half widths (bit k: patch size 2k+1), so
@@ -393,16 +395,14 @@ void nlsar_scale_pass(const ScaleArgs a)
             }
         }
     };
-    auto step3 = [&](const int dy, const int dx, float* sR, float* sW) {
+    auto step3_slot = [&](const int dy, const int dx, float* sR, float* sW) {
         [[maybe_unused]] const int adx = dx < 0 ? -dx : dx;
         // union of the tile (for w(c, d)) and the tile shifted by -d (for w(c, -d))
         [[maybe_unused]] const int RY = TY - dy, RX = TX + adx;
         [[maybe_unused]] const int rxo = dx > 0 ? -dx : 0;   // region x origin relative to tx0
         [[maybe_unused]] const int DY = RY + 2 * P, DX = RX + 2 * P;
-        // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for
-        //    runs of 4 rows in pairs of columns, and the LUT weight
-        //    (compute_weights.cl:19-56).  Element i: run ry, column pair xi = i - ry * nrx,
-        //    at LDS byte offset ry * (16 row bytes - 8 nrx) + 8 i.
+        // 3. (one patch size, one offset slot at a time) column sums and LUT weights as in
+        //    step3 below, element i of the slot taken by thread i.
         {
             const int nrx = (RX + 1) >> 1;
             const int nry = (RY + 3) >> 2;
@@ -444,6 +444,87 @@ void nlsar_scale_pass(const ScaleArgs a)
                     }
                 }
             }
+        }
+    };
+    // 3. column sums (patch_similarities_col_pass.cl:52-60), top to bottom, for runs of 4
+    //    rows in pairs of columns, and the LUT weight (compute_weights.cl:19-56), of every
+    //    offset slot of the phase and every patch size.  The (slot, patch) groups of
+    //    elements are laid end to end, each padded to whole waves, so every wave of the
+    //    block takes a share and works on one group (its patch size is a compile-time
+    //    branch, uniform in the wave).  Element j of a group: run ry, column pair
+    //    xi = j - ry * nrx, at LDS byte offset ry * (16 row bytes - 8 nrx) + 8 j.
+    auto step3 = [&](const int* oys, const int* oxs) {
+        constexpr int NG = OPB * NP;
+        int n1[OPB], base[NG];
+        unsigned int ostep[OPB];
+        float inv[OPB];
+        int total = 0;
+#pragma unroll
+        for (int o = 0; o < OPB; o++) {
+            const int ady = oys[o] < 0 ? -oys[o] : oys[o];
+            const int adx = oxs[o] < 0 ? -oxs[o] : oxs[o];
+            const int RY = TY + ady, RX = TX + adx;
+            const int nrx = (RX + 1) >> 1;
+            n1[o] = ((RY + 3) >> 2) * nrx;
+            inv[o] = nlsar_rcp(nrx);
+            ostep[o] = (unsigned int)(16 * RXM - 8 * nrx);
+#pragma unroll
+            for (int k = 0; k < NP; k++) {
+                base[o * NP + k] = total;
+                total += (n1[o] + 63) & ~63;
+            }
+        }
+        auto body = [&](auto kc, const int o, const unsigned int offb) {
+            constexpr int k = decltype(kc)::value;
+            constexpr int psh = pmask_bit(PMASK, k);
+            float* sR = smem + Lay::off_R(NP, OPB, 0) + o * (Lay::off_R(NP, OPB, 1) - Lay::off_R(NP, OPB, 0));
+            float* sW = smem + o * (Lay::off_D(NP, 1) - Lay::off_D(NP, 0));
+            const float* src = reinterpret_cast<const float*>(reinterpret_cast<const char*>(sR + k * RPL) + offb);
+            float w0[W2], w1[W2];
+#pragma unroll
+            for (int t = 0; t < W2; t++) {
+                const float2 v = *reinterpret_cast<const float2*>(src + t * RXM);
+                w0[t] = v.x;
+                w1[t] = v.y;
+            }
+            const float* lut = sL + k * a.lut_stride;
+            float* dst = reinterpret_cast<float*>(reinterpret_cast<char*>(sW + k * WPL) + offb);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                float s0 = w0[m + P - psh], s1 = w1[m + P - psh];
+#pragma unroll
+                for (int jj = -P + 1; jj <= P; jj++)
+                    if (jj > -psh && jj <= psh) {
+                        s0 = s0 + w0[m + P + jj];
+                        s1 = s1 + w1[m + P + jj];
+                    }
+                const float x0 = lut_weight(s0, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                const float x1 = lut_weight(s1, a.dmin[k], a.dmax[k], a.rng[k], a.rcp[k], a.scl[k], lut);
+                *reinterpret_cast<float2*>(dst + m * RXM) = make_float2(x0, x1);
+            }
+        };
+        constexpr int N1MAX = (((TX + WMAX + 3) / 4) * ((TX + WMAX + 1) / 2) + 63) & ~63;
+        constexpr int KC = (NG * N1MAX + NT - 1) / NT;
+#pragma unroll
+        for (int kk = 0; kk < KC; kk++) {
+            const int e = tid + kk * NT;
+            // the wave's first element: groups start at multiples of 64, so the group is
+            // the same for the whole wave
+            const int ew = uni(e) & ~63;
+            if (ew >= total) break;
+            int g = 0;
+#pragma unroll
+            for (int q = 1; q < NG; q++)
+                if (ew >= base[q]) g = q;
+            const int o = OPB == 1 ? 0 : g / NP;
+            const int k = g - o * NP;
+            const int j = e - base[g];
+            if (j >= n1[o]) continue;
+            const unsigned int ry = (unsigned int)(((float)j + 0.5f) * inv[o]);
+            const unsigned int offb = __umul24(ry, ostep[o]) + 8u * (unsigned int)j;
+            if (k == 0) body(std::integral_constant<int, 0>{}, o, offb);
+            if constexpr (NP > 1) if (k == 1) body(std::integral_constant<int, (NP > 1 ? 1 : 0)>{}, o, offb);
+            if constexpr (NP > 2) if (k == 2) body(std::integral_constant<int, (NP > 2 ? 2 : 0)>{}, o, offb);
         }
     };
     auto step4 = [&](const int dy, const int dx, float* sW) {
@@ -586,8 +667,14 @@ void nlsar_scale_pass(const ScaleArgs a)
 #pragma unroll
         for (int o = 0; o < OPB; o++) step2(oy[o], ox[o], smem + Lay::off_D(NP, o), smem + Lay::off_R(NP, OPB, o));
         __syncthreads();
+        // a single patch size keeps the per-slot form (its two slots spread over the block's
+        // waves measured 3 % slower); several patch sizes are spread over every wave
+        if constexpr (NP == 1) {
 #pragma unroll
-        for (int o = 0; o < OPB; o++) step3(oy[o], ox[o], smem + Lay::off_R(NP, OPB, o), smem + Lay::off_D(NP, o));
+            for (int o = 0; o < OPB; o++) step3_slot(oy[o], ox[o], smem + Lay::off_R(NP, OPB, o), smem + Lay::off_D(NP, o));
+        } else {
+            step3(oy, ox);
+        }
         __syncthreads();
 #pragma unroll
         for (int o = 0; o < OPB; o++) step4(oy[o], ox[o], smem + Lay::off_D(NP, o));
