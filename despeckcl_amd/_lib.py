@@ -85,6 +85,12 @@ SIGNATURES = {
     "dkx_nlsar_batch_get_tile": (_I, [_P]),
     "dkx_nlsar_batch_autotune": (_I, [_P, _P, _I, _IP, _P]),
     "dkx_nlsar_batch_ub_pixels": (_I, [_P, _ULLP]),
+    "dkx_nlinsar_batch_create": (_I, [C.POINTER(C.c_void_p)]),
+    "dkx_nlinsar_batch_destroy": (None, [_P]),
+    "dkx_nlinsar_batch_set": (_I, [_P, C.POINTER(DkxImage), _I, _I, _I, _I, _I]),
+    "dkx_nlinsar_batch_enqueue": (_I, [_P, _P]),
+    "dkx_nlinsar_batch_graph_launch": (_I, [_P, _P]),
+    "dkx_nlinsar_batch_smoothed": (_I, [_P, _ULLP]),
     "dkx_server_create": (_I, [C.c_ulonglong, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F, _I, _I, _I, _I,
                                C.POINTER(C.c_void_p)]),
     "dkx_server_submit": (_I, [_P, C.POINTER(DkxRequest), C.c_double, C.POINTER(C.c_longlong)]),

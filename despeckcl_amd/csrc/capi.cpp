@@ -293,8 +293,8 @@ struct NlinsarCall {
     NlinsarPlan plan;
 };
 
-static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int p, int niter,
-                           int lmin)
+int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, int tile,
+                    NlinsarProblem* pb)
 {
     // nlinsar.cpp:65-78
     float h_theo;
@@ -308,7 +308,15 @@ static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int
     const int patch_area = p * p;
     const float h_para = h_theo * patch_area;
     const float T_para = (float)(2.0 / h_para * M_PI / 4 * patch_area);
-    call.pb = NlinsarProblem{height, width, sw, p, niter, lmin, h_para, T_para, 80, g_tile[1].load()};
+    *pb = NlinsarProblem{height, width, sw, p, niter, lmin, h_para, T_para, 80, tile};
+    return 0;
+}
+
+static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int p, int niter,
+                           int lmin)
+{
+    int rc = nlinsar_problem(height, width, sw, p, niter, lmin, g_tile[1].load(), &call.pb);
+    if (rc) return rc;
     return nlinsar_plan(call.pb, &call.plan);
 }
 

@@ -3,6 +3,8 @@
 
 #include "dkx_common.h"
 
+#include <vector>
+
 namespace dkx {
 
 struct NlinsarProblem {
@@ -30,10 +32,49 @@ struct NlinsarPlan {
 };
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);
+
+// The reference's parameters for a search window, patch size, iteration count and lmin
+// (h and T from the patch size, 80-pixel tiles), with the given tile shape.
+int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, int tile,
+                    NlinsarProblem* pb);
+
 // d_ref/d_phase/d_coh: in = first-iteration estimates, out = result.
 int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* d_a1,
                 const float* d_a2, const float* d_phi, float* d_ref, float* d_phase,
                 float* d_coh, void* d_ws, unsigned long long* n_smoothed_host,
                 hipStream_t stream);
+
+
+// ---- batches: many images with the same parameters, each stage one launch over all ----
+
+// One image of a batch: device arrays and its workspace (NlinsarPlan::ws_bytes).
+struct NlinsarBatchItem {
+    const float* a1; const float* a2; const float* phi;
+    float* ref; float* phase; float* coh;   // in: first-iteration estimates, out: the result
+    int H, W;
+    char* ws;
+};
+
+// What a batch's launches need on the host, and the layout of its device tables.
+struct NlinsarBatchLaunch {
+    int nimg, niter, tile, psh, sw, lmin;
+    int total_tiles;        // weights tiles of all images
+    int persistent_blocks;  // grid of the persistent weights kernel
+    long long max_n;        // pixels of the largest image
+    int heap_block;
+    size_t off_img, off_w, off_s, off_q, bytes;   // device tables, relative to their base
+};
+
+// Per-image plans of a batch (all with the batch's tile shape; their ws_bytes give each
+// image's workspace) and the launch description.  common: the parameters (H, W unused).
+int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W, int n,
+                       std::vector<NlinsarPlan>& plans, NlinsarBatchLaunch* L);
+// Host image of the device tables for tables placed at d_tables (L.bytes of them).
+int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<NlinsarPlan>& plans,
+                         const NlinsarBatchItem* items, const NlinsarBatchLaunch& L,
+                         char* d_tables, std::vector<unsigned char>& host);
+// Enqueues every iteration of every image on `stream` (the tables must be on the device).
+int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch& L,
+                          char* d_tables, hipStream_t stream);
 
 }  // namespace dkx

@@ -13,6 +13,7 @@
 #include "dkx_nlinsar.h"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 #include <cmath>
 
@@ -21,12 +22,10 @@ namespace dkx {
 // The estimates as the KL divergence uses them (nlinsar/sim_measures.h:42-49): per element
 // of the padded grid E, float4 (ref, coh * cos phase, coh * sin phase,
 // u = 1 / (ref * (1 - coh^2))).
-__global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const float* __restrict__ pha,
-                                       const float* __restrict__ coh, int H, int W, Grid E)
+__device__ __forceinline__ void nl_estimate_record(const float* __restrict__ ref, const float* __restrict__ pha,
+                                                   const float* __restrict__ coh, int H, int W,
+                                                   const Grid& E, int gy, int gx)
 {
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int gy = blockIdx.y;
-    if (gx >= E.w || gy >= E.h) return;
     const int y = min(H - 1, max(0, gy - E.oy));
     const int x = min(W - 1, max(0, gx - E.ox));
     const long long i = (long long)y * W + x;
@@ -35,16 +34,22 @@ __global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const floa
     reinterpret_cast<float4*>(E.base)[o] = make_float4(r, c * cosf(p), c * sinf(p), 1.0f / (r * (1.0f - c * c)));
 }
 
-// The raw data on the padded grid R: float4 (a1, a2, cos phase, sin phase), clamped like the
-// reference's tile padding (src/data/sub_images.cpp:37-40); a1 alone also into A (same
-// element layout as R) for the smoothing's amplitude test.
-__global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __restrict__ a2,
-                                 const float* __restrict__ phi, int H, int W, Grid R,
-                                 float* __restrict__ A)
+__global__ void nlinsar_fill_estimates(const float* __restrict__ ref, const float* __restrict__ pha,
+                                       const float* __restrict__ coh, int H, int W, Grid E)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
-    if (gx >= R.w || gy >= R.h) return;
+    if (gx >= E.w || gy >= E.h) return;
+    nl_estimate_record(ref, pha, coh, H, W, E, gy, gx);
+}
+
+// The raw data on the padded grid R: float4 (a1, a2, cos phase, sin phase), clamped like the
+// reference's tile padding (src/data/sub_images.cpp:37-40); a1 alone also into A (same
+// element layout as R) for the smoothing's amplitude test.
+__device__ __forceinline__ void nl_raw_record(const float* __restrict__ a1, const float* __restrict__ a2,
+                                              const float* __restrict__ phi, int H, int W,
+                                              const Grid& R, float* __restrict__ A, int gy, int gx)
+{
     const int y = min(H - 1, max(0, gy - R.oy));
     const int x = min(W - 1, max(0, gx - R.ox));
     const long long i = (long long)y * W + x;
@@ -53,17 +58,32 @@ __global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __re
     A[(long long)gy * R.pitch + gx] = a1[i];
 }
 
+__global__ void nlinsar_fill_raw(const float* __restrict__ a1, const float* __restrict__ a2,
+                                 const float* __restrict__ phi, int H, int W, Grid R,
+                                 float* __restrict__ A)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gy = blockIdx.y;
+    if (gx >= R.w || gy >= R.h) return;
+    nl_raw_record(a1, a2, phi, H, W, R, A, gy, gx);
+}
+
 // precompute_filter_values.cl:19-39: a = (a1^2 + a2^2)/2, x = a1*a2*exp(-i*phase), with
 // cos(-phase) = cos(phase) and sin(-phase) = -sin(phase)
+__device__ __forceinline__ void nl_filter_record(const Grid& R, const Grid& F, int gy, int gx)
+{
+    const float4 r = reinterpret_cast<const float4*>(R.base)[R.at(gy - F.oy, gx - F.ox)];
+    const float a1 = r.x, a2 = r.y;
+    reinterpret_cast<float4*>(F.base)[(long long)gy * F.pitch + gx] =
+        make_float4(0.5f * (a1 * a1 + a2 * a2), a1 * a2 * r.z, a1 * a2 * -r.w, 0.f);
+}
+
 __global__ void nlinsar_filter_values(Grid R, Grid F)
 {
     const int gx = blockIdx.x * blockDim.x + threadIdx.x;
     const int gy = blockIdx.y;
     if (gx >= F.w || gy >= F.h) return;
-    const float4 r = reinterpret_cast<const float4*>(R.base)[R.at(gy - F.oy, gx - F.ox)];
-    const float a1 = r.x, a2 = r.y;
-    reinterpret_cast<float4*>(F.base)[(long long)gy * F.pitch + gx] =
-        make_float4(0.5f * (a1 * a1 + a2 * a2), a1 * a2 * r.z, a1 * a2 * -r.w, 0.f);
+    nl_filter_record(R, F, gy, gx);
 }
 
 // flag[0] = 1 if any reflectivity estimate is nonzero: otherwise every KL divergence is
@@ -182,6 +202,7 @@ struct WeightArgs {
     float* new_coh;
     int* list;
     unsigned int* cnt;
+    int block0;           // batch: first tile of this image in the batch's tile order
 };
 
 constexpr int NLI_BAND = 4;   // block rows per band of the block order
@@ -213,7 +234,7 @@ __device__ __forceinline__ float nl_pixel_kl_sym(float4 e1, float4 e2, float sec
 //   outputs:  threads tid < NG*TX (NG row groups) own TY/NG vertically adjacent pixels of one
 //             column, slot tid + m*NG*TX in the block's similarity / weight chunk.
 template <int TY, int TX, int PSH, bool FIRST, int NTW>
-__global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
+__device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const int t)
 {
     // output row groups: as many as the threads allow, dividing TY
     constexpr int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
@@ -250,14 +271,9 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     // reference tile row * nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile
     // of a reference tile is cut at the tile's end.  The one-row / one-column mappings have a
     // single block row / column starting at the mapping's origin.
-    // Linear block ids are remapped so that the blocks the hardware sends to one XCD (id
-    // mod 8) take a contiguous run of tiles, ordered column by column within bands of
-    // NLI_BAND block rows: the blocks resident on one XCD then cover a compact patch of the
-    // image, whose neighbourhoods overlap in that XCD's L2.
-    const int nb = a.bstart[4];
-    const int bq = nb >> 3, br = nb & 7;
-    const int xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
-    const int t = xcd * bq + min(xcd, br) + kk;   // tile index (also the chunk index)
+    // Tile t (the launch gives each XCD a contiguous run of tiles) is ordered column by
+    // column within bands of NLI_BAND block rows: the blocks resident on one XCD then cover
+    // a compact patch of the image, whose neighbourhoods overlap in that XCD's L2.
     int m = 0;
     while (m < 3 && t >= a.bstart[m + 1]) m++;
     const Launch L = m == 0 ? a.L[0] : m == 1 ? a.L[1] : m == 2 ? a.L[2] : a.L[3];
@@ -589,6 +605,48 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     }
 }
 
+// One image per launch, one tile per block.  Linear block ids are remapped so that the
+// blocks the hardware sends to one XCD (id mod 8) take a contiguous run of tiles.
+template <int TY, int TX, int PSH, bool FIRST, int NTW>
+__global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
+{
+    const int nb = a.bstart[4];
+    const int bq = nb >> 3, br = nb & 7;
+    const int xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW>(a, xcd * bq + min(xcd, br) + kk);
+}
+
+// A batch of images (table: one entry per image, the images' tiles end to end from
+// entry.block0, `total` in all) on a persistent grid: each block takes tiles from the queue
+// of its XCD (block id mod 8) until that queue is empty.  Queue x holds the x-th eighth of
+// the tiles, a contiguous run, like the block order of a single image; the grid has at least
+// eight blocks, so every queue is drained.  queue: 8 counters, zero before the launch.
+template <int TY, int TX, int PSH, bool FIRST, int NTW>
+__global__ __launch_bounds__(NTW) void nlinsar_weights_batch(const WeightArgs* __restrict__ table, int nimg,
+                                                             int total, unsigned int* __restrict__ queue)
+{
+    __shared__ int s_item;
+    const int bq = total >> 3, br = total & 7;
+    const int xcd = blockIdx.x & 7;
+    const int mine = bq + (xcd < br ? 1 : 0), first = xcd * bq + min(xcd, br);
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(queue + xcd, 1u);
+        __syncthreads();
+        const int k = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (k >= mine) break;
+        const int T = first + k;
+        // the image holding tile T: the last entry with block0 <= T
+        int lo = 0, hi = nimg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].block0 <= T) lo = mid;
+            else hi = mid - 1;
+        }
+        nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW>(table[lo], T - table[lo].block0);
+    }
+}
+
 struct SmoothArgs {
     Grid R, F;
     int wsh;
@@ -606,6 +664,14 @@ struct SmoothArgs {
     unsigned int* cnt;  // [0] queued this iteration, [1] running total, [3] exact queue
     int* xlist;         // entries of list that need the exact queue
     const float* A;     // a1 on R's element layout
+};
+
+// The smoothing launches: one image (`one`) or a batch (`table`, one entry per image); the
+// blocks work through the queue of every image in turn.
+struct SmoothBatch {
+    const SmoothArgs* table;
+    SmoothArgs one;
+    int nimg;
 };
 
 // std::priority_queue<weight_el> of smoothing.cpp:27-36 (top = smallest weight): sift
@@ -808,13 +874,15 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 // the candidate flags of each run of CH offsets in LDS (entry c of a thread at
 // c * blockDim.x), which the second pass reads instead of the amplitudes.
 template <int CH>
-__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a, unsigned int qlo, unsigned int qhi)
+__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
 {
     using Flags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
     static_assert(CH <= 32, "candidate flags of a run are kept in one 32-bit word");
     extern __shared__ float4 smem_flags[];
+    for (int img = 0; img < b.nimg; img++) {
+    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[0];
-    if (total < qlo || total >= qhi) return;
+    if (total < qlo || total >= qhi) continue;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
     const int lmin = a.lmin;
@@ -914,16 +982,19 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothArgs a, unsign
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
     }
+    }
 }
 
 // The pixels nlinsar_smooth queued for the exact heap (ties at the smallest kept weight);
 // the variant with runs of CH offsets serves exact-queue lengths in [qlo, qhi).
 template <int CH>
-__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a, unsigned int qlo, unsigned int qhi)
+__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
 {
     extern __shared__ float smem[];
+    for (int img = 0; img < b.nimg; img++) {
+    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[3];
-    if (total < qlo || total >= qhi) return;
+    if (total < qlo || total >= qhi) continue;
     const int bs = blockDim.x;
     float* hw = smem + threadIdx.x;
     int* hi = reinterpret_cast<int*>(smem + a.lmin * bs + threadIdx.x);
@@ -936,6 +1007,7 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a, 
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
     }
+    }
 }
 
 // Short queues, one wave per pixel.  Per thread, the chain of dependent load runs of the
@@ -946,11 +1018,13 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothArgs a, 
 // arithmetic of nl_smooth_exact.  One block is one wave.  TIES: the pixels of the exact
 // queue that nlinsar_smooth left (when that queue is short), instead of the whole queue.
 template <bool TIES>
-__global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothArgs a, unsigned int qhi)
+__global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, unsigned int qhi)
 {
     extern __shared__ float4 smem_wave[];
+    for (int img = 0; img < b.nimg; img++) {
+    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = TIES ? a.cnt[3] : a.cnt[0];
-    if (total >= qhi) return;
+    if (total >= qhi) continue;
     if (!TIES && blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
@@ -1058,6 +1132,7 @@ __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothArgs a, un
                         a.new_coh);
         }
         __syncthreads();
+    }
     }
 }
 
@@ -1184,6 +1259,9 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->ws_bytes = off;
     return 0;
 }
+
+static int launch_smoothing(const SmoothBatch& sb, int sw, int lmin, int heap_block, long long n, int ncu,
+                            hipStream_t stream);
 
 template <int TY, int TX, int PSH, bool FIRST, int NT>
 static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t stream)
@@ -1313,43 +1391,12 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         sa.list = list; sa.cnt = cnt;
         sa.xlist = (int*)(ws + plan.off_xlist);
         sa.A = (const float*)(ws + plan.off_A);
-        const size_t lds = (size_t)plan.heap_block * 2 * pb.lmin * 4;
-        const int nfull = pb.sw * pb.sw;
-        // candidate flags: one byte per run of NLI_SCH offsets; blocks are narrowed until
-        // they fit 48 KB
-        const size_t fl = (size_t)(nfull + NLI_SCH - 1) / NLI_SCH;
-        int fbs = 256;
-        while (fbs > 64 && fbs * fl > 48 * 1024) fbs /= 2;
-        const int blocks = std::max(1, std::min((int)((n + fbs - 1) / fbs), ncu * 8));
-        const int xblocks = std::max(1, std::min((int)((n + plan.heap_block - 1) / plan.heap_block), ncu * 8));
-        // one wave per pixel for short queues: window in LDS, then the heap
-        const size_t wlds = 4 * (4 * (size_t)((nfull + 3) & ~3) + (size_t)(nfull + 63) / 64 * 2 + 2 * (size_t)pb.lmin);
-        const int wblocks = (int)std::min<size_t>(n, NLI_QWAVE);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing_on()) {
-            DKX_HIP_CHECK(hipEventCreate(&e0));
-            DKX_HIP_CHECK(hipEventCreate(&e1));
-            DKX_HIP_CHECK(hipEventRecord(e0, stream));
-        }
-        // the variants split the queue lengths between them: each returns at once outside
-        // its range (the lengths are known on the device only)
-        if (wlds > 64 * 1024) { set_error("nlinsar: search window too large for the smoothing"); return -2; }
-        // short queue: one wave per pixel; else the register selection, whose tied pixels go
-        // to the waves again when they are few and to the per-thread heap otherwise
-        hipLaunchKernelGGL(nlinsar_smooth_wave<false>, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
-        DKX_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(nlinsar_smooth<NLI_SCH>, dim3(blocks), dim3(fbs), fbs * fl, stream, sa,
-                           NLI_QWAVE, 0xffffffffu);
-        DKX_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(nlinsar_smooth_wave<true>, dim3(wblocks), dim3(64), wlds, stream, sa, NLI_QWAVE);
-        DKX_HIP_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(nlinsar_smooth_exact<XSCH>, dim3(xblocks), dim3(plan.heap_block), lds, stream, sa,
-                           NLI_QWAVE, 0xffffffffu);
-        DKX_HIP_CHECK(hipGetLastError());
-        if (e0) {
-            DKX_HIP_CHECK(hipEventRecord(e1, stream));
-            timing_push(STAGE_NLINSAR_SMOOTH, e0, e1);
-        }
+        SmoothBatch sb;
+        memset(&sb, 0, sizeof(sb));
+        sb.one = sa;
+        sb.nimg = 1;
+        rc = launch_smoothing(sb, pb.sw, pb.lmin, plan.heap_block, (long long)n, ncu, stream);
+        if (rc) return rc;
         // the estimates just written become the next iteration's input
         for (int k = 0; k < 3; k++) {
             float* t = nxt[k];
@@ -1373,6 +1420,375 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         DKX_HIP_CHECK(hipMemcpyAsync(&tot, cnt + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
         DKX_HIP_CHECK(hipStreamSynchronize(stream));
         *n_smoothed_host = tot;
+    }
+    return 0;
+}
+
+
+// ---- batches ---------------------------------------------------------------------------
+// Many images with the same parameters: every stage is one launch over all images (the
+// elementwise stages on a (rows, image) grid, the weights on the persistent tile queue, the
+// smoothing variants looping over the images' queues), so a batch of small images fills the
+// GPU as one large image does.  Each image's stages, arithmetic and summation orders are the
+// ones of nlinsar_run on that image alone with the batch's tile shape, so its result is the
+// same bits.
+
+// Per image: inputs, caller arrays (first-iteration estimates in, result out), its grids.
+struct NliImage {
+    const float* a1;
+    const float* a2;
+    const float* phi;
+    float* io[3];
+    long long n;
+    int H, W;
+    Grid R, E, F;
+    float* A;
+    unsigned int* cnt;    // [0] queued, [1] running total, [2] estimate flag, [3] exact queue
+};
+
+// The estimate arrays per iteration: iteration 0 reads the caller's arrays and writes set 0;
+// odd iterations read set 0 and write set 1; even ones from 2 on read set 1 and write set 0
+// (the rotation of nlinsar_run).  Weight / smoothing tables exist per kind.
+static int iteration_kind(int it) { return it == 0 ? 0 : ((it & 1) ? 1 : 2); }
+
+__global__ void nlinsar_b_clear(const NliImage* __restrict__ im, int nimg, unsigned int* __restrict__ queue,
+                                int clear_total)
+{
+    for (int i = threadIdx.x; i < nimg; i += blockDim.x) {
+        unsigned int* c = im[i].cnt;
+        c[0] = 0u;
+        if (clear_total) c[1] = 0u;
+        c[2] = 0u;
+        c[3] = 0u;
+    }
+    if (threadIdx.x < 8) queue[threadIdx.x] = 0u;
+}
+
+// Grid (x, image): blocks x, x + gridDim.x, ... take the rows of their image.
+__global__ __launch_bounds__(256) void nlinsar_b_fill_raw(const NliImage* __restrict__ im)
+{
+    const NliImage& g = im[blockIdx.y];
+    for (int gy = blockIdx.x; gy < g.R.h; gy += gridDim.x)
+        for (int gx = threadIdx.x; gx < g.R.w; gx += blockDim.x)
+            nl_raw_record(g.a1, g.a2, g.phi, g.H, g.W, g.R, g.A, gy, gx);
+}
+
+__global__ __launch_bounds__(256) void nlinsar_b_filter_values(const NliImage* __restrict__ im)
+{
+    const NliImage& g = im[blockIdx.y];
+    for (int gy = blockIdx.x; gy < g.F.h; gy += gridDim.x)
+        for (int gx = threadIdx.x; gx < g.F.w; gx += blockDim.x)
+            nl_filter_record(g.R, g.F, gy, gx);
+}
+
+// The estimates that the iteration of `wt` reads (wt[i].old_*) on image i's grid E.
+__global__ __launch_bounds__(256) void nlinsar_b_fill_estimates(const NliImage* __restrict__ im,
+                                                                const WeightArgs* __restrict__ wt)
+{
+    const NliImage& g = im[blockIdx.y];
+    const WeightArgs& a = wt[blockIdx.y];
+    for (int gy = blockIdx.x; gy < g.E.h; gy += gridDim.x)
+        for (int gx = threadIdx.x; gx < g.E.w; gx += blockDim.x)
+            nl_estimate_record(a.old_ref, a.old_pha, a.old_coh, g.H, g.W, g.E, gy, gx);
+}
+
+// nlinsar_any_estimate per image
+__global__ __launch_bounds__(256) void nlinsar_b_any_estimate(const NliImage* __restrict__ im,
+                                                              const WeightArgs* __restrict__ wt)
+{
+    const NliImage& g = im[blockIdx.y];
+    const float* ref = wt[blockIdx.y].old_ref;
+    bool nz = false;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < g.n; k += (long long)gridDim.x * blockDim.x)
+        nz |= ref[k] != 0.0f;
+    if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(g.cnt + 2, 1u);
+}
+
+// The last iteration's estimates (wt[i].new_*) into the caller's arrays.
+__global__ __launch_bounds__(256) void nlinsar_b_copy_out(const NliImage* __restrict__ im,
+                                                          const WeightArgs* __restrict__ wt)
+{
+    const NliImage& g = im[blockIdx.y];
+    const WeightArgs& a = wt[blockIdx.y];
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < g.n; k += (long long)gridDim.x * blockDim.x) {
+        g.io[0][k] = a.new_ref[k];
+        g.io[1][k] = a.new_pha[k];
+        g.io[2][k] = a.new_coh[k];
+    }
+}
+
+static int weights_tile_shape(int tile, int* TY, int* TX)
+{
+    *TY = kTY[tile];
+    *TX = kTX[tile];
+    return 0;
+}
+
+int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W, int n,
+                       std::vector<NlinsarPlan>& plans, NlinsarBatchLaunch* L)
+{
+    if (n < 1) { set_error("nlinsar batch: no images"); return -1; }
+    if (n > 65535) { set_error("nlinsar batch: at most 65535 images"); return -1; }
+    long long px = 0, max_n = 0;
+    for (int i = 0; i < n; i++) {
+        if (H[i] <= 0 || W[i] <= 0) { set_error("nlinsar batch: empty image " + std::to_string(i)); return -1; }
+        px += (long long)H[i] * W[i];
+        max_n = std::max(max_n, (long long)H[i] * W[i]);
+    }
+    // the batch's tile shape: the single-image rule applied to the batch's pixel count
+    NlinsarProblem pb = common;
+    pb.H = (int)std::min<long long>(px, 1ll << 20);
+    pb.W = 1;
+    NlinsarPlan probe;
+    int rc = nlinsar_plan(pb, &probe);
+    if (rc) return rc;
+    const int tile = probe.tile;
+    plans.assign(n, NlinsarPlan{});
+    int total = 0;
+    for (int i = 0; i < n; i++) {
+        pb = common;
+        pb.H = H[i];
+        pb.W = W[i];
+        pb.tile = tile;
+        rc = nlinsar_plan(pb, &plans[i]);
+        if (rc) return rc;
+        if ((long long)total + plans[i].blocks >= (1ll << 30)) { set_error("nlinsar batch: too many tiles"); return -4; }
+        total += plans[i].blocks;
+    }
+    L->nimg = n;
+    L->niter = common.niter;
+    L->tile = tile;
+    L->psh = probe.psh;
+    L->sw = common.sw;
+    L->lmin = common.lmin;
+    L->total_tiles = total;
+    L->persistent_blocks = 0;   // set by the first enqueue (it needs the device)
+    L->max_n = max_n;
+    L->heap_block = probe.heap_block;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t q = off; off += (bytes + 255) / 256 * 256; return q; };
+    L->off_img = take(sizeof(NliImage) * (size_t)n);
+    L->off_w = take(sizeof(WeightArgs) * 3 * (size_t)n);
+    L->off_s = take(sizeof(SmoothArgs) * 3 * (size_t)n);
+    L->off_q = take(sizeof(unsigned int) * 8);
+    L->bytes = off;
+    return 0;
+}
+
+int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<NlinsarPlan>& plans,
+                         const NlinsarBatchItem* items, const NlinsarBatchLaunch& L,
+                         char* d_tables, std::vector<unsigned char>& host)
+{
+    const int n = L.nimg;
+    host.assign(L.bytes, 0);
+    NliImage* img = reinterpret_cast<NliImage*>(host.data() + L.off_img);
+    WeightArgs* wt = reinterpret_cast<WeightArgs*>(host.data() + L.off_w);
+    SmoothArgs* st = reinterpret_cast<SmoothArgs*>(host.data() + L.off_s);
+    int TY, TX;
+    weights_tile_shape(L.tile, &TY, &TX);
+    int block0 = 0;
+    for (int i = 0; i < n; i++) {
+        const NlinsarPlan& plan = plans[i];
+        const NlinsarBatchItem& it = items[i];
+        char* ws = it.ws;
+        const long long npx = (long long)it.H * it.W;
+        NliImage& g = img[i];
+        g.a1 = it.a1; g.a2 = it.a2; g.phi = it.phi;
+        g.io[0] = it.ref; g.io[1] = it.phase; g.io[2] = it.coh;
+        g.n = npx;
+        g.H = it.H; g.W = it.W;
+        g.R = plan.R; g.E = plan.E; g.F = plan.F;
+        g.R.base = (float*)(ws + plan.off_R);
+        g.E.base = (float*)(ws + plan.off_E);
+        g.F.base = (float*)(ws + plan.off_F);
+        g.A = (float*)(ws + plan.off_A);
+        g.cnt = (unsigned int*)(ws + plan.off_cnt);
+        float* est = (float*)(ws + plan.off_est);
+        float* set0[3] = {est, est + npx, est + 2 * npx};
+        float* set1[3] = {est + 3 * npx, est + 4 * npx, est + 5 * npx};
+        for (int kind = 0; kind < 3; kind++) {
+            float* const* cur = kind == 0 ? g.io : (kind == 1 ? set0 : set1);
+            float* const* nxt = kind == 1 ? set1 : set0;
+            WeightArgs& a = wt[kind * n + i];
+            a.R4 = reinterpret_cast<const float4*>(g.R.base);
+            a.E4 = reinterpret_cast<const float4*>(g.E.base);
+            a.F4 = reinterpret_cast<const float4*>(g.F.base);
+            a.rpitch = g.R.pitch;
+            a.rorg = g.R.oy * g.R.pitch + g.R.ox;
+            a.fpitch = g.F.pitch;
+            a.forg = g.F.oy * g.F.pitch + g.F.ox;
+            a.wsh = plan.wsh; a.step = plan.step;
+            a.wsim = common.ref_tile - common.sw + 1;
+            a.rh = 1.0f / common.h; a.rT = 1.0f / common.T; a.lmin = (float)common.lmin;
+            a.H = it.H; a.W = it.W;
+            nlinsar_mappings(it.H, it.W, plan.eh, plan.ew, a.L);
+            const int nb = nlinsar_grid(plan.step, TY, TX, a.L, a.gx, a.nsx, a.nsy, a.bstart);
+            if (nb != plan.blocks) { set_error("nlinsar batch: inconsistent tile grid"); return -1; }
+            a.scache = (float*)(ws + plan.off_s);
+            a.wt = (float*)(ws + plan.off_w);
+            a.kl_on = g.cnt + 2;
+            a.old_ref = cur[0]; a.old_pha = cur[1]; a.old_coh = cur[2];
+            a.new_ref = nxt[0]; a.new_pha = nxt[1]; a.new_coh = nxt[2];
+            a.list = (int*)(ws + plan.off_list);
+            a.cnt = g.cnt;
+            a.block0 = block0;
+            SmoothArgs& sa = st[kind * n + i];
+            sa.R = g.R; sa.F = g.F; sa.wsh = plan.wsh; sa.lmin = common.lmin;
+            sa.H = it.H; sa.W = it.W; sa.eh = plan.eh; sa.ew = plan.ew;
+            sa.npix = TY * TX;
+            sa.wt = a.wt;
+            sa.old_ref = cur[0]; sa.old_pha = cur[1]; sa.old_coh = cur[2];
+            sa.new_ref = nxt[0]; sa.new_pha = nxt[1]; sa.new_coh = nxt[2];
+            sa.list = a.list; sa.cnt = g.cnt;
+            sa.xlist = (int*)(ws + plan.off_xlist);
+            sa.A = g.A;
+        }
+        block0 += plan.blocks;
+    }
+    (void)d_tables;
+    return 0;
+}
+
+template <int TY, int TX, int PSH, bool FIRST, int NT>
+static int launch_weights_batch(const NlinsarBatchLaunch& L, const WeightArgs* table, unsigned int* queue,
+                                int ncu, hipStream_t stream)
+{
+    const size_t lds = nlinsar_lds_bytes(TY, TX, (L.sw - 1) / 2, PSH);
+    if (lds + 16 > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
+    const void* fn = (const void*)nlinsar_weights_batch<TY, TX, PSH, FIRST, NT>;
+    DKX_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    DKX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds));
+    per_cu = std::max(per_cu, 1);
+    // every XCD queue needs a block: at least 8, at most what is resident at once
+    const int grid = std::max(8, std::min(per_cu * ncu, (L.total_tiles + 7) / 8 * 8));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timing_on();
+    if (timed) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    hipLaunchKernelGGL((nlinsar_weights_batch<TY, TX, PSH, FIRST, NT>), dim3(grid), dim3(NT), lds, stream, table,
+                       L.nimg, L.total_tiles, queue);
+    DKX_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
+    }
+    return 0;
+}
+
+template <int PSH, bool FIRST>
+static int launch_weights_batch_tile(const NlinsarBatchLaunch& L, const WeightArgs* table, unsigned int* queue,
+                                     int ncu, hipStream_t stream)
+{
+    switch (L.tile) {
+    case 0: return launch_weights_batch<27, 54, PSH, FIRST, 512>(L, table, queue, ncu, stream);
+    case 1: return launch_weights_batch<18, 54, PSH, FIRST, 512>(L, table, queue, ncu, stream);
+    default: return launch_weights_batch<54, 54, PSH, FIRST, 1024>(L, table, queue, ncu, stream);
+    }
+}
+
+// The four smoothing launches of one iteration (nlinsar_run's sequence); n: pixels of the
+// largest image.
+static int launch_smoothing(const SmoothBatch& sb, int sw, int lmin, int heap_block, long long n, int ncu,
+                            hipStream_t stream)
+{
+    const int nfull = sw * sw;
+    const size_t lds = (size_t)heap_block * 2 * lmin * 4;
+    // candidate flags: one byte per run of NLI_SCH offsets; blocks are narrowed until they
+    // fit 48 KB
+    const size_t fl = (size_t)(nfull + NLI_SCH - 1) / NLI_SCH;
+    int fbs = 256;
+    while (fbs > 64 && fbs * fl > 48 * 1024) fbs /= 2;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>((n + fbs - 1) / fbs, ncu * 8));
+    const int xblocks = (int)std::max<long long>(1, std::min<long long>((n + heap_block - 1) / heap_block, ncu * 8));
+    // one wave per pixel for short queues: window in LDS, then the heap
+    const size_t wlds = 4 * (4 * (size_t)((nfull + 3) & ~3) + (size_t)(nfull + 63) / 64 * 2 + 2 * (size_t)lmin);
+    const int wblocks = (int)std::min<long long>(n, NLI_QWAVE);
+    if (wlds > 64 * 1024) { set_error("nlinsar: search window too large for the smoothing"); return -2; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing_on()) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    // the variants split the queue lengths between them: each returns at once outside its
+    // range (the lengths are known on the device only).  Short queue: one wave per pixel;
+    // else the register selection, whose tied pixels go to the waves again when they are few
+    // and to the per-thread heap otherwise
+    hipLaunchKernelGGL(nlinsar_smooth_wave<false>, dim3(wblocks), dim3(64), wlds, stream, sb, NLI_QWAVE);
+    DKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(nlinsar_smooth<NLI_SCH>, dim3(blocks), dim3(fbs), fbs * fl, stream, sb, NLI_QWAVE,
+                       0xffffffffu);
+    DKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(nlinsar_smooth_wave<true>, dim3(wblocks), dim3(64), wlds, stream, sb, NLI_QWAVE);
+    DKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(nlinsar_smooth_exact<XSCH>, dim3(xblocks), dim3(heap_block), lds, stream, sb, NLI_QWAVE,
+                       0xffffffffu);
+    DKX_HIP_CHECK(hipGetLastError());
+    if (e0) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(STAGE_NLINSAR_SMOOTH, e0, e1);
+    }
+    return 0;
+}
+
+int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch& L,
+                          char* d_tables, hipStream_t stream)
+{
+    (void)common;
+    const int n = L.nimg;
+    const NliImage* img = reinterpret_cast<const NliImage*>(d_tables + L.off_img);
+    const WeightArgs* wt = reinterpret_cast<const WeightArgs*>(d_tables + L.off_w);
+    const SmoothArgs* st = reinterpret_cast<const SmoothArgs*>(d_tables + L.off_s);
+    unsigned int* queue = reinterpret_cast<unsigned int*>(d_tables + L.off_q);
+    int hdev = 0;
+    DKX_HIP_CHECK(hipGetDevice(&hdev));
+    int ncu = 256;
+    DKX_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hdev));
+    // (rows, image) grids of the elementwise stages: about 8 blocks per CU in all
+    const int bx = std::max(1, ncu * 8 / n);
+    const dim3 egrid(bx, n), eblk(256);
+    hipLaunchKernelGGL(nlinsar_b_clear, dim3(1), dim3(256), 0, stream, img, n, queue, 1);
+    DKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(nlinsar_b_fill_raw, egrid, eblk, 0, stream, img);
+    DKX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(nlinsar_b_filter_values, egrid, eblk, 0, stream, img);
+    DKX_HIP_CHECK(hipGetLastError());
+    for (int it = 0; it < L.niter; it++) {
+        const int kind = iteration_kind(it);
+        const WeightArgs* wk = wt + (size_t)kind * n;
+        hipLaunchKernelGGL(nlinsar_b_fill_estimates, egrid, eblk, 0, stream, img, wk);
+        DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_b_clear, dim3(1), dim3(256), 0, stream, img, n, queue, 0);
+        DKX_HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(nlinsar_b_any_estimate, egrid, eblk, 0, stream, img, wk);
+        DKX_HIP_CHECK(hipGetLastError());
+        int rc;
+        if (L.psh == 1)
+            rc = it == 0 ? launch_weights_batch_tile<1, true>(L, wk, queue, ncu, stream)
+                         : launch_weights_batch_tile<1, false>(L, wk, queue, ncu, stream);
+        else if (L.psh == 3)
+            rc = it == 0 ? launch_weights_batch_tile<3, true>(L, wk, queue, ncu, stream)
+                         : launch_weights_batch_tile<3, false>(L, wk, queue, ncu, stream);
+        else {
+            set_error("nlinsar: only patch sizes 3 and 7 are supported");
+            rc = -4;
+        }
+        if (rc) return rc;
+        SmoothBatch sb;
+        memset(&sb, 0, sizeof(sb));
+        sb.table = st + (size_t)kind * n;
+        sb.nimg = n;
+        rc = launch_smoothing(sb, L.sw, L.lmin, L.heap_block, L.max_n, ncu, stream);
+        if (rc) return rc;
+    }
+    if (L.niter > 0) {
+        hipLaunchKernelGGL(nlinsar_b_copy_out, egrid, eblk, 0, stream, img,
+                           wt + (size_t)iteration_kind(L.niter - 1) * n);
+        DKX_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }

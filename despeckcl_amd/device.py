@@ -146,6 +146,56 @@ class NlsarBatch:
         return [out[i] for i in range(self.n)]
 
 
+class NlinsarBatch:
+    """Batched NL-InSAR on device buffers: many images of any size, one parameter set, one
+    launch per stage over all images (dkx_nlinsar_batch_* in the C ABI).  The output arrays
+    hold the first iteration's estimates on entry (zero for a plain filter run), as for
+    dkx_nlinsar_device.
+
+        b = NlinsarBatch()
+        b.set(images, 21, 7, 5, 10)   # images: (a1, a2, phase, ref, phase_filt, coh, h, w)
+        b.enqueue(stream)             # or b.graph_launch(stream) to capture once and replay
+    """
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().dkx_nlinsar_batch_create(C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            lib().dkx_nlinsar_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, images, search_window_size, patch_size, niter, lmin):
+        from ._lib import DkxImage
+        arr = (DkxImage * len(images))()
+        for i, (a1, a2, ph, ref, phf, coh, hh, ww) in enumerate(images):
+            arr[i] = DkxImage(a1, a2, ph, ref, phf, coh, int(hh), int(ww))
+        check(lib().dkx_nlinsar_batch_set(self._h, arr, len(images), int(search_window_size), int(patch_size),
+                                          int(niter), int(lmin)))
+        self.n = len(images)
+
+    def enqueue(self, stream=0):
+        check(lib().dkx_nlinsar_batch_enqueue(self._h, stream or None))
+
+    def graph_launch(self, stream):
+        check(lib().dkx_nlinsar_batch_graph_launch(self._h, stream))
+
+    def smoothed(self):
+        """Per image, the pixel-iterations that went through the lmin smoothing (waits)."""
+        out = (C.c_ulonglong * max(self.n, 1))()
+        check(lib().dkx_nlinsar_batch_smoothed(self._h, out))
+        return [int(out[i]) for i in range(self.n)]
+
+
 class Server:
     """Request loop on the device (dkx_server_* in the C ABI): fixed HBM budget, resident
     NL-SAR LUTs, continuous batching of NL-SAR requests, NL-InSAR requests overlapped on

@@ -278,6 +278,35 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch batch, void* stream, int reps, int*
  * parameter selection is undefined (see dkx_nlsar_insar_device). */
 int dkx_nlsar_batch_ub_pixels(dkx_nlsar_batch batch, unsigned long long* counts);
 
+/* ---- batched NL-InSAR --------------------------------------------------------------------
+ * No counterpart in the reference, which filters one image per call (despeckcl::nlinsar,
+ * include/despeckcl.h:97-109) in 80 x 80 tiles, one tile after another with a host round trip
+ * per tile (src/compute/nlinsar/nlinsar.cpp:103-139).  For callers that filter many
+ * interferograms of any size with the same parameters: every stage runs as one launch over
+ * all images (the weights on a persistent grid working through a queue of tiles per XCD),
+ * and the whole iteration sequence can be captured in a HIP graph.  Each image's result
+ * equals the result of dkx_nlinsar_device on that image alone. */
+
+typedef dkx_nlsar_image dkx_nlinsar_image;   /* ref/phase/coh_filt: estimates in, result out */
+typedef struct dkx_nlinsar_batch_s* dkx_nlinsar_batch;
+
+int dkx_nlinsar_batch_create(dkx_nlinsar_batch* batch);
+void dkx_nlinsar_batch_destroy(dkx_nlinsar_batch batch);
+/* (Re)defines the images and the parameters (same meaning and checks as dkx_nlinsar).  Waits
+ * for the batch's previous run, grows its device workspace if needed and uploads the image
+ * tables (synchronous). */
+int dkx_nlinsar_batch_set(dkx_nlinsar_batch batch, const dkx_nlinsar_image* images, int n_images,
+                          int search_window_size, int patch_size, int niter, int lmin);
+/* Enqueues every iteration of every image on `stream` (a hipStream_t); capturable in a HIP
+ * graph. */
+int dkx_nlinsar_batch_enqueue(dkx_nlinsar_batch batch, void* stream);
+/* Captures the batch's stage sequence into a HIP graph on first use (after each _set), then
+ * replays it on `stream` (not the null stream). */
+int dkx_nlinsar_batch_graph_launch(dkx_nlinsar_batch batch, void* stream);
+/* Waits for the last run and reads each image's number of pixel-iterations that went
+ * through the lmin smoothing. */
+int dkx_nlinsar_batch_smoothed(dkx_nlinsar_batch batch, unsigned long long* counts);
+
 /* ---- serving ------------------------------------------------------------------------------
  * No counterpart in the reference (which has no request loop).  A server owns a fixed HBM
  * budget (one arena, sub-allocated per request: no allocation while serving), keeps the NL-SAR

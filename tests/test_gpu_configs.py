@@ -8,6 +8,8 @@
     iteration by iteration.
   Batched NL-SAR, the seeded 64 images with sides in [256, 2048]: every image bit for bit
     equal to its single call, eager and replayed from the HIP graph.
+  Batched NL-InSAR, ten images of uneven size: every image bit for bit equal to its single
+    call, eager and replayed from the HIP graph.
   Served stream, the seeded 2000 requests (70 % NL-SAR / 30 % NL-InSAR, sides in
     [128, 4096]) all at once: 100 seeded requests bit for bit equal to the same request
     filtered alone.
@@ -124,6 +126,70 @@ def test_batched_64_images_equal_single_calls(D, stats):
         for inp, out, single, _h, _w in imgs:
             for d in inp + out + single:
                 d.free()
+
+
+NLI_BATCH_SHAPES = [(1, 7), (5, 3), (64, 64), (80, 80), (81, 200), (300, 257), (512, 512), (97, 1000),
+                    (700, 130), (256, 256)]
+
+
+@pytest.mark.parametrize("sw,p,lmin,niter,tile", [(21, 7, 10, 3, -1), (21, 7, 10, 2, 2), (11, 3, 4, 2, -1)])
+def test_nlinsar_batch_equals_single_calls(D, sw, p, lmin, niter, tile):
+    """Ten images of uneven size (one row, a few pixels, up to 512 x 512), half of them from
+    nonzero first-iteration estimates: every image bit for bit equal to its single call, eager
+    and on two replays of the HIP graph; the smoothing counts too."""
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+    old = DV.get_tile(1)
+    DV.set_tile(1, tile)
+    imgs = []
+    try:
+        for i, (h, w) in enumerate(NLI_BATCH_SHAPES):
+            a1, a2, ph = S.interferogram(h, w, S.SEED_BASE + 5000 + i, n_regions=8, coh_range=(0.2, 0.95))
+            if i % 2:
+                init = [(0.5 * (a1 * a1 + a2 * a2) + 1e-3).astype(np.float32), ph.copy(),
+                        np.full((h, w), 0.3, np.float32)]
+            else:
+                init = [np.zeros((h, w), np.float32) for _ in range(3)]
+            imgs.append(([HIP.DeviceArray.from_host(x) for x in (a1, a2, ph)],
+                         [HIP.DeviceArray.from_host(x) for x in init],
+                         [HIP.DeviceArray.from_host(x) for x in init], init, h, w))
+        stream = HIP.Stream()
+        b = DV.NlinsarBatch()
+        try:
+            b.set([(*[d.ptr for d in inp], *[d.ptr for d in out], h, w) for inp, out, _s, _i, h, w in imgs],
+                  sw, p, niter, lmin)
+            b.enqueue(stream.handle)
+            stream.synchronize()
+            eager = [[d.download() for d in out] for _i, out, _s, _n, _h, _w in imgs]
+            counts = b.smoothed()
+            print("smoothed pixel-iterations", counts)
+            for i, (inp, _o, single, _n, h, w) in enumerate(imgs):
+                n_single = DV.nlinsar(*[d.ptr for d in inp], *[d.ptr for d in single], h, w, sw, p, niter,
+                                      lmin, stream=stream.handle, want_count=True)
+                assert counts[i] == n_single, f"image {i} ({h}x{w}) smoothing count"
+                for k in range(3):
+                    assert np.array_equal(eager[i][k], single[k].download(), equal_nan=True), \
+                        f"image {i} ({h}x{w}) output {k}"
+            for replay in range(2):
+                for _inp, out, _s, init, _h, _w in imgs:
+                    for d, x in zip(out, init):
+                        d.upload(x)
+                HIP.synchronize()
+                b.graph_launch(stream.handle)
+                stream.synchronize()
+                for i, (_inp, out, _s, _n, h, w) in enumerate(imgs):
+                    for k in range(3):
+                        assert np.array_equal(out[k].download(), eager[i][k], equal_nan=True), \
+                            f"graph replay {replay}, image {i} ({h}x{w}) output {k}"
+        finally:
+            b.close()
+    finally:
+        DV.set_tile(1, old)
+        for inp, out, single, _n, _h, _w in imgs:
+            for d in inp + out + single:
+                d.free()
+        D.lib().dkx_release_workspace()
 
 
 def test_served_stream_2000_requests(D, stats):
