@@ -464,16 +464,18 @@ def served_run(n_requests, rate, budget_gib):
     srv = DV.Server(budget_gib << 30, SW, PATCHES, SCALES, stats, HPAR, CPAR, 21, 7, 5, 10)
     srv.run_trace(reqs[:8], [0.0] * 8)       # warm-up
     lat_sat, span_sat = srv.run_trace(reqs, [0.0] * n)
-    b_sat = srv.counters()
+    b_sat, nb_sat = srv.counters(), srv.nlinsar_counters()
     log(f"served: saturated {span_sat:.2f} s")
     lat, span = srv.run_trace(reqs, arrivals.tolist())
-    b_all = srv.counters()
+    b_all, nb_all = srv.counters(), srv.nlinsar_counters()
     srv.close()
     log(f"served: Poisson stream {span:.2f} s")
 
     stream = HIP.Stream()
     t0 = time.perf_counter()
+    by_kind = [0.0, 0.0]
     for (k, a1, a2, ph, r1, r2, r3, h, w) in reqs:
+        tk = time.perf_counter()
         if k == 0:
             DV.nlsar_insar(a1, a2, ph, r1, r2, r3, h, w, SW, PATCHES, SCALES, stats, HPAR, CPAR, stream=stream.handle)
         else:
@@ -482,6 +484,7 @@ def served_run(n_requests, rate, budget_gib):
             HIP.hip().hipMemsetAsync(C_void(r3), 0, C_size(h * w * 4), C_void(stream.handle))
             DV.nlinsar(a1, a2, ph, r1, r2, r3, h, w, 21, 7, 5, 10, stream=stream.handle)
         stream.synchronize()
+        by_kind[k] += time.perf_counter() - tk
     serial = time.perf_counter() - t0
     D.lib().dkx_release_workspace()
     lat = np.array(lat)
@@ -499,9 +502,11 @@ def served_run(n_requests, rate, budget_gib):
         "stream_seconds": round(span, 3),
         "saturated": {"seconds": round(span_sat, 3), "mpx_s": round(px / 1e6 / span_sat, 3),
                       "p50_latency_ms": round(float(np.percentile(lat_sat, 50)), 3)},
-        "serial_plain": {"seconds": round(serial, 3), "mpx_s": round(px / 1e6 / serial, 3)},
+        "serial_plain": {"seconds": round(serial, 3), "mpx_s": round(px / 1e6 / serial, 3),
+                         "nlsar_seconds": round(by_kind[0], 3), "nlinsar_seconds": round(by_kind[1], 3)},
         "speedup_vs_serial": round(serial / span_sat, 3),
         "nlsar_batches": b_all[0] - b_sat[0], "nlsar_batched_images": b_all[1] - b_sat[1],
+        "nlinsar_batches": nb_all[0] - nb_sat[0], "nlinsar_batched_images": nb_all[1] - nb_sat[1],
     }
     return out
 

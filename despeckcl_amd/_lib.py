@@ -99,6 +99,7 @@ SIGNATURES = {
     "dkx_server_run_trace": (_I, [_P, C.POINTER(DkxRequest), C.POINTER(C.c_double), _I,
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "dkx_server_counters": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "dkx_server_nlinsar_counters": (_I, [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "dkx_server_destroy": (None, [_P]),
 }
 

@@ -73,8 +73,13 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
 int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<NlinsarPlan>& plans,
                          const NlinsarBatchItem* items, const NlinsarBatchLaunch& L,
                          char* d_tables, std::vector<unsigned char>& host);
-// Enqueues every iteration of every image on `stream` (the tables must be on the device).
+// Enqueues every iteration of every image on `stream` (the tables must be on the device);
+// zero_init: first clear the images' estimate arrays (a run from zero estimates).
 int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch& L,
-                          char* d_tables, hipStream_t stream);
+                          char* d_tables, hipStream_t stream, int zero_init = 0);
+// Bytes of the device tables of a batch of n images (NlinsarBatchLaunch::bytes).
+size_t nlinsar_batch_table_bytes(int n);
+// Upper bound of the workspace of an image under any tile shape a batch may pick.
+int nlinsar_batch_image_bytes(const NlinsarProblem& common, int H, int W, size_t* bytes);
 
 }  // namespace dkx

@@ -1504,6 +1504,17 @@ __global__ __launch_bounds__(256) void nlinsar_b_any_estimate(const NliImage* __
     if (__any(nz) && (threadIdx.x & 63) == 0) atomicOr(g.cnt + 2, 1u);
 }
 
+// Zero first-iteration estimates in the caller's arrays (a filter run from scratch).
+__global__ __launch_bounds__(256) void nlinsar_b_zero(const NliImage* __restrict__ im)
+{
+    const NliImage& g = im[blockIdx.y];
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < g.n; k += (long long)gridDim.x * blockDim.x) {
+        g.io[0][k] = 0.0f;
+        g.io[1][k] = 0.0f;
+        g.io[2][k] = 0.0f;
+    }
+}
+
 // The last iteration's estimates (wt[i].new_*) into the caller's arrays.
 __global__ __launch_bounds__(256) void nlinsar_b_copy_out(const NliImage* __restrict__ im,
                                                           const WeightArgs* __restrict__ wt)
@@ -1522,6 +1533,19 @@ static int weights_tile_shape(int tile, int* TY, int* TX)
     *TY = kTY[tile];
     *TX = kTX[tile];
     return 0;
+}
+
+// Device tables of a batch of n images: descriptors, weight and smoothing arguments of the
+// three iteration kinds, the XCD queues.
+static void batch_table_layout(int n, NlinsarBatchLaunch* L)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t q = off; off += (bytes + 255) / 256 * 256; return q; };
+    L->off_img = take(sizeof(NliImage) * (size_t)n);
+    L->off_w = take(sizeof(WeightArgs) * 3 * (size_t)n);
+    L->off_s = take(sizeof(SmoothArgs) * 3 * (size_t)n);
+    L->off_q = take(sizeof(unsigned int) * 8);
+    L->bytes = off;
 }
 
 int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W, int n,
@@ -1565,13 +1589,33 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
     L->persistent_blocks = 0;   // set by the first enqueue (it needs the device)
     L->max_n = max_n;
     L->heap_block = probe.heap_block;
-    size_t off = 0;
-    auto take = [&](size_t bytes) { size_t q = off; off += (bytes + 255) / 256 * 256; return q; };
-    L->off_img = take(sizeof(NliImage) * (size_t)n);
-    L->off_w = take(sizeof(WeightArgs) * 3 * (size_t)n);
-    L->off_s = take(sizeof(SmoothArgs) * 3 * (size_t)n);
-    L->off_q = take(sizeof(unsigned int) * 8);
-    L->bytes = off;
+    batch_table_layout(n, L);
+    return 0;
+}
+
+size_t nlinsar_batch_table_bytes(int n)
+{
+    NlinsarBatchLaunch L;
+    batch_table_layout(n, &L);
+    return L.bytes;
+}
+
+int nlinsar_batch_image_bytes(const NlinsarProblem& common, int H, int W, size_t* bytes)
+{
+    size_t best = 0;
+    for (int t = 0; t < 3; t++) {
+        if (common.tile != NLINSAR_TILE_AUTO && t != common.tile) continue;
+        if (common.tile == NLINSAR_TILE_AUTO && t == 0) continue;   // the automatic rule picks 1 or 2
+        NlinsarProblem pb = common;
+        pb.H = H;
+        pb.W = W;
+        pb.tile = t;
+        NlinsarPlan plan;
+        const int rc = nlinsar_plan(pb, &plan);
+        if (rc) return rc;
+        best = std::max(best, plan.ws_bytes);
+    }
+    *bytes = best;
     return 0;
 }
 
@@ -1736,7 +1780,7 @@ static int launch_smoothing(const SmoothBatch& sb, int sw, int lmin, int heap_bl
 }
 
 int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch& L,
-                          char* d_tables, hipStream_t stream)
+                          char* d_tables, hipStream_t stream, int zero_init)
 {
     (void)common;
     const int n = L.nimg;
@@ -1753,6 +1797,10 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
     const dim3 egrid(bx, n), eblk(256);
     hipLaunchKernelGGL(nlinsar_b_clear, dim3(1), dim3(256), 0, stream, img, n, queue, 1);
     DKX_HIP_CHECK(hipGetLastError());
+    if (zero_init) {
+        hipLaunchKernelGGL(nlinsar_b_zero, egrid, eblk, 0, stream, img);
+        DKX_HIP_CHECK(hipGetLastError());
+    }
     hipLaunchKernelGGL(nlinsar_b_fill_raw, egrid, eblk, 0, stream, img);
     DKX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(nlinsar_b_filter_values, egrid, eblk, 0, stream, img);

@@ -1,9 +1,9 @@
 // Serving: a request loop around the filters.  Requests of mixed filter type and image size
 // arrive at any time; a scheduler thread admits them against a fixed HBM budget (one arena,
-// sub-allocated per request, so serving never calls hipMalloc), batches the NL-SAR requests
-// that are waiting into one launch per stage (continuous batching: a new batch forms as soon
-// as a batch lane and memory are free, while earlier work still runs), runs NL-InSAR
-// requests on streams of their own so that their work overlaps the batches, and records each
+// sub-allocated per request, so serving never calls hipMalloc), batches the waiting requests
+// of each filter into one launch per stage (continuous batching: a new batch forms as soon as
+// a lane of that filter and memory are free, while earlier work still runs; the NL-SAR and
+// NL-InSAR lanes run on streams of their own, so their work overlaps), and records each
 // request's completion time from a host callback on its stream.  The weight LUTs and
 // Gaussian taps stay resident on the device for the server's lifetime.
 #include "../../include/despeckcl_amd.h"
@@ -86,7 +86,7 @@ struct Request {
 
 struct Server;
 
-// One unit of GPU work: an NL-SAR batch or one NL-InSAR request.
+// One unit of GPU work: an NL-SAR or an NL-InSAR batch.
 struct Flight {
     Server* srv = nullptr;
     int kind = 0, lane = 0;
@@ -96,8 +96,12 @@ struct Flight {
 };
 
 constexpr int SAR_LANES = 2;
-constexpr int NLI_LANES = 4;
+constexpr int NLI_LANES = 2;
 constexpr int MAX_BATCH = 256;
+constexpr int MAX_NLI_BATCH = 128;
+// an NL-InSAR batch stops growing once it holds this many pixels (one 4096 x 4096 image), so
+// that small requests do not wait behind a long batch while the other lane is free
+constexpr long long NLI_BATCH_PIXELS = 1ll << 24;
 
 struct Server {
     // NL-SAR parameters (owned copies) and resident constants
@@ -108,8 +112,7 @@ struct Server {
     NlsarCommon cm{};
     float* consts = nullptr;
     // NL-InSAR parameters
-    int nl_sw = 21, nl_p = 7, nl_niter = 5, nl_lmin = 10;
-    float nl_h = 0.f, nl_T = 0.f;
+    NlinsarProblem nl_pb{};
     // device memory and streams
     Arena arena;
     void* arena_ptr = nullptr;
@@ -117,6 +120,8 @@ struct Server {
     hipStream_t nli_stream[NLI_LANES] = {};
     NlsarImageDesc* pinned_table[SAR_LANES] = {};   // staging of a lane's batch table
     NlsarImageDesc* dev_table[SAR_LANES] = {};      // the table itself (MAX_BATCH entries)
+    unsigned char* nli_pinned[NLI_LANES] = {};     // staging of a lane's NL-InSAR tables
+    char* nli_table[NLI_LANES] = {};               // the tables (MAX_NLI_BATCH images)
     Flight* sar_busy[SAR_LANES] = {};
     Flight* nli_busy[NLI_LANES] = {};
     // queues
@@ -132,6 +137,7 @@ struct Server {
     std::string sched_error;
     // counters
     long long batches = 0, batched_images = 0;
+    long long nli_batches = 0, nli_batched_images = 0;
 };
 
 // Completion callback on a flight's stream.  Once `done` is set the scheduler may retire and
@@ -143,18 +149,6 @@ static void host_done(void* p)
     f->t_done.store(now_s());
     f->done.store(true);
     s->cv.notify_all();
-}
-
-static int nlinsar_params(int p, float* h, float* T)
-{
-    float h_theo;
-    if (p == 3) h_theo = 0.58f;
-    else if (p == 7) h_theo = 0.244f;
-    else return -1;
-    const int area = p * p;
-    *h = h_theo * area;
-    *T = (float)(2.0 / *h * M_PI / 4 * area);
-    return 0;
 }
 
 static void fail_request(Server* s, Request* r, const std::string& msg)
@@ -243,54 +237,78 @@ static void launch_sar(Server* s, int lane)
     s->sar_busy[lane] = f;
 }
 
-__global__ void serve_zero3(float* a, float* b, float* c, long long n)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        a[i] = 0.f;
-        b[i] = 0.f;
-        c[i] = 0.f;
-    }
-}
-
 static void launch_nli(Server* s, int lane)
 {
     if (s->wait_nli.empty()) return;
-    Request* r = s->wait_nli.front();
-    NlinsarProblem pb{r->r.height, r->r.width, s->nl_sw, s->nl_p, s->nl_niter, s->nl_lmin, s->nl_h, s->nl_T, 80,
-                      NLINSAR_TILE_AUTO};
-    NlinsarPlan plan;
-    if (nlinsar_plan(pb, &plan)) {
-        s->wait_nli.pop_front();
-        fail_request(s, r, last_error());
-        return;
-    }
-    size_t off;
-    if (!s->arena.alloc(plan.ws_bytes, &off)) return;   // wait for memory to free up
-    s->wait_nli.pop_front();
-    r->off = off;
-    r->bytes = plan.ws_bytes;
-    r->status = 1;
     Flight* f = new Flight();
     f->srv = s;
     f->kind = 1;
     f->lane = lane;
-    f->reqs.push_back(r);
-    hipStream_t st = s->nli_stream[lane];
-    const long long n = (long long)r->r.height * r->r.width;
-    // the first iteration starts from zero estimates, like the Python API
-    hipLaunchKernelGGL(serve_zero3, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, st,
-                       r->r.ref_filt, r->r.phase_filt, r->r.coh_filt, n);
-    int rc = hipGetLastError() == hipSuccess ? 0 : -5;
-    if (!rc) rc = nlinsar_run(pb, plan, r->r.ampl_master, r->r.ampl_slave, r->r.phase, r->r.ref_filt,
-                              r->r.phase_filt, r->r.coh_filt, s->arena.base + off, nullptr, st);
-    if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
-    if (rc) {
-        (void)hipStreamSynchronize(st);
-        s->arena.release(r->off, r->bytes);
-        fail_request(s, r, last_error());
+    std::vector<int> H, W;
+    long long px = 0;
+    while (!s->wait_nli.empty() && (int)f->reqs.size() < MAX_NLI_BATCH && px < NLI_BATCH_PIXELS) {
+        Request* r = s->wait_nli.front();
+        size_t bytes = 0;
+        if (nlinsar_batch_image_bytes(s->nl_pb, r->r.height, r->r.width, &bytes)) {
+            s->wait_nli.pop_front();
+            fail_request(s, r, last_error());
+            continue;
+        }
+        size_t off;
+        if (!s->arena.alloc(bytes, &off)) break;   // wait for memory to free up
+        s->wait_nli.pop_front();
+        r->off = off;
+        r->bytes = bytes;
+        r->status = 1;
+        f->reqs.push_back(r);
+        H.push_back(r->r.height);
+        W.push_back(r->r.width);
+        px += (long long)r->r.height * r->r.width;
+    }
+    if (f->reqs.empty()) {
         delete f;
         return;
     }
+    const int n = (int)f->reqs.size();
+    hipStream_t st = s->nli_stream[lane];
+    std::vector<NlinsarPlan> plans;
+    NlinsarBatchLaunch L;
+    int rc = nlinsar_batch_plan(s->nl_pb, H.data(), W.data(), n, plans, &L);
+    std::vector<NlinsarBatchItem> items(n);
+    for (int i = 0; i < n && !rc; i++) {
+        const Request* r = f->reqs[i];
+        if (plans[i].ws_bytes > r->bytes) {
+            set_error("serve: NL-InSAR workspace bound exceeded");
+            rc = -1;
+            break;
+        }
+        items[i] = NlinsarBatchItem{r->r.ampl_master, r->r.ampl_slave, r->r.phase, r->r.ref_filt, r->r.phase_filt,
+                                    r->r.coh_filt, r->r.height, r->r.width, s->arena.base + r->off};
+    }
+    std::vector<unsigned char> host;
+    if (!rc) rc = nlinsar_batch_tables(s->nl_pb, plans, items.data(), L, s->nli_table[lane], host);
+    // the lane's tables were reserved at creation; its previous batch has finished (the lane
+    // was free) and the stream orders the copy after it
+    if (!rc) {
+        memcpy(s->nli_pinned[lane], host.data(), host.size());
+        if (hipMemcpyAsync(s->nli_table[lane], s->nli_pinned[lane], host.size(), hipMemcpyHostToDevice, st) !=
+            hipSuccess)
+            rc = -5;
+    }
+    // every request starts from zero estimates, like the Python API
+    if (!rc) rc = nlinsar_batch_enqueue(s->nl_pb, L, s->nli_table[lane], st, 1);
+    if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
+    if (rc) {
+        (void)hipStreamSynchronize(st);
+        for (Request* r : f->reqs) {
+            s->arena.release(r->off, r->bytes);
+            fail_request(s, r, last_error());
+        }
+        delete f;
+        return;
+    }
+    s->nli_batches++;
+    s->nli_batched_images += n;
     s->nli_busy[lane] = f;
 }
 
@@ -354,14 +372,17 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
         int rc = check_stats(stats, n_stats, patch_sizes, n_patch_sizes, scale_sizes, n_scale_sizes, views,
                              &last_unused);
         if (rc) return rc;
-        float nh, nT;
-        if (nlinsar_params(nl_patch_size, &nh, &nT)) {
-            set_error("serve: NL-InSAR patch sizes 3 and 7 only");
-            return DKX_EUNSUPPORTED;
-        }
-        if (nl_niter < 0 || nl_lmin < 1 || nl_search_window_size < 3 || nl_search_window_size % 2 == 0) {
-            set_error("serve: invalid NL-InSAR parameters");
-            return DKX_EINVAL;
+        NlinsarProblem nl_pb;
+        rc = nlinsar_problem(0, 0, nl_search_window_size, nl_patch_size, nl_niter, nl_lmin, NLINSAR_TILE_AUTO,
+                             &nl_pb);
+        if (rc) return rc;
+        {
+            // the parameter checks of a plan, on a one-pixel image
+            NlinsarProblem probe = nl_pb;
+            probe.H = probe.W = 1;
+            NlinsarPlan plan;
+            rc = nlinsar_plan(probe, &plan);
+            if (rc) return rc;
         }
         Server* s = new Server();
         s->P.assign(patch_sizes, patch_sizes + n_patch_sizes);
@@ -385,12 +406,7 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
             delete s;
             return rc;
         }
-        s->nl_sw = nl_search_window_size;
-        s->nl_p = nl_patch_size;
-        s->nl_niter = nl_niter;
-        s->nl_lmin = nl_lmin;
-        s->nl_h = nh;
-        s->nl_T = nT;
+        s->nl_pb = nl_pb;
         auto cleanup = [&](const std::string& msg) {
             set_error(msg);
             if (s->consts) (void)hipFree(s->consts);
@@ -400,8 +416,11 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
                 if (s->pinned_table[i]) (void)hipHostFree(s->pinned_table[i]);
                 if (s->dev_table[i]) (void)hipFree(s->dev_table[i]);
             }
-            for (int i = 0; i < NLI_LANES; i++)
+            for (int i = 0; i < NLI_LANES; i++) {
                 if (s->nli_stream[i]) (void)hipStreamDestroy(s->nli_stream[i]);
+                if (s->nli_pinned[i]) (void)hipHostFree(s->nli_pinned[i]);
+                if (s->nli_table[i]) (void)hipFree(s->nli_table[i]);
+            }
             delete s;
             return DKX_ERESOURCE;
         };
@@ -418,8 +437,14 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
             if (hipMalloc((void**)&s->dev_table[i], sizeof(NlsarImageDesc) * MAX_BATCH) != hipSuccess)
                 return cleanup("serve: batch table allocation failed");
         }
-        for (int i = 0; i < NLI_LANES; i++)
+        const size_t nli_tables = nlinsar_batch_table_bytes(MAX_NLI_BATCH);
+        for (int i = 0; i < NLI_LANES; i++) {
             if (hipStreamCreateWithFlags(&s->nli_stream[i], hipStreamNonBlocking) != hipSuccess) return cleanup("serve: stream creation failed");
+            if (hipHostMalloc((void**)&s->nli_pinned[i], nli_tables, 0) != hipSuccess)
+                return cleanup("serve: pinned staging allocation failed");
+            if (hipMalloc((void**)&s->nli_table[i], nli_tables) != hipSuccess)
+                return cleanup("serve: batch table allocation failed");
+        }
         s->sched = std::thread(scheduler, s);
         *out = reinterpret_cast<dkx_server>(s);
         return 0;
@@ -521,6 +546,18 @@ int dkx_server_run_trace(dkx_server h, const dkx_request* reqs, const double* ar
     });
 }
 
+int dkx_server_nlinsar_counters(dkx_server h, long long* batches, long long* batched_images)
+{
+    return dkx::guarded<int>([&]() -> int {
+        Server* s = reinterpret_cast<Server*>(h);
+        if (!s) { set_error("serve: null handle"); return DKX_EINVAL; }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (batches) *batches = s->nli_batches;
+        if (batched_images) *batched_images = s->nli_batched_images;
+        return 0;
+    });
+}
+
 int dkx_server_counters(dkx_server h, long long* batches, long long* batched_images)
 {
     return dkx::guarded<int>([&]() -> int {
@@ -550,6 +587,8 @@ void dkx_server_destroy(dkx_server h)
         for (int i = 0; i < NLI_LANES; i++) {
             (void)hipStreamSynchronize(s->nli_stream[i]);
             (void)hipStreamDestroy(s->nli_stream[i]);
+            (void)hipHostFree(s->nli_pinned[i]);
+            (void)hipFree(s->nli_table[i]);
         }
         for (auto& kv : s->reqs) delete kv.second;
         (void)hipFree(s->arena_ptr);

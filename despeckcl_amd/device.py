@@ -249,6 +249,12 @@ class Server:
         check(lib().dkx_server_counters(self._h, C.byref(b), C.byref(im)))
         return b.value, im.value
 
+    def nlinsar_counters(self):
+        """(NL-InSAR batches launched, requests they held)."""
+        b, im = C.c_longlong(0), C.c_longlong(0)
+        check(lib().dkx_server_nlinsar_counters(self._h, C.byref(b), C.byref(im)))
+        return b.value, im.value
+
     def close(self):
         if self._h:
             lib().dkx_server_destroy(self._h)

@@ -310,9 +310,9 @@ int dkx_nlinsar_batch_smoothed(dkx_nlinsar_batch batch, unsigned long long* coun
 /* ---- serving ------------------------------------------------------------------------------
  * No counterpart in the reference (which has no request loop).  A server owns a fixed HBM
  * budget (one arena, sub-allocated per request: no allocation while serving), keeps the NL-SAR
- * weight LUTs resident, admits requests as memory frees up, batches the waiting NL-SAR
- * requests into one launch per stage while earlier work still runs, and runs NL-InSAR
- * requests on streams of their own, overlapping the batches.  Each request's result equals
+ * weight LUTs resident, admits requests as memory frees up, and batches the waiting requests
+ * of each filter into one launch per stage while earlier work still runs (NL-SAR and NL-InSAR
+ * batches on streams of their own, overlapping).  Each request's result equals
  * the single-call result (dkx_nlsar_insar_device with the default pass variant, or
  * dkx_nlinsar_device from zero estimates). */
 
@@ -347,7 +347,10 @@ int dkx_server_wait(dkx_server server, long long id, double* latency_ms);
  * returns each latency (ms, -1 for a failed request) and the time to the last completion. */
 int dkx_server_run_trace(dkx_server server, const dkx_request* requests, const double* arrival_s,
                          int n, double* latency_ms, double* makespan_s);
+/* NL-SAR batches launched and the requests they held. */
 int dkx_server_counters(dkx_server server, long long* batches, long long* batched_images);
+/* The same for NL-InSAR. */
+int dkx_server_nlinsar_counters(dkx_server server, long long* batches, long long* batched_images);
 /* Finishes the queued work, then frees the server. */
 void dkx_server_destroy(dkx_server server);
 

@@ -221,11 +221,14 @@ def test_served_stream_2000_requests(D, stats):
         try:
             lat, span = srv.run_trace(reqs, [0.0] * n)
             batches, batched = srv.counters()
+            nbatches, nbatched = srv.nlinsar_counters()
         finally:
             srv.close()
-        print(f"2000 requests in {span:.2f} s, {batches} NL-SAR batches of {batched} images")
+        print(f"2000 requests in {span:.2f} s, {batches} NL-SAR batches of {batched} images, "
+              f"{nbatches} NL-InSAR batches of {nbatched}")
         assert min(lat) >= 0
         assert batched == int((kinds == 0).sum()) and batches < batched
+        assert nbatched == int((kinds == 1).sum()) and nbatches < nbatched
         rng = np.random.default_rng(2000)
         sample = sorted(rng.choice(n, 100, replace=False).tolist())
         assert {int(kinds[i]) for i in sample} == {0, 1}
