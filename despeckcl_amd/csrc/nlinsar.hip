@@ -674,6 +674,15 @@ struct SmoothBatch {
     int nimg;
 };
 
+// The work items of a batch's smoothing launch are the queued pixels of all images end to end
+// (image i's items from base_i on): the first item at or after `base` that a worker `me` of
+// `stride` workers takes (items me, me + stride, ...).
+__device__ __forceinline__ unsigned long long nl_first_item(unsigned long long me, unsigned long long stride,
+                                                            unsigned long long base)
+{
+    return me >= base ? me : me + (base - me + stride - 1) / stride * stride;
+}
+
 // std::priority_queue<weight_el> of smoothing.cpp:27-36 (top = smallest weight): sift
 // rules of libstdc++'s push_heap / pop_heap, which decide which of several equal weights
 // stays in the queue.  Entry j of a thread's heap: weight hw[j * bs], index hi[j * bs] (the
@@ -879,10 +888,15 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsig
     using Flags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
     static_assert(CH <= 32, "candidate flags of a run are kept in one 32-bit word");
     extern __shared__ float4 smem_flags[];
+    const unsigned long long me = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    unsigned long long base = 0;
     for (int img = 0; img < b.nimg; img++) {
     const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[0];
     if (total < qlo || total >= qhi) continue;
+    const unsigned long long first_item = nl_first_item(me, stride, base);
+    base += total;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
     const int lmin = a.lmin;
@@ -890,8 +904,8 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsig
     Flags* cf = reinterpret_cast<Flags*>(smem_flags) + threadIdx.x;
     const int bs = blockDim.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
-    for (unsigned int li = blockIdx.x * blockDim.x + threadIdx.x; li < total;
-         li += gridDim.x * blockDim.x) {
+    for (unsigned long long g = first_item; g < base; g += stride) {
+        const unsigned int li = (unsigned int)(g - (base - total));
         long long opix;
         const SmoothPixel px = nl_smooth_pixel(a, li, &opix);
         bool exact = lmin >= NLI_TOPK;
@@ -991,15 +1005,20 @@ template <int CH>
 __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
 {
     extern __shared__ float smem[];
+    const unsigned long long me = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    unsigned long long base = 0;
     for (int img = 0; img < b.nimg; img++) {
     const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[3];
     if (total < qlo || total >= qhi) continue;
+    const unsigned long long first_item = nl_first_item(me, stride, base);
+    base += total;
     const int bs = blockDim.x;
     float* hw = smem + threadIdx.x;
     int* hi = reinterpret_cast<int*>(smem + a.lmin * bs + threadIdx.x);
-    for (unsigned int xi_ = blockIdx.x * blockDim.x + threadIdx.x; xi_ < total;
-         xi_ += gridDim.x * blockDim.x) {
+    for (unsigned long long g = first_item; g < base; g += stride) {
+        const unsigned int xi_ = (unsigned int)(g - (base - total));
         long long opix;
         const SmoothPixel px = nl_smooth_pixel(a, (unsigned int)a.xlist[xi_], &opix);
         float N, at_, xr, xi;
@@ -1021,10 +1040,13 @@ template <bool TIES>
 __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, unsigned int qhi)
 {
     extern __shared__ float4 smem_wave[];
+    unsigned long long base = 0;
     for (int img = 0; img < b.nimg; img++) {
     const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = TIES ? a.cnt[3] : a.cnt[0];
     if (total >= qhi) continue;
+    const unsigned long long first_item = nl_first_item(blockIdx.x, gridDim.x, base);
+    base += total;
     if (!TIES && blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
     const int sw = 2 * a.wsh + 1;
     const int nfull = sw * sw;
@@ -1038,7 +1060,8 @@ __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, u
     unsigned int* C = reinterpret_cast<unsigned int*>(Fz + np4);   // candidate flags
     float* hw = reinterpret_cast<float*>(C + nwords);
     int* hi = reinterpret_cast<int*>(hw + a.lmin);
-    for (unsigned int qi = blockIdx.x; qi < total; qi += gridDim.x) {
+    for (unsigned long long g = first_item; g < base; g += gridDim.x) {
+        const unsigned int qi = (unsigned int)(g - (base - total));
         const unsigned int li = TIES ? (unsigned int)a.xlist[qi] : qi;
         long long opix;
         const SmoothPixel px = nl_smooth_pixel(a, li, &opix);
