@@ -37,6 +37,7 @@ struct NlsarBatch {
     float* consts = nullptr;
     size_t consts_cap = 0;
     NlsarImageDesc* table = nullptr;   // lives in ws
+    unsigned int* queues = nullptr;    // the persistent passes' tile counters, in ws
     hipEvent_t done = nullptr;         // completion of the last enqueue
     bool pending = false;
     hipGraph_t graph = nullptr;
@@ -90,7 +91,8 @@ static int choose_tile(NlsarBatch* b)
 static int batch_enqueue(NlsarBatch* b, hipStream_t stream)
 {
     if (b->descs.empty()) { set_error("nlsar batch: no images set"); return DKX_EINVAL; }
-    int rc = nlsar_enqueue(b->pp, b->cm, b->descs.data(), (int)b->descs.size(), b->table, b->consts, stream);
+    int rc = nlsar_enqueue(b->pp, b->cm, b->descs.data(), (int)b->descs.size(), b->table, b->consts, stream,
+                           b->queues);
     if (rc) return rc;
     return 0;
 }
@@ -170,7 +172,9 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         rc = nlsar_common(nb.pp, &nb.cm);
         if (rc) return rc;
         std::vector<NlsarImagePlan> plans(n);
-        size_t off = (sizeof(NlsarImageDesc) * (size_t)n + 255) / 256 * 256;   // the table first
+        // the table first, then the tile counters
+        const size_t qoff = (sizeof(NlsarImageDesc) * (size_t)n + 255) / 256 * 256;
+        size_t off = qoff + (NLSAR_QUEUE_BYTES + 255) / 256 * 256;
         std::vector<size_t> offs(n);
         for (int i = 0; i < n; i++) {
             rc = nlsar_image_plan(nb.cm, images[i].height, images[i].width, -1, -1, &plans[i]);
@@ -231,6 +235,7 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         b->descs = std::move(nb.descs);
         b->ws_bytes = off;
         b->table = (NlsarImageDesc*)b->ws;
+        b->queues = (unsigned int*)((char*)b->ws + qoff);
         return choose_tile(b);
     });
 }

@@ -127,10 +127,16 @@ void nlsar_image_desc(const NlsarImagePlan& ip, char* ws_image, const float* a1,
 // numbers the rows and pass blocks of a batch; returns the totals
 void nlsar_batch_layout(NlsarImageDesc* d, int n, int* rows_c, int* rows_a, int* blocks,
                         int* max_cw, int* max_aw);
+// Tile-queue counters a persistent batch needs (8 per pass, NLSAR_MAX_PASSES passes).
+constexpr int NLSAR_MAX_PASSES = 64;
+constexpr size_t NLSAR_QUEUE_BYTES = sizeof(unsigned int) * 8 * NLSAR_MAX_PASSES;
 // Enqueues the filter on `stream`.  d_table: device copy of descs[0..n) (after
 // nlsar_batch_layout), or null for a single image (n == 1), which is passed by value.
 // d_const: device copy of nlsar_build_constants' blob, unchanged until the work has run.
+// d_queues: NLSAR_QUEUE_BYTES of device memory for a batch's persistent passes (cleared
+// here by a kernel), or null for one tile per block.
 int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImageDesc* descs,
-                  int n, const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream);
+                  int n, const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream,
+                  unsigned int* d_queues = nullptr);
 
 }  // namespace dkx

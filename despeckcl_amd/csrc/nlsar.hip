@@ -23,6 +23,8 @@ struct PrepArgs {
     const NlsarImageDesc* table;   // batch: descriptors in device memory, else null
     NlsarImageDesc one;            // the image of a single call
     int nimg;
+    unsigned int* queues;          // persistent passes' tile counters (nqueues), or null
+    int nqueues;
     int cblocks;                   // blocks per grid row
     const float* gauss;            // scale average: s x s taps
     int s;                         // scale size
@@ -50,6 +52,7 @@ __global__ void nlsar_zero_counts(const PrepArgs a)
     } else if (i < a.nimg) {
         *a.table[i].ub_count = 0ull;
     }
+    if (a.queues && i < a.nqueues) a.queues[i] = 0u;
 }
 
 // Covariance of the input pixel on the padded grid C (records of nlsar_c_float4(MODE)
@@ -351,7 +354,8 @@ void nlsar_batch_layout(NlsarImageDesc* d, int n, int* rows_c, int* rows_a, int*
 }
 
 int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImageDesc* descs, int n,
-                  const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream)
+                  const NlsarImageDesc* d_table, const float* d_const, hipStream_t stream,
+                  unsigned int* d_queues)
 {
     if (n < 1 || (!d_table && n != 1)) { set_error("nlsar: internal error, bad image set"); return -1; }
     const float* d_lut = d_const;
@@ -373,7 +377,11 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
     pa.table = d_table;
     if (!d_table) pa.one = descs[0];
     pa.nimg = n;
-    hipLaunchKernelGGL(nlsar_zero_counts, dim3((n + 255) / 256), dim3(256), 0, stream, pa);
+    if (d_table && d_queues) {
+        pa.queues = d_queues;
+        pa.nqueues = 8 * NLSAR_MAX_PASSES;
+    }
+    hipLaunchKernelGGL(nlsar_zero_counts, dim3((std::max(n, pa.nqueues) + 255) / 256), dim3(256), 0, stream, pa);
     DKX_HIP_CHECK(hipGetLastError());
     pa.cblocks = (max_cw + 255) / 256;
     const dim3 gprep((unsigned)rows_c * pa.cblocks);
@@ -426,6 +434,10 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
         passes.assign(1, Pass{smax_i, std::vector<int>{pmax_i}, true});
     }
 
+    if (passes.size() > (size_t)NLSAR_MAX_PASSES) {
+        set_error("nlsar: too many patch and scale sizes for one call");
+        return -1;
+    }
     int cur_scale = -1;
     for (size_t oi = 0; oi < passes.size(); oi++) {
         const Pass& ps = passes[oi];
@@ -469,6 +481,7 @@ int nlsar_enqueue(const NlsarParams& pp, const NlsarCommon& cm, const NlsarImage
             sa.lut_size[k] = pst[idx]->lut_size;
         }
         sa.alpha_zero = ps.last && pp.last_unused ? 1 : 0;
+        sa.queue = pa.queues ? pa.queues + 8 * oi : nullptr;
         if (ps.last && !pp.last_unused && pis.back() != pmax_i) {
             set_error("nlsar: internal error, the last pass must end with the largest patch");
             return -1;

@@ -120,6 +120,7 @@ struct Server {
     hipStream_t nli_stream[NLI_LANES] = {};
     NlsarImageDesc* pinned_table[SAR_LANES] = {};   // staging of a lane's batch table
     NlsarImageDesc* dev_table[SAR_LANES] = {};      // the table itself (MAX_BATCH entries)
+    unsigned int* dev_queues[SAR_LANES] = {};       // its persistent passes' tile counters
     unsigned char* nli_pinned[NLI_LANES] = {};     // staging of a lane's NL-InSAR tables
     char* nli_table[NLI_LANES] = {};               // the tables (MAX_NLI_BATCH images)
     Flight* sar_busy[SAR_LANES] = {};
@@ -220,7 +221,7 @@ static void launch_sar(Server* s, int lane)
     memcpy(s->pinned_table[lane], descs.data(), table_bytes);
     NlsarImageDesc* d_table = s->dev_table[lane];
     if (hipMemcpyAsync(d_table, s->pinned_table[lane], table_bytes, hipMemcpyHostToDevice, st) != hipSuccess) rc = -5;
-    if (!rc) rc = nlsar_enqueue(s->pp, s->cm, descs.data(), n, d_table, s->consts, st);
+    if (!rc) rc = nlsar_enqueue(s->pp, s->cm, descs.data(), n, d_table, s->consts, st, s->dev_queues[lane]);
     if (!rc && hipLaunchHostFunc(st, host_done, f) != hipSuccess) rc = -5;
     if (rc) {
         // nothing reached the GPU that still needs the memory once the stream drains
@@ -415,6 +416,7 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
                 if (s->sar_stream[i]) (void)hipStreamDestroy(s->sar_stream[i]);
                 if (s->pinned_table[i]) (void)hipHostFree(s->pinned_table[i]);
                 if (s->dev_table[i]) (void)hipFree(s->dev_table[i]);
+                if (s->dev_queues[i]) (void)hipFree(s->dev_queues[i]);
             }
             for (int i = 0; i < NLI_LANES; i++) {
                 if (s->nli_stream[i]) (void)hipStreamDestroy(s->nli_stream[i]);
@@ -435,6 +437,8 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
             if (hipHostMalloc((void**)&s->pinned_table[i], sizeof(NlsarImageDesc) * MAX_BATCH, 0) != hipSuccess)
                 return cleanup("serve: pinned staging allocation failed");
             if (hipMalloc((void**)&s->dev_table[i], sizeof(NlsarImageDesc) * MAX_BATCH) != hipSuccess)
+                return cleanup("serve: batch table allocation failed");
+            if (hipMalloc((void**)&s->dev_queues[i], NLSAR_QUEUE_BYTES) != hipSuccess)
                 return cleanup("serve: batch table allocation failed");
         }
         const size_t nli_tables = nlinsar_batch_table_bytes(MAX_NLI_BATCH);
@@ -583,6 +587,7 @@ void dkx_server_destroy(dkx_server h)
             (void)hipStreamDestroy(s->sar_stream[i]);
             (void)hipHostFree(s->pinned_table[i]);
             (void)hipFree(s->dev_table[i]);
+            (void)hipFree(s->dev_queues[i]);
         }
         for (int i = 0; i < NLI_LANES; i++) {
             (void)hipStreamSynchronize(s->nli_stream[i]);
