@@ -9,9 +9,10 @@ A step is one NL-SAR InSAR filtering of the seeded 2048x2048 interferogram (sear
 (patch, scale) key's statistics with their bias-corrected ENL and alpha maps, then the
 selected key's weighted mean and decomposition, inputs and outputs resident in HBM.  Prints
 one JSON line with Mpixel/s, the roofline of the dominant kernel (the last scale pass), the
-same output from the selected key's statistics alone, the NL-InSAR 4096^2 x 10 and 64-image
-batch summaries, the served stream's p99 latency (500-request sample) and the CPU baseline
-(the C restatement in oracle/, timed on this host's cores on a 1536x1536 crop).
+same output from the selected key's statistics alone, the NL-InSAR 4096^2 x 10 summary, the
+64-image NL-SAR and NL-InSAR batch summaries, the served stream's p99 latency (500-request
+sample) and the CPU baseline (the C restatement in oracle/, timed on this host's cores on a
+1536x1536 crop).
 """
 from __future__ import annotations
 
@@ -150,6 +151,86 @@ def batched_run(tiles, steps, warmup):
 def nlinsar_form(args):
     print(json.dumps(nlinsar_run(args.size or 4096, args.niter, args.steps, args.warmup, args.tile,
                                  not args.no_cpu_baseline)), flush=True)
+
+
+def nlinsar_batched_run(tiles, steps, warmup, lo=128, hi=1024, niter=5):
+    """Batched NL-InSAR: `tiles` seeded interferograms with sides log-uniform in [lo, hi]
+    (aspect <= 4:1), search window 21, patch 7, `niter` iterations, lmin 10, estimates from
+    zero; plain per-call loop vs one batch (eager launches and HIP-graph replay), on one
+    stream with inputs resident in HBM.  Every timed form first clears the estimate arrays
+    (hipMemsetAsync on the stream), so each run starts from zero estimates."""
+    import despeckcl_amd as D
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+
+    shapes = S.tile_shapes(tiles, S.SEED_BASE + 6, lo=lo, hi=hi)
+    stream = HIP.Stream()
+    imgs = []
+    for i, (h, w) in enumerate(shapes):
+        host = S.interferogram(h, w, S.SEED_BASE + 6000 + i, n_regions=8, coh_range=(0.3, 0.9))
+        imgs.append(([HIP.DeviceArray.from_host(x) for x in host], [HIP.DeviceArray((h, w)) for _ in range(3)],
+                     h, w))
+    px = sum(h * w for _i, _o, h, w in imgs)
+    log(f"nlinsar batched: {len(imgs)} images, {px / 1e6:.1f} Mpixel")
+
+    def zero_all():
+        for _i, out, h, w in imgs:
+            for d in out:
+                HIP.hip().hipMemsetAsync(C_void(d.ptr), 0, C_size(h * w * 4), C_void(stream.handle))
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        stream.synchronize()
+        e0, e1 = HIP.Event(), HIP.Event()
+        e0.record(stream.handle)
+        for _ in range(steps):
+            fn()
+        e1.record(stream.handle)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    def plain():
+        zero_all()
+        for inp, out, h, w in imgs:
+            DV.nlinsar(*[d.ptr for d in inp], *[d.ptr for d in out], h, w, 21, 7, niter, 10, stream=stream.handle)
+
+    plain_ms = timed(plain)
+    D.lib().dkx_release_workspace()
+    b = DV.NlinsarBatch()
+    try:
+        b.set([(*[d.ptr for d in inp], *[d.ptr for d in out], h, w) for inp, out, h, w in imgs], 21, 7, niter, 10)
+
+        def eager():
+            zero_all()
+            b.enqueue(stream.handle)
+
+        def graph():
+            zero_all()
+            b.graph_launch(stream.handle)
+
+        eager_ms = timed(eager)
+        graph_ms = timed(graph)
+    finally:
+        b.close()
+        for inp, out, _h, _w in imgs:
+            for d in inp + out:
+                d.free()
+    best = min(eager_ms, graph_ms)
+    return {
+        "metric": f"Mpixel/s batched NL-InSAR ({len(imgs)} interferograms, sides {lo}-{hi}, {niter} iterations, 1 GPU)",
+        "value": round(px / 1e6 / (best / 1e3), 3), "unit": "Mpixel/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(best, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded speckle model of insarsim_simu.cpp, 8 regions, coherence U[0.3,0.9])",
+        "config": {"workload": f"NL-InSAR, {len(imgs)} interferograms per batch", "pixels": px,
+                   "shapes_seed": S.SEED_BASE + 6, "search_window": 21, "patch": 7, "niter": niter, "lmin": 10},
+        "plain_per_call": {"ms": round(plain_ms, 3), "mpx_s": round(px / 1e6 / (plain_ms / 1e3), 3)},
+        "batched_eager": {"ms": round(eager_ms, 3), "mpx_s": round(px / 1e6 / (eager_ms / 1e3), 3)},
+        "batched_graph": {"ms": round(graph_ms, 3), "mpx_s": round(px / 1e6 / (graph_ms / 1e3), 3)},
+        "speedup_vs_plain": round(plain_ms / best, 3),
+    }
 
 
 def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
@@ -531,7 +612,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the NL-InSAR 4096^2 and 64-image batch summaries")
     ap.add_argument("--tile", type=int, default=None)
-    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "served", "covmat", "boxcar",
+    ap.add_argument("--form", choices=("plain", "batched", "nlinsar", "nlinsar_batched", "served", "covmat", "boxcar",
                                        "goldstein"), default="plain")
     ap.add_argument("--window", type=int, default=5, help="window width (--form boxcar)")
     ap.add_argument("--dim", type=int, default=3, help="covariance dimension (--form covmat)")
@@ -551,6 +632,9 @@ def main():
         return batched(args)
     if args.form == "nlinsar":
         return nlinsar_form(args)
+    if args.form == "nlinsar_batched":
+        print(json.dumps(nlinsar_batched_run(args.tiles, args.steps, args.warmup)), flush=True)
+        return
     if args.form == "served":
         return served(args)
     if args.form == "covmat":
@@ -686,13 +770,17 @@ def main():
         bt = batched_run(64, 3, 1)
         out["batched_64"] = {k: bt[k] for k in ("config", "value", "ms_per_step", "plain_per_call",
                                                  "batched_eager", "batched_graph", "speedup_vs_plain")}
+        nb = nlinsar_batched_run(64, 2, 1)
+        out["nlinsar_batched_64"] = {k: nb[k] for k in ("config", "value", "ms_per_step", "plain_per_call",
+                                                         "batched_eager", "batched_graph", "speedup_vs_plain")}
     if not args.no_served:
         # the mixed stream's latency: a 500-request sample of the served workload
         # (the full 2000-request run is `--form served`)
         sv = served_run(500, 40.0, 150)
         out["p99_latency_ms"] = sv["p99_latency_ms"]
         out["served_sample"] = {k: sv[k] for k in ("config", "p50_latency_ms", "p99_latency_ms", "mean_latency_ms",
-                                                   "saturated", "serial_plain", "speedup_vs_serial")}
+                                                   "saturated", "serial_plain", "speedup_vs_serial",
+                                                   "nlsar_batches", "nlinsar_batches")}
     print(json.dumps(out), flush=True)
 
 

@@ -882,19 +882,19 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 // the heap, so that neither kernel's waves wait on the other path.  The first pass keeps
 // the candidate flags of each run of CH offsets in LDS (entry c of a thread at
 // c * blockDim.x), which the second pass reads instead of the amplitudes.
+// One image of nlinsar_smooth: its queued pixels from work item `base` on (base advances past
+// them).
 template <int CH>
-__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
+__device__ __forceinline__ void nl_smooth_image(const SmoothArgs& a, unsigned int qlo, unsigned int qhi,
+                                                unsigned long long& base)
 {
     using Flags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
     static_assert(CH <= 32, "candidate flags of a run are kept in one 32-bit word");
     extern __shared__ float4 smem_flags[];
     const unsigned long long me = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    unsigned long long base = 0;
-    for (int img = 0; img < b.nimg; img++) {
-    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[0];
-    if (total < qlo || total >= qhi) continue;
+    if (total < qlo || total >= qhi) return;
     const unsigned long long first_item = nl_first_item(me, stride, base);
     base += total;
     const int sw = 2 * a.wsh + 1;
@@ -996,22 +996,30 @@ __global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsig
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
     }
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void nlinsar_smooth(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
+{
+    unsigned long long base = 0;
+    if (!b.table) {
+        nl_smooth_image<CH>(b.one, qlo, qhi, base);
+        return;
     }
+    for (int img = 0; img < b.nimg; img++) nl_smooth_image<CH>(b.table[img], qlo, qhi, base);
 }
 
 // The pixels nlinsar_smooth queued for the exact heap (ties at the smallest kept weight);
 // the variant with runs of CH offsets serves exact-queue lengths in [qlo, qhi).
 template <int CH>
-__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
+__device__ __forceinline__ void nl_smooth_exact_image(const SmoothArgs& a, unsigned int qlo, unsigned int qhi,
+                                                      unsigned long long& base)
 {
     extern __shared__ float smem[];
     const unsigned long long me = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-    unsigned long long base = 0;
-    for (int img = 0; img < b.nimg; img++) {
-    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = a.cnt[3];
-    if (total < qlo || total >= qhi) continue;
+    if (total < qlo || total >= qhi) return;
     const unsigned long long first_item = nl_first_item(me, stride, base);
     base += total;
     const int bs = blockDim.x;
@@ -1026,7 +1034,17 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b,
         nl_estimate(N, at_, xr, xi, opix, a.old_ref, a.old_pha, a.old_coh, a.new_ref, a.new_pha,
                     a.new_coh);
     }
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b, unsigned int qlo, unsigned int qhi)
+{
+    unsigned long long base = 0;
+    if (!b.table) {
+        nl_smooth_exact_image<CH>(b.one, qlo, qhi, base);
+        return;
     }
+    for (int img = 0; img < b.nimg; img++) nl_smooth_exact_image<CH>(b.table[img], qlo, qhi, base);
 }
 
 // Short queues, one wave per pixel.  Per thread, the chain of dependent load runs of the
@@ -1037,14 +1055,11 @@ __global__ __launch_bounds__(256) void nlinsar_smooth_exact(const SmoothBatch b,
 // arithmetic of nl_smooth_exact.  One block is one wave.  TIES: the pixels of the exact
 // queue that nlinsar_smooth left (when that queue is short), instead of the whole queue.
 template <bool TIES>
-__global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, unsigned int qhi)
+__device__ __forceinline__ void nl_smooth_wave_image(const SmoothArgs& a, unsigned int qhi, unsigned long long& base)
 {
     extern __shared__ float4 smem_wave[];
-    unsigned long long base = 0;
-    for (int img = 0; img < b.nimg; img++) {
-    const SmoothArgs& a = b.table ? b.table[img] : b.one;
     const unsigned int total = TIES ? a.cnt[3] : a.cnt[0];
-    if (total >= qhi) continue;
+    if (total >= qhi) return;
     const unsigned long long first_item = nl_first_item(blockIdx.x, gridDim.x, base);
     base += total;
     if (!TIES && blockIdx.x == 0 && threadIdx.x == 0) a.cnt[1] += total;
@@ -1156,7 +1171,17 @@ __global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, u
         }
         __syncthreads();
     }
+}
+
+template <bool TIES>
+__global__ __launch_bounds__(64) void nlinsar_smooth_wave(const SmoothBatch b, unsigned int qhi)
+{
+    unsigned long long base = 0;
+    if (!b.table) {
+        nl_smooth_wave_image<TIES>(b.one, qhi, base);
+        return;
     }
+    for (int img = 0; img < b.nimg; img++) nl_smooth_wave_image<TIES>(b.table[img], qhi, base);
 }
 
 static int round_up(int v, int m) { return (v + m - 1) / m * m; }
