@@ -85,6 +85,11 @@ SIGNATURES = {
     "dkx_nlsar_batch_get_tile": (_I, [_P]),
     "dkx_nlsar_batch_autotune": (_I, [_P, _P, _I, _IP, _P]),
     "dkx_nlsar_batch_ub_pixels": (_I, [_P, _ULLP]),
+    "dkx_nlsar_insar_device_autotune": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F,
+                                             _I, _I, _P, _IP, _P]),
+    "dkx_nlinsar_device_autotune": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _IP, _P]),
+    "dkx_get_tuned_tile": (_I, [_I, _I, _I, _I, _IP, _I, _IP, _I, _I]),
+    "dkx_clear_tuned_tiles": (None, []),
     "dkx_nlinsar_batch_create": (_I, [C.POINTER(C.c_void_p)]),
     "dkx_nlinsar_batch_destroy": (None, [_P]),
     "dkx_nlinsar_batch_set": (_I, [_P, C.POINTER(DkxImage), _I, _I, _I, _I, _I]),

@@ -45,15 +45,11 @@ struct NlsarBatch {
     hipStream_t graph_stream = nullptr;
 };
 
-// autotuner results: (search window, patch set, scale set, log2 of the batch's pixels) -> tile
-static std::mutex g_tune_mu;
-static std::map<std::tuple<int, std::vector<int>, std::vector<int>, int>, int> g_tune;
-
-static int pixel_bucket(const NlsarBatch& b)
+static double batch_pixels(const NlsarBatch& b)
 {
     double px = 0;
     for (const NlsarImageDesc& d : b.descs) px += (double)d.H * d.W;
-    return (int)std::floor(std::log2(std::max(px, 1.0)));
+    return px;
 }
 
 static int batch_wait(NlsarBatch* b)
@@ -77,10 +73,10 @@ static int choose_tile(NlsarBatch* b)
 {
     int t = b->tile;
     if (t < 0) {
-        t = NLSAR_TILE_AUTO;   // untuned: per pass, as the single-call entry points
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune.find(std::make_tuple(b->pp.sw, b->P, b->S, pixel_bucket(*b)));
-        if (it != g_tune.end()) t = it->second;
+        // the autotuner's record for this batch shape, else per pass as the single calls
+        t = tuned_tile(0, b->diagnostics, b->pp.sw, b->P.data(), (int)b->P.size(), b->S.data(), (int)b->S.size(),
+                       batch_pixels(*b));
+        if (t < 0) t = NLSAR_TILE_AUTO;
     }
     b->tile_used = t;
     b->pp.tile = t;
@@ -372,10 +368,8 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch h, void* stream_, int reps, int* ti
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            g_tune[std::make_tuple(b->pp.sw, b->P, b->S, pixel_bucket(*b))] = best;
-        }
+        record_tuned_tile(0, b->diagnostics, b->pp.sw, b->P.data(), (int)b->P.size(), b->S.data(), (int)b->S.size(),
+                          batch_pixels(*b), best);
         if (tile_out) *tile_out = best;
         b->tile = -1;
         return choose_tile(b);

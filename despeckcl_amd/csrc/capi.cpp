@@ -250,8 +250,13 @@ int nlsar_prepare(NlsarCall& call, int dim, int all_keys, int height, int width,
     int last_unused = 0;
     int rc = check_stats(stats, n_stats, P, nP, S, nS, call.views, &last_unused);
     if (rc) return rc;
+    // the pass variant: the caller's setting, else the autotuner's record for this parameter
+    // set and image size, else chosen per pass
+    int tile = g_tile[0].load();
+    if (tile == NLSAR_TILE_AUTO) tile = tuned_tile(0, all_keys ? 1 : 0, sw, P, nP, S, nS, (double)height * width);
+    if (tile < 0) tile = NLSAR_TILE_AUTO;
     call.pp = NlsarParams{dim, sw, P, nP, S, nS, call.views.data(), (int)call.views.size(), h, c,
-                          g_tile[0].load(), all_keys, last_unused};
+                          tile, all_keys, last_unused};
     rc = nlsar_common(call.pp, &call.cm);
     if (rc) return rc;
     return nlsar_image_plan(call.cm, height, width, -1, -1, &call.ip);
@@ -315,7 +320,12 @@ int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, i
 static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int p, int niter,
                            int lmin)
 {
-    int rc = nlinsar_problem(height, width, sw, p, niter, lmin, g_tile[1].load(), &call.pb);
+    // the tile shape: the caller's setting, else the autotuner's record for these parameters
+    // and image size, else the built-in rule
+    int tile = g_tile[1].load();
+    if (tile == NLINSAR_TILE_AUTO) tile = tuned_tile(1, 0, sw, &p, 1, nullptr, 0, (double)height * width);
+    if (tile < 0) tile = NLINSAR_TILE_AUTO;
+    int rc = nlinsar_problem(height, width, sw, p, niter, lmin, tile, &call.pb);
     if (rc) return rc;
     return nlinsar_plan(call.pb, &call.plan);
 }
@@ -362,6 +372,35 @@ struct HostStage {
     void* ptr = nullptr;
     ~HostStage() { if (ptr) (void)hipFree(ptr); }
 };
+
+// Times run(): prep() then run() once as a warm-up, then reps times prep() and run(), the
+// runs alone between two events on `stream`; ms: their mean.
+template <class Prep, class Run>
+static int time_runs(hipStream_t stream, int reps, Prep prep, Run run, float* ms)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    DKX_HIP_CHECK(hipEventCreate(&e0));
+    DKX_HIP_CHECK(hipEventCreate(&e1));
+    int rc = prep();
+    if (!rc) rc = run();
+    float total = 0.f;
+    for (int r = 0; r < reps && !rc; r++) {
+        rc = prep();
+        if (rc) break;
+        if (hipEventRecord(e0, stream) != hipSuccess) { rc = DKX_EHIP; break; }
+        rc = run();
+        if (rc) break;
+        if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = DKX_EHIP; break; }
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, e0, e1) != hipSuccess) { rc = DKX_EHIP; break; }
+        total += t;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == DKX_EHIP) set_error("autotune: event timing failed");
+    *ms = total / (float)std::max(reps, 1);
+    return rc;
+}
 
 }  // namespace dkx
 
@@ -527,6 +566,116 @@ int dkx_nlinsar_device(const float* a1, const float* a2, const float* phi, float
         if (!w) { set_error("no HIP device"); return DKX_EHIP; }
         std::lock_guard<std::mutex> lk(w->mu);
         return nlinsar_device_impl(call, a1, a2, phi, ref, pha, coh, nsm, (hipStream_t)stream, w);
+    });
+}
+
+int dkx_nlsar_insar_device_autotune(const float* a1, const float* a2, const float* phi, float* ref,
+                                    float* pha, float* coh, int height, int width, int sw, const int* P,
+                                    int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                                    int n_stats, float h, float c, int every_key, int reps, void* stream_,
+                                    int* tile_out, float* ms_out)
+{
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlsar: null array argument"); return DKX_EINVAL; }
+        NlsarCall call;
+        int rc = nlsar_prepare(call, 0, every_key ? 1 : 0, height, width, sw, P, nP, S, nS, stats, n_stats, h, c);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        const hipStream_t stream = (hipStream_t)stream_;
+        reps = std::max(reps, 1);
+        int best = -1;
+        float best_ms = 0.f;
+        for (int t = 0; t < NLSAR_NTILES; t++) {
+            call.pp.tile = t;
+            call.cm.tile = t;
+            float ms = 0.f;
+            {
+                std::lock_guard<std::mutex> lk(w->mu);
+                rc = time_runs(stream, reps, [] { return 0; },
+                               [&] {
+                                   return nlsar_device_impl(call, a1, a2, phi, nullptr, ref, pha, coh, nullptr,
+                                                            nullptr, nullptr, nullptr, stream, w);
+                               },
+                               &ms);
+            }
+            if (rc) return rc;
+            if (ms_out) ms_out[t] = ms;
+            if (best < 0 || ms < best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+        record_tuned_tile(0, every_key ? 1 : 0, sw, P, nP, S, nS, (double)height * width, best);
+        if (tile_out) *tile_out = best;
+        return 0;
+    });
+}
+
+int dkx_nlinsar_device_autotune(const float* a1, const float* a2, const float* phi, float* ref, float* pha,
+                                float* coh, int height, int width, int sw, int p, int niter, int lmin,
+                                int reps, void* stream_, int* tile_out, float* ms_out)
+{
+    return dkx::guarded<int>([&]() -> int {
+        if (!a1 || !a2 || !phi || !ref || !pha || !coh) { set_error("nlinsar: null array argument"); return DKX_EINVAL; }
+        NlinsarCall call;
+        int rc = nlinsar_prepare(call, height, width, sw, p, niter, lmin);
+        if (rc) return rc;
+        DeviceWorkspace* w = workspace_for_current_device();
+        if (!w) { set_error("no HIP device"); return DKX_EHIP; }
+        const hipStream_t stream = (hipStream_t)stream_;
+        reps = std::max(reps, 1);
+        // the first-iteration estimates, restored before every run (the runs overwrite them)
+        const size_t n = (size_t)height * width;
+        float* init = nullptr;
+        if (hipMalloc(&init, 3 * n * sizeof(float)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("nlinsar autotune: allocation of the estimate copy failed");
+            return DKX_ERESOURCE;
+        }
+        float* io[3] = {ref, pha, coh};
+        rc = 0;
+        for (int k = 0; k < 3 && !rc; k++)
+            if (hipMemcpyAsync(init + k * n, io[k], n * sizeof(float), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+                rc = DKX_EHIP;
+        auto restore = [&]() -> int {
+            for (int k = 0; k < 3; k++)
+                DKX_HIP_CHECK(hipMemcpyAsync(io[k], init + k * n, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+            return 0;
+        };
+        int best = -1;
+        float best_ms = 0.f;
+        for (int t = 0; t < 3 && !rc; t++) {
+            NlinsarCall ct = call;
+            ct.pb.tile = t;
+            if (nlinsar_plan(ct.pb, &ct.plan)) {
+                if (ms_out) ms_out[t] = -1.f;   // a shape these parameters do not allow
+                continue;
+            }
+            float ms = -1.f;
+            {
+                std::lock_guard<std::mutex> lk(w->mu);
+                rc = time_runs(stream, reps, restore,
+                               [&] { return nlinsar_device_impl(ct, a1, a2, phi, ref, pha, coh, nullptr, stream, w); },
+                               &ms);
+            }
+            if (rc) break;
+            if (ms_out) ms_out[t] = ms;
+            if (ms >= 0.f && (best < 0 || ms < best_ms)) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+        if (!rc && best < 0) {
+            set_error("nlinsar autotune: no tile shape runs with these parameters");
+            rc = DKX_EUNSUPPORTED;
+        }
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(init);
+        if (rc) return rc;
+        record_tuned_tile(1, 0, sw, &p, 1, nullptr, 0, (double)height * width, best);
+        if (tile_out) *tile_out = best;
+        return 0;
     });
 }
 

@@ -109,6 +109,14 @@ enum StageKind {
     STAGE_NLINSAR_WEIGHTS = 11,  // NL-InSAR weights / ENL / estimates
     STAGE_NLINSAR_SMOOTH = 12,   // NL-InSAR lmin smoothing
 };
+// Autotuner records (tune.cpp): the fastest tile variant for filter 0 (NL-SAR; variant 1 =
+// every key's statistics, 0 = the selected key's; P, S = patch and scale sizes) or 1
+// (NL-InSAR; P = the patch size, no S) at a search window and log2 of the pixel count
+// (a batch: of all its images).  tuned_tile returns -1 when nothing is recorded.
+int tuned_tile(int filter, int variant, int sw, const int* P, int nP, const int* S, int nS, double pixels);
+void record_tuned_tile(int filter, int variant, int sw, const int* P, int nP, const int* S, int nS,
+                       double pixels, int tile);
+
 bool timing_on();
 void timing_push(int kind, hipEvent_t start, hipEvent_t stop);
 

@@ -1250,13 +1250,18 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
         // 54 x 54 blocks when the image fills the GPU with them, a reference tile interior
         // is a whole number of them, and their LDS fits; 18 x 54 otherwise
         const bool big = (long long)pb.H * pb.W >= (1ll << 20) && plan->step % kTY[2] == 0 &&
-                         nlinsar_lds_bytes(kTY[2], kTX[2], plan->wsh, plan->psh) <= 160 * 1024;
+                         nlinsar_lds_bytes(kTY[2], kTX[2], plan->wsh, plan->psh) + 16 <= 160 * 1024;
         plan->tile = big ? 2 : 1;
     } else {
         plan->tile = std::min(std::max(pb.tile, 0), 2);
     }
     plan->TY = kTY[plan->tile];
     plan->TX = kTX[plan->tile];
+    // the weights kernel's LDS (16 bytes more on a batch's persistent grid)
+    if (nlinsar_lds_bytes(plan->TY, plan->TX, plan->wsh, plan->psh) + 16 > 160 * 1024) {
+        set_error("nlinsar: search window and patch too large for the LDS of this tile shape");
+        return -2;
+    }
     // virtual position written to the last row/column by the reference's 80x80 tiling
     // (tile_iterator.cpp:37-44, sub_images.cpp:47-69): the last tile's last interior row
     auto vmax = [&](int n) {

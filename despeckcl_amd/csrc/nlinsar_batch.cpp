@@ -104,8 +104,15 @@ int dkx_nlinsar_batch_set(dkx_nlinsar_batch h, const dkx_nlinsar_image* images, 
             W[i] = im.width;
         }
         // everything that can fail without the device is checked before touching the batch
+        // the tile shape: dkx_set_tile's, else the autotuner's record for the batch's pixel
+        // count, else the built-in rule
+        double px = 0;
+        for (int i = 0; i < n; i++) px += (double)H[i] * W[i];
+        int tile = dkx_get_tile(1);
+        if (tile == NLINSAR_TILE_AUTO) tile = tuned_tile(1, 0, sw, &p, 1, nullptr, 0, px);
+        if (tile < 0) tile = NLINSAR_TILE_AUTO;
         NlinsarProblem pb;
-        int rc = nlinsar_problem(0, 0, sw, p, niter, lmin, dkx_get_tile(1), &pb);
+        int rc = nlinsar_problem(0, 0, sw, p, niter, lmin, tile, &pb);
         if (rc) return rc;
         std::vector<NlinsarPlan> plans;
         NlinsarBatchLaunch L;

@@ -79,6 +79,49 @@ def get_tile(filter_index):
     return lib().dkx_get_tile(int(filter_index))
 
 
+def nlsar_insar_autotune(a1, a2, phase, ref, phase_filt, coh, height, width, search_window_size,
+                         patch_sizes, scale_sizes, nlsar_stats, h=15.0, c=49.0, every_key=False, reps=2,
+                         stream=0):
+    """Time every NL-SAR pass variant on this image (dkx_nlsar_insar_device_autotune), record
+    the fastest for its parameters and size; returns (variant, [ms per variant])."""
+    from . import _stats_structs
+    P, nP = int_array(patch_sizes)
+    S, nS = int_array(scale_sizes)
+    arr, ns, keep = _stats_structs(nlsar_stats)
+    t = C.c_int(-1)
+    ms = (C.c_float * 3)()
+    check(lib().dkx_nlsar_insar_device_autotune(a1, a2, phase, ref, phase_filt, coh, int(height), int(width),
+                                                int(search_window_size), P, nP, S, nS, arr, ns, float(h),
+                                                float(c), 1 if every_key else 0, int(reps), stream or None,
+                                                C.byref(t), ms))
+    return t.value, [ms[i] for i in range(3)]
+
+
+def nlinsar_autotune(a1, a2, phase, ref, phase_filt, coh, height, width, search_window_size, patch_size,
+                     niter, lmin, reps=1, stream=0):
+    """Time every NL-InSAR tile shape on this image (dkx_nlinsar_device_autotune; the
+    estimates in ref/phase_filt/coh are restored before each run), record the fastest for
+    its parameters and size; returns (shape, [ms per shape, -1 where not allowed])."""
+    t = C.c_int(-1)
+    ms = (C.c_float * 3)()
+    check(lib().dkx_nlinsar_device_autotune(a1, a2, phase, ref, phase_filt, coh, int(height), int(width),
+                                            int(search_window_size), int(patch_size), int(niter), int(lmin),
+                                            int(reps), stream or None, C.byref(t), ms))
+    return t.value, [ms[i] for i in range(3)]
+
+
+def tuned_tile(filter_index, height, width, search_window_size, patch_sizes, scale_sizes=(), every_key=False):
+    """The autotuner's recorded variant for these parameters and image size, -1 if none."""
+    P, nP = int_array(patch_sizes)
+    S, nS = int_array(scale_sizes) if len(scale_sizes) else (None, 0)
+    return lib().dkx_get_tuned_tile(int(filter_index), int(height), int(width), int(search_window_size), P, nP,
+                                    S, nS, 1 if every_key else 0)
+
+
+def clear_tuned_tiles():
+    lib().dkx_clear_tuned_tiles()
+
+
 class NlsarBatch:
     """Batched NL-SAR on device buffers: many images of any size, one parameter set, one
     launch per stage over the tiles of all images (dkx_nlsar_batch_* in the C ABI).

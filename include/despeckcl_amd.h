@@ -210,6 +210,38 @@ double dkx_chi2_inv(double p, double nu);
 int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
 int dkx_get_tile(int filter);
 
+/* ---- autotuner -----------------------------------------------------------------------------
+ * No counterpart in the reference, whose tile_size (src/compute/nlsar/tile_size.cpp:31-114)
+ * picks host tile dimensions from the device memory size alone; here every image is one
+ * device call and the choice that matters is the kernels' tile variant.  The entry points
+ * below run the filter on the caller's image once per variant (one warm-up run, then `reps`
+ * timed runs each), record the fastest for the parameter set and the image size (log2 of the
+ * pixel count) and return it; ms (optional, 3 entries) receives each variant's mean time, -1
+ * for a variant the parameters do not allow.  While dkx_set_tile holds -1 for the filter,
+ * single calls, batches (keyed by their total pixel count) and the server use a recorded
+ * choice where one exists, else the built-in rule.  The outputs hold the filter's result
+ * afterwards (every variant gives the same bits); NL-InSAR restores the first-iteration
+ * estimates in ref/phase/coh_filt before every run. */
+int dkx_nlsar_insar_device_autotune(const float* d_ampl_master, const float* d_ampl_slave,
+                                    const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                                    float* d_coh_filt, int height, int width, int search_window_size,
+                                    const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                                    int n_scale_sizes, const dkx_nlsar_stats* stats, int n_stats,
+                                    float h_param, float c_param, int every_key, int reps, void* stream,
+                                    int* tile, float* ms);
+int dkx_nlinsar_device_autotune(const float* d_ampl_master, const float* d_ampl_slave,
+                                const float* d_phase, float* d_ref_filt, float* d_phase_filt,
+                                float* d_coh_filt, int height, int width, int search_window_size,
+                                int patch_size, int niter, int lmin, int reps, void* stream, int* tile,
+                                float* ms);
+/* The recorded variant for a filter (0 NL-SAR: every_key selects the every-key records; 1
+ * NL-InSAR: patch_sizes[0] is the patch size, scale sizes unused), -1 if none. */
+int dkx_get_tuned_tile(int filter, int height, int width, int search_window_size,
+                       const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                       int n_scale_sizes, int every_key);
+/* Forgets every record. */
+void dkx_clear_tuned_tiles(void);
+
 /* Stage timing.  When enabled, the main launch of every fused stage is bracketed by HIP
  * events on the caller's stream.  dkx_stage_times() synchronises with those events and
  * returns the stages of the calls since the previous dkx_stage_times() call: kinds

@@ -157,6 +157,18 @@ def test_tile_settings(D):
     assert DV.get_tile(0) == -1 and DV.get_tile(1) == -1
 
 
+def test_tuned_tile_records_without_device(D):
+    """The autotuner's record lookup needs no device: nothing is recorded in a fresh process
+    (or after clearing), and invalid filters or sizes read as no record."""
+    from despeckcl_amd import device as DV
+    DV.clear_tuned_tiles()
+    assert DV.tuned_tile(0, 2048, 2048, 21, [3, 5, 7], [1, 3]) == -1
+    assert DV.tuned_tile(0, 2048, 2048, 21, [3, 5, 7], [1, 3], every_key=True) == -1
+    assert DV.tuned_tile(1, 4096, 4096, 21, [7]) == -1
+    assert DV.tuned_tile(2, 64, 64, 21, [7]) == -1
+    assert DV.tuned_tile(1, 0, 64, 21, [7]) == -1
+
+
 def test_training_needs_a_device(D):
     """Training runs on the GPU only: without a device it fails loudly (arguments are still
     checked first)."""
