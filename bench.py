@@ -31,11 +31,11 @@ H = W = 2048
 SW, PATCHES, SCALES = 21, [3, 5, 7], [1, 3]
 HPAR, CPAR = 15.0, 49.0
 PEAK_FP32_TFLOPS = 157.3
-# HBM-side bytes per launch of the selected key's pass: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this benchmark (KiB per dispatch), scaled by what the same counters read for
-# a 256 MiB device-to-device copy (FETCH x 2.000, WRITE x 1.000);
-# profiles/nlsar_2048_scale_pass_pmc.md
-TRAFFIC_LAST_PASS_BYTES = round((462552 * 2.000 + 49344 * 1.000) * 1024)
+# HBM-side bytes per launch of the every-key last pass (the roofline's kernel): rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE on this benchmark (KiB per dispatch, mean of the dispatches
+# that write the ENL / alpha maps), scaled by what the same counters read for a 256 MiB
+# device-to-device copy (FETCH x 2.000, WRITE x 1.000); profiles/nlsar_2048_scale_pass_pmc.md
+TRAFFIC_LAST_PASS_BYTES = round((5146314 * 2.000 + 156480 * 1.000) * 1024)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),
@@ -258,6 +258,16 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
         return DV.nlinsar(*[d.ptr for d in d_in], *[d.ptr for d in d_out], n, n, 21, 7, niter, 10,
                           stream=stream.handle, want_count=want)
 
+    tuned = None
+    if tile is None:
+        # the autotuner on this image (the estimates are restored before every run)
+        for d in d_out:
+            d.zero()
+        t, tms = DV.nlinsar_autotune(*[d.ptr for d in d_in], *[d.ptr for d in d_out], n, n, 21, 7, niter, 10,
+                                     reps=1, stream=stream.handle)
+        tuned = {"shape": t, "ms_per_shape": [round(x, 3) for x in tms],
+                 "note": "tile shapes 0 / 1 / 2 = 27 x 54, 18 x 54 (512 threads), 54 x 54 (1024 threads); one run each"}
+        log(f"nlinsar autotune: {tuned}")
     nsm = run(True)
     stream.synchronize()
     lib = D.lib()
@@ -297,6 +307,7 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
         "tflops": round(flop / (ms * 1e-3) / 1e12, 3),
         "weights_ms": [round(x, 3) for x in wts], "smooth_ms": [round(x, 3) for x in sms],
         "smoothed_pixel_iterations": int(nsm),
+        "autotune": tuned,
     }
     # the dominant kernel: a later iteration's weights launch (similarity cache, KL, weights,
     # ENL and weighted means), 31,791 algorithmic FLOP per pixel
@@ -668,6 +679,14 @@ def main():
     # the same output from the selected key's statistics alone (best_params.cpp:27-53)
     call_sel = lambda: DV.nlsar_insar(*[d.ptr for d in d_in], *[d.ptr for d in d_out], H, W, SW, PATCHES,
                                       SCALES, stats, HPAR, CPAR, stream=stream.handle)
+    # the autotuner: every pass variant timed on this image, for the every-key and the
+    # selected-key forms; the automatic choice of the calls below takes its records
+    tuned = {}
+    for ek in (True, False):
+        t, tms = DV.nlsar_insar_autotune(*[d.ptr for d in d_in], *[d.ptr for d in d_out], H, W, SW, PATCHES,
+                                         SCALES, stats, HPAR, CPAR, every_key=ek, reps=2, stream=stream.handle)
+        tuned["every_key" if ek else "selected_key"] = {"variant": t, "ms_per_variant": [round(x, 4) for x in tms]}
+    log(f"autotune: {tuned}")
     ub = DV.nlsar_insar(*[d.ptr for d in d_in], *[d.ptr for d in d_out], H, W, SW, PATCHES, SCALES,
                         stats, HPAR, CPAR, stream=stream.handle, want_ub=True)
     lib = D.lib()
@@ -743,6 +762,8 @@ def main():
                               "ms_per_step": round(ms_sel, 4), "value": round(H * W / 1e6 / (ms_sel / 1e3), 3),
                               "pass_ms": round(sel_ms, 4), "pass_tflops": round(sel_ach, 3),
                               "flop_per_px_pass": FLOP_PER_PX_LAST_KEY_PASS},
+        "autotune": dict(tuned, note="pass variants 0 / 1 / 2 = 4 / 2 / 1 rows per thread, 32 x 32 tiles; "
+                                     "mean of 2 runs each on this image"),
         "gpu": gpu_info(),
         "p99_latency_ms": None,
         "ub_pixels": int(ub),
@@ -766,7 +787,7 @@ def main():
         # NL-InSAR 4096^2, 10 iterations with point scatterers, and the 64-image batch
         nl = nlinsar_run(4096, 10, 2, 1)
         out["nlinsar_4096"] = {k: nl[k] for k in ("config", "ms_per_step", "value", "mpx_iter_s", "tflops",
-                                                   "roofline", "weights_ms", "smooth_ms")}
+                                                   "roofline", "weights_ms", "smooth_ms", "autotune")}
         bt = batched_run(64, 3, 1)
         out["batched_64"] = {k: bt[k] for k in ("config", "value", "ms_per_step", "plain_per_call",
                                                  "batched_eager", "batched_graph", "speedup_vs_plain")}

@@ -213,6 +213,13 @@ static void launch_sar(Server* s, int lane)
     const int n = (int)descs.size();
     int rows_c, rows_a, blocks, mcw, maw;
     nlsar_batch_layout(descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
+    // the pass variant: the autotuner's record for this batch's pixel count, else per pass
+    double px = 0;
+    for (const NlsarImageDesc& d : descs) px += (double)d.H * d.W;
+    int tile = tuned_tile(0, 0, s->pp.sw, s->P.data(), (int)s->P.size(), s->S.data(), (int)s->S.size(), px);
+    if (tile < 0) tile = NLSAR_TILE_AUTO;
+    s->pp.tile = tile;
+    s->cm.tile = tile;
     // the lane's table was reserved at creation; its previous batch has finished (the lane
     // was free) and the stream orders the copy after it
     const size_t table_bytes = sizeof(NlsarImageDesc) * (size_t)n;
@@ -399,7 +406,8 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
             s->views.push_back(v);
         }
         s->pp = NlsarParams{0, search_window_size, s->P.data(), n_patch_sizes, s->S.data(), n_scale_sizes,
-                            s->views.data(), (int)s->views.size(), h_param, c_param, 1, 0, last_unused};
+                            s->views.data(), (int)s->views.size(), h_param, c_param, NLSAR_TILE_AUTO, 0,
+                            last_unused};
         rc = nlsar_common(s->pp, &s->cm);
         std::vector<float> blob;
         if (!rc) rc = nlsar_build_constants(s->pp, s->cm, blob);

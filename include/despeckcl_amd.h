@@ -260,8 +260,8 @@ void dkx_release_workspace(void);
  * (src/compute/nlsar/nlsar.cpp:81-136, map_filter_tiles.h:37-104).  For callers that filter
  * many interferograms of any size with the same parameters: every stage runs as one launch
  * over the tiles of all images, and the whole stage sequence can be captured in a HIP graph.
- * Each image's result equals the result of dkx_nlsar_insar_device on that image alone with
- * the same pass variant (dkx_set_tile / dkx_nlsar_batch_set_tile). */
+ * Each image's result equals the result of dkx_nlsar_insar_device on that image alone (every
+ * pass variant gives the same bits). */
 
 /* One image of a batch: device pointers, row-major height x width float32 planes. */
 typedef struct {
@@ -345,7 +345,7 @@ int dkx_nlinsar_batch_smoothed(dkx_nlinsar_batch batch, unsigned long long* coun
  * weight LUTs resident, admits requests as memory frees up, and batches the waiting requests
  * of each filter into one launch per stage while earlier work still runs (NL-SAR and NL-InSAR
  * batches on streams of their own, overlapping).  Each request's result equals
- * the single-call result (dkx_nlsar_insar_device with the default pass variant, or
+ * the single-call result (dkx_nlsar_insar_device, or
  * dkx_nlinsar_device from zero estimates). */
 
 typedef struct {
