@@ -292,6 +292,7 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
     lib.dkx_set_timing(0)
     wts = [times[i] for i in range(k) if kinds[i] == 11]
     sms = [times[i] for i in range(k) if kinds[i] == 12]
+
     px = n * n
     flop = 31791.0 * px * niter
     out = {
@@ -310,7 +311,8 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
         "autotune": tuned,
     }
     # the dominant kernel: a later iteration's weights launch (similarity cache, KL, weights,
-    # ENL and weighted means), 31,791 algorithmic FLOP per pixel
+    # ENL and weighted means), 31,791 algorithmic FLOP per pixel; a single call's launch
+    # covers every pixel (the interior and the last row / column mappings)
     later = wts[1:] if len(wts) > 1 else wts
     wk_ms = float(np.mean(later))
     ach = 31791.0 * px / (wk_ms * 1e-3) / 1e12

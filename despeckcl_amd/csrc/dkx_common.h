@@ -108,6 +108,7 @@ enum StageKind {
     STAGE_NLSAR_LAST = 2,        // NL-SAR last pass (statistics + weighted mean + outputs)
     STAGE_NLINSAR_WEIGHTS = 11,  // NL-InSAR weights / ENL / estimates
     STAGE_NLINSAR_SMOOTH = 12,   // NL-InSAR lmin smoothing
+    STAGE_NLINSAR_EDGE = 13,     // NL-InSAR weights of the last row / column (thin blocks)
 };
 // Autotuner records (tune.cpp): the fastest tile variant for filter 0 (NL-SAR; variant 1 =
 // every key's statistics, 0 = the selected key's; P, S = patch and scale sizes) or 1

@@ -14,11 +14,23 @@ struct NlinsarProblem {
     float h, T;       // weight parameters (nlinsar.cpp:65-78)
     int ref_tile;     // side of the reference's fixed tiles (80, nlinsar.cpp:106)
     int tile;         // tile shape index, or NLINSAR_TILE_AUTO
+    int thin_edges = 0;   // 1: the last row / column in thin blocks of their own (batches)
 };
 
 // Tile shape chosen from the image size: 54 x 54 with 1024 threads from 2^20 pixels up,
 // 18 x 54 with 512 threads below (where the larger blocks would leave CUs idle).
 constexpr int NLINSAR_TILE_AUTO = -1;
+
+// Block shapes of the weights kernel per launch mapping.  Shape 0 is the tile shape.  The
+// last row and the corner (shape 1) and the last column (shape 2), which the reference's
+// tiling computes at virtual positions one row / column wide, either run in shape 0 with
+// the interior (single calls: one launch, the edge blocks' latency hidden among the
+// interior's) or, for batches (NlinsarProblem::thin_edges), in thin blocks of their own
+// (1 x 54 and 54 x 3), whose launches cover the edges of all images at once.
+constexpr int NLI_SHAPES = 3;
+// A queued pixel's weight slot: the shape in the bits from NLI_SLOT_BITS up, below them the
+// slot within that shape's weight region.
+constexpr int NLI_SLOT_BITS = 29;
 
 struct NlinsarPlan {
     int tile, TY, TX;
@@ -27,8 +39,9 @@ struct NlinsarPlan {
     Grid R, E, F;                  // float4 grids: raw (a1, a2, phase), estimates (ref, phase,
                                    // coh), filter values (a, Re x, Im x)
     int heap_block;                // threads per block of the smoothing kernel
-    int blocks;                    // blocks of the weights kernel (weight / cache chunks)
-    size_t off_R, off_A, off_E, off_F, off_est, off_w, off_s, off_list, off_xlist, off_cnt, ws_bytes;
+    int blocks[NLI_SHAPES];        // blocks of the weights kernel per shape (weight / cache chunks)
+    size_t off_w[NLI_SHAPES], off_s[NLI_SHAPES];   // weight and similarity-cache regions per shape
+    size_t off_R, off_A, off_E, off_F, off_est, off_list, off_xlist, off_cnt, ws_bytes;
 };
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);
@@ -58,11 +71,14 @@ struct NlinsarBatchItem {
 // What a batch's launches need on the host, and the layout of its device tables.
 struct NlinsarBatchLaunch {
     int nimg, niter, tile, psh, sw, lmin;
-    int total_tiles;        // weights tiles of all images
+    int total_tiles[NLI_SHAPES];   // weights tiles of all images, per shape
     int persistent_blocks;  // grid of the persistent weights kernel
     long long max_n;        // pixels of the largest image
     int heap_block;
-    size_t off_img, off_w, off_s, off_q, bytes;   // device tables, relative to their base
+    size_t off_img, off_w, off_s, off_q, bytes;   // device tables, relative to their base:
+                                                  // weights args [shape][kind][image],
+                                                  // smoothing args [kind][image], 8 queue
+                                                  // counters per shape
 };
 
 // Per-image plans of a batch (all with the batch's tile shape; their ws_bytes give each

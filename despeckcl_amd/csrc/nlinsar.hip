@@ -147,15 +147,26 @@ __device__ __forceinline__ float nl_pixel_kl_pre(float4 e1, float4 e2)
 
 __device__ __forceinline__ bool nl_is_valid(float x) { return !(isnan(x) || isinf(x) || x == 0); }
 
+// The new estimate's reflectivity, phase and coherence (compute_insar.cl:95-101), in a
+// function of its own that every kernel calls rather than inlines: inlined, the device
+// library's atan2f was compiled to different instructions in different kernels (one ulp
+// apart in the phase), and every kernel that writes an estimate must give the same bits
+// (the single-image and batch paths compute the edge pixels in different block shapes).
+__device__ __attribute__((noinline)) float4 nl_estimate_values(float N, float at, float xr, float xi)
+{
+    return make_float4(at / N, atan2f(xi, xr), sqrtf(xr * xr + xi * xi) / at, 0.0f);
+}
+
 // compute_insar.cl:95-117: new estimate, or the previous one where the value is invalid
 __device__ __forceinline__ void nl_estimate(float N, float at, float xr, float xi, long long opix,
                                             const float* old_ref, const float* old_pha,
                                             const float* old_coh, float* nref, float* npha,
                                             float* ncoh)
 {
-    const float reflectivity = at / N;
-    const float iphase = atan2f(xi, xr);
-    const float coherence = sqrtf(xr * xr + xi * xi) / at;
+    const float4 v = nl_estimate_values(N, at, xr, xi);
+    const float reflectivity = v.x;
+    const float iphase = v.y;
+    const float coherence = v.z;
     const float pi_f = 3.14159274101257324219f;
     nref[opix] = nl_is_valid(reflectivity) ? reflectivity : old_ref[opix];
     if (nl_is_valid(iphase)) {
@@ -203,6 +214,7 @@ struct WeightArgs {
     int* list;
     unsigned int* cnt;
     int block0;           // batch: first tile of this image in the batch's tile order
+    int shape;            // block shape (NLI_SHAPES) of the launch: tags the weight slots
 };
 
 constexpr int NLI_BAND = 4;   // block rows per band of the block order
@@ -596,7 +608,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
             if (queued) {
                 const unsigned int k = base + __popcll(qmask & ((1ull << lane) - 1ull));
                 a.list[2 * k] = (int)opix[q];
-                a.list[2 * k + 1] = t * NPIX + tid + q * NCT;   // weight slot
+                a.list[2 * k + 1] = (a.shape << NLI_SLOT_BITS) | (t * NPIX + tid + q * NCT);   // weight slot
             }
         }
         if (valid[q] && !queued)
@@ -652,8 +664,8 @@ struct SmoothArgs {
     int wsh;
     int lmin;
     int H, W, eh, ew;
-    int npix;           // weight slots per block (TY * TX of the weights kernel)
-    const float* wt;
+    int npix[NLI_SHAPES];        // weight slots per block of each shape (TY * TX)
+    const float* wt[NLI_SHAPES]; // weight region of each shape
     const float* old_ref;
     const float* old_pha;
     const float* old_coh;
@@ -748,6 +760,7 @@ struct SmoothPixel {
     const float* A;       // a1 on R's element layout
     const float4* F4;
     const float* wts;     // weight of offset k at wts[k * npix]
+    int npix;             // plane stride of the weights (slots per block of their shape)
     unsigned int rb, fb;  // R / F element of window offset (0, 0) (grids hold < 2^30)
     float as2;            // 2 * amplitude of the pixel
 };
@@ -771,7 +784,7 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
         for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull) {
                 at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
-                wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                wv[j] = px.wts[(unsigned int)((k0 + j) * px.npix)];
             }
             if (++ww == sw) { ww = 0; hh++; }
         }
@@ -821,7 +834,7 @@ __device__ void nl_smooth_exact(const SmoothArgs& a, const SmoothPixel& px, floa
 #pragma unroll
         for (int j = 0; j < XSCH; j++) {
             if (k0 + j < nfull) {
-                wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                wv[j] = px.wts[(unsigned int)((k0 + j) * px.npix)];
                 f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
             }
             if (++ww == sw) { ww = 0; hh++; }
@@ -850,14 +863,19 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 {
     const int nfull = (2 * a.wsh + 1) * (2 * a.wsh + 1);
     *opix = a.list[2 * li];
-    const int slot = a.list[2 * li + 1];
+    const int tagged = a.list[2 * li + 1];
+    const int shape = tagged >> NLI_SLOT_BITS;
+    const int slot = tagged & ((1 << NLI_SLOT_BITS) - 1);
+    const int npix = shape == 0 ? a.npix[0] : shape == 1 ? a.npix[1] : a.npix[2];
+    const float* wbase = shape == 0 ? a.wt[0] : shape == 1 ? a.wt[1] : a.wt[2];
     const int oy = (int)(*opix / a.W), ox = (int)(*opix - (long long)oy * a.W);
     const int vy = oy == a.H - 1 ? oy + a.eh : oy;
     const int vx = ox == a.W - 1 ? ox + a.ew : ox;
     SmoothPixel px;
     px.A = a.A;
     px.F4 = reinterpret_cast<const float4*>(a.F.base);
-    px.wts = a.wt + ((long long)(slot / a.npix) * nfull * a.npix + slot % a.npix);
+    px.wts = wbase + ((long long)(slot / npix) * nfull * npix + slot % npix);
+    px.npix = npix;
     px.rb = (unsigned int)a.R.at(vy - a.wsh, vx - a.wsh);
     px.fb = (unsigned int)a.F.at(vy - a.wsh, vx - a.wsh);
     // at < 2.0 * (double)as in the reference: 2 * as is exact in float unless it
@@ -924,7 +942,7 @@ __device__ __forceinline__ void nl_smooth_image(const SmoothArgs& a, unsigned in
                 for (int j = 0; j < CH; j++) {
                     if (k0 + j < nfull) {
                         at[j] = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)];
-                        wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                        wv[j] = px.wts[(unsigned int)((k0 + j) * px.npix)];
                     }
                     if (++ww == sw) { ww = 0; hh++; }
                 }
@@ -973,7 +991,7 @@ __device__ __forceinline__ void nl_smooth_image(const SmoothArgs& a, unsigned in
 #pragma unroll
             for (int j = 0; j < CH; j++) {
                 if (k0 + j < nfull) {
-                    wv[j] = px.wts[(unsigned int)((k0 + j) * a.npix)];
+                    wv[j] = px.wts[(unsigned int)((k0 + j) * px.npix)];
                     f[j] = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
                 }
                 if (++ww == sw) { ww = 0; hh++; }
@@ -1086,7 +1104,7 @@ __device__ __forceinline__ void nl_smooth_wave_image(const SmoothArgs& a, unsign
             if (k < nfull) {
                 const int hh = k / sw, ww = k - (k / sw) * sw;
                 cand = px.A[px.rb + (unsigned int)(hh * a.R.pitch + ww)] < px.as2;
-                W[k] = px.wts[(unsigned int)(k * a.npix)];
+                W[k] = px.wts[(unsigned int)(k * px.npix)];
                 const float4 f = px.F4[px.fb + (unsigned int)(hh * a.F.pitch + ww)];
                 Fx[k] = f.x;
                 Fy[k] = f.y;
@@ -1198,6 +1216,29 @@ static size_t nlinsar_lds_bytes(int TY, int TX, int wsh, int psh)
 
 static const int kTY[3] = {27, 18, 54};
 static const int kTX[3] = {54, 54, 54};
+// the thin shapes of the last row / corner and of the last column (NLI_SHAPES)
+// (512 threads: a thin block is a latency-bound chain of 441 offsets, so each thread takes
+// at most two positions per step)
+constexpr int NLI_ROW_TY = 1, NLI_ROW_TX = 54, NLI_ROW_NT = 512;
+constexpr int NLI_COL_TY = 54, NLI_COL_TX = 3, NLI_COL_NT = 512;
+
+static void shape_dims(int shape, int tile, int* TY, int* TX)
+{
+    *TY = shape == 0 ? kTY[tile] : shape == 1 ? NLI_ROW_TY : NLI_COL_TY;
+    *TX = shape == 0 ? kTX[tile] : shape == 1 ? NLI_ROW_TX : NLI_COL_TX;
+}
+
+// The launch mappings a shape covers (the others emptied).  thin: 0 the interior, 1 the last
+// row and the corner, 2 the last column; else shape 0 covers all four.
+static void shape_mappings(const Launch* L, int shape, bool thin, Launch* out)
+{
+    for (int i = 0; i < 4; i++) {
+        const bool mine = !thin ? shape == 0
+                        : shape == 0 ? i == 0 : shape == 1 ? (i == 1 || i == 3) : i == 2;
+        out[i] = L[i];
+        if (!mine) out[i].ny = out[i].nx = 0;
+    }
+}
 
 // Block grid of the weights kernel over the four launch mappings: reference tiles covered,
 // times sub-tiles per reference tile; the one-row / one-column mappings use a single block
@@ -1272,7 +1313,9 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->eh = vmax(pb.H) - (pb.H - 1);
     plan->ew = vmax(pb.W) - (pb.W - 1);
     const int H = pb.H, W = pb.W;
-    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->step) + plan->TY, H - 1 + plan->eh + plan->TY);
+    // rows: the interior's blocks and the last column's 54-row blocks; columns: 54 wide blocks
+    const int TYg = std::max(plan->TY, NLI_COL_TY);
+    const int Hx = std::max(round_up(std::max(H - 1, 1), plan->step) + TYg, H - 1 + plan->eh + TYg);
     const int Wx = std::max(round_up(std::max(W - 1, 1), plan->step) + plan->TX, W - 1 + plan->ew + plan->TX);
     const int o = plan->overlap;
     plan->R = Grid{nullptr, 0, round_up(Wx + 2 * o, 64), o, o, Hx + 2 * o, Wx + 2 * o};
@@ -1302,10 +1345,20 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     Launch L[4];
     nlinsar_mappings(H, W, plan->eh, plan->ew, L);
     int gx[4], nsx[4], nsy[4], bstart[5];
-    plan->blocks = nlinsar_grid(plan->step, plan->TY, plan->TX, L, gx, nsx, nsy, bstart);
-    const size_t slots = (size_t)plan->blocks * plan->TY * plan->TX;
-    plan->off_w = take(sizeof(float) * (size_t)nfull * slots);
-    plan->off_s = take(sizeof(float) * (size_t)nfull * slots);
+    for (int sh = 0; sh < NLI_SHAPES; sh++) {
+        int TY, TX;
+        shape_dims(sh, plan->tile, &TY, &TX);
+        Launch Ls[4];
+        shape_mappings(L, sh, pb.thin_edges != 0, Ls);
+        plan->blocks[sh] = nlinsar_grid(plan->step, TY, TX, Ls, gx, nsx, nsy, bstart);
+        const size_t slots = (size_t)plan->blocks[sh] * TY * TX;
+        if (slots >= (1ull << NLI_SLOT_BITS)) {
+            set_error("nlinsar: image too large for one call (split it into bands)");
+            return -4;
+        }
+        plan->off_w[sh] = take(sizeof(float) * (size_t)nfull * slots);
+        plan->off_s[sh] = take(sizeof(float) * (size_t)nfull * slots);
+    }
     plan->off_list = take(2 * sizeof(int) * n);
     plan->off_xlist = take(sizeof(int) * n);
     plan->off_cnt = take(sizeof(unsigned int) * 4);
@@ -1316,8 +1369,9 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
 static int launch_smoothing(const SmoothBatch& sb, int sw, int lmin, int heap_block, long long n, int ncu,
                             hipStream_t stream);
 
+// One weights launch of one shape over the mappings L (the others emptied); timed as `stage`.
 template <int TY, int TX, int PSH, bool FIRST, int NT>
-static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t stream)
+static int launch_weights(const WeightArgs& base, const Launch* L, int stage, hipStream_t stream)
 {
     const size_t lds = nlinsar_lds_bytes(TY, TX, base.wsh, PSH);
     if (lds > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
@@ -1338,18 +1392,35 @@ static int launch_weights(const WeightArgs& base, const Launch* L, hipStream_t s
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
         DKX_HIP_CHECK(hipEventRecord(e1, stream));
-        timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
+        timing_push(stage, e0, e1);
     }
     return 0;
 }
 
+// The weights of one iteration: with thin edges the last-row / corner and last-column
+// shapes, then the interior with the tile shape; else one launch of the tile shape.  a: arguments without mappings, shape or cache / weight
+// regions; wt / sc: each shape's regions.
 template <int PSH, bool FIRST>
-static int launch_weights_tile(int tile, const WeightArgs& a, const Launch* L, hipStream_t stream)
+static int launch_weights_all(int tile, bool thin, const WeightArgs& a, const Launch* L, float* const* wt,
+                              float* const* sc, hipStream_t stream)
 {
+    Launch Ls[4];
+    WeightArgs w = a;
+    int rc;
+    shape_mappings(L, 1, thin, Ls);
+    w.shape = 1; w.wt = wt[1]; w.scache = sc[1];
+    rc = launch_weights<NLI_ROW_TY, NLI_ROW_TX, PSH, FIRST, NLI_ROW_NT>(w, Ls, STAGE_NLINSAR_EDGE, stream);
+    if (rc) return rc;
+    shape_mappings(L, 2, thin, Ls);
+    w.shape = 2; w.wt = wt[2]; w.scache = sc[2];
+    rc = launch_weights<NLI_COL_TY, NLI_COL_TX, PSH, FIRST, NLI_COL_NT>(w, Ls, STAGE_NLINSAR_EDGE, stream);
+    if (rc) return rc;
+    shape_mappings(L, 0, thin, Ls);
+    w.shape = 0; w.wt = wt[0]; w.scache = sc[0];
     switch (tile) {
-    case 0: return launch_weights<27, 54, PSH, FIRST, 512>(a, L, stream);
-    case 1: return launch_weights<18, 54, PSH, FIRST, 512>(a, L, stream);
-    default: return launch_weights<54, 54, PSH, FIRST, 1024>(a, L, stream);
+    case 0: return launch_weights<27, 54, PSH, FIRST, 512>(w, Ls, STAGE_NLINSAR_WEIGHTS, stream);
+    case 1: return launch_weights<18, 54, PSH, FIRST, 512>(w, Ls, STAGE_NLINSAR_WEIGHTS, stream);
+    default: return launch_weights<54, 54, PSH, FIRST, 1024>(w, Ls, STAGE_NLINSAR_WEIGHTS, stream);
     }
 }
 
@@ -1365,12 +1436,16 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     E.base = (float*)(ws + plan.off_E);
     F.base = (float*)(ws + plan.off_F);
     float* est = (float*)(ws + plan.off_est);  // two sets of (ref, phase, coh)
-    float* wt = (float*)(ws + plan.off_w);
+    float* wt[NLI_SHAPES];
+    float* scache[NLI_SHAPES];
+    for (int sh = 0; sh < NLI_SHAPES; sh++) {
+        wt[sh] = (float*)(ws + plan.off_w[sh]);
+        scache[sh] = (float*)(ws + plan.off_s[sh]);
+    }
     int* list = (int*)(ws + plan.off_list);
     unsigned int* cnt = (unsigned int*)(ws + plan.off_cnt);
 
     unsigned int* flag = cnt + 2;
-    float* scache = (float*)(ws + plan.off_s);
     hipLaunchKernelGGL(nlinsar_clear_counters, dim3(1), dim3(64), 0, stream, cnt, flag, 1);
     DKX_HIP_CHECK(hipGetLastError());
     {
@@ -1416,19 +1491,18 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         wa.wsim = pb.ref_tile - pb.sw + 1;
         wa.rh = 1.0f / pb.h; wa.rT = 1.0f / pb.T; wa.lmin = (float)pb.lmin;
         wa.H = H; wa.W = W;
-        wa.scache = scache;
-        wa.wt = wt;
         wa.kl_on = flag;
         wa.old_ref = cur[0]; wa.old_pha = cur[1]; wa.old_coh = cur[2];
         wa.new_ref = nxt[0]; wa.new_pha = nxt[1]; wa.new_coh = nxt[2];
         wa.list = list; wa.cnt = cnt;
+        wa.block0 = 0;
         int rc;
         if (plan.psh == 1)
-            rc = it == 0 ? launch_weights_tile<1, true>(plan.tile, wa, L, stream)
-                         : launch_weights_tile<1, false>(plan.tile, wa, L, stream);
+            rc = it == 0 ? launch_weights_all<1, true>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream)
+                         : launch_weights_all<1, false>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream);
         else if (plan.psh == 3)
-            rc = it == 0 ? launch_weights_tile<3, true>(plan.tile, wa, L, stream)
-                         : launch_weights_tile<3, false>(plan.tile, wa, L, stream);
+            rc = it == 0 ? launch_weights_all<3, true>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream)
+                         : launch_weights_all<3, false>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream);
         else {
             set_error("nlinsar: only patch sizes 3 and 7 are supported");
             rc = -4;
@@ -1437,8 +1511,12 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         SmoothArgs sa;
         sa.R = R; sa.F = F; sa.wsh = plan.wsh; sa.lmin = pb.lmin;
         sa.H = H; sa.W = W; sa.eh = plan.eh; sa.ew = plan.ew;
-        sa.npix = plan.TY * plan.TX;
-        sa.wt = wt;
+        for (int sh = 0; sh < NLI_SHAPES; sh++) {
+            int TY, TX;
+            shape_dims(sh, plan.tile, &TY, &TX);
+            sa.npix[sh] = TY * TX;
+            sa.wt[sh] = wt[sh];
+        }
         sa.old_ref = cur[0]; sa.old_pha = cur[1]; sa.old_coh = cur[2];
         sa.new_ref = nxt[0]; sa.new_pha = nxt[1]; sa.new_coh = nxt[2];
         sa.list = list; sa.cnt = cnt;
@@ -1514,7 +1592,7 @@ __global__ void nlinsar_b_clear(const NliImage* __restrict__ im, int nimg, unsig
         c[2] = 0u;
         c[3] = 0u;
     }
-    if (threadIdx.x < 8) queue[threadIdx.x] = 0u;
+    if (threadIdx.x < 8 * NLI_SHAPES) queue[threadIdx.x] = 0u;
 }
 
 // Grid (x, image): blocks x, x + gridDim.x, ... take the rows of their image.
@@ -1581,13 +1659,6 @@ __global__ __launch_bounds__(256) void nlinsar_b_copy_out(const NliImage* __rest
     }
 }
 
-static int weights_tile_shape(int tile, int* TY, int* TX)
-{
-    *TY = kTY[tile];
-    *TX = kTX[tile];
-    return 0;
-}
-
 // Device tables of a batch of n images: descriptors, weight and smoothing arguments of the
 // three iteration kinds, the XCD queues.
 static void batch_table_layout(int n, NlinsarBatchLaunch* L)
@@ -1595,9 +1666,9 @@ static void batch_table_layout(int n, NlinsarBatchLaunch* L)
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t q = off; off += (bytes + 255) / 256 * 256; return q; };
     L->off_img = take(sizeof(NliImage) * (size_t)n);
-    L->off_w = take(sizeof(WeightArgs) * 3 * (size_t)n);
+    L->off_w = take(sizeof(WeightArgs) * NLI_SHAPES * 3 * (size_t)n);
     L->off_s = take(sizeof(SmoothArgs) * 3 * (size_t)n);
-    L->off_q = take(sizeof(unsigned int) * 8);
+    L->off_q = take(sizeof(unsigned int) * 8 * NLI_SHAPES);
     L->bytes = off;
 }
 
@@ -1614,6 +1685,7 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
     }
     // the batch's tile shape: the single-image rule applied to the batch's pixel count
     NlinsarProblem pb = common;
+    pb.thin_edges = 1;
     pb.H = (int)std::min<long long>(px, 1ll << 20);
     pb.W = 1;
     NlinsarPlan probe;
@@ -1621,16 +1693,19 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
     if (rc) return rc;
     const int tile = probe.tile;
     plans.assign(n, NlinsarPlan{});
-    int total = 0;
+    long long total[NLI_SHAPES] = {0, 0, 0};
     for (int i = 0; i < n; i++) {
         pb = common;
+        pb.thin_edges = 1;
         pb.H = H[i];
         pb.W = W[i];
         pb.tile = tile;
         rc = nlinsar_plan(pb, &plans[i]);
         if (rc) return rc;
-        if ((long long)total + plans[i].blocks >= (1ll << 30)) { set_error("nlinsar batch: too many tiles"); return -4; }
-        total += plans[i].blocks;
+        for (int sh = 0; sh < NLI_SHAPES; sh++) {
+            total[sh] += plans[i].blocks[sh];
+            if (total[sh] >= (1ll << 30)) { set_error("nlinsar batch: too many tiles"); return -4; }
+        }
     }
     L->nimg = n;
     L->niter = common.niter;
@@ -1638,7 +1713,7 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
     L->psh = probe.psh;
     L->sw = common.sw;
     L->lmin = common.lmin;
-    L->total_tiles = total;
+    for (int sh = 0; sh < NLI_SHAPES; sh++) L->total_tiles[sh] = (int)total[sh];
     L->persistent_blocks = 0;   // set by the first enqueue (it needs the device)
     L->max_n = max_n;
     L->heap_block = probe.heap_block;
@@ -1660,6 +1735,7 @@ int nlinsar_batch_image_bytes(const NlinsarProblem& common, int H, int W, size_t
         if (common.tile != NLINSAR_TILE_AUTO && t != common.tile) continue;
         if (common.tile == NLINSAR_TILE_AUTO && t == 0) continue;   // the automatic rule picks 1 or 2
         NlinsarProblem pb = common;
+        pb.thin_edges = 1;
         pb.H = H;
         pb.W = W;
         pb.tile = t;
@@ -1681,9 +1757,7 @@ int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<Nlinsar
     NliImage* img = reinterpret_cast<NliImage*>(host.data() + L.off_img);
     WeightArgs* wt = reinterpret_cast<WeightArgs*>(host.data() + L.off_w);
     SmoothArgs* st = reinterpret_cast<SmoothArgs*>(host.data() + L.off_s);
-    int TY, TX;
-    weights_tile_shape(L.tile, &TY, &TX);
-    int block0 = 0;
+    int block0[NLI_SHAPES] = {0, 0, 0};
     for (int i = 0; i < n; i++) {
         const NlinsarPlan& plan = plans[i];
         const NlinsarBatchItem& it = items[i];
@@ -1703,53 +1777,67 @@ int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<Nlinsar
         float* est = (float*)(ws + plan.off_est);
         float* set0[3] = {est, est + npx, est + 2 * npx};
         float* set1[3] = {est + 3 * npx, est + 4 * npx, est + 5 * npx};
+        Launch Lall[4];
+        nlinsar_mappings(it.H, it.W, plan.eh, plan.ew, Lall);
         for (int kind = 0; kind < 3; kind++) {
             float* const* cur = kind == 0 ? g.io : (kind == 1 ? set0 : set1);
             float* const* nxt = kind == 1 ? set1 : set0;
-            WeightArgs& a = wt[kind * n + i];
-            a.R4 = reinterpret_cast<const float4*>(g.R.base);
-            a.E4 = reinterpret_cast<const float4*>(g.E.base);
-            a.F4 = reinterpret_cast<const float4*>(g.F.base);
-            a.rpitch = g.R.pitch;
-            a.rorg = g.R.oy * g.R.pitch + g.R.ox;
-            a.fpitch = g.F.pitch;
-            a.forg = g.F.oy * g.F.pitch + g.F.ox;
-            a.wsh = plan.wsh; a.step = plan.step;
-            a.wsim = common.ref_tile - common.sw + 1;
-            a.rh = 1.0f / common.h; a.rT = 1.0f / common.T; a.lmin = (float)common.lmin;
-            a.H = it.H; a.W = it.W;
-            nlinsar_mappings(it.H, it.W, plan.eh, plan.ew, a.L);
-            const int nb = nlinsar_grid(plan.step, TY, TX, a.L, a.gx, a.nsx, a.nsy, a.bstart);
-            if (nb != plan.blocks) { set_error("nlinsar batch: inconsistent tile grid"); return -1; }
-            a.scache = (float*)(ws + plan.off_s);
-            a.wt = (float*)(ws + plan.off_w);
-            a.kl_on = g.cnt + 2;
-            a.old_ref = cur[0]; a.old_pha = cur[1]; a.old_coh = cur[2];
-            a.new_ref = nxt[0]; a.new_pha = nxt[1]; a.new_coh = nxt[2];
-            a.list = (int*)(ws + plan.off_list);
-            a.cnt = g.cnt;
-            a.block0 = block0;
+            for (int sh = 0; sh < NLI_SHAPES; sh++) {
+                int TY, TX;
+                shape_dims(sh, L.tile, &TY, &TX);
+                WeightArgs& a = wt[((size_t)sh * 3 + kind) * n + i];
+                a.R4 = reinterpret_cast<const float4*>(g.R.base);
+                a.E4 = reinterpret_cast<const float4*>(g.E.base);
+                a.F4 = reinterpret_cast<const float4*>(g.F.base);
+                a.rpitch = g.R.pitch;
+                a.rorg = g.R.oy * g.R.pitch + g.R.ox;
+                a.fpitch = g.F.pitch;
+                a.forg = g.F.oy * g.F.pitch + g.F.ox;
+                a.wsh = plan.wsh; a.step = plan.step;
+                a.wsim = common.ref_tile - common.sw + 1;
+                a.rh = 1.0f / common.h; a.rT = 1.0f / common.T; a.lmin = (float)common.lmin;
+                a.H = it.H; a.W = it.W;
+                shape_mappings(Lall, sh, true, a.L);
+                const int nb = nlinsar_grid(plan.step, TY, TX, a.L, a.gx, a.nsx, a.nsy, a.bstart);
+                if (nb != plan.blocks[sh]) { set_error("nlinsar batch: inconsistent tile grid"); return -1; }
+                a.scache = (float*)(ws + plan.off_s[sh]);
+                a.wt = (float*)(ws + plan.off_w[sh]);
+                a.kl_on = g.cnt + 2;
+                a.old_ref = cur[0]; a.old_pha = cur[1]; a.old_coh = cur[2];
+                a.new_ref = nxt[0]; a.new_pha = nxt[1]; a.new_coh = nxt[2];
+                a.list = (int*)(ws + plan.off_list);
+                a.cnt = g.cnt;
+                a.block0 = block0[sh];
+                a.shape = sh;
+            }
             SmoothArgs& sa = st[kind * n + i];
             sa.R = g.R; sa.F = g.F; sa.wsh = plan.wsh; sa.lmin = common.lmin;
             sa.H = it.H; sa.W = it.W; sa.eh = plan.eh; sa.ew = plan.ew;
-            sa.npix = TY * TX;
-            sa.wt = a.wt;
+            for (int sh = 0; sh < NLI_SHAPES; sh++) {
+                int TY, TX;
+                shape_dims(sh, L.tile, &TY, &TX);
+                sa.npix[sh] = TY * TX;
+                sa.wt[sh] = (const float*)(ws + plan.off_w[sh]);
+            }
             sa.old_ref = cur[0]; sa.old_pha = cur[1]; sa.old_coh = cur[2];
             sa.new_ref = nxt[0]; sa.new_pha = nxt[1]; sa.new_coh = nxt[2];
-            sa.list = a.list; sa.cnt = g.cnt;
+            sa.list = (const int*)(ws + plan.off_list); sa.cnt = g.cnt;
             sa.xlist = (int*)(ws + plan.off_xlist);
             sa.A = g.A;
         }
-        block0 += plan.blocks;
+        for (int sh = 0; sh < NLI_SHAPES; sh++) block0[sh] += plan.blocks[sh];
     }
     (void)d_tables;
     return 0;
 }
 
+// One persistent weights launch of one shape over `total` tiles of the batch (table: that
+// shape's arguments of every image, queue: its 8 counters); timed as `stage`.
 template <int TY, int TX, int PSH, bool FIRST, int NT>
-static int launch_weights_batch(const NlinsarBatchLaunch& L, const WeightArgs* table, unsigned int* queue,
-                                int ncu, hipStream_t stream)
+static int launch_weights_batch(const NlinsarBatchLaunch& L, const WeightArgs* table, int total,
+                                unsigned int* queue, int stage, int ncu, hipStream_t stream)
 {
+    if (total <= 0) return 0;
     const size_t lds = nlinsar_lds_bytes(TY, TX, (L.sw - 1) / 2, PSH);
     if (lds + 16 > 160 * 1024) { set_error("nlinsar: patch too large for the LDS budget"); return -2; }
     const void* fn = (const void*)nlinsar_weights_batch<TY, TX, PSH, FIRST, NT>;
@@ -1758,7 +1846,7 @@ static int launch_weights_batch(const NlinsarBatchLaunch& L, const WeightArgs* t
     DKX_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, lds));
     per_cu = std::max(per_cu, 1);
     // every XCD queue needs a block: at least 8, at most what is resident at once
-    const int grid = std::max(8, std::min(per_cu * ncu, (L.total_tiles + 7) / 8 * 8));
+    const int grid = std::max(8, std::min(per_cu * ncu, (total + 7) / 8 * 8));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = timing_on();
     if (timed) {
@@ -1767,23 +1855,32 @@ static int launch_weights_batch(const NlinsarBatchLaunch& L, const WeightArgs* t
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
     hipLaunchKernelGGL((nlinsar_weights_batch<TY, TX, PSH, FIRST, NT>), dim3(grid), dim3(NT), lds, stream, table,
-                       L.nimg, L.total_tiles, queue);
+                       L.nimg, total, queue);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
         DKX_HIP_CHECK(hipEventRecord(e1, stream));
-        timing_push(STAGE_NLINSAR_WEIGHTS, e0, e1);
+        timing_push(stage, e0, e1);
     }
     return 0;
 }
 
+// The weights of one iteration of a batch: the thin shapes, then the interior.  wk: the
+// iteration kind's arguments of shape 0; shape s follows at wk + s * 3 * nimg.
 template <int PSH, bool FIRST>
-static int launch_weights_batch_tile(const NlinsarBatchLaunch& L, const WeightArgs* table, unsigned int* queue,
-                                     int ncu, hipStream_t stream)
+static int launch_weights_batch_all(const NlinsarBatchLaunch& L, const WeightArgs* wk, unsigned int* queue,
+                                    int ncu, hipStream_t stream)
 {
+    const size_t per_shape = (size_t)3 * L.nimg;
+    int rc = launch_weights_batch<NLI_ROW_TY, NLI_ROW_TX, PSH, FIRST, NLI_ROW_NT>(
+        L, wk + per_shape, L.total_tiles[1], queue + 8, STAGE_NLINSAR_EDGE, ncu, stream);
+    if (rc) return rc;
+    rc = launch_weights_batch<NLI_COL_TY, NLI_COL_TX, PSH, FIRST, NLI_COL_NT>(
+        L, wk + 2 * per_shape, L.total_tiles[2], queue + 16, STAGE_NLINSAR_EDGE, ncu, stream);
+    if (rc) return rc;
     switch (L.tile) {
-    case 0: return launch_weights_batch<27, 54, PSH, FIRST, 512>(L, table, queue, ncu, stream);
-    case 1: return launch_weights_batch<18, 54, PSH, FIRST, 512>(L, table, queue, ncu, stream);
-    default: return launch_weights_batch<54, 54, PSH, FIRST, 1024>(L, table, queue, ncu, stream);
+    case 0: return launch_weights_batch<27, 54, PSH, FIRST, 512>(L, wk, L.total_tiles[0], queue, STAGE_NLINSAR_WEIGHTS, ncu, stream);
+    case 1: return launch_weights_batch<18, 54, PSH, FIRST, 512>(L, wk, L.total_tiles[0], queue, STAGE_NLINSAR_WEIGHTS, ncu, stream);
+    default: return launch_weights_batch<54, 54, PSH, FIRST, 1024>(L, wk, L.total_tiles[0], queue, STAGE_NLINSAR_WEIGHTS, ncu, stream);
     }
 }
 
@@ -1860,7 +1957,8 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
     DKX_HIP_CHECK(hipGetLastError());
     for (int it = 0; it < L.niter; it++) {
         const int kind = iteration_kind(it);
-        const WeightArgs* wk = wt + (size_t)kind * n;
+        const WeightArgs* wk = wt + (size_t)kind * n;   // shape 0 of this kind (the per-image
+                                                         // estimate pointers are alike in all shapes)
         hipLaunchKernelGGL(nlinsar_b_fill_estimates, egrid, eblk, 0, stream, img, wk);
         DKX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(nlinsar_b_clear, dim3(1), dim3(256), 0, stream, img, n, queue, 0);
@@ -1869,11 +1967,11 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
         DKX_HIP_CHECK(hipGetLastError());
         int rc;
         if (L.psh == 1)
-            rc = it == 0 ? launch_weights_batch_tile<1, true>(L, wk, queue, ncu, stream)
-                         : launch_weights_batch_tile<1, false>(L, wk, queue, ncu, stream);
+            rc = it == 0 ? launch_weights_batch_all<1, true>(L, wk, queue, ncu, stream)
+                         : launch_weights_batch_all<1, false>(L, wk, queue, ncu, stream);
         else if (L.psh == 3)
-            rc = it == 0 ? launch_weights_batch_tile<3, true>(L, wk, queue, ncu, stream)
-                         : launch_weights_batch_tile<3, false>(L, wk, queue, ncu, stream);
+            rc = it == 0 ? launch_weights_batch_all<3, true>(L, wk, queue, ncu, stream)
+                         : launch_weights_batch_all<3, false>(L, wk, queue, ncu, stream);
         else {
             set_error("nlinsar: only patch sizes 3 and 7 are supported");
             rc = -4;
