@@ -90,6 +90,7 @@ SIGNATURES = {
     "dkx_nlinsar_device_autotune": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _P, _IP, _P]),
     "dkx_get_tuned_tile": (_I, [_I, _I, _I, _I, _IP, _I, _IP, _I, _I]),
     "dkx_clear_tuned_tiles": (None, []),
+    "dkx_nlinsar_simulate_h": (_I, [_I, C.c_uint, C.POINTER(C.c_float)]),
     "dkx_nlinsar_batch_create": (_I, [C.POINTER(C.c_void_p)]),
     "dkx_nlinsar_batch_destroy": (None, [_P]),
     "dkx_nlinsar_batch_set": (_I, [_P, C.POINTER(DkxImage), _I, _I, _I, _I, _I]),

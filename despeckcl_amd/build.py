@@ -22,7 +22,7 @@ BUILD_DIR = os.path.join(ROOT, "_build", "despeckcl_amd")
 ARCH = os.environ.get("DKX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["nlsar.hip", "nlinsar.hip", "server.hip", "noalloc.hip", "capi.cpp", "batch.cpp", "nlinsar_batch.cpp", "tune.cpp", "stats.cpp", "stats_io.cpp", "training.hip", "boxcar.hip", "goldstein.hip",
+SOURCES = ["nlsar.hip", "nlinsar.hip", "server.hip", "noalloc.hip", "capi.cpp", "batch.cpp", "nlinsar_batch.cpp", "nlinsar_simu.cpp", "tune.cpp", "stats.cpp", "stats_io.cpp", "training.hip", "boxcar.hip", "goldstein.hip",
            "despeckcl_api.cpp"] + [
     f"nlsar_pass_{kind}_{shape}.hip" for kind in ("stat", "last")
     for shape in ("t32r4_w10", "t32r2_w10", "t32r1_w10", "t32r4_w20", "t32r2_w10_cov2", "t32r4_w20_cov2",

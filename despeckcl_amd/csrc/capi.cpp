@@ -301,15 +301,13 @@ struct NlinsarCall {
 int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, int tile,
                     NlinsarProblem* pb)
 {
-    // nlinsar.cpp:65-78
-    float h_theo;
-    if (p == 3) h_theo = 0.58f;
-    else if (p == 7) h_theo = 0.244f;
-    else {
-        set_error("nlinsar: only patch sizes 3 and 7 are supported (the reference estimates h "
-                  "for other sizes by a non-deterministic Monte-Carlo simulation)");
-        return DKX_EUNSUPPORTED;
+    // nlinsar.cpp:65-78: h from the patch size (tabulated for 3 and 7, else the seeded
+    // restatement of the reference's Monte-Carlo quantile, nlinsar_simu.cpp)
+    if (p < 1 || p % 2 == 0 || p > 11) {
+        set_error("nlinsar: patch sizes must be odd, up to 11 (the weights kernels are compiled for those)");
+        return DKX_EINVAL;
     }
+    const float h_theo = nlinsar_h_theo(p);
     const int patch_area = p * p;
     const float h_para = h_theo * patch_area;
     const float T_para = (float)(2.0 / h_para * M_PI / 4 * patch_area);

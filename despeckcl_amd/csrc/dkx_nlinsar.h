@@ -46,6 +46,14 @@ struct NlinsarPlan {
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);
 
+// Seed of the Monte-Carlo h for patch sizes other than 3 and 7 (std::mt19937's default).
+constexpr unsigned int NLINSAR_H_SEED = 5489u;
+// quantile_insar (insarsim_simu.cpp:62-82) with a seeded generator (nlinsar_simu.cpp).
+float nlinsar_simulate_h(int patch_size, float alpha, unsigned int seed);
+// h_theo of nlinsar.cpp:65-76: 0.58 for patch 3, 0.244 for patch 7, else the simulation at
+// alpha 0.92 with NLINSAR_H_SEED (computed once per patch size).
+float nlinsar_h_theo(int patch_size);
+
 // The reference's parameters for a search window, patch size, iteration count and lmin
 // (h and T from the patch size, 80-pixel tiles), with the given tile shape.
 int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, int tile,

@@ -1278,6 +1278,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
         set_error("nlinsar: search window and patch sizes must be odd");
         return -1;
     }
+    if (pb.p > 11) { set_error("nlinsar: patch sizes must be odd, up to 11"); return -1; }
     if (pb.niter < 0 || pb.lmin < 1 || pb.lmin > pb.sw * pb.sw) {
         set_error("nlinsar: niter must be >= 0 and 1 <= lmin <= search_window_size^2");
         return -1;
@@ -1496,16 +1497,19 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         wa.new_ref = nxt[0]; wa.new_pha = nxt[1]; wa.new_coh = nxt[2];
         wa.list = list; wa.cnt = cnt;
         wa.block0 = 0;
+        const bool thin = pb.thin_edges != 0;
         int rc;
-        if (plan.psh == 1)
-            rc = it == 0 ? launch_weights_all<1, true>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream)
-                         : launch_weights_all<1, false>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream);
-        else if (plan.psh == 3)
-            rc = it == 0 ? launch_weights_all<3, true>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream)
-                         : launch_weights_all<3, false>(plan.tile, pb.thin_edges != 0, wa, L, wt, scache, stream);
-        else {
-            set_error("nlinsar: only patch sizes 3 and 7 are supported");
-            rc = -4;
+        switch (plan.psh) {
+#define DKX_NLI_PSH(P)                                                                        \
+        case P:                                                                               \
+            rc = it == 0 ? launch_weights_all<P, true>(plan.tile, thin, wa, L, wt, scache, stream)  \
+                         : launch_weights_all<P, false>(plan.tile, thin, wa, L, wt, scache, stream); \
+            break;
+        DKX_NLI_PSH(0) DKX_NLI_PSH(1) DKX_NLI_PSH(2) DKX_NLI_PSH(3) DKX_NLI_PSH(4) DKX_NLI_PSH(5)
+#undef DKX_NLI_PSH
+        default:
+            set_error("nlinsar: patch sizes must be odd, up to 11");
+            rc = -1;
         }
         if (rc) return rc;
         SmoothArgs sa;
@@ -1966,15 +1970,17 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
         hipLaunchKernelGGL(nlinsar_b_any_estimate, egrid, eblk, 0, stream, img, wk);
         DKX_HIP_CHECK(hipGetLastError());
         int rc;
-        if (L.psh == 1)
-            rc = it == 0 ? launch_weights_batch_all<1, true>(L, wk, queue, ncu, stream)
-                         : launch_weights_batch_all<1, false>(L, wk, queue, ncu, stream);
-        else if (L.psh == 3)
-            rc = it == 0 ? launch_weights_batch_all<3, true>(L, wk, queue, ncu, stream)
-                         : launch_weights_batch_all<3, false>(L, wk, queue, ncu, stream);
-        else {
-            set_error("nlinsar: only patch sizes 3 and 7 are supported");
-            rc = -4;
+        switch (L.psh) {
+#define DKX_NLI_PSH(P)                                                                        \
+        case P:                                                                               \
+            rc = it == 0 ? launch_weights_batch_all<P, true>(L, wk, queue, ncu, stream)       \
+                         : launch_weights_batch_all<P, false>(L, wk, queue, ncu, stream);     \
+            break;
+        DKX_NLI_PSH(0) DKX_NLI_PSH(1) DKX_NLI_PSH(2) DKX_NLI_PSH(3) DKX_NLI_PSH(4) DKX_NLI_PSH(5)
+#undef DKX_NLI_PSH
+        default:
+            set_error("nlinsar: patch sizes must be odd, up to 11");
+            rc = -1;
         }
         if (rc) return rc;
         SmoothBatch sb;

@@ -67,6 +67,15 @@ def goldstein(a1, a2, phase, ref, phase_filt, height, width, patch_size, overlap
                                      int(patch_size), int(overlap), float(alpha), stream or None))
 
 
+def nlinsar_simulate_h(patch_size, seed=5489):
+    """The reference's Monte-Carlo h_theo for an NL-InSAR patch size (quantile_insar at alpha
+    0.92), with a seeded generator; the filter uses seed 5489 for patch sizes other than 3
+    and 7 (dkx_nlinsar_simulate_h)."""
+    h = C.c_float(0.0)
+    check(lib().dkx_nlinsar_simulate_h(int(patch_size), int(seed), C.byref(h)))
+    return h.value
+
+
 def set_tile(filter_index, tile_index):
     """Block shape of the single-call entry points (dkx_set_tile).  filter 0 (NL-SAR): 0, 1, 2
     = 4, 2, 1 rows per thread; -1 (default) = per pass, 1 row for one patch size, else 2.

@@ -109,11 +109,19 @@ int dkx_nlsar_covmat_device(const float* d_covmat, float* d_covmat_filt, int hei
 
 /* despeckcl::nlinsar (include/despeckcl.h:97-109; src/compute/nlinsar/nlinsar.cpp:42-146).
  * ref_filt / phase_filt / coh_filt are read as the first iteration's estimates and
- * overwritten with the result, like the reference (nlinsar.cpp:59-61).  Patch sizes 3 and
- * 7 only (the reference draws h at random for the others, insarsim_simu.cpp:62-82). */
+ * overwritten with the result, like the reference (nlinsar.cpp:59-61).  Odd patch sizes up
+ * to 11.  h: 0.58 for patch 3 and 0.244 for patch 7 as tabulated by the reference; for the
+ * others the reference's Monte-Carlo quantile (insarsim_simu.cpp:62-82), which it seeds from
+ * std::random_device, here with a fixed seed (dkx_nlinsar_simulate_h with seed 5489). */
 int dkx_nlinsar(const float* ampl_master, const float* ampl_slave, const float* phase,
                 float* ref_filt, float* phase_filt, float* coh_filt, int height, int width,
                 int search_window_size, int patch_size, int niter, int lmin);
+
+/* nlinsar::simu::quantile_insar(patch_size, 0.92) (src/compute/nlinsar/insarsim_simu.cpp:62-82):
+ * the h_theo the reference uses for patch sizes other than 3 and 7, simulated with a
+ * std::mt19937 seeded with `seed` in place of the reference's std::random_device.  Host
+ * only. */
+int dkx_nlinsar_simulate_h(int patch_size, unsigned int seed, float* h_theo);
 
 /* Same on device buffers.  n_smoothed: optional host pointer (synchronises) receiving the
  * number of pixel-iterations that went through the lmin smoothing. */

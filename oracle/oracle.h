@@ -143,6 +143,8 @@ long long orc_training_dissims(const float* covmat, int height, int width, int d
 /* despeckcl::nlinsar, nlinsar.cpp:42-146.  ref/phase/coh are in/out: their input values are
  * the first iteration's estimates (the Python API passes zeros).  tile: reference tile side
  * (80).  *n_smoothed counts pixels that went through the lmin smoothing. */
+/* h_theo for a patch size other than 3 and 7 (the reference simulates it at random). */
+int orc_nlinsar_set_h_theo(int patch_size, float h_theo);
 int orc_nlinsar(const float* a1, const float* a2, const float* phi,
                 float* ref_filt, float* phase_filt, float* coh_filt,
                 int height, int width, int search_window_size, int patch_size, int niter,

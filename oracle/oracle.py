@@ -321,6 +321,12 @@ def training_dissims(covmat, dim, patch_size, scale_size):
     return out[:n]
 
 
+def set_nlinsar_h_theo(patch_size, h_theo):
+    """h_theo for a patch size other than 3 and 7 (the reference draws it at random)."""
+    if lib().orc_nlinsar_set_h_theo(C.c_int(patch_size), C.c_float(h_theo)) != 0:
+        raise ValueError("invalid patch size")
+
+
 def nlinsar(a1, a2, phi, search_window_size, patch_size, niter, lmin, init=None, tile=80,
             want_frag=False):
     """Returns (ref, phase, coh, n_smoothed), plus the last iteration's fragility flags

@@ -452,6 +452,19 @@ def test_nlinsar_small_windows(D, oracle, sw, p, lmin, shape):
     nlinsar_trajectory(D, oracle, a1, a2, ph, sw, p, 2, lmin, f"nlinsar sw {sw} p {p}", frag_frac=0.6)
 
 
+@pytest.mark.parametrize("p,sw,lmin,shape", [(1, 11, 5, (50, 61)), (5, 11, 5, (70, 90)), (9, 15, 10, (64, 70)),
+                                             (11, 11, 5, (60, 75))])
+def test_nlinsar_other_patch_sizes(D, oracle, p, sw, lmin, shape):
+    """Patch sizes whose h the reference simulates (quantile_insar, seeded here): the oracle
+    gets the filter's own h, three iterations checked as a trajectory."""
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import synthetic as S
+    h = DV.nlinsar_simulate_h(p)
+    oracle.set_nlinsar_h_theo(p, h)
+    a1, a2, ph = S.interferogram(*shape, S.SEED_BASE + 7300 + p, n_regions=6, coh_range=(0.3, 0.9))
+    nlinsar_trajectory(D, oracle, a1, a2, ph, sw, p, 3, lmin, f"nlinsar patch {p} (h_theo {h:.4f})", frag_frac=0.6)
+
+
 def test_nlinsar_large_window_large_image(D):
     """A search window of 41 on a 1024 x 1024 image: the automatic choice must not pick the
     54 x 54 blocks, whose LDS (and reference-tile interior of 34) do not suit this window;

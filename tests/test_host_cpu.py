@@ -100,8 +100,10 @@ def test_argument_errors_without_device(D):
         D.goldstein(a, a, a, 32, 16, 0.5)
     with pytest.raises(ValueError, match="quarter"):
         D.goldstein(a, a, a, 16, 5, 0.5)
-    with pytest.raises(ValueError, match="patch sizes 3 and 7"):
-        D.nlinsar(a, a, a, 21, 5, 2, 10)
+    with pytest.raises(ValueError, match="odd, up to 11"):
+        D.nlinsar(a, a, a, 21, 13, 2, 10)
+    with pytest.raises(ValueError, match="odd"):
+        D.nlinsar(a, a, a, 21, 4, 2, 10)
     with pytest.raises(ValueError, match="odd"):
         D.nlinsar(a, a, a, 22, 7, 2, 10)
     with pytest.raises(ValueError, match="lmin"):
@@ -167,6 +169,28 @@ def test_tuned_tile_records_without_device(D):
     assert DV.tuned_tile(1, 4096, 4096, 21, [7]) == -1
     assert DV.tuned_tile(2, 64, 64, 21, [7]) == -1
     assert DV.tuned_tile(1, 0, 64, 21, [7]) == -1
+
+
+def test_nlinsar_simulated_h_reproduces_the_tabulated_values(D):
+    """quantile_insar (insarsim_simu.cpp:62-82) restated with a seeded generator.  The reference
+    tabulates its result for patch 3 (0.58) and 7 (0.244) at alpha 0.92 (nlinsar.cpp:65-76):
+    the simulation must give them within its Monte-Carlo spread, per seed (6 %) and on the mean
+    of four seeds (2 %).  A wrong quantile, a missing negation, the patch side instead of its
+    area, or correlated pixels fail these bounds.  The value decreases with the patch size and
+    depends on the seed only."""
+    from despeckcl_amd import device as DV
+    for p, want in ((3, 0.58), (7, 0.244)):
+        vals = [DV.nlinsar_simulate_h(p, seed) for seed in (5489, 1, 2, 3)]
+        print(p, vals)
+        assert all(abs(v - want) <= 0.06 * want for v in vals), (p, vals)
+        assert abs(np.mean(vals) - want) <= 0.02 * want, (p, vals)
+    hs = [DV.nlinsar_simulate_h(p) for p in (1, 3, 5, 7, 9, 11)]
+    assert all(a > b for a, b in zip(hs, hs[1:])), hs
+    assert DV.nlinsar_simulate_h(5, 7) == DV.nlinsar_simulate_h(5, 7)
+    assert DV.nlinsar_simulate_h(5, 7) != DV.nlinsar_simulate_h(5, 8)
+    for bad in (0, 2, 13):
+        with pytest.raises(ValueError, match="odd"):
+            DV.nlinsar_simulate_h(bad)
 
 
 def test_training_needs_a_device(D):
