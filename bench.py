@@ -292,6 +292,7 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
     lib.dkx_set_timing(0)
     wts = [times[i] for i in range(k) if kinds[i] == 11]
     sms = [times[i] for i in range(k) if kinds[i] == 12]
+    fls = [times[i] for i in range(k) if kinds[i] == 14]
 
     px = n * n
     flop = 31791.0 * px * niter
@@ -307,6 +308,7 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
         "mpx_iter_s": round(px * niter / 1e6 / (ms / 1e3), 3),
         "tflops": round(flop / (ms * 1e-3) / 1e12, 3),
         "weights_ms": [round(x, 3) for x in wts], "smooth_ms": [round(x, 3) for x in sms],
+        "fill_ms": [round(x, 3) for x in fls],
         "smoothed_pixel_iterations": int(nsm),
         "autotune": tuned,
     }
@@ -789,7 +791,8 @@ def main():
         # NL-InSAR 4096^2, 10 iterations with point scatterers, and the 64-image batch
         nl = nlinsar_run(4096, 10, 2, 1)
         out["nlinsar_4096"] = {k: nl[k] for k in ("config", "ms_per_step", "value", "mpx_iter_s", "tflops",
-                                                   "roofline", "weights_ms", "smooth_ms", "autotune")}
+                                                   "roofline", "weights_ms", "smooth_ms", "fill_ms",
+                                                   "autotune")}
         bt = batched_run(64, 3, 1)
         out["batched_64"] = {k: bt[k] for k in ("config", "value", "ms_per_step", "plain_per_call",
                                                  "batched_eager", "batched_graph", "speedup_vs_plain")}

@@ -57,6 +57,8 @@ SIGNATURES = {
     "dkx_nlsar_covmat_device": (_I, [_P, _P, _I, _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F,
                                      _P, _P, _ULLP, _P]),
     "dkx_nlinsar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I]),
+    "dkx_set_nlinsar_weight_store": (_I, [_I]),
+    "dkx_get_nlinsar_weight_store": (_I, []),
     "dkx_boxcar": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I]),
     "dkx_goldstein": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F]),
     "dkx_goldstein_device": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P]),

@@ -151,6 +151,7 @@ static int ensure_stream(DeviceWorkspace* w)
 
 // NL-SAR: chosen per pass (NLSAR_TILE_AUTO); NL-InSAR: chosen per image (NLINSAR_TILE_AUTO)
 static std::atomic<int> g_tile[2] = {{NLSAR_TILE_AUTO}, {NLINSAR_TILE_AUTO}};
+static std::atomic<int> g_nli_store{NLINSAR_STORE_AUTO};   // NLINSAR_STORE_* of single calls
 
 static std::atomic<bool> g_timing{false};
 static std::mutex g_timing_mu;
@@ -325,6 +326,7 @@ static int nlinsar_prepare(NlinsarCall& call, int height, int width, int sw, int
     if (tile < 0) tile = NLINSAR_TILE_AUTO;
     int rc = nlinsar_problem(height, width, sw, p, niter, lmin, tile, &call.pb);
     if (rc) return rc;
+    call.pb.store = g_nli_store.load();
     return nlinsar_plan(call.pb, &call.plan);
 }
 
@@ -422,6 +424,20 @@ int dkx_set_tile(int filter, int tile_index)
 }
 
 int dkx_get_tile(int filter) { return (filter >= 0 && filter <= 1) ? g_tile[filter].load() : -1; }
+
+int dkx_set_nlinsar_weight_store(int mode)
+{
+    return dkx::guarded<int>([&]() -> int {
+        if (mode < NLINSAR_STORE_AUTO || mode > NLINSAR_STORE_NONE) {
+            set_error("dkx_set_nlinsar_weight_store: mode must be -1, 0, 1 or 2");
+            return DKX_EINVAL;
+        }
+        g_nli_store.store(mode);
+        return 0;
+    });
+}
+
+int dkx_get_nlinsar_weight_store(void) { return g_nli_store.load(); }
 
 int dkx_set_timing(int enabled)
 {

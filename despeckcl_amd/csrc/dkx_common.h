@@ -23,6 +23,19 @@ __device__ __forceinline__ DKX_GLOBAL T* gptr(T* p) { return (DKX_GLOBAL T*)p; }
 // an address-space-qualified object; the native vector types do)
 typedef float dkx_f4 __attribute__((ext_vector_type(4)));
 typedef float dkx_f2 __attribute__((ext_vector_type(2)));
+#ifdef __HIP__
+// Values every lane of a wave holds alike, moved to scalar registers (HIP sources only).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)u);
+    const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(u >> 32));
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+#endif
+
 __device__ __forceinline__ float4 gload4(const DKX_GLOBAL void* p)
 {
     const dkx_f4 v = *(const DKX_GLOBAL dkx_f4*)p;
@@ -109,6 +122,7 @@ enum StageKind {
     STAGE_NLINSAR_WEIGHTS = 11,  // NL-InSAR weights / ENL / estimates
     STAGE_NLINSAR_SMOOTH = 12,   // NL-InSAR lmin smoothing
     STAGE_NLINSAR_EDGE = 13,     // NL-InSAR weights of the last row / column (thin blocks)
+    STAGE_NLINSAR_FILL = 14,     // NL-InSAR weights of queued pixels the weights launch did not store
 };
 // Autotuner records (tune.cpp): the fastest tile variant for filter 0 (NL-SAR; variant 1 =
 // every key's statistics, 0 = the selected key's; P, S = patch and scale sizes) or 1

@@ -7,6 +7,17 @@
 
 namespace dkx {
 
+// Which weights the weights launches of a single call store for the lmin smoothing, which
+// needs the weights of the pixels it queues (ENL < lmin); the queued pixels whose weights
+// were not stored get them recomputed, the same bits, so the setting changes the memory
+// traffic and not the result.  Automatic: see nli_store_mode (nlinsar.hip).  Batches store
+// all.
+constexpr int NLINSAR_STORE_AUTO = -1;
+constexpr int NLINSAR_STORE_ALL = 0;    // every weight
+constexpr int NLINSAR_STORE_PRED = 1;   // the pixels the previous iteration queued and their
+                                        // neighbours (none in the first iteration)
+constexpr int NLINSAR_STORE_NONE = 2;   // none: every queued pixel's weights recomputed
+
 struct NlinsarProblem {
     int H, W;
     int sw, p;
@@ -15,10 +26,11 @@ struct NlinsarProblem {
     int ref_tile;     // side of the reference's fixed tiles (80, nlinsar.cpp:106)
     int tile;         // tile shape index, or NLINSAR_TILE_AUTO
     int thin_edges = 0;   // 1: the last row / column in thin blocks of their own (batches)
+    int store = NLINSAR_STORE_AUTO;   // which weights the weights launches store (NLINSAR_STORE_*)
 };
 
-// Tile shape chosen from the image size: 54 x 54 with 1024 threads from 2^20 pixels up,
-// 18 x 54 with 512 threads below (where the larger blocks would leave CUs idle).
+// Tile shape of the built-in rule: 18 x 54 with 512 threads (the autotuner may record
+// another per parameter set and image size).
 constexpr int NLINSAR_TILE_AUTO = -1;
 
 // Block shapes of the weights kernel per launch mapping.  Shape 0 is the tile shape.  The
@@ -31,6 +43,9 @@ constexpr int NLI_SHAPES = 3;
 // A queued pixel's weight slot: the shape in the bits from NLI_SLOT_BITS up, below them the
 // slot within that shape's weight region.
 constexpr int NLI_SLOT_BITS = 29;
+// Slot tag of a compact weight slot (single calls that store only some pixels' weights): the
+// sw*sw weights of the pixel contiguous at slot * sw*sw of the shape-0 weight region.
+constexpr int NLI_COMPACT = NLI_SHAPES;
 
 struct NlinsarPlan {
     int tile, TY, TX;
@@ -41,7 +56,7 @@ struct NlinsarPlan {
     int heap_block;                // threads per block of the smoothing kernel
     int blocks[NLI_SHAPES];        // blocks of the weights kernel per shape (weight / cache chunks)
     size_t off_w[NLI_SHAPES], off_s[NLI_SHAPES];   // weight and similarity-cache regions per shape
-    size_t off_R, off_A, off_E, off_F, off_est, off_list, off_xlist, off_cnt, ws_bytes;
+    size_t off_R, off_A, off_E, off_F, off_est, off_pred, off_caux, off_list, off_xlist, off_cnt, ws_bytes;
 };
 
 int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan);

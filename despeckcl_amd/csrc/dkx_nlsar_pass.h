@@ -131,17 +131,6 @@ __device__ __forceinline__ float lut_weight(float s, float dmin, float dmax, flo
     return s > dmax ? 0.f : w;
 }
 
-// Values every lane of a wave holds alike, moved to scalar registers.
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-template <class T>
-__device__ __forceinline__ T* uni(T* p)
-{
-    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
-    const unsigned int lo = __builtin_amdgcn_readfirstlane((unsigned int)u);
-    const unsigned int hi = __builtin_amdgcn_readfirstlane((unsigned int)(u >> 32));
-    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
-}
-
 __device__ __forceinline__ const NlsarImageDesc& nlsar_image(const ScaleArgs& a, int img)
 {
     return a.table ? a.table[img] : a.one;

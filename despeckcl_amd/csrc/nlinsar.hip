@@ -102,6 +102,8 @@ __global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, in
         cnt[0] = 0u;
         if (clear_total) cnt[1] = 0u;
         cnt[3] = 0u;
+        cnt[4] = 0u;
+        if (clear_total) cnt[5] = 0u;
         *flag = 0u;
     }
 }
@@ -215,10 +217,91 @@ struct WeightArgs {
     unsigned int* cnt;
     int block0;           // batch: first tile of this image in the batch's tile order
     int shape;            // block shape (NLI_SHAPES) of the launch: tags the weight slots
+    // Which pixels' weights the launch stores for the lmin smoothing (NLI_STORE_*; zero, the
+    // batches' value, stores all); nlinsar_fill_weights recomputes the others' when they are
+    // queued.
+    int store;
+    int tag;                      // NLI_STORE_PRED: pixels with pred[pixel] == tag are stored
+    const unsigned char* pred;    // per image pixel, row-major
+    // Compact weights: the stored pixels of a wave take a run of k consecutive slots c0 ..
+    // c0 + k - 1, whose weights lie offset-major at wc + c0*sw*sw (offset p of the run's
+    // pixel r at p*k + r, so that a store of one offset is contiguous); caux[c] = (k << 8) | r
+    // for the queued ones.
+    float* wc;
+    int* caux;
+    const unsigned int* dense;    // NLI_STORE_PRED: nonzero when this iteration stores all instead
+};
+
+// Weight store modes of a weights launch.  The stored weights and the ones
+// nlinsar_fill_weights recomputes are the same bits, so the mode changes only the traffic.
+constexpr int NLI_STORE_ALL = 0;    // every pixel's
+constexpr int NLI_STORE_PRED = 1;   // the pixels marked with the launch's tag
+constexpr int NLI_STORE_NONE = 2;   // none
+constexpr int NLI_STORE_KL = 3;     // all when the KL term is on, none when it is off (a first
+                                    // iteration from zero estimates, where a weight is a
+                                    // function of the cached similarity alone)
+
+// Every pixel's weights stored, in the block-major layout of the tile slots (no compact
+// slots, nothing to recompute).
+__device__ __forceinline__ bool nl_store_dense(const WeightArgs& a, bool kl)
+{
+    return a.store == NLI_STORE_ALL || (a.store == NLI_STORE_KL && kl) ||
+           (a.store == NLI_STORE_PRED && *a.dense != 0u);
+}
+
+// Otherwise: whether the weights launch stores the weights of output pixel opix, in a compact
+// slot (so that sparse stores fill whole cache lines).
+__device__ __forceinline__ bool nl_weight_stored(const WeightArgs& a, long long opix)
+{
+    return a.store == NLI_STORE_PRED && a.pred[opix] == (unsigned char)a.tag;
+}
+
+// Position of weights tile t (a block of TY x TX outputs): its first virtual output position,
+// how many of its rows / columns the reference tile interior holds, the origin of that
+// reference tile's interior, and the launch mapping it belongs to.
+struct NlTile {
+    int ty0, tx0, rows_lim, cols_lim, Y0, X0;
+    Launch L;
 };
 
 constexpr int NLI_BAND = 4;   // block rows per band of the block order
 constexpr int NLI_RSEG = 3;   // row sums per thread (a run of adjacent outputs)
+
+// The four launch mappings (interior, last row, last column, corner) are laid end to end in
+// one grid.  Blocks are sub-tiles of the interiors (step x step) of the reference's tiles, so
+// every output of a block belongs to one reference tile: block row by = reference tile row *
+// nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile of a reference tile is
+// cut at the tile's end.  The one-row / one-column mappings have a single block row / column
+// starting at the mapping's origin.  Tile t (the launch gives each XCD a contiguous run of
+// tiles) is ordered column by column within bands of NLI_BAND block rows: the blocks
+// resident on one XCD then cover a compact patch of the image, whose neighbourhoods overlap
+// in that XCD's L2.
+__device__ __forceinline__ NlTile nl_tile(const WeightArgs& a, int t, int TY, int TX)
+{
+    int m = 0;
+    while (m < 3 && t >= a.bstart[m + 1]) m++;
+    NlTile g;
+    g.L = m == 0 ? a.L[0] : m == 1 ? a.L[1] : m == 2 ? a.L[2] : a.L[3];
+    const int nsy = a.nsy[m], nsx = a.nsx[m];
+    const int local = t - a.bstart[m];
+    const int gxm = a.gx[m];
+    const int gym = (a.bstart[m + 1] - a.bstart[m]) / gxm;
+    const int band = local / (NLI_BAND * gxm);
+    const int within = local - band * NLI_BAND * gxm;
+    const int bh = min(NLI_BAND, gym - band * NLI_BAND);
+    const int blx = within / bh;
+    const int bly = band * NLI_BAND + (within - blx * bh);
+    const int ry = bly / nsy, sy = bly - ry * nsy;
+    const int rx = blx / nsx, sx_ = blx - rx * nsx;
+    g.ty0 = g.L.vy0 + ry * a.step + sy * TY;
+    g.tx0 = g.L.vx0 + rx * a.step + sx_ * TX;
+    g.rows_lim = min(TY, a.step - sy * TY);
+    g.cols_lim = min(TX, a.step - sx_ * TX);
+    // interior origin of the reference tile that computes this block's outputs
+    g.Y0 = (g.ty0 / a.step) * a.step;
+    g.X0 = (g.tx0 / a.step) * a.step;
+    return g;
+}
 
 // nl_pixel_kl_pre(e1, e2) with the pixel the reference passes second given by `second`:
 // r, and a1 + a2, are the same bits for either argument order (products and fmaf commute,
@@ -245,7 +328,9 @@ __device__ __forceinline__ float nl_pixel_kl_sym(float4 e1, float4 e2, float sec
 //   row sums: runs of NLI_RSEG adjacent outputs of one row;
 //   outputs:  threads tid < NG*TX (NG row groups) own TY/NG vertically adjacent pixels of one
 //             column, slot tid + m*NG*TX in the block's similarity / weight chunk.
-template <int TY, int TX, int PSH, bool FIRST, int NTW>
+// SPARSE: the launch may store only some pixels' weights (WeightArgs::store); batches, which
+// store all, compile without that path.
+template <int TY, int TX, int PSH, bool FIRST, int NTW, bool SPARSE>
 __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const int t)
 {
     // output row groups: as many as the threads allow, dividing TY
@@ -277,36 +362,11 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
 
     const int tid = threadIdx.x;
     const bool kl = *a.kl_on != 0u;
-    // The four launch mappings (interior, last row, last column, corner) are laid end to end
-    // in one grid.  Blocks are sub-tiles of the interiors (step x step) of the reference's
-    // tiles, so every output of a block belongs to one reference tile: block row by =
-    // reference tile row * nsub_y + sub-tile row (nsub = ceil(step / TY)); the last sub-tile
-    // of a reference tile is cut at the tile's end.  The one-row / one-column mappings have a
-    // single block row / column starting at the mapping's origin.
-    // Tile t (the launch gives each XCD a contiguous run of tiles) is ordered column by
-    // column within bands of NLI_BAND block rows: the blocks resident on one XCD then cover
-    // a compact patch of the image, whose neighbourhoods overlap in that XCD's L2.
-    int m = 0;
-    while (m < 3 && t >= a.bstart[m + 1]) m++;
-    const Launch L = m == 0 ? a.L[0] : m == 1 ? a.L[1] : m == 2 ? a.L[2] : a.L[3];
-    const int nsy = a.nsy[m], nsx = a.nsx[m];
-    const int local = t - a.bstart[m];
-    const int gxm = a.gx[m];
-    const int gym = (a.bstart[m + 1] - a.bstart[m]) / gxm;
-    const int band = local / (NLI_BAND * gxm);
-    const int within = local - band * NLI_BAND * gxm;
-    const int bh = min(NLI_BAND, gym - band * NLI_BAND);
-    const int blx = within / bh;
-    const int bly = band * NLI_BAND + (within - blx * bh);
-    const int ry = bly / nsy, sy = bly - ry * nsy;
-    const int rx = blx / nsx, sx_ = blx - rx * nsx;
-    const int ty0 = L.vy0 + ry * a.step + sy * TY;
-    const int tx0 = L.vx0 + rx * a.step + sx_ * TX;
-    const int rows_lim = min(TY, a.step - sy * TY);
-    const int cols_lim = min(TX, a.step - sx_ * TX);
-    // interior origin of the reference tile that computes this block's outputs
-    const int Y0 = (ty0 / a.step) * a.step;
-    const int X0 = (tx0 / a.step) * a.step;
+    const bool dense = !SPARSE || nl_store_dense(a, kl);
+    const NlTile tg = nl_tile(a, t, TY, TX);
+    const Launch L = tg.L;
+    const int ty0 = tg.ty0, tx0 = tg.tx0, rows_lim = tg.rows_lim, cols_lim = tg.cols_lim;
+    const int Y0 = tg.Y0, X0 = tg.X0;
 
     // this block's chunk of the similarity cache and of the weights
     const long long chunk = (long long)t * (sw * sw) * NPIX;
@@ -320,6 +380,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
     float acc[RPT][5];
     long long opix[RPT];
     bool valid[RPT];
+    bool stq[RPT];            // the pixel's weights are stored
     unsigned int fofs[RPT];   // F grid element of the pixel (offset 0)
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
@@ -328,7 +389,27 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
         const int iy = ty0 + py - L.vy0, ix = tx0 + px - L.vx0;
         valid[q] = owner && py < rows_lim && px < cols_lim && iy < L.ny && ix < L.nx;
         opix[q] = valid[q] ? (long long)(L.oy0 + iy) * a.W + (L.ox0 + ix) : 0;
+        stq[q] = valid[q] && (dense || nl_weight_stored(a, opix[q]));
         fofs[q] = (unsigned int)(a.forg + (ty0 + py) * a.fpitch + (tx0 + px));
+    }
+    // Compact slots of the stored pixels (not dense): a run of nst slots per wave from the
+    // compact counter (cnt[4]), pixel q of a lane at rank cof[q] in the run.
+    const int wlane = tid & 63;
+    unsigned int cof[RPT];
+    float* wcw = a.wc;
+    int cslot0 = 0, nst = 0;
+    if (!dense) {
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const unsigned long long bal = __ballot(stq[q]);
+            cof[q] = (unsigned int)(nst + __popcll(bal & ((1ull << wlane) - 1ull)));
+            nst += __popcll(bal);
+        }
+        unsigned int wb0 = 0;
+        if (nst && wlane == 0) wb0 = atomicAdd(a.cnt + 4, (unsigned int)nst);
+        cslot0 = __builtin_amdgcn_readfirstlane((int)wb0);
+        nst = __builtin_amdgcn_readfirstlane(nst);
+        wcw = uni(a.wc + (long long)cslot0 * (sw * sw));
     }
 
     // stage the neighbourhood: rows ty0 - o .. ty0 + TY + o, columns tx0 - o .. tx0 + TX + o
@@ -442,7 +523,10 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                 }
                 float w = expf(sv[q] * a.rh + v * a.rT);
                 if (isnan(w) || isinf(w) || self) w = 0.0f;
-                if (valid[q]) wb[(unsigned int)(tid + q * NCT)] = w;
+                if (stq[q]) {
+                    if (dense) wb[(unsigned int)(tid + q * NCT)] = w;
+                    else wcw[(unsigned int)(plane * nst) + cof[q]] = w;
+                }
                 const float4 f = fv[q];
                 acc[q][0] += w;
                 acc[q][1] = fmaf(w, w, acc[q][1]);
@@ -581,7 +665,10 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                     }
                     float w = expf(u * a.rh + v * a.rT);
                     if (isnan(w) || isinf(w) || self) w = 0.0f;
-                    if (valid[q]) a.wt[pbase + tid + q * NCT] = w;
+                    if (stq[q]) {
+                        if (dense) a.wt[pbase + tid + q * NCT] = w;
+                        else wcw[(unsigned int)(plane * nst) + cof[q]] = w;
+                    }
                     const float4 f = fv[q];
                     acc[q][0] += w;
                     acc[q][1] = fmaf(w, w, acc[q][1]);
@@ -608,7 +695,14 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
             if (queued) {
                 const unsigned int k = base + __popcll(qmask & ((1ull << lane) - 1ull));
                 a.list[2 * k] = (int)opix[q];
-                a.list[2 * k + 1] = (a.shape << NLI_SLOT_BITS) | (t * NPIX + tid + q * NCT);   // weight slot
+                // weight slot: the tile slot (dense, or recomputed by nlinsar_fill_weights), or
+                // the compact slot of stored weights
+                if (dense || !stq[q]) {
+                    a.list[2 * k + 1] = (a.shape << NLI_SLOT_BITS) | (t * NPIX + tid + q * NCT);
+                } else {
+                    a.list[2 * k + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (cslot0 + (int)cof[q]);
+                    a.caux[cslot0 + (int)cof[q]] = (nst << 8) | (int)cof[q];
+                }
             }
         }
         if (valid[q] && !queued)
@@ -625,7 +719,7 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights(const WeightArgs a)
     const int nb = a.bstart[4];
     const int bq = nb >> 3, br = nb & 7;
     const int xcd = blockIdx.x & 7, kk = blockIdx.x >> 3;
-    nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW>(a, xcd * bq + min(xcd, br) + kk);
+    nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW, true>(a, xcd * bq + min(xcd, br) + kk);
 }
 
 // A batch of images (table: one entry per image, the images' tiles end to end from
@@ -655,8 +749,126 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights_batch(const WeightArgs* _
             if (table[mid].block0 <= T) lo = mid;
             else hi = mid - 1;
         }
-        nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW>(table[lo], T - table[lo].block0);
+        nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW, false>(table[lo], T - table[lo].block0);
     }
+}
+
+// The weights of the queued pixels whose weights the weights launch of this iteration did not
+// store (their queue entries still carry the tile slot), for the lmin smoothing: each such
+// pixel's sw*sw weights, recomputed from the cached similarity sums and the estimate records
+// with the weights kernel's operations in its order (the KL field with its orientation rule,
+// row sums left to right, then the patch rows top to bottom, the exponential, NaN / inf and the
+// centre to 0), so they are the same bits, into a compact slot of their own; the queue entry
+// then names that slot.  One wave per block: it tests 64 queue entries at once, then handles
+// its pixels one after another, with the pixel's (sw + p - 1)^2 estimate records staged in
+// LDS and the lanes over the search-window offsets.  a: the arguments of the weights launch
+// (one image; TY x TX blocks of NTW threads).
+template <int TY, int TX, int PSH, int NTW>
+__global__ __launch_bounds__(64) void nlinsar_fill_weights(const WeightArgs a)
+{
+    constexpr int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
+    constexpr int RPT = TY / NG;
+    constexpr int NCT = NG * TX;
+    constexpr int NPIX = TY * TX;
+    constexpr int NP2 = 2 * PSH;
+    extern __shared__ float4 stg[];
+    const bool kl = *a.kl_on != 0u;
+    if (nl_store_dense(a, kl)) return;
+    const unsigned int total = a.cnt[0];
+    const int lane = threadIdx.x;
+    const int wsh = a.wsh, sw = 2 * wsh + 1, nfull = sw * sw, centre = wsh * sw + wsh;
+    const int SE = sw + NP2;   // staged records per side
+    for (unsigned int b = blockIdx.x * 64u; b < total; b += gridDim.x * 64u) {
+        const unsigned int li = b + (unsigned int)lane;
+        const bool miss = li < total && (a.list[2 * li + 1] >> NLI_SLOT_BITS) != NLI_COMPACT;
+        unsigned long long m = __ballot(miss);
+        while (m) {
+            const unsigned int e = b + (unsigned int)(__ffsll((long long)m) - 1);
+            m &= m - 1ull;
+            const int slot = __builtin_amdgcn_readfirstlane(a.list[2 * e + 1]) & ((1 << NLI_SLOT_BITS) - 1);
+            const int t = slot / NPIX, local = slot - t * NPIX;
+            const int q = local / NCT, tid = local - q * NCT;
+            const int og = tid / TX, oc = tid - og * TX;
+            const int py = og * RPT + q, px = oc;
+            const NlTile tg = nl_tile(a, t, TY, TX);
+            const float* sb = a.scache + ((long long)t * nfull * NPIX + local);
+            unsigned int cs = 0;
+            if (lane == 0) cs = atomicAdd(a.cnt + 4, 1u);
+            cs = (unsigned int)__builtin_amdgcn_readfirstlane((int)cs);
+            float* wq = a.wc + (long long)cs * nfull;
+            // patch position (0, 0) of the pixel
+            const int gy0 = tg.ty0 - PSH + py, gx0 = tg.tx0 - PSH + px;
+            if (kl) {
+                // records of rows gy0 - wsh .. gy0 + NP2 + wsh, same columns
+                const unsigned int g0 = (unsigned int)(a.rorg + (gy0 - wsh) * a.rpitch + (gx0 - wsh));
+                for (int i = lane; i < SE * SE; i += 64) {
+                    const int r = i / SE, c = i - r * SE;
+                    stg[i] = a.E4[g0 + (unsigned int)(r * a.rpitch + c)];
+                }
+                __syncthreads();
+            }
+            for (int plane = lane; plane < nfull; plane += 64) {
+                const int hh = plane / sw;
+                const int dy = hh - wsh, dx = plane - hh * sw - wsh;
+                float v = -0.0f;
+                if (kl && plane != centre) {
+                    for (int j = 0; j <= NP2; j++) {
+                        const int gy = gy0 + j;
+                        const int sh = gy - tg.Y0 + PSH;
+                        const float4* rowp = stg + (wsh + j) * SE + wsh;
+                        float x[NP2 + 1];
+#pragma unroll
+                        for (int jj = 0; jj <= NP2; jj++) {
+                            const float4 fe = rowp[jj];
+                            const float4 vd = rowp[jj + dy * SE + dx];
+                            const int sx = gx0 + jj - tg.X0 + PSH;
+                            // orientation rule of the similarities, as in the weights kernel
+                            const bool swp = dy < 0 && sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
+                            x[jj] = nl_pixel_kl_sym(fe, vd, swp ? fe.x : vd.x);
+                        }
+                        float r = x[0];
+#pragma unroll
+                        for (int jj = 1; jj <= NP2; jj++) r += x[jj];
+                        v = j == 0 ? r : v + r;
+                    }
+                }
+                float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
+                if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
+                wq[plane] = w;
+            }
+            if (lane == 0) {
+                a.list[2 * e + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (int)cs;
+                a.caux[cs] = 1 << 8;   // a run of one
+            }
+            // the next pixel's records overwrite these
+            __syncthreads();
+        }
+    }
+}
+
+// Marks, for the next iteration's weights launch (NLI_STORE_PRED), the pixels queued in this
+// iteration and their eight neighbours with `tag`: pixels the lmin smoothing queued tend to
+// be queued again.  When more than `most` pixels were queued, the next launch stores all
+// weights instead (cnt[5] = 1): most would be stored anyway.
+__global__ void nlinsar_mark_pred(const int* __restrict__ list, unsigned int* __restrict__ cnt,
+                                  unsigned char* __restrict__ pred, int H, int W, int tag, unsigned int most)
+{
+    const unsigned int total = cnt[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[5] = total > most ? 1u : 0u;
+    if (total > most) return;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int opix = list[2 * i];
+        const int y = opix / W, x = opix - y * W;
+        for (int yy = max(y - 1, 0); yy <= min(y + 1, H - 1); yy++)
+            for (int xx = max(x - 1, 0); xx <= min(x + 1, W - 1); xx++)
+                pred[(long long)yy * W + xx] = (unsigned char)tag;
+    }
+}
+
+__global__ void nlinsar_zero_bytes(unsigned char* __restrict__ p, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0;
 }
 
 struct SmoothArgs {
@@ -666,6 +878,8 @@ struct SmoothArgs {
     int H, W, eh, ew;
     int npix[NLI_SHAPES];        // weight slots per block of each shape (TY * TX)
     const float* wt[NLI_SHAPES]; // weight region of each shape
+    const float* wc;             // compact weight slots (single calls), see WeightArgs
+    const int* caux;
     const float* old_ref;
     const float* old_pha;
     const float* old_coh;
@@ -760,7 +974,8 @@ struct SmoothPixel {
     const float* A;       // a1 on R's element layout
     const float4* F4;
     const float* wts;     // weight of offset k at wts[k * npix]
-    int npix;             // plane stride of the weights (slots per block of their shape)
+    int npix;             // offset stride of the weights (slots per block of their shape; 1
+                          // for a compact slot)
     unsigned int rb, fb;  // R / F element of window offset (0, 0) (grids hold < 2^30)
     float as2;            // 2 * amplitude of the pixel
 };
@@ -866,16 +1081,25 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
     const int tagged = a.list[2 * li + 1];
     const int shape = tagged >> NLI_SLOT_BITS;
     const int slot = tagged & ((1 << NLI_SLOT_BITS) - 1);
-    const int npix = shape == 0 ? a.npix[0] : shape == 1 ? a.npix[1] : a.npix[2];
-    const float* wbase = shape == 0 ? a.wt[0] : shape == 1 ? a.wt[1] : a.wt[2];
     const int oy = (int)(*opix / a.W), ox = (int)(*opix - (long long)oy * a.W);
     const int vy = oy == a.H - 1 ? oy + a.eh : oy;
     const int vx = ox == a.W - 1 ? ox + a.ew : ox;
     SmoothPixel px;
     px.A = a.A;
     px.F4 = reinterpret_cast<const float4*>(a.F.base);
-    px.wts = wbase + ((long long)(slot / npix) * nfull * npix + slot % npix);
-    px.npix = npix;
+    if (shape == NLI_COMPACT) {
+        // a compact slot: pixel r of a run of k, offset-major
+        const int aux = a.caux[slot];
+        const int k = aux >> 8, r = aux & 255;
+        px.wts = a.wc + ((long long)(slot - r) * nfull + r);
+        px.npix = k;
+    } else {
+        // a tile slot: weights of the block's pixels interleaved, offset-major
+        const int npix = shape == 0 ? a.npix[0] : shape == 1 ? a.npix[1] : a.npix[2];
+        const float* wbase = shape == 0 ? a.wt[0] : shape == 1 ? a.wt[1] : a.wt[2];
+        px.wts = wbase + ((long long)(slot / npix) * nfull * npix + slot % npix);
+        px.npix = npix;
+    }
     px.rb = (unsigned int)a.R.at(vy - a.wsh, vx - a.wsh);
     px.fb = (unsigned int)a.F.at(vy - a.wsh, vx - a.wsh);
     // at < 2.0 * (double)as in the reference: 2 * as is exact in float unless it
@@ -1289,11 +1513,10 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->step = pb.ref_tile - 2 * plan->overlap;
     if (plan->step <= 0) { set_error("nlinsar: search window and patch too large for the 80-pixel tiles"); return -1; }
     if (pb.tile == NLINSAR_TILE_AUTO) {
-        // 54 x 54 blocks when the image fills the GPU with them, a reference tile interior
-        // is a whole number of them, and their LDS fits; 18 x 54 otherwise
-        const bool big = (long long)pb.H * pb.W >= (1ll << 20) && plan->step % kTY[2] == 0 &&
-                         nlinsar_lds_bytes(kTY[2], kTX[2], plan->wsh, plan->psh) + 16 <= 160 * 1024;
-        plan->tile = big ? 2 : 1;
+        // 18 x 54 blocks of 512 threads: two per CU (LDS), and the fastest shape at every
+        // measured size since the weight stores became sparse (4096^2 with scatterers, 10
+        // iterations: 279 ms against 300 ms for 54 x 54 and 356 ms for 27 x 54)
+        plan->tile = 1;
     } else {
         plan->tile = std::min(std::max(pb.tile, 0), 2);
     }
@@ -1360,9 +1583,11 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
         plan->off_w[sh] = take(sizeof(float) * (size_t)nfull * slots);
         plan->off_s[sh] = take(sizeof(float) * (size_t)nfull * slots);
     }
+    plan->off_pred = take(n);
+    plan->off_caux = take(sizeof(int) * n);
     plan->off_list = take(2 * sizeof(int) * n);
     plan->off_xlist = take(sizeof(int) * n);
-    plan->off_cnt = take(sizeof(unsigned int) * 4);
+    plan->off_cnt = take(sizeof(unsigned int) * 8);
     plan->ws_bytes = off;
     return 0;
 }
@@ -1425,6 +1650,58 @@ static int launch_weights_all(int tile, bool thin, const WeightArgs& a, const La
     }
 }
 
+// The missing weights of queued pixels (nlinsar_fill_weights) of one launch of the tile shape.
+template <int PSH>
+static int launch_fill(int tile, const WeightArgs& a, int ncu, hipStream_t stream)
+{
+    WeightArgs w = a;
+    const int total = nlinsar_grid(a.step, kTY[tile], kTX[tile], a.L, w.gx, w.nsx, w.nsy, w.bstart);
+    if (total <= 0) return 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool timed = timing_on();
+    if (timed) {
+        DKX_HIP_CHECK(hipEventCreate(&e0));
+        DKX_HIP_CHECK(hipEventCreate(&e1));
+        DKX_HIP_CHECK(hipEventRecord(e0, stream));
+    }
+    const int se = 2 * a.wsh + 1 + 2 * PSH;
+    const size_t lds = sizeof(float4) * (size_t)se * se;
+    if (lds > 64 * 1024) { set_error("nlinsar: search window and patch too large"); return -2; }
+    const void* fn = tile == 0 ? (const void*)nlinsar_fill_weights<27, 54, PSH, 512>
+                   : tile == 1 ? (const void*)nlinsar_fill_weights<18, 54, PSH, 512>
+                               : (const void*)nlinsar_fill_weights<54, 54, PSH, 1024>;
+    DKX_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const dim3 grd(ncu * 16), blk(64);
+    switch (tile) {
+    case 0: hipLaunchKernelGGL((nlinsar_fill_weights<27, 54, PSH, 512>), grd, blk, lds, stream, w); break;
+    case 1: hipLaunchKernelGGL((nlinsar_fill_weights<18, 54, PSH, 512>), grd, blk, lds, stream, w); break;
+    default: hipLaunchKernelGGL((nlinsar_fill_weights<54, 54, PSH, 1024>), grd, blk, lds, stream, w); break;
+    }
+    DKX_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        DKX_HIP_CHECK(hipEventRecord(e1, stream));
+        timing_push(STAGE_NLINSAR_FILL, e0, e1);
+    }
+    return 0;
+}
+
+// Store mode of iteration it's weights launch (NlinsarProblem::store).  Automatic: the first
+// iteration stores its weights only when the KL term is on (from zero estimates a weight is a
+// function of the cached similarity alone); the next two store all (the lmin queue changes
+// too much from one iteration to the next early on: from 0.04 % to 11 % to 88 % of the pixels
+// on a 512^2 crop of the 4096^2 scatterer image); later ones store the pixels queued in the
+// previous iteration and their neighbours (there 94 % / 19 % / 9 % ... of the pixels, missing
+// 0.6 % / 0.3 % / 0.1 % ... that the fill recomputes).
+static int nli_store_mode(int setting, int it)
+{
+    switch (setting) {
+    case NLINSAR_STORE_ALL: return NLI_STORE_ALL;
+    case NLINSAR_STORE_PRED: return NLI_STORE_PRED;
+    case NLINSAR_STORE_NONE: return NLI_STORE_NONE;
+    default: return it == 0 ? NLI_STORE_KL : it <= 2 ? NLI_STORE_ALL : NLI_STORE_PRED;
+    }
+}
+
 int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* d_a1,
                 const float* d_a2, const float* d_phi, float* d_ref, float* d_phase,
                 float* d_coh, void* d_ws, unsigned long long* n_smoothed_host, hipStream_t stream)
@@ -1445,9 +1722,19 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     }
     int* list = (int*)(ws + plan.off_list);
     unsigned int* cnt = (unsigned int*)(ws + plan.off_cnt);
+    unsigned char* pred = (unsigned char*)(ws + plan.off_pred);
+    int* caux = (int*)(ws + plan.off_caux);
+    int hdev = 0;
+    DKX_HIP_CHECK(hipGetDevice(&hdev));
+    int ncu = 256;
+    DKX_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hdev));
 
     unsigned int* flag = cnt + 2;
     hipLaunchKernelGGL(nlinsar_clear_counters, dim3(1), dim3(64), 0, stream, cnt, flag, 1);
+    DKX_HIP_CHECK(hipGetLastError());
+    // no pixel marked for storing (marks are the iteration's tag, 1..255)
+    hipLaunchKernelGGL(nlinsar_zero_bytes, dim3(std::max(1, std::min(ncu * 4, (int)((n + 255) / 256)))), dim3(256), 0,
+                       stream, pred, (long long)n);
     DKX_HIP_CHECK(hipGetLastError());
     {
         dim3 blk(256), grd((R.w + 255) / 256, R.h);
@@ -1464,11 +1751,18 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
     float* alt[3] = {est + 3 * n, est + 4 * n, est + 5 * n};
     Launch L[4];
     nlinsar_mappings(H, W, plan.eh, plan.ew, L);
-    int hdev = 0;
-    DKX_HIP_CHECK(hipGetDevice(&hdev));
-    int ncu = 256;
-    DKX_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, hdev));
     for (int it = 0; it < pb.niter; it++) {
+        const int store = nli_store_mode(pb.store, it);
+        const int tag = it % 255 + 1;
+        if (store == NLI_STORE_PRED && it > 0) {
+            // the previous iteration's queue (its length still in cnt[0]) and neighbours
+            // automatic: all weights stored after an iteration that queued more than a quarter
+            // of the pixels
+            const unsigned int most = pb.store == NLINSAR_STORE_AUTO ? (unsigned int)(n / 4) : 0xffffffffu;
+            hipLaunchKernelGGL(nlinsar_mark_pred, dim3(ncu * 4), dim3(256), 0, stream, (const int*)list, cnt,
+                               pred, H, W, tag, most);
+            DKX_HIP_CHECK(hipGetLastError());
+        }
         {
             dim3 blk(256), grd((E.w + 255) / 256, E.h);
             hipLaunchKernelGGL(nlinsar_fill_estimates, grd, blk, 0, stream, (const float*)cur[0],
@@ -1497,7 +1791,18 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         wa.new_ref = nxt[0]; wa.new_pha = nxt[1]; wa.new_coh = nxt[2];
         wa.list = list; wa.cnt = cnt;
         wa.block0 = 0;
+        wa.shape = 0;
+        wa.store = store;
+        wa.tag = tag;
+        wa.pred = pred;
+        wa.wc = wt[0];
+        wa.caux = caux;
+        wa.dense = cnt + 5;
         const bool thin = pb.thin_edges != 0;
+        if (thin && store != NLI_STORE_ALL) {
+            set_error("nlinsar: thin edge blocks store every weight");
+            return -1;
+        }
         int rc;
         switch (plan.psh) {
 #define DKX_NLI_PSH(P)                                                                        \
@@ -1512,6 +1817,22 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
             rc = -1;
         }
         if (rc) return rc;
+        if (store != NLI_STORE_ALL) {
+            // the weights the launch did not store, for the queued pixels
+            WeightArgs fa = wa;
+            for (int i = 0; i < 4; i++) fa.L[i] = L[i];
+            fa.wt = wt[0];
+            fa.scache = scache[0];
+            switch (plan.psh) {
+            case 0: rc = launch_fill<0>(plan.tile, fa, ncu, stream); break;
+            case 1: rc = launch_fill<1>(plan.tile, fa, ncu, stream); break;
+            case 2: rc = launch_fill<2>(plan.tile, fa, ncu, stream); break;
+            case 3: rc = launch_fill<3>(plan.tile, fa, ncu, stream); break;
+            case 4: rc = launch_fill<4>(plan.tile, fa, ncu, stream); break;
+            default: rc = launch_fill<5>(plan.tile, fa, ncu, stream); break;
+            }
+            if (rc) return rc;
+        }
         SmoothArgs sa;
         sa.R = R; sa.F = F; sa.wsh = plan.wsh; sa.lmin = pb.lmin;
         sa.H = H; sa.W = W; sa.eh = plan.eh; sa.ew = plan.ew;
@@ -1526,6 +1847,8 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         sa.list = list; sa.cnt = cnt;
         sa.xlist = (int*)(ws + plan.off_xlist);
         sa.A = (const float*)(ws + plan.off_A);
+        sa.wc = wt[0];
+        sa.caux = caux;
         SmoothBatch sb;
         memset(&sb, 0, sizeof(sb));
         sb.one = sa;

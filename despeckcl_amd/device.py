@@ -79,13 +79,26 @@ def nlinsar_simulate_h(patch_size, seed=5489):
 def set_tile(filter_index, tile_index):
     """Block shape of the single-call entry points (dkx_set_tile).  filter 0 (NL-SAR): 0, 1, 2
     = 4, 2, 1 rows per thread; -1 (default) = per pass, 1 row for one patch size, else 2.
-    filter 1 (NL-InSAR): 0 = 27 x 54, 1 = 18 x 54, 2 = 54 x 54; -1 (default) = by image size.
+    filter 1 (NL-InSAR): 0 = 27 x 54, 1 = 18 x 54, 2 = 54 x 54; -1 (default) = the autotuner's
+    record for the parameters and size, else 18 x 54.
     Results do not depend on the setting."""
     check(lib().dkx_set_tile(int(filter_index), int(tile_index)))
 
 
 def get_tile(filter_index):
     return lib().dkx_get_tile(int(filter_index))
+
+
+def set_nlinsar_weight_store(mode):
+    """Which weights the NL-InSAR weights launches of single calls store for the lmin
+    smoothing (dkx_set_nlinsar_weight_store): -1 automatic (default), 0 all, 1 the pixels
+    smoothed in the previous iteration and their neighbours, 2 none.  The smoothed pixels'
+    missing weights are recomputed, so results do not depend on the setting."""
+    check(lib().dkx_set_nlinsar_weight_store(int(mode)))
+
+
+def get_nlinsar_weight_store():
+    return lib().dkx_get_nlinsar_weight_store()
 
 
 def nlsar_insar_autotune(a1, a2, phase, ref, phase_filt, coh, height, width, search_window_size,

@@ -212,11 +212,21 @@ double dkx_chi2_inv(double p, double nu);
  * 4, 2 or 1 rows per thread; -1 (default) = per pass, 2 for a pass of one patch size (the
  * default selected-key pass) and 1 for several (search windows up to 21; larger ones use the
  * 4-row variant whatever the setting).  NL-InSAR: 0 = 27 x 54 and 1 = 18 x 54 (512 threads),
- * 2 = 54 x 54 (1024 threads), -1 (default) = 2 for images of 2^20 pixels or more whose
- * 80-pixel reference tiles have a 54 x 54 interior (search window + patch = 28, e.g. 21 and 7)
- * and whose LDS fits, else 1.  Results do not depend on the tile shape. */
+ * 2 = 54 x 54 (1024 threads), -1 (default) = the autotuner's record for the parameters and
+ * image size, else 1.  Results do not depend on the tile shape. */
 int dkx_set_tile(int filter /* 0 NL-SAR, 1 NL-InSAR */, int tile_index);
 int dkx_get_tile(int filter);
+
+/* No counterpart in the reference, whose weights kernels write every pixel's sw^2 weights and
+ * whose host smoothing (src/compute/nlinsar/smoothing.cpp:39-76) reads back those of the
+ * pixels with ENL < lmin.  Which weights the NL-InSAR weights launches of the single-image
+ * entry points store for the lmin smoothing: -1 automatic (the default), 0 every weight,
+ * 1 those of the pixels the previous iteration smoothed and their neighbours, 2 none.  The
+ * weights of a smoothed pixel that were not stored are recomputed, the same bits, so every
+ * mode gives the same results; the mode changes only the memory traffic.  Batches and the
+ * server store every weight. */
+int dkx_set_nlinsar_weight_store(int mode);
+int dkx_get_nlinsar_weight_store(void);
 
 /* ---- autotuner -----------------------------------------------------------------------------
  * No counterpart in the reference, whose tile_size (src/compute/nlsar/tile_size.cpp:31-114)
