@@ -103,6 +103,7 @@ __global__ void nlinsar_clear_counters(unsigned int* cnt, unsigned int* flag, in
         if (clear_total) cnt[1] = 0u;
         cnt[3] = 0u;
         cnt[4] = 0u;
+        cnt[6] = 0u;
         if (clear_total) cnt[5] = 0u;
         *flag = 0u;
     }
@@ -753,18 +754,35 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights_batch(const WeightArgs* _
     }
 }
 
-// The weights of the queued pixels whose weights the weights launch of this iteration did not
-// store (their queue entries still carry the tile slot), for the lmin smoothing: each such
-// pixel's sw*sw weights, recomputed from the cached similarity sums and the estimate records
-// with the weights kernel's operations in its order (the KL field with its orientation rule,
-// row sums left to right, then the patch rows top to bottom, the exponential, NaN / inf and the
-// centre to 0), so they are the same bits, into a compact slot of their own; the queue entry
-// then names that slot.  One wave per block: it tests 64 queue entries at once, then handles
-// its pixels one after another, with the pixel's (sw + p - 1)^2 estimate records staged in
-// LDS and the lanes over the search-window offsets.  a: the arguments of the weights launch
-// (one image; TY x TX blocks of NTW threads).
+// The queue entries whose weights the weights launch of this iteration did not store (their
+// entries still carry the tile slot): appended to `miss` (count in cnt[6]), a wave's at once.
+__global__ __launch_bounds__(256) void nlinsar_fill_scan(const WeightArgs a, int* __restrict__ miss)
+{
+    const bool kl = *a.kl_on != 0u;
+    if (nl_store_dense(a, kl)) return;
+    const unsigned int total = a.cnt[0];
+    const int lane = threadIdx.x & 63;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i - lane < total; i += gridDim.x * blockDim.x) {
+        const bool m = i < total && (a.list[2 * i + 1] >> NLI_SLOT_BITS) != NLI_COMPACT;
+        const unsigned long long b = __ballot(m);
+        if (!b) continue;
+        unsigned int base = 0;
+        if (lane == 0) base = atomicAdd(a.cnt + 6, (unsigned int)__popcll(b));
+        base = __shfl(base, 0);
+        if (m) miss[base + __popcll(b & ((1ull << lane) - 1ull))] = (int)i;
+    }
+}
+
+// The weights of those queued pixels, for the lmin smoothing: each pixel's sw*sw weights,
+// recomputed from the cached similarity sums and the estimate records with the weights
+// kernel's operations in its order (the KL field with its orientation rule, row sums left to
+// right, then the patch rows top to bottom, the exponential, NaN / inf and the centre to 0),
+// so they are the same bits, into a compact slot of its own; the queue entry then names that
+// slot.  One wave per block and per pixel (pixels miss[w], miss[w + waves], ...), with the
+// pixel's (sw + p - 1)^2 estimate records staged in LDS and the lanes over the search-window
+// offsets.  a: the arguments of the weights launch (one image; TY x TX blocks of NTW threads).
 template <int TY, int TX, int PSH, int NTW>
-__global__ __launch_bounds__(64) void nlinsar_fill_weights(const WeightArgs a)
+__global__ __launch_bounds__(64) void nlinsar_fill_weights(const WeightArgs a, const int* __restrict__ miss)
 {
     constexpr int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
     constexpr int RPT = TY / NG;
@@ -774,75 +792,69 @@ __global__ __launch_bounds__(64) void nlinsar_fill_weights(const WeightArgs a)
     extern __shared__ float4 stg[];
     const bool kl = *a.kl_on != 0u;
     if (nl_store_dense(a, kl)) return;
-    const unsigned int total = a.cnt[0];
+    const unsigned int total = a.cnt[6];
     const int lane = threadIdx.x;
     const int wsh = a.wsh, sw = 2 * wsh + 1, nfull = sw * sw, centre = wsh * sw + wsh;
     const int SE = sw + NP2;   // staged records per side
-    for (unsigned int b = blockIdx.x * 64u; b < total; b += gridDim.x * 64u) {
-        const unsigned int li = b + (unsigned int)lane;
-        const bool miss = li < total && (a.list[2 * li + 1] >> NLI_SLOT_BITS) != NLI_COMPACT;
-        unsigned long long m = __ballot(miss);
-        while (m) {
-            const unsigned int e = b + (unsigned int)(__ffsll((long long)m) - 1);
-            m &= m - 1ull;
-            const int slot = __builtin_amdgcn_readfirstlane(a.list[2 * e + 1]) & ((1 << NLI_SLOT_BITS) - 1);
-            const int t = slot / NPIX, local = slot - t * NPIX;
-            const int q = local / NCT, tid = local - q * NCT;
-            const int og = tid / TX, oc = tid - og * TX;
-            const int py = og * RPT + q, px = oc;
-            const NlTile tg = nl_tile(a, t, TY, TX);
-            const float* sb = a.scache + ((long long)t * nfull * NPIX + local);
-            unsigned int cs = 0;
-            if (lane == 0) cs = atomicAdd(a.cnt + 4, 1u);
-            cs = (unsigned int)__builtin_amdgcn_readfirstlane((int)cs);
-            float* wq = a.wc + (long long)cs * nfull;
-            // patch position (0, 0) of the pixel
-            const int gy0 = tg.ty0 - PSH + py, gx0 = tg.tx0 - PSH + px;
-            if (kl) {
-                // records of rows gy0 - wsh .. gy0 + NP2 + wsh, same columns
-                const unsigned int g0 = (unsigned int)(a.rorg + (gy0 - wsh) * a.rpitch + (gx0 - wsh));
-                for (int i = lane; i < SE * SE; i += 64) {
-                    const int r = i / SE, c = i - r * SE;
-                    stg[i] = a.E4[g0 + (unsigned int)(r * a.rpitch + c)];
-                }
-                __syncthreads();
+    for (unsigned int k = blockIdx.x; k < total; k += gridDim.x) {
+        const int e = __builtin_amdgcn_readfirstlane(miss[k]);
+        const int slot = __builtin_amdgcn_readfirstlane(a.list[2 * e + 1]) & ((1 << NLI_SLOT_BITS) - 1);
+        const int t = slot / NPIX, local = slot - t * NPIX;
+        const int q = local / NCT, tid = local - q * NCT;
+        const int og = tid / TX, oc = tid - og * TX;
+        const int py = og * RPT + q, px = oc;
+        const NlTile tg = nl_tile(a, t, TY, TX);
+        const float* sb = a.scache + ((long long)t * nfull * NPIX + local);
+        unsigned int cs = 0;
+        if (lane == 0) cs = atomicAdd(a.cnt + 4, 1u);
+        cs = (unsigned int)__builtin_amdgcn_readfirstlane((int)cs);
+        float* wq = a.wc + (long long)cs * nfull;
+        // patch position (0, 0) of the pixel
+        const int gy0 = tg.ty0 - PSH + py, gx0 = tg.tx0 - PSH + px;
+        if (kl) {
+            // records of rows gy0 - wsh .. gy0 + NP2 + wsh, same columns
+            const unsigned int g0 = (unsigned int)(a.rorg + (gy0 - wsh) * a.rpitch + (gx0 - wsh));
+            for (int i = lane; i < SE * SE; i += 64) {
+                const int r = i / SE, c = i - r * SE;
+                stg[i] = a.E4[g0 + (unsigned int)(r * a.rpitch + c)];
             }
-            for (int plane = lane; plane < nfull; plane += 64) {
-                const int hh = plane / sw;
-                const int dy = hh - wsh, dx = plane - hh * sw - wsh;
-                float v = -0.0f;
-                if (kl && plane != centre) {
-                    for (int j = 0; j <= NP2; j++) {
-                        const int gy = gy0 + j;
-                        const int sh = gy - tg.Y0 + PSH;
-                        const float4* rowp = stg + (wsh + j) * SE + wsh;
-                        float x[NP2 + 1];
-#pragma unroll
-                        for (int jj = 0; jj <= NP2; jj++) {
-                            const float4 fe = rowp[jj];
-                            const float4 vd = rowp[jj + dy * SE + dx];
-                            const int sx = gx0 + jj - tg.X0 + PSH;
-                            // orientation rule of the similarities, as in the weights kernel
-                            const bool swp = dy < 0 && sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
-                            x[jj] = nl_pixel_kl_sym(fe, vd, swp ? fe.x : vd.x);
-                        }
-                        float r = x[0];
-#pragma unroll
-                        for (int jj = 1; jj <= NP2; jj++) r += x[jj];
-                        v = j == 0 ? r : v + r;
-                    }
-                }
-                float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
-                if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
-                wq[plane] = w;
-            }
-            if (lane == 0) {
-                a.list[2 * e + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (int)cs;
-                a.caux[cs] = 1 << 8;   // a run of one
-            }
-            // the next pixel's records overwrite these
             __syncthreads();
         }
+        for (int plane = lane; plane < nfull; plane += 64) {
+            const int hh = plane / sw;
+            const int dy = hh - wsh, dx = plane - hh * sw - wsh;
+            float v = -0.0f;
+            if (kl && plane != centre) {
+                for (int j = 0; j <= NP2; j++) {
+                    const int gy = gy0 + j;
+                    const int sh = gy - tg.Y0 + PSH;
+                    const float4* rowp = stg + (wsh + j) * SE + wsh;
+                    float x[NP2 + 1];
+#pragma unroll
+                    for (int jj = 0; jj <= NP2; jj++) {
+                        const float4 fe = rowp[jj];
+                        const float4 vd = rowp[jj + dy * SE + dx];
+                        const int sx = gx0 + jj - tg.X0 + PSH;
+                        // orientation rule of the similarities, as in the weights kernel
+                        const bool swp = dy < 0 && sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
+                        x[jj] = nl_pixel_kl_sym(fe, vd, swp ? fe.x : vd.x);
+                    }
+                    float r = x[0];
+#pragma unroll
+                    for (int jj = 1; jj <= NP2; jj++) r += x[jj];
+                    v = j == 0 ? r : v + r;
+                }
+            }
+            float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
+            if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
+            wq[plane] = w;
+        }
+        if (lane == 0) {
+            a.list[2 * e + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (int)cs;
+            a.caux[cs] = 1 << 8;   // a run of one
+        }
+        // the next pixel's records overwrite these
+        __syncthreads();
     }
 }
 
@@ -1587,6 +1599,8 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
     plan->off_caux = take(sizeof(int) * n);
     plan->off_list = take(2 * sizeof(int) * n);
     plan->off_xlist = take(sizeof(int) * n);
+    // counters: [0] queued this iteration, [1] queued in all, [2] KL on, [3] exact queue,
+    // [4] compact weight slots, [5] store all (NLI_STORE_PRED), [6] missing weights
     plan->off_cnt = take(sizeof(unsigned int) * 8);
     plan->ws_bytes = off;
     return 0;
@@ -1651,8 +1665,9 @@ static int launch_weights_all(int tile, bool thin, const WeightArgs& a, const La
 }
 
 // The missing weights of queued pixels (nlinsar_fill_weights) of one launch of the tile shape.
+// miss: one int per pixel of scratch (the missing entries).
 template <int PSH>
-static int launch_fill(int tile, const WeightArgs& a, int ncu, hipStream_t stream)
+static int launch_fill(int tile, const WeightArgs& a, int* miss, int ncu, hipStream_t stream)
 {
     WeightArgs w = a;
     const int total = nlinsar_grid(a.step, kTY[tile], kTX[tile], a.L, w.gx, w.nsx, w.nsy, w.bstart);
@@ -1671,11 +1686,13 @@ static int launch_fill(int tile, const WeightArgs& a, int ncu, hipStream_t strea
                    : tile == 1 ? (const void*)nlinsar_fill_weights<18, 54, PSH, 512>
                                : (const void*)nlinsar_fill_weights<54, 54, PSH, 1024>;
     DKX_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(nlinsar_fill_scan, dim3(ncu * 2), dim3(256), 0, stream, w, miss);
+    DKX_HIP_CHECK(hipGetLastError());
     const dim3 grd(ncu * 16), blk(64);
     switch (tile) {
-    case 0: hipLaunchKernelGGL((nlinsar_fill_weights<27, 54, PSH, 512>), grd, blk, lds, stream, w); break;
-    case 1: hipLaunchKernelGGL((nlinsar_fill_weights<18, 54, PSH, 512>), grd, blk, lds, stream, w); break;
-    default: hipLaunchKernelGGL((nlinsar_fill_weights<54, 54, PSH, 1024>), grd, blk, lds, stream, w); break;
+    case 0: hipLaunchKernelGGL((nlinsar_fill_weights<27, 54, PSH, 512>), grd, blk, lds, stream, w, miss); break;
+    case 1: hipLaunchKernelGGL((nlinsar_fill_weights<18, 54, PSH, 512>), grd, blk, lds, stream, w, miss); break;
+    default: hipLaunchKernelGGL((nlinsar_fill_weights<54, 54, PSH, 1024>), grd, blk, lds, stream, w, miss); break;
     }
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
@@ -1818,18 +1835,20 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         }
         if (rc) return rc;
         if (store != NLI_STORE_ALL) {
-            // the weights the launch did not store, for the queued pixels
+            // the weights the launch did not store, for the queued pixels (the exact queue's
+            // list, which the smoothing fills later, holds the missing entries meanwhile)
+            int* miss = (int*)(ws + plan.off_xlist);
             WeightArgs fa = wa;
             for (int i = 0; i < 4; i++) fa.L[i] = L[i];
             fa.wt = wt[0];
             fa.scache = scache[0];
             switch (plan.psh) {
-            case 0: rc = launch_fill<0>(plan.tile, fa, ncu, stream); break;
-            case 1: rc = launch_fill<1>(plan.tile, fa, ncu, stream); break;
-            case 2: rc = launch_fill<2>(plan.tile, fa, ncu, stream); break;
-            case 3: rc = launch_fill<3>(plan.tile, fa, ncu, stream); break;
-            case 4: rc = launch_fill<4>(plan.tile, fa, ncu, stream); break;
-            default: rc = launch_fill<5>(plan.tile, fa, ncu, stream); break;
+            case 0: rc = launch_fill<0>(plan.tile, fa, miss, ncu, stream); break;
+            case 1: rc = launch_fill<1>(plan.tile, fa, miss, ncu, stream); break;
+            case 2: rc = launch_fill<2>(plan.tile, fa, miss, ncu, stream); break;
+            case 3: rc = launch_fill<3>(plan.tile, fa, miss, ncu, stream); break;
+            case 4: rc = launch_fill<4>(plan.tile, fa, miss, ncu, stream); break;
+            default: rc = launch_fill<5>(plan.tile, fa, miss, ncu, stream); break;
             }
             if (rc) return rc;
         }
