@@ -264,7 +264,9 @@ void dkx_clear_tuned_tiles(void);
  * events on the caller's stream.  dkx_stage_times() synchronises with those events and
  * returns the stages of the calls since the previous dkx_stage_times() call: kinds
  * 1 = NL-SAR scale pass, 2 = NL-SAR last scale pass (statistics, weighted mean, outputs),
- * 11 = NL-InSAR weights, 12 = NL-InSAR smoothing; durations in milliseconds.  Returns the
+ * 11 = NL-InSAR weights, 12 = NL-InSAR smoothing, 13 = NL-InSAR weights of the last row /
+ * column in thin blocks (batches), 14 = NL-InSAR recomputation of unstored weights; durations
+ * in milliseconds.  Returns the
  * number of entries (entries beyond max_entries are dropped). */
 int dkx_set_timing(int enabled);
 int dkx_stage_times(int* kinds, float* ms, int max_entries);
