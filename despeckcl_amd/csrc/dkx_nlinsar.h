@@ -10,8 +10,8 @@ namespace dkx {
 // Which weights the weights launches of a single call store for the lmin smoothing, which
 // needs the weights of the pixels it queues (ENL < lmin); the queued pixels whose weights
 // were not stored get them recomputed, the same bits, so the setting changes the memory
-// traffic and not the result.  Automatic: see nli_store_mode (nlinsar.hip).  Batches store
-// all.
+// traffic and not the result.  Automatic: see nli_store_mode (nlinsar.hip); batches follow
+// the same rule (nlinsar_batch_enqueue).
 constexpr int NLINSAR_STORE_AUTO = -1;
 constexpr int NLINSAR_STORE_ALL = 0;    // every weight
 constexpr int NLINSAR_STORE_PRED = 1;   // the pixels the previous iteration queued and their
@@ -94,6 +94,7 @@ struct NlinsarBatchItem {
 // What a batch's launches need on the host, and the layout of its device tables.
 struct NlinsarBatchLaunch {
     int nimg, niter, tile, psh, sw, lmin;
+    int store;              // NlinsarProblem::store of the batch
     int total_tiles[NLI_SHAPES];   // weights tiles of all images, per shape
     int persistent_blocks;  // grid of the persistent weights kernel
     long long max_n;        // pixels of the largest image

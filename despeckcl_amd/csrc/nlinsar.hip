@@ -231,6 +231,7 @@ struct WeightArgs {
     float* wc;
     int* caux;
     const unsigned int* dense;    // NLI_STORE_PRED: nonzero when this iteration stores all instead
+    int* miss;                    // the queue entries whose weights must be recomputed (cnt[6])
 };
 
 // Weight store modes of a weights launch.  The stored weights and the ones
@@ -329,8 +330,7 @@ __device__ __forceinline__ float nl_pixel_kl_sym(float4 e1, float4 e2, float sec
 //   row sums: runs of NLI_RSEG adjacent outputs of one row;
 //   outputs:  threads tid < NG*TX (NG row groups) own TY/NG vertically adjacent pixels of one
 //             column, slot tid + m*NG*TX in the block's similarity / weight chunk.
-// SPARSE: the launch may store only some pixels' weights (WeightArgs::store); batches, which
-// store all, compile without that path.
+// SPARSE: the launch may store only some pixels' weights (WeightArgs::store).
 template <int TY, int TX, int PSH, bool FIRST, int NTW, bool SPARSE>
 __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const int t)
 {
@@ -750,14 +750,34 @@ __global__ __launch_bounds__(NTW) void nlinsar_weights_batch(const WeightArgs* _
             if (table[mid].block0 <= T) lo = mid;
             else hi = mid - 1;
         }
-        nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW, false>(table[lo], T - table[lo].block0);
+        nlinsar_weights_tile<TY, TX, PSH, FIRST, NTW, true>(table[lo], T - table[lo].block0);
     }
 }
 
-// The queue entries whose weights the weights launch of this iteration did not store (their
-// entries still carry the tile slot): appended to `miss` (count in cnt[6]), a wave's at once.
-__global__ __launch_bounds__(256) void nlinsar_fill_scan(const WeightArgs a, int* __restrict__ miss)
+// The work items of a batch's smoothing or fill launch are the queued pixels of all images end
+// to end (image i's items from base_i on): the first item at or after `base` that a worker `me` of
+// `stride` workers takes (items me, me + stride, ...).
+__device__ __forceinline__ unsigned long long nl_first_item(unsigned long long me, unsigned long long stride,
+                                                            unsigned long long base)
 {
+    return me >= base ? me : me + (base - me + stride - 1) / stride * stride;
+}
+
+// The weights launches whose stores nlinsar_fill_scan / nlinsar_fill_weights complete: one
+// image (`one`, shape 0) or a batch (`table`: shape s of image i at table[s * stride + i]).
+struct FillBatch {
+    const WeightArgs* table;
+    WeightArgs one;
+    int stride, nimg;
+    int ty[NLI_SHAPES], tx[NLI_SHAPES], nt[NLI_SHAPES];   // block shape and threads per shape
+};
+
+// The queue entries whose weights the weights launch of this iteration did not store (their
+// entries still carry a tile slot): appended to the image's `miss` list (count in cnt[6]), a
+// wave's at once.  Grid (x, image).
+__global__ __launch_bounds__(256) void nlinsar_fill_scan(const FillBatch f)
+{
+    const WeightArgs& a = f.table ? f.table[blockIdx.y] : f.one;
     const bool kl = *a.kl_on != 0u;
     if (nl_store_dense(a, kl)) return;
     const unsigned int total = a.cnt[0];
@@ -769,7 +789,7 @@ __global__ __launch_bounds__(256) void nlinsar_fill_scan(const WeightArgs a, int
         unsigned int base = 0;
         if (lane == 0) base = atomicAdd(a.cnt + 6, (unsigned int)__popcll(b));
         base = __shfl(base, 0);
-        if (m) miss[base + __popcll(b & ((1ull << lane) - 1ull))] = (int)i;
+        if (m) a.miss[base + __popcll(b & ((1ull << lane) - 1ull))] = (int)i;
     }
 }
 
@@ -778,83 +798,92 @@ __global__ __launch_bounds__(256) void nlinsar_fill_scan(const WeightArgs a, int
 // kernel's operations in its order (the KL field with its orientation rule, row sums left to
 // right, then the patch rows top to bottom, the exponential, NaN / inf and the centre to 0),
 // so they are the same bits, into a compact slot of its own; the queue entry then names that
-// slot.  One wave per block and per pixel (pixels miss[w], miss[w + waves], ...), with the
-// pixel's (sw + p - 1)^2 estimate records staged in LDS and the lanes over the search-window
-// offsets.  a: the arguments of the weights launch (one image; TY x TX blocks of NTW threads).
-template <int TY, int TX, int PSH, int NTW>
-__global__ __launch_bounds__(64) void nlinsar_fill_weights(const WeightArgs a, const int* __restrict__ miss)
+// slot.  One wave per block and per pixel: the missing pixels of all images are numbered end
+// to end and dealt out round robin over the blocks; each pixel's (sw + p - 1)^2 estimate
+// records are staged in LDS and the lanes take the search-window offsets.  The block shape
+// of a pixel's weights launch comes from its slot's shape tag.
+template <int PSH>
+__global__ __launch_bounds__(64) void nlinsar_fill_weights(const FillBatch f)
 {
-    constexpr int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
-    constexpr int RPT = TY / NG;
-    constexpr int NCT = NG * TX;
-    constexpr int NPIX = TY * TX;
     constexpr int NP2 = 2 * PSH;
     extern __shared__ float4 stg[];
-    const bool kl = *a.kl_on != 0u;
-    if (nl_store_dense(a, kl)) return;
-    const unsigned int total = a.cnt[6];
     const int lane = threadIdx.x;
-    const int wsh = a.wsh, sw = 2 * wsh + 1, nfull = sw * sw, centre = wsh * sw + wsh;
-    const int SE = sw + NP2;   // staged records per side
-    for (unsigned int k = blockIdx.x; k < total; k += gridDim.x) {
-        const int e = __builtin_amdgcn_readfirstlane(miss[k]);
-        const int slot = __builtin_amdgcn_readfirstlane(a.list[2 * e + 1]) & ((1 << NLI_SLOT_BITS) - 1);
-        const int t = slot / NPIX, local = slot - t * NPIX;
-        const int q = local / NCT, tid = local - q * NCT;
-        const int og = tid / TX, oc = tid - og * TX;
-        const int py = og * RPT + q, px = oc;
-        const NlTile tg = nl_tile(a, t, TY, TX);
-        const float* sb = a.scache + ((long long)t * nfull * NPIX + local);
-        unsigned int cs = 0;
-        if (lane == 0) cs = atomicAdd(a.cnt + 4, 1u);
-        cs = (unsigned int)__builtin_amdgcn_readfirstlane((int)cs);
-        float* wq = a.wc + (long long)cs * nfull;
-        // patch position (0, 0) of the pixel
-        const int gy0 = tg.ty0 - PSH + py, gx0 = tg.tx0 - PSH + px;
-        if (kl) {
-            // records of rows gy0 - wsh .. gy0 + NP2 + wsh, same columns
-            const unsigned int g0 = (unsigned int)(a.rorg + (gy0 - wsh) * a.rpitch + (gx0 - wsh));
-            for (int i = lane; i < SE * SE; i += 64) {
-                const int r = i / SE, c = i - r * SE;
-                stg[i] = a.E4[g0 + (unsigned int)(r * a.rpitch + c)];
+    unsigned long long base = 0;
+    for (int img = 0; img < f.nimg; img++) {
+        const WeightArgs& a0 = f.table ? f.table[img] : f.one;
+        const bool kl = *a0.kl_on != 0u;
+        if (nl_store_dense(a0, kl)) continue;
+        const unsigned int total = a0.cnt[6];
+        const int wsh = a0.wsh, sw = 2 * wsh + 1, nfull = sw * sw, centre = wsh * sw + wsh;
+        const int SE = sw + NP2;   // staged records per side
+        for (unsigned long long g = nl_first_item(blockIdx.x, gridDim.x, base); g < base + total; g += gridDim.x) {
+            const int e = __builtin_amdgcn_readfirstlane(a0.miss[(unsigned int)(g - base)]);
+            const int tagged = __builtin_amdgcn_readfirstlane(a0.list[2 * e + 1]);
+            const int sh = tagged >> NLI_SLOT_BITS;
+            const int slot = tagged & ((1 << NLI_SLOT_BITS) - 1);
+            const WeightArgs& a = f.table ? f.table[sh * f.stride + img] : f.one;
+            // the slot of pixel (q, thread) of tile t, as the weights kernel numbers them
+            const int TY = f.ty[sh], TX = f.tx[sh], NTW = f.nt[sh];
+            const int NG = (NTW / TX) >= TY ? TY : (NTW / TX);
+            const int RPT = TY / NG, NCT = NG * TX, NPIX = TY * TX;
+            const int t = slot / NPIX, local = slot - t * NPIX;
+            const int q = local / NCT, tid = local - q * NCT;
+            const int og = tid / TX, oc = tid - og * TX;
+            const int py = og * RPT + q, px = oc;
+            const NlTile tg = nl_tile(a, t, TY, TX);
+            const float* sb = a.scache + ((long long)t * nfull * NPIX + local);
+            unsigned int cs = 0;
+            if (lane == 0) cs = atomicAdd(a0.cnt + 4, 1u);
+            cs = (unsigned int)__builtin_amdgcn_readfirstlane((int)cs);
+            float* wq = a0.wc + (long long)cs * nfull;
+            // patch position (0, 0) of the pixel
+            const int gy0 = tg.ty0 - PSH + py, gx0 = tg.tx0 - PSH + px;
+            if (kl) {
+                // records of rows gy0 - wsh .. gy0 + NP2 + wsh, same columns
+                const unsigned int g0 = (unsigned int)(a.rorg + (gy0 - wsh) * a.rpitch + (gx0 - wsh));
+                for (int i = lane; i < SE * SE; i += 64) {
+                    const int r = i / SE, c = i - r * SE;
+                    stg[i] = a.E4[g0 + (unsigned int)(r * a.rpitch + c)];
+                }
+                __syncthreads();
             }
+            for (int plane = lane; plane < nfull; plane += 64) {
+                const int hh = plane / sw;
+                const int dy = hh - wsh, dx = plane - hh * sw - wsh;
+                float v = -0.0f;
+                if (kl && plane != centre) {
+                    for (int j = 0; j <= NP2; j++) {
+                        const int gy = gy0 + j;
+                        const int shy = gy - tg.Y0 + PSH;
+                        const float4* rowp = stg + (wsh + j) * SE + wsh;
+                        float x[NP2 + 1];
+#pragma unroll
+                        for (int jj = 0; jj <= NP2; jj++) {
+                            const float4 fe = rowp[jj];
+                            const float4 vd = rowp[jj + dy * SE + dx];
+                            const int sx = gx0 + jj - tg.X0 + PSH;
+                            // orientation rule of the similarities, as in the weights kernel
+                            const bool swp = dy < 0 && shy >= wsh && sx >= wsh && sx < a.wsim - wsh;
+                            x[jj] = nl_pixel_kl_sym(fe, vd, swp ? fe.x : vd.x);
+                        }
+                        float r = x[0];
+#pragma unroll
+                        for (int jj = 1; jj <= NP2; jj++) r += x[jj];
+                        v = j == 0 ? r : v + r;
+                    }
+                }
+                float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
+                if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
+                wq[plane] = w;
+            }
+            if (lane == 0) {
+                a0.list[2 * e + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (int)cs;
+                a0.caux[cs] = 1 << 8;   // a run of one
+            }
+            // the next pixel's records overwrite these
             __syncthreads();
         }
-        for (int plane = lane; plane < nfull; plane += 64) {
-            const int hh = plane / sw;
-            const int dy = hh - wsh, dx = plane - hh * sw - wsh;
-            float v = -0.0f;
-            if (kl && plane != centre) {
-                for (int j = 0; j <= NP2; j++) {
-                    const int gy = gy0 + j;
-                    const int sh = gy - tg.Y0 + PSH;
-                    const float4* rowp = stg + (wsh + j) * SE + wsh;
-                    float x[NP2 + 1];
-#pragma unroll
-                    for (int jj = 0; jj <= NP2; jj++) {
-                        const float4 fe = rowp[jj];
-                        const float4 vd = rowp[jj + dy * SE + dx];
-                        const int sx = gx0 + jj - tg.X0 + PSH;
-                        // orientation rule of the similarities, as in the weights kernel
-                        const bool swp = dy < 0 && sh >= wsh && sx >= wsh && sx < a.wsim - wsh;
-                        x[jj] = nl_pixel_kl_sym(fe, vd, swp ? fe.x : vd.x);
-                    }
-                    float r = x[0];
-#pragma unroll
-                    for (int jj = 1; jj <= NP2; jj++) r += x[jj];
-                    v = j == 0 ? r : v + r;
-                }
-            }
-            float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
-            if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
-            wq[plane] = w;
-        }
-        if (lane == 0) {
-            a.list[2 * e + 1] = (NLI_COMPACT << NLI_SLOT_BITS) | (int)cs;
-            a.caux[cs] = 1 << 8;   // a run of one
-        }
-        // the next pixel's records overwrite these
-        __syncthreads();
+        base += total;
     }
 }
 
@@ -911,15 +940,6 @@ struct SmoothBatch {
     SmoothArgs one;
     int nimg;
 };
-
-// The work items of a batch's smoothing launch are the queued pixels of all images end to end
-// (image i's items from base_i on): the first item at or after `base` that a worker `me` of
-// `stride` workers takes (items me, me + stride, ...).
-__device__ __forceinline__ unsigned long long nl_first_item(unsigned long long me, unsigned long long stride,
-                                                            unsigned long long base)
-{
-    return me >= base ? me : me + (base - me + stride - 1) / stride * stride;
-}
 
 // std::priority_queue<weight_el> of smoothing.cpp:27-36 (top = smallest weight): sift
 // rules of libstdc++'s push_heap / pop_heap, which decide which of several equal weights
@@ -1592,7 +1612,8 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
             set_error("nlinsar: image too large for one call (split it into bands)");
             return -4;
         }
-        plan->off_w[sh] = take(sizeof(float) * (size_t)nfull * slots);
+        // shape 0's weight region also holds the compact slots (at most one per pixel)
+        plan->off_w[sh] = take(sizeof(float) * (size_t)nfull * (sh == 0 ? std::max(slots, n) : slots));
         plan->off_s[sh] = take(sizeof(float) * (size_t)nfull * slots);
     }
     plan->off_pred = take(n);
@@ -1664,14 +1685,24 @@ static int launch_weights_all(int tile, bool thin, const WeightArgs& a, const La
     }
 }
 
-// The missing weights of queued pixels (nlinsar_fill_weights) of one launch of the tile shape.
-// miss: one int per pixel of scratch (the missing entries).
-template <int PSH>
-static int launch_fill(int tile, const WeightArgs& a, int* miss, int ncu, hipStream_t stream)
+// The shapes of a weights launch's blocks (NLI_SHAPES), for the fill kernels.
+static void fill_shapes(int tile, FillBatch* f)
 {
-    WeightArgs w = a;
-    const int total = nlinsar_grid(a.step, kTY[tile], kTX[tile], a.L, w.gx, w.nsx, w.nsy, w.bstart);
-    if (total <= 0) return 0;
+    f->ty[0] = kTY[tile]; f->tx[0] = kTX[tile]; f->nt[0] = tile == 2 ? 1024 : 512;
+    f->ty[1] = NLI_ROW_TY; f->tx[1] = NLI_ROW_TX; f->nt[1] = NLI_ROW_NT;
+    f->ty[2] = NLI_COL_TY; f->tx[2] = NLI_COL_TX; f->nt[2] = NLI_COL_NT;
+}
+
+// The weights the weights launches did not store, for the queued pixels: the scan over each
+// image's queue, then the recomputation (timed as one stage).  grid_x: scan blocks per image.
+template <int PSH>
+static int launch_fill(const FillBatch& f, int wsh, int grid_x, int ncu, hipStream_t stream)
+{
+    const int se = 2 * wsh + 1 + 2 * PSH;
+    const size_t lds = sizeof(float4) * (size_t)se * se;
+    if (lds > 64 * 1024) { set_error("nlinsar: search window and patch too large"); return -2; }
+    DKX_HIP_CHECK(hipFuncSetAttribute((const void*)nlinsar_fill_weights<PSH>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     const bool timed = timing_on();
     if (timed) {
@@ -1679,27 +1710,27 @@ static int launch_fill(int tile, const WeightArgs& a, int* miss, int ncu, hipStr
         DKX_HIP_CHECK(hipEventCreate(&e1));
         DKX_HIP_CHECK(hipEventRecord(e0, stream));
     }
-    const int se = 2 * a.wsh + 1 + 2 * PSH;
-    const size_t lds = sizeof(float4) * (size_t)se * se;
-    if (lds > 64 * 1024) { set_error("nlinsar: search window and patch too large"); return -2; }
-    const void* fn = tile == 0 ? (const void*)nlinsar_fill_weights<27, 54, PSH, 512>
-                   : tile == 1 ? (const void*)nlinsar_fill_weights<18, 54, PSH, 512>
-                               : (const void*)nlinsar_fill_weights<54, 54, PSH, 1024>;
-    DKX_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(nlinsar_fill_scan, dim3(ncu * 2), dim3(256), 0, stream, w, miss);
+    hipLaunchKernelGGL(nlinsar_fill_scan, dim3(grid_x, f.nimg), dim3(256), 0, stream, f);
     DKX_HIP_CHECK(hipGetLastError());
-    const dim3 grd(ncu * 16), blk(64);
-    switch (tile) {
-    case 0: hipLaunchKernelGGL((nlinsar_fill_weights<27, 54, PSH, 512>), grd, blk, lds, stream, w, miss); break;
-    case 1: hipLaunchKernelGGL((nlinsar_fill_weights<18, 54, PSH, 512>), grd, blk, lds, stream, w, miss); break;
-    default: hipLaunchKernelGGL((nlinsar_fill_weights<54, 54, PSH, 1024>), grd, blk, lds, stream, w, miss); break;
-    }
+    hipLaunchKernelGGL(nlinsar_fill_weights<PSH>, dim3(ncu * 16), dim3(64), lds, stream, f);
     DKX_HIP_CHECK(hipGetLastError());
     if (timed) {
         DKX_HIP_CHECK(hipEventRecord(e1, stream));
         timing_push(STAGE_NLINSAR_FILL, e0, e1);
     }
     return 0;
+}
+
+static int launch_fill_psh(int psh, const FillBatch& f, int wsh, int grid_x, int ncu, hipStream_t stream)
+{
+    switch (psh) {
+    case 0: return launch_fill<0>(f, wsh, grid_x, ncu, stream);
+    case 1: return launch_fill<1>(f, wsh, grid_x, ncu, stream);
+    case 2: return launch_fill<2>(f, wsh, grid_x, ncu, stream);
+    case 3: return launch_fill<3>(f, wsh, grid_x, ncu, stream);
+    case 4: return launch_fill<4>(f, wsh, grid_x, ncu, stream);
+    default: return launch_fill<5>(f, wsh, grid_x, ncu, stream);
+    }
 }
 
 // Store mode of iteration it's weights launch (NlinsarProblem::store).  Automatic: the first
@@ -1815,6 +1846,8 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         wa.wc = wt[0];
         wa.caux = caux;
         wa.dense = cnt + 5;
+        // the exact queue's list, which the smoothing fills later, holds the missing entries
+        wa.miss = (int*)(ws + plan.off_xlist);
         const bool thin = pb.thin_edges != 0;
         if (thin && store != NLI_STORE_ALL) {
             set_error("nlinsar: thin edge blocks store every weight");
@@ -1835,21 +1868,17 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
         }
         if (rc) return rc;
         if (store != NLI_STORE_ALL) {
-            // the weights the launch did not store, for the queued pixels (the exact queue's
-            // list, which the smoothing fills later, holds the missing entries meanwhile)
-            int* miss = (int*)(ws + plan.off_xlist);
-            WeightArgs fa = wa;
-            for (int i = 0; i < 4; i++) fa.L[i] = L[i];
-            fa.wt = wt[0];
-            fa.scache = scache[0];
-            switch (plan.psh) {
-            case 0: rc = launch_fill<0>(plan.tile, fa, miss, ncu, stream); break;
-            case 1: rc = launch_fill<1>(plan.tile, fa, miss, ncu, stream); break;
-            case 2: rc = launch_fill<2>(plan.tile, fa, miss, ncu, stream); break;
-            case 3: rc = launch_fill<3>(plan.tile, fa, miss, ncu, stream); break;
-            case 4: rc = launch_fill<4>(plan.tile, fa, miss, ncu, stream); break;
-            default: rc = launch_fill<5>(plan.tile, fa, miss, ncu, stream); break;
-            }
+            // the weights the launch did not store, for the queued pixels
+            FillBatch fb;
+            memset(&fb, 0, sizeof(fb));
+            fb.one = wa;
+            for (int i = 0; i < 4; i++) fb.one.L[i] = L[i];
+            nlinsar_grid(plan.step, plan.TY, plan.TX, L, fb.one.gx, fb.one.nsx, fb.one.nsy, fb.one.bstart);
+            fb.one.wt = wt[0];
+            fb.one.scache = scache[0];
+            fb.nimg = 1;
+            fill_shapes(plan.tile, &fb);
+            rc = launch_fill_psh(plan.psh, fb, plan.wsh, ncu * 2, ncu, stream);
             if (rc) return rc;
         }
         SmoothArgs sa;
@@ -1920,7 +1949,9 @@ struct NliImage {
     int H, W;
     Grid R, E, F;
     float* A;
-    unsigned int* cnt;    // [0] queued, [1] running total, [2] estimate flag, [3] exact queue
+    unsigned int* cnt;    // the counters of nlinsar_plan (off_cnt)
+    unsigned char* pred;  // store marks (NLI_STORE_PRED)
+    const int* list;      // the queue
 };
 
 // The estimate arrays per iteration: iteration 0 reads the caller's arrays and writes set 0;
@@ -1937,8 +1968,45 @@ __global__ void nlinsar_b_clear(const NliImage* __restrict__ im, int nimg, unsig
         if (clear_total) c[1] = 0u;
         c[2] = 0u;
         c[3] = 0u;
+        c[4] = 0u;
+        if (clear_total) c[5] = 0u;
+        c[6] = 0u;
     }
     if (threadIdx.x < 8 * NLI_SHAPES) queue[threadIdx.x] = 0u;
+}
+
+// NLI_STORE_PRED iterations of a batch, before the queue counters are cleared: every
+// image's weights launch stores all (dense = 1), or only the marked pixels, which the two
+// kernels below mark from the image's previous queue (the marks of earlier iterations
+// cleared first; tag 1).  Grid (x, image) for the marks.
+__global__ void nlinsar_b_set_dense(const NliImage* __restrict__ im, int nimg)
+{
+    for (int i = threadIdx.x; i < nimg; i += blockDim.x) im[i].cnt[5] = 1u;
+}
+
+__global__ __launch_bounds__(256) void nlinsar_b_clear_marks(const NliImage* __restrict__ im)
+{
+    const NliImage& g = im[blockIdx.y];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += (long long)gridDim.x * blockDim.x)
+        g.pred[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void nlinsar_b_mark(const NliImage* __restrict__ im, int quarter)
+{
+    const NliImage& g = im[blockIdx.y];
+    const unsigned int total = g.cnt[0];
+    // automatic (quarter): all weights stored after an iteration that queued more than a
+    // quarter of the pixels
+    const bool all = quarter && (long long)total > g.n / 4;
+    if (blockIdx.x == 0 && threadIdx.x == 0) g.cnt[5] = all ? 1u : 0u;
+    if (all) return;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int opix = g.list[2 * i];
+        const int y = opix / g.W, x = opix - y * g.W;
+        for (int yy = max(y - 1, 0); yy <= min(y + 1, g.H - 1); yy++)
+            for (int xx = max(x - 1, 0); xx <= min(x + 1, g.W - 1); xx++)
+                g.pred[(long long)yy * g.W + xx] = 1;
+    }
 }
 
 // Grid (x, image): blocks x, x + gridDim.x, ... take the rows of their image.
@@ -2059,6 +2127,7 @@ int nlinsar_batch_plan(const NlinsarProblem& common, const int* H, const int* W,
     L->psh = probe.psh;
     L->sw = common.sw;
     L->lmin = common.lmin;
+    L->store = common.store;
     for (int sh = 0; sh < NLI_SHAPES; sh++) L->total_tiles[sh] = (int)total[sh];
     L->persistent_blocks = 0;   // set by the first enqueue (it needs the device)
     L->max_n = max_n;
@@ -2120,6 +2189,8 @@ int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<Nlinsar
         g.F.base = (float*)(ws + plan.off_F);
         g.A = (float*)(ws + plan.off_A);
         g.cnt = (unsigned int*)(ws + plan.off_cnt);
+        g.pred = (unsigned char*)(ws + plan.off_pred);
+        g.list = (const int*)(ws + plan.off_list);
         float* est = (float*)(ws + plan.off_est);
         float* set0[3] = {est, est + npx, est + 2 * npx};
         float* set1[3] = {est + 3 * npx, est + 4 * npx, est + 5 * npx};
@@ -2155,6 +2226,20 @@ int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<Nlinsar
                 a.cnt = g.cnt;
                 a.block0 = block0[sh];
                 a.shape = sh;
+                // store modes (NlinsarProblem::store): the first iteration's kind stores per the
+                // KL flag, the others predict (all stored in iterations 2-3 of the automatic
+                // mode, by the dense flag)
+                a.store = L.store == NLINSAR_STORE_ALL    ? NLI_STORE_ALL
+                        : L.store == NLINSAR_STORE_NONE   ? NLI_STORE_NONE
+                        : L.store == NLINSAR_STORE_PRED   ? NLI_STORE_PRED
+                        : kind == 0                       ? NLI_STORE_KL
+                                                          : NLI_STORE_PRED;
+                a.tag = 1;
+                a.pred = g.pred;
+                a.wc = (float*)(ws + plan.off_w[0]);
+                a.caux = (int*)(ws + plan.off_caux);
+                a.dense = g.cnt + 5;
+                a.miss = (int*)(ws + plan.off_xlist);
             }
             SmoothArgs& sa = st[kind * n + i];
             sa.R = g.R; sa.F = g.F; sa.wsh = plan.wsh; sa.lmin = common.lmin;
@@ -2170,6 +2255,8 @@ int nlinsar_batch_tables(const NlinsarProblem& common, const std::vector<Nlinsar
             sa.list = (const int*)(ws + plan.off_list); sa.cnt = g.cnt;
             sa.xlist = (int*)(ws + plan.off_xlist);
             sa.A = g.A;
+            sa.wc = (const float*)(ws + plan.off_w[0]);
+            sa.caux = (const int*)(ws + plan.off_caux);
         }
         for (int sh = 0; sh < NLI_SHAPES; sh++) block0[sh] += plan.blocks[sh];
     }
@@ -2301,10 +2388,25 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
     DKX_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(nlinsar_b_filter_values, egrid, eblk, 0, stream, img);
     DKX_HIP_CHECK(hipGetLastError());
+    const bool sparse = L.store != NLINSAR_STORE_ALL;
     for (int it = 0; it < L.niter; it++) {
         const int kind = iteration_kind(it);
         const WeightArgs* wk = wt + (size_t)kind * n;   // shape 0 of this kind (the per-image
                                                          // estimate pointers are alike in all shapes)
+        if (sparse && it > 0 && L.store != NLINSAR_STORE_NONE) {
+            // which weights this iteration's launches store (before the counters are cleared:
+            // the marks come from the previous queue)
+            if (L.store == NLINSAR_STORE_AUTO && it <= 2) {
+                hipLaunchKernelGGL(nlinsar_b_set_dense, dim3(1), dim3(256), 0, stream, img, n);
+                DKX_HIP_CHECK(hipGetLastError());
+            } else {
+                hipLaunchKernelGGL(nlinsar_b_clear_marks, egrid, eblk, 0, stream, img);
+                DKX_HIP_CHECK(hipGetLastError());
+                hipLaunchKernelGGL(nlinsar_b_mark, egrid, eblk, 0, stream, img,
+                                   L.store == NLINSAR_STORE_AUTO ? 1 : 0);
+                DKX_HIP_CHECK(hipGetLastError());
+            }
+        }
         hipLaunchKernelGGL(nlinsar_b_fill_estimates, egrid, eblk, 0, stream, img, wk);
         DKX_HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(nlinsar_b_clear, dim3(1), dim3(256), 0, stream, img, n, queue, 0);
@@ -2325,6 +2427,17 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
             rc = -1;
         }
         if (rc) return rc;
+        if (sparse) {
+            // the weights the launches did not store, for the queued pixels of every image
+            FillBatch fb;
+            memset(&fb, 0, sizeof(fb));
+            fb.table = wk;
+            fb.stride = 3 * n;
+            fb.nimg = n;
+            fill_shapes(L.tile, &fb);
+            rc = launch_fill_psh(L.psh, fb, (L.sw - 1) / 2, bx, ncu, stream);
+            if (rc) return rc;
+        }
         SmoothBatch sb;
         memset(&sb, 0, sizeof(sb));
         sb.table = st + (size_t)kind * n;

@@ -114,6 +114,7 @@ int dkx_nlinsar_batch_set(dkx_nlinsar_batch h, const dkx_nlinsar_image* images, 
         NlinsarProblem pb;
         int rc = nlinsar_problem(0, 0, sw, p, niter, lmin, tile, &pb);
         if (rc) return rc;
+        pb.store = dkx_get_nlinsar_weight_store();
         std::vector<NlinsarPlan> plans;
         NlinsarBatchLaunch L;
         rc = nlinsar_batch_plan(pb, H.data(), W.data(), n, plans, &L);

@@ -384,6 +384,7 @@ int dkx_server_create(unsigned long long hbm_budget_bytes, int search_window_siz
         rc = nlinsar_problem(0, 0, nl_search_window_size, nl_patch_size, nl_niter, nl_lmin, NLINSAR_TILE_AUTO,
                              &nl_pb);
         if (rc) return rc;
+        nl_pb.store = dkx_get_nlinsar_weight_store();
         {
             // the parameter checks of a plan, on a one-pixel image
             NlinsarProblem probe = nl_pb;

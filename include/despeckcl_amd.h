@@ -219,12 +219,13 @@ int dkx_get_tile(int filter);
 
 /* No counterpart in the reference, whose weights kernels write every pixel's sw^2 weights and
  * whose host smoothing (src/compute/nlinsar/smoothing.cpp:39-76) reads back those of the
- * pixels with ENL < lmin.  Which weights the NL-InSAR weights launches of the single-image
- * entry points store for the lmin smoothing: -1 automatic (the default), 0 every weight,
+ * pixels with ENL < lmin.  Which weights the NL-InSAR weights launches store for the lmin
+ * smoothing: -1 automatic (the default), 0 every weight,
  * 1 those of the pixels the previous iteration smoothed and their neighbours, 2 none.  The
  * weights of a smoothed pixel that were not stored are recomputed, the same bits, so every
- * mode gives the same results; the mode changes only the memory traffic.  Batches and the
- * server store every weight. */
+ * mode gives the same results; the mode changes only the memory traffic.  A batch
+ * (dkx_nlinsar_batch_set) and a server (dkx_server_create) keep the mode in force when they
+ * are set up. */
 int dkx_set_nlinsar_weight_store(int mode);
 int dkx_get_nlinsar_weight_store(void);
 

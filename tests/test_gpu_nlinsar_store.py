@@ -104,3 +104,66 @@ def test_store_mode_rejected(D):
     with pytest.raises(Exception):
         DV.set_nlinsar_weight_store(3)
     assert DV.get_nlinsar_weight_store() == -1
+
+
+BATCH_SHAPES = [(1, 7), (5, 3), (64, 64), (81, 200), (300, 257), (200, 130), (256, 256), (97, 400)]
+
+
+def test_batch_store_modes_bit_identical(D):
+    """Batches under every store mode (the mode in force when the batch is set): the
+    predicted iterations' marks, the dense decision and the recomputation run over all images
+    at once, the thin edge blocks' stored pixels go to compact slots too.  Every image equals
+    its single call with all weights stored, eagerly and on two replays of the HIP graph
+    (marks and recomputation captured)."""
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    from despeckcl_amd import synthetic as S
+    sw, p, niter, lmin = 21, 7, 6, 10
+    old = DV.get_nlinsar_weight_store()
+    imgs = []
+    try:
+        for i, (h, w) in enumerate(BATCH_SHAPES):
+            a1, a2, ph = S.interferogram(h, w, 7100 + i, n_regions=6, scatterer_fraction=0.05)
+            imgs.append(([HIP.DeviceArray.from_host(x) for x in (a1, a2, ph)],
+                         [HIP.DeviceArray((h, w)) for _ in range(3)], h, w))
+        stream = HIP.Stream()
+        ref = []
+        DV.set_nlinsar_weight_store(0)
+        for inp, out, h, w in imgs:
+            for d in out:
+                d.zero()
+            n = DV.nlinsar(*[d.ptr for d in inp], *[d.ptr for d in out], h, w, sw, p, niter, lmin,
+                           stream=stream.handle, want_count=True)
+            ref.append(([d.download() for d in out], n))
+        assert sum(r[1] for r in ref) > 0
+        for mode in (0, 1, 2, -1):
+            DV.set_nlinsar_weight_store(mode)
+            b = DV.NlinsarBatch()
+            try:
+                b.set([(*[d.ptr for d in inp], *[d.ptr for d in out], h, w) for inp, out, h, w in imgs],
+                      sw, p, niter, lmin)
+                for run in range(3):
+                    for _inp, out, _h, _w in imgs:
+                        for d in out:
+                            d.zero()
+                    HIP.synchronize()
+                    if run == 0:
+                        b.enqueue(stream.handle)
+                    else:
+                        b.graph_launch(stream.handle)
+                    stream.synchronize()
+                    counts = b.smoothed()
+                    for i, (_inp, out, h, w) in enumerate(imgs):
+                        if run == 0:
+                            assert counts[i] == ref[i][1], f"mode {mode}, image {i} ({h}x{w}) smoothing count"
+                        for k in range(3):
+                            assert np.array_equal(out[k].download(), ref[i][0][k], equal_nan=True), \
+                                f"mode {mode}, run {run}, image {i} ({h}x{w}) output {k}"
+            finally:
+                b.close()
+    finally:
+        DV.set_nlinsar_weight_store(old)
+        for inp, out, _h, _w in imgs:
+            for d in inp + out:
+                d.free()
+        D.lib().dkx_release_workspace()
