@@ -31,11 +31,18 @@ H = W = 2048
 SW, PATCHES, SCALES = 21, [3, 5, 7], [1, 3]
 HPAR, CPAR = 15.0, 49.0
 PEAK_FP32_TFLOPS = 157.3
-# HBM-side bytes per launch of the every-key last pass (the roofline's kernel): rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE on this benchmark (KiB per dispatch, mean of the dispatches
-# that write the ENL / alpha maps), scaled by what the same counters read for a 256 MiB
-# device-to-device copy (FETCH x 2.000, WRITE x 1.000); profiles/nlsar_2048_scale_pass_pmc.md
-TRAFFIC_LAST_PASS_BYTES = round((5146314 * 2.000 + 156480 * 1.000) * 1024)
+# HBM-side bytes per launch of the every-key last pass (the roofline's kernel), per pass
+# variant (1: two rows per thread, 2: one row): rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on
+# this benchmark with --tile 1 / --tile 2 (KiB per dispatch: the mean FETCH and the WRITE of
+# the dispatches that write the ENL / alpha maps), scaled by what the same counters read for
+# a 256 MiB device-to-device copy (FETCH x 2.000, WRITE x 1.000);
+# profiles/nlsar_2048_scale_pass_pmc.md
+TRAFFIC_LAST_PASS_BYTES = {1: round((5141421 * 2.000 + 156482 * 1.000) * 1024),
+                           2: round((864173 * 2.000 + 148032 * 1.000) * 1024)}
+# the same for a later NL-InSAR weights launch at 4096^2 (18 x 54 blocks): the mean over
+# iterations 2-10 of one call (profiles/nlinsar_per_call.md)
+TRAFFIC_NLINSAR_WEIGHTS_BYTES = round((3 * 6.15e10 + 3.95e10 + 3.56e10 + 3.45e10 + 3.39e10 + 3.35e10 +
+                                       3.33e10) / 9)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),
@@ -319,9 +326,12 @@ def nlinsar_run(n, niter, steps, warmup, tile=None, with_cpu=False):
     wk_ms = float(np.mean(later))
     ach = 31791.0 * px / (wk_ms * 1e-3) / 1e12
     out["roofline"] = {"bound": "valu", "achieved": round(ach, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                       "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                       "kernel": "nlinsar_weights (an iteration after the first)", "kernel_ms": round(wk_ms, 4),
-                       "note": f"31791 algorithmic FLOP/px x {px} px per launch"}
+                       "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                       "traffic": TRAFFIC_NLINSAR_WEIGHTS_BYTES if n == 4096 and tuned and tuned["shape"] == 1 else None,
+                       "kernel": "nlinsar_weights (iterations 2 to niter, mean)", "kernel_ms": round(wk_ms, 4),
+                       "note": f"31791 algorithmic FLOP/px x {px} px per launch; traffic: the mean HBM-side bytes "
+                               "of those launches (all weights stored in iterations 2-4, the predicted "
+                               "pixels' after)"}
     for d in d_in + d_out:
         d.free()
     lib.dkx_release_workspace()
@@ -752,7 +762,7 @@ def main():
                    "tile": DV.get_tile(0)},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": TRAFFIC_LAST_PASS_BYTES,
+                     "traffic": TRAFFIC_LAST_PASS_BYTES.get(tuned["every_key"]["variant"]),
                      "kernel": "nlsar_scale_pass, last pass (patches 3/5/7 at scale 3, weighted mean, "
                                "decomposition)",
                      "kernel_ms": round(last_ms, 4),
