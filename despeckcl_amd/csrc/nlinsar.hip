@@ -412,6 +412,11 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
         nst = __builtin_amdgcn_readfirstlane(nst);
         wcw = uni(a.wc + (long long)cslot0 * (sw * sw));
     }
+    // the lane's weight-store offsets from an offset's base: its tile slot (dense) or its rank
+    // in the wave's run (compact)
+    unsigned int soff[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) soff[q] = dense ? (unsigned int)(tid + q * NCT) : cof[q];
 
     // stage the neighbourhood: rows ty0 - o .. ty0 + TY + o, columns tx0 - o .. tx0 + TX + o
     {
@@ -508,12 +513,20 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
         auto accumulate = [&](int plane, const float (&sv)[RPT], const float4 (&fv)[RPT],
                               const float* rows) {
             float* wb = a.wt + (chunk + (long long)plane * NPIX);
+            // this offset's weight stores: a wave-uniform base in the global address space and
+            // the lane's offset.  Loads and stores share one counter (vmcnt), so a use of a
+            // loaded value after a store waits for the store too: every load of this offset is
+            // waited for before the weights are formed (s_waitcnt vmcnt(0), expcnt and lgkmcnt
+            // left free), and the stores then stay in flight across the next offset's LDS phase.
+            DKX_GLOBAL float* const sbase = gptr(dense ? wb : wcw + (unsigned int)(plane * nst));
             const bool self = plane == centre;
+            float wq[RPT];
             float ck[RPT + NP2];
             if (rows) {
 #pragma unroll
                 for (int j = 0; j < RPT + NP2; j++) ck[j] = rows[(og * RPT + j) * TX + oc];
             }
+            __builtin_amdgcn_s_waitcnt(0x0F70);
 #pragma unroll
             for (int q = 0; q < RPT; q++) {
                 float v = -0.0f;
@@ -524,10 +537,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                 }
                 float w = expf(sv[q] * a.rh + v * a.rT);
                 if (isnan(w) || isinf(w) || self) w = 0.0f;
-                if (stq[q]) {
-                    if (dense) wb[(unsigned int)(tid + q * NCT)] = w;
-                    else wcw[(unsigned int)(plane * nst) + cof[q]] = w;
-                }
+                wq[q] = w;
                 const float4 f = fv[q];
                 acc[q][0] += w;
                 acc[q][1] = fmaf(w, w, acc[q][1]);
@@ -535,6 +545,9 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                 acc[q][3] = fmaf(w, f.y, acc[q][3]);
                 acc[q][4] = fmaf(w, f.z, acc[q][4]);
             }
+#pragma unroll
+            for (int q = 0; q < RPT; q++)
+                if (stq[q]) sbase[soff[q]] = wq[q];
         };
         if (!kl) {
             for (int plane = 0, dy = -wsh, dx = -wsh; plane < npl; plane++) {
@@ -645,6 +658,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                 // 3. rows accumulated top to bottom (precompute_patch_similarities.cl:53-58),
                 //    weight (compute_weights.cl:19-30), ENL sums (compute_number_of_looks.cl)
                 //    and weighted means (compute_insar.cl:81-93) in plane order
+                DKX_GLOBAL float* const sbase = gptr(dense ? a.wt + pbase : wcw + (unsigned int)(plane * nst));
                 float cs[RPT + NP2], ck[RPT + NP2];
 #pragma unroll
                 for (int j = 0; j < RPT + NP2; j++) cs[j] = rS[(og * RPT + j) * TX + oc];
@@ -652,12 +666,17 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
 #pragma unroll
                     for (int j = 0; j < RPT + NP2; j++) ck[j] = rK[(og * RPT + j) * TX + oc];
                 }
+                // the similarity sums and weights are stored after the accumulation, this
+                // offset's loads waited for first (loads and stores share one counter: see the
+                // later iterations' accumulate)
+                __builtin_amdgcn_s_waitcnt(0x0F70);
+                float uq[RPT], wq[RPT];
 #pragma unroll
                 for (int q = 0; q < RPT; q++) {
                     float u = cs[q];
 #pragma unroll
                     for (int j = 1; j <= NP2; j++) u += cs[q + j];
-                    if (valid[q] && !self) a.scache[pbase + tid + q * NCT] = u;
+                    uq[q] = u;
                     float v = -0.0f;
                     if (kl) {
                         v = ck[q];
@@ -666,16 +685,18 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
                     }
                     float w = expf(u * a.rh + v * a.rT);
                     if (isnan(w) || isinf(w) || self) w = 0.0f;
-                    if (stq[q]) {
-                        if (dense) a.wt[pbase + tid + q * NCT] = w;
-                        else wcw[(unsigned int)(plane * nst) + cof[q]] = w;
-                    }
+                    wq[q] = w;
                     const float4 f = fv[q];
                     acc[q][0] += w;
                     acc[q][1] = fmaf(w, w, acc[q][1]);
                     acc[q][2] = fmaf(w, f.x, acc[q][2]);
                     acc[q][3] = fmaf(w, f.y, acc[q][3]);
                     acc[q][4] = fmaf(w, f.z, acc[q][4]);
+                }
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    if (valid[q] && !self) a.scache[pbase + tid + q * NCT] = uq[q];
+                    if (stq[q]) sbase[soff[q]] = wq[q];
                 }
             }
         }
