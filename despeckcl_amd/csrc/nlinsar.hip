@@ -305,6 +305,30 @@ __device__ __forceinline__ NlTile nl_tile(const WeightArgs& a, int t, int TY, in
     return g;
 }
 
+// Weight of a patch similarity u and KL divergence v, e^(u rh + v rT) with rh = 1/h and
+// rT = 1/T (compute_weights.cl:23-27: exp(sim / h + kl / T), 0 when NaN or infinite), and 0
+// for the centre offset.  e^x is the device library's expf, operation for operation: x log2 e
+// carried in two parts (the product, its FMA residual plus x times the remainder of log2 e),
+// its nearest integer n split off, v_exp_f32 of the rest and v_ldexp_f32 by n.  The library
+// sends x < -103.27893 to 0 and x > 88.72284 to infinity, both of which end as weight 0
+// here, so its two range selections and the NaN / infinity test become one range test of x
+// and one class test of the result (the same bits for every float x, 2^24 of them compared
+// on the device including both range ends, infinities and NaN).  Every kernel that forms
+// weights calls this one function.
+__device__ __forceinline__ float nl_weight(float u, float v, float rh, float rT, bool centre)
+{
+    const float x = u * rh + v * rT;
+    const float L = 0x1.715476p+0f;        // log2 e, leading float
+    const float Llo = 0x1.4ae0bep-26f;     // log2 e - L
+    const float ph = x * L;
+    float pl = fmaf(x, L, -ph);
+    pl = fmaf(x, Llo, pl);
+    const float n = __builtin_rintf(ph);
+    const float r = __builtin_ldexpf(__builtin_amdgcn_exp2f((ph - n) + pl), (int)n);
+    const bool in = __builtin_amdgcn_fmed3f(x, -0x1.9d1da0p+6f, 0x1.62e430p+6f) == x;
+    return (in & __builtin_isfinite(r) & !centre) ? r : 0.0f;
+}
+
 // nl_pixel_kl_pre(e1, e2) with the pixel the reference passes second given by `second`:
 // r, and a1 + a2, are the same bits for either argument order (products and fmaf commute,
 // and a2 + a1 == a1 + a2), so only the zero test depends on the order.
@@ -535,8 +559,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
 #pragma unroll
                     for (int j = 1; j <= NP2; j++) v += ck[q + j];
                 }
-                float w = expf(sv[q] * a.rh + v * a.rT);
-                if (isnan(w) || isinf(w) || self) w = 0.0f;
+                const float w = nl_weight(sv[q], v, a.rh, a.rT, self);
                 wq[q] = w;
                 const float4 f = fv[q];
                 acc[q][0] += w;
@@ -683,8 +706,7 @@ __device__ __forceinline__ void nlinsar_weights_tile(const WeightArgs& a, const 
 #pragma unroll
                         for (int j = 1; j <= NP2; j++) v += ck[q + j];
                     }
-                    float w = expf(u * a.rh + v * a.rT);
-                    if (isnan(w) || isinf(w) || self) w = 0.0f;
+                    const float w = nl_weight(u, v, a.rh, a.rT, self);
                     wq[q] = w;
                     const float4 f = fv[q];
                     acc[q][0] += w;
@@ -893,8 +915,7 @@ __global__ __launch_bounds__(64) void nlinsar_fill_weights(const FillBatch f)
                         v = j == 0 ? r : v + r;
                     }
                 }
-                float w = expf(sb[(long long)plane * NPIX] * a.rh + v * a.rT);
-                if (isnan(w) || isinf(w) || plane == centre) w = 0.0f;
+                const float w = nl_weight(sb[(long long)plane * NPIX], v, a.rh, a.rT, plane == centre);
                 wq[plane] = w;
             }
             if (lane == 0) {
