@@ -304,8 +304,8 @@ int nlinsar_problem(int height, int width, int sw, int p, int niter, int lmin, i
 {
     // nlinsar.cpp:65-78: h from the patch size (tabulated for 3 and 7, else the seeded
     // restatement of the reference's Monte-Carlo quantile, nlinsar_simu.cpp)
-    if (p < 1 || p % 2 == 0 || p > 11) {
-        set_error("nlinsar: patch sizes must be odd, up to 11 (the weights kernels are compiled for those)");
+    if (p < 1 || p % 2 == 0 || p > 2 * NLI_MAX_PSH + 1) {
+        set_error("nlinsar: patch sizes must be odd, up to 21 (the weights kernels are compiled for those)");
         return DKX_EINVAL;
     }
     const float h_theo = nlinsar_h_theo(p);

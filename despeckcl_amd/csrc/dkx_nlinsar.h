@@ -39,6 +39,8 @@ constexpr int NLINSAR_TILE_AUTO = -1;
 // the interior (single calls: one launch, the edge blocks' latency hidden among the
 // interior's) or, for batches (NlinsarProblem::thin_edges), in thin blocks of their own
 // (1 x 54 and 54 x 3), whose launches cover the edges of all images at once.
+// largest patch half width the weights kernels are instantiated for (patch sizes 1-21)
+constexpr int NLI_MAX_PSH = 10;
 constexpr int NLI_SHAPES = 3;
 // A queued pixel's weight slot: the shape in the bits from NLI_SLOT_BITS up, below them the
 // slot within that shape's weight region.

@@ -1624,7 +1624,7 @@ int nlinsar_plan(const NlinsarProblem& pb, NlinsarPlan* plan)
         set_error("nlinsar: search window and patch sizes must be odd");
         return -1;
     }
-    if (pb.p > 11) { set_error("nlinsar: patch sizes must be odd, up to 11"); return -1; }
+    if (pb.p > 2 * NLI_MAX_PSH + 1) { set_error("nlinsar: patch sizes must be odd, up to 21"); return -1; }
     if (pb.niter < 0 || pb.lmin < 1 || pb.lmin > pb.sw * pb.sw) {
         set_error("nlinsar: niter must be >= 0 and 1 <= lmin <= search_window_size^2");
         return -1;
@@ -1819,7 +1819,13 @@ static int launch_fill_psh(int psh, const FillBatch& f, int wsh, int grid_x, int
     case 2: return launch_fill<2>(f, wsh, grid_x, ncu, stream);
     case 3: return launch_fill<3>(f, wsh, grid_x, ncu, stream);
     case 4: return launch_fill<4>(f, wsh, grid_x, ncu, stream);
-    default: return launch_fill<5>(f, wsh, grid_x, ncu, stream);
+    case 5: return launch_fill<5>(f, wsh, grid_x, ncu, stream);
+    case 6: return launch_fill<6>(f, wsh, grid_x, ncu, stream);
+    case 7: return launch_fill<7>(f, wsh, grid_x, ncu, stream);
+    case 8: return launch_fill<8>(f, wsh, grid_x, ncu, stream);
+    case 9: return launch_fill<9>(f, wsh, grid_x, ncu, stream);
+    case 10: return launch_fill<10>(f, wsh, grid_x, ncu, stream);
+    default: set_error("nlinsar: patch sizes must be odd, up to 21"); return -1;
     }
 }
 
@@ -1951,9 +1957,10 @@ int nlinsar_run(const NlinsarProblem& pb, const NlinsarPlan& plan, const float* 
                          : launch_weights_all<P, false>(plan.tile, thin, wa, L, wt, scache, stream); \
             break;
         DKX_NLI_PSH(0) DKX_NLI_PSH(1) DKX_NLI_PSH(2) DKX_NLI_PSH(3) DKX_NLI_PSH(4) DKX_NLI_PSH(5)
+        DKX_NLI_PSH(6) DKX_NLI_PSH(7) DKX_NLI_PSH(8) DKX_NLI_PSH(9) DKX_NLI_PSH(10)
 #undef DKX_NLI_PSH
         default:
-            set_error("nlinsar: patch sizes must be odd, up to 11");
+            set_error("nlinsar: patch sizes must be odd, up to 21");
             rc = -1;
         }
         if (rc) return rc;
@@ -2513,9 +2520,10 @@ int nlinsar_batch_enqueue(const NlinsarProblem& common, const NlinsarBatchLaunch
                          : launch_weights_batch_all<P, false>(L, wk, queue, ncu, stream);     \
             break;
         DKX_NLI_PSH(0) DKX_NLI_PSH(1) DKX_NLI_PSH(2) DKX_NLI_PSH(3) DKX_NLI_PSH(4) DKX_NLI_PSH(5)
+        DKX_NLI_PSH(6) DKX_NLI_PSH(7) DKX_NLI_PSH(8) DKX_NLI_PSH(9) DKX_NLI_PSH(10)
 #undef DKX_NLI_PSH
         default:
-            set_error("nlinsar: patch sizes must be odd, up to 11");
+            set_error("nlinsar: patch sizes must be odd, up to 21");
             rc = -1;
         }
         if (rc) return rc;

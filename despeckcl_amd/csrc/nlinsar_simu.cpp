@@ -99,8 +99,8 @@ extern "C" int dkx_nlinsar_simulate_h(int patch_size, unsigned int seed, float* 
 {
     return dkx::guarded<int>([&]() -> int {
         if (!h_theo) { dkx::set_error("nlinsar: null output pointer"); return DKX_EINVAL; }
-        if (patch_size < 1 || patch_size % 2 == 0 || patch_size > 11) {
-            dkx::set_error("nlinsar: patch sizes must be odd, up to 11");
+        if (patch_size < 1 || patch_size % 2 == 0 || patch_size > 2 * dkx::NLI_MAX_PSH + 1) {
+            dkx::set_error("nlinsar: patch sizes must be odd, up to 21");
             return DKX_EINVAL;
         }
         *h_theo = dkx::nlinsar_simulate_h(patch_size, 0.92f, seed);

@@ -110,7 +110,7 @@ int dkx_nlsar_covmat_device(const float* d_covmat, float* d_covmat_filt, int hei
 /* despeckcl::nlinsar (include/despeckcl.h:97-109; src/compute/nlinsar/nlinsar.cpp:42-146).
  * ref_filt / phase_filt / coh_filt are read as the first iteration's estimates and
  * overwritten with the result, like the reference (nlinsar.cpp:59-61).  Odd patch sizes up
- * to 11.  h: 0.58 for patch 3 and 0.244 for patch 7 as tabulated by the reference; for the
+ * to 21.  h: 0.58 for patch 3 and 0.244 for patch 7 as tabulated by the reference; for the
  * others the reference's Monte-Carlo quantile (insarsim_simu.cpp:62-82), which it seeds from
  * std::random_device, here with a fixed seed (dkx_nlinsar_simulate_h with seed 5489). */
 int dkx_nlinsar(const float* ampl_master, const float* ampl_slave, const float* phase,

@@ -1154,11 +1154,11 @@ float orc_nlinsar_pixel_kl(float ref1, float dp1, float coh1, float ref2, float 
 /* h_theo for patch sizes without a tabulated value (index (p - 1) / 2): the reference draws
  * it by a Monte-Carlo quantile seeded from std::random_device (insarsim_simu.cpp:62-82), so
  * the checker takes the value the filter under test used; 0 = not set. */
-static float g_h_theo[8];
+static float g_h_theo[11];
 
 int orc_nlinsar_set_h_theo(int patch_size, float h_theo)
 {
-    if (patch_size < 1 || patch_size % 2 == 0 || patch_size > 15) return -1;
+    if (patch_size < 1 || patch_size % 2 == 0 || patch_size > 21) return -1;
     g_h_theo[(patch_size - 1) / 2] = h_theo;
     return 0;
 }
@@ -1170,7 +1170,7 @@ int orc_nlinsar_params(int patch_size, float* h_para, float* T_para)
     float h_theo;
     if (patch_size == 3) h_theo = 0.58f;
     else if (patch_size == 7) h_theo = 0.244f;
-    else if (patch_size >= 1 && patch_size % 2 == 1 && patch_size <= 15 && g_h_theo[(patch_size - 1) / 2] != 0.0f)
+    else if (patch_size >= 1 && patch_size % 2 == 1 && patch_size <= 21 && g_h_theo[(patch_size - 1) / 2] != 0.0f)
         h_theo = g_h_theo[(patch_size - 1) / 2];
     else return -1;
     *h_para = h_theo * patch_area;

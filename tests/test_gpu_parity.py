@@ -453,7 +453,7 @@ def test_nlinsar_small_windows(D, oracle, sw, p, lmin, shape):
 
 
 @pytest.mark.parametrize("p,sw,lmin,shape", [(1, 11, 5, (50, 61)), (5, 11, 5, (70, 90)), (9, 15, 10, (64, 70)),
-                                             (11, 11, 5, (60, 75))])
+                                             (11, 11, 5, (60, 75)), (13, 11, 5, (58, 66)), (21, 9, 5, (47, 52))])
 def test_nlinsar_other_patch_sizes(D, oracle, p, sw, lmin, shape):
     """Patch sizes whose h the reference simulates (quantile_insar, seeded here): the oracle
     gets the filter's own h, three iterations checked as a trajectory."""

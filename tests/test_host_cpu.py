@@ -100,8 +100,8 @@ def test_argument_errors_without_device(D):
         D.goldstein(a, a, a, 32, 16, 0.5)
     with pytest.raises(ValueError, match="quarter"):
         D.goldstein(a, a, a, 16, 5, 0.5)
-    with pytest.raises(ValueError, match="odd, up to 11"):
-        D.nlinsar(a, a, a, 21, 13, 2, 10)
+    with pytest.raises(ValueError, match="odd, up to 21"):
+        D.nlinsar(a, a, a, 21, 23, 2, 10)
     with pytest.raises(ValueError, match="odd"):
         D.nlinsar(a, a, a, 21, 4, 2, 10)
     with pytest.raises(ValueError, match="odd"):
@@ -188,7 +188,7 @@ def test_nlinsar_simulated_h_reproduces_the_tabulated_values(D):
     assert all(a > b for a, b in zip(hs, hs[1:])), hs
     assert DV.nlinsar_simulate_h(5, 7) == DV.nlinsar_simulate_h(5, 7)
     assert DV.nlinsar_simulate_h(5, 7) != DV.nlinsar_simulate_h(5, 8)
-    for bad in (0, 2, 13):
+    for bad in (0, 2, 23):
         with pytest.raises(ValueError, match="odd"):
             DV.nlinsar_simulate_h(bad)
 
