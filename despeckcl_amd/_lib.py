@@ -31,6 +31,10 @@ class DkxImage(C.Structure):
                 ("height", C.c_int), ("width", C.c_int)]
 
 
+class DkxCovmatImage(C.Structure):
+    _fields_ = [("covmat", C.c_void_p), ("covmat_filt", C.c_void_p), ("height", C.c_int), ("width", C.c_int)]
+
+
 class DkxRequest(C.Structure):
     _fields_ = [("kind", C.c_int), ("ampl_master", C.c_void_p), ("ampl_slave", C.c_void_p),
                 ("phase", C.c_void_p), ("ref_filt", C.c_void_p), ("phase_filt", C.c_void_p),
@@ -80,6 +84,8 @@ SIGNATURES = {
     "dkx_nlsar_batch_create": (_I, [C.POINTER(C.c_void_p)]),
     "dkx_nlsar_batch_destroy": (None, [_P]),
     "dkx_nlsar_batch_set": (_I, [_P, C.POINTER(DkxImage), _I, _I, _IP, _I, _IP, _I, _SP, _I, _F, _F]),
+    "dkx_nlsar_batch_set_covmat": (_I, [_P, C.POINTER(DkxCovmatImage), _I, _I, _I, _IP, _I, _IP, _I, _SP, _I, _F,
+                                        _F]),
     "dkx_nlsar_batch_enqueue": (_I, [_P, _P]),
     "dkx_nlsar_batch_graph_launch": (_I, [_P, _P]),
     "dkx_nlsar_batch_set_tile": (_I, [_P, _I]),

@@ -72,7 +72,8 @@ static int drop_graph(NlsarBatch* b)
 static int choose_tile(NlsarBatch* b)
 {
     int t = b->tile;
-    if (t < 0) {
+    if (b->pp.dim != 0) t = NLSAR_TILE_AUTO;   // covariance input: one pass variant per dimension
+    else if (t < 0) {
         // the autotuner's record for this batch shape, else per pass as the single calls
         t = tuned_tile(0, b->diagnostics, b->pp.sw, b->P.data(), (int)b->P.size(), b->S.data(), (int)b->S.size(),
                        batch_pixels(*b));
@@ -101,46 +102,31 @@ static int batch_mark(NlsarBatch* b, hipStream_t stream)
     return 0;
 }
 
-}  // namespace dkx
+// The images of a batch: InSAR triples (dim 0) or covariance matrices of one dimension.
+struct BatchImages {
+    const dkx_nlsar_image* insar = nullptr;
+    const dkx_nlsar_covmat_image* cov = nullptr;
+    int n = 0, dim = 0;
+    int height(int i) const { return insar ? insar[i].height : cov[i].height; }
+    int width(int i) const { return insar ? insar[i].width : cov[i].width; }
+};
 
-using namespace dkx;
-
-extern "C" {
-
-int dkx_nlsar_batch_create(dkx_nlsar_batch* out)
+// dkx_nlsar_batch_set / _set_covmat
+static int batch_set(NlsarBatch* b, const BatchImages& im, int sw, const int* P, int nP, const int* S, int nS,
+                     const dkx_nlsar_stats* stats, int n_stats, float hpar, float cpar)
 {
-    return dkx::guarded<int>([&]() -> int {
-        if (!out) { set_error("nlsar batch: null handle pointer"); return DKX_EINVAL; }
-        *out = reinterpret_cast<dkx_nlsar_batch>(new NlsarBatch());
-        return 0;
-    });
-}
-
-void dkx_nlsar_batch_destroy(dkx_nlsar_batch h)
-{
-    dkx::guarded_void([&] {
-        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-        if (!b) return;
-        (void)batch_wait(b);
-        (void)drop_graph(b);
-        if (b->done) (void)hipEventDestroy(b->done);
-        if (b->ws) (void)hipFree(b->ws);
-        if (b->consts) (void)hipFree(b->consts);
-        delete b;
-    });
-}
-
-int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n, int sw, const int* P,
-                        int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
-                        float hpar, float cpar)
-{
-    return dkx::guarded<int>([&]() -> int {
-        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
-        if (!b || !images || n < 1) { set_error("nlsar batch: no images"); return DKX_EINVAL; }
+        const int n = im.n;
+        if (!b || (!im.insar && !im.cov) || n < 1) { set_error("nlsar batch: no images"); return DKX_EINVAL; }
         if (!P || !S || nP < 1 || nS < 1) { set_error("nlsar batch: empty patch or scale size list"); return DKX_EINVAL; }
+        if (im.cov && im.dim != 2 && im.dim != 3) {
+            set_error("nlsar batch: covariance matrices of dimension 2 or 3 are supported");
+            return im.dim == 1 ? DKX_EUNSUPPORTED : DKX_EINVAL;
+        }
         for (int i = 0; i < n; i++) {
-            const dkx_nlsar_image& im = images[i];
-            if (!im.ampl_master || !im.ampl_slave || !im.phase || !im.ref_filt || !im.phase_filt || !im.coh_filt) {
+            const bool null_array = im.insar ? (!im.insar[i].ampl_master || !im.insar[i].ampl_slave || !im.insar[i].phase ||
+                                                !im.insar[i].ref_filt || !im.insar[i].phase_filt || !im.insar[i].coh_filt)
+                                             : (!im.cov[i].covmat || !im.cov[i].covmat_filt);
+            if (null_array) {
                 set_error("nlsar batch: null array in image " + std::to_string(i));
                 return DKX_EINVAL;
             }
@@ -163,7 +149,7 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
             v.chi2cdf_inv = nb.lut_c[k].data();
             nb.views.push_back(v);
         }
-        nb.pp = NlsarParams{0, sw, nb.P.data(), nP, nb.S.data(), nS, nb.views.data(), (int)nb.views.size(),
+        nb.pp = NlsarParams{im.cov ? im.dim : 0, sw, nb.P.data(), nP, nb.S.data(), nS, nb.views.data(), (int)nb.views.size(),
                             hpar, cpar, 1, b->diagnostics, last_unused};
         rc = nlsar_common(nb.pp, &nb.cm);
         if (rc) return rc;
@@ -173,7 +159,7 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         size_t off = qoff + (NLSAR_QUEUE_BYTES + 255) / 256 * 256;
         std::vector<size_t> offs(n);
         for (int i = 0; i < n; i++) {
-            rc = nlsar_image_plan(nb.cm, images[i].height, images[i].width, -1, -1, &plans[i]);
+            rc = nlsar_image_plan(nb.cm, im.height(i), im.width(i), -1, -1, &plans[i]);
             if (rc) return rc;
             offs[i] = off;
             off += plans[i].bytes;
@@ -210,9 +196,16 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         DKX_HIP_CHECK(hipMemcpy(b->consts, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice));
         nb.descs.resize(n);
         for (int i = 0; i < n; i++) {
-            const dkx_nlsar_image& im = images[i];
-            nlsar_image_desc(plans[i], (char*)b->ws + offs[i], im.ampl_master, im.ampl_slave, im.phase, nullptr,
-                             im.ref_filt, im.phase_filt, im.coh_filt, nullptr, nullptr, nullptr, &nb.descs[i]);
+            char* ws_i = (char*)b->ws + offs[i];
+            if (im.insar) {
+                const dkx_nlsar_image& x = im.insar[i];
+                nlsar_image_desc(plans[i], ws_i, x.ampl_master, x.ampl_slave, x.phase, nullptr, x.ref_filt, x.phase_filt,
+                                 x.coh_filt, nullptr, nullptr, nullptr, &nb.descs[i]);
+            } else {
+                const dkx_nlsar_covmat_image& x = im.cov[i];
+                nlsar_image_desc(plans[i], ws_i, nullptr, nullptr, nullptr, x.covmat, nullptr, nullptr, nullptr,
+                                 x.covmat_filt, nullptr, nullptr, &nb.descs[i]);
+            }
         }
         int rows_c, rows_a, blocks, mcw, maw;
         nlsar_batch_layout(nb.descs.data(), n, &rows_c, &rows_a, &blocks, &mcw, &maw);
@@ -233,6 +226,59 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n,
         b->table = (NlsarImageDesc*)b->ws;
         b->queues = (unsigned int*)((char*)b->ws + qoff);
         return choose_tile(b);
+}
+
+}  // namespace dkx
+
+using namespace dkx;
+
+extern "C" {
+
+int dkx_nlsar_batch_create(dkx_nlsar_batch* out)
+{
+    return dkx::guarded<int>([&]() -> int {
+        if (!out) { set_error("nlsar batch: null handle pointer"); return DKX_EINVAL; }
+        *out = reinterpret_cast<dkx_nlsar_batch>(new NlsarBatch());
+        return 0;
+    });
+}
+
+void dkx_nlsar_batch_destroy(dkx_nlsar_batch h)
+{
+    dkx::guarded_void([&] {
+        NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
+        if (!b) return;
+        (void)batch_wait(b);
+        (void)drop_graph(b);
+        if (b->done) (void)hipEventDestroy(b->done);
+        if (b->ws) (void)hipFree(b->ws);
+        if (b->consts) (void)hipFree(b->consts);
+        delete b;
+    });
+}
+
+int dkx_nlsar_batch_set(dkx_nlsar_batch h, const dkx_nlsar_image* images, int n, int sw, const int* P,
+                        int nP, const int* S, int nS, const dkx_nlsar_stats* stats, int n_stats,
+                        float hpar, float cpar)
+{
+    return dkx::guarded<int>([&]() -> int {
+        BatchImages im;
+        im.insar = images;
+        im.n = n;
+        return batch_set(reinterpret_cast<NlsarBatch*>(h), im, sw, P, nP, S, nS, stats, n_stats, hpar, cpar);
+    });
+}
+
+int dkx_nlsar_batch_set_covmat(dkx_nlsar_batch h, const dkx_nlsar_covmat_image* images, int n, int dim, int sw,
+                               const int* P, int nP, const int* S, int nS, const dkx_nlsar_stats* stats,
+                               int n_stats, float hpar, float cpar)
+{
+    return dkx::guarded<int>([&]() -> int {
+        BatchImages im;
+        im.cov = images;
+        im.n = n;
+        im.dim = dim;
+        return batch_set(reinterpret_cast<NlsarBatch*>(h), im, sw, P, nP, S, nS, stats, n_stats, hpar, cpar);
     });
 }
 
@@ -335,6 +381,13 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch h, void* stream_, int reps, int* ti
     return dkx::guarded<int>([&]() -> int {
         NlsarBatch* b = reinterpret_cast<NlsarBatch*>(h);
         if (!b || b->descs.empty()) { set_error("nlsar batch: no images set"); return DKX_EINVAL; }
+        if (b->pp.dim != 0) {
+            // covariance input: one pass variant per dimension, nothing to choose or record
+            if (tile_out) *tile_out = -1;
+            if (ms_out)
+                for (int t = 0; t < NLSAR_NTILES; t++) ms_out[t] = 0.f;
+            return 0;
+        }
         hipStream_t stream = (hipStream_t)stream_;
         reps = std::max(reps, 1);
         int rc = batch_wait(b);

@@ -150,6 +150,7 @@ class NlsarBatch:
 
         b = NlsarBatch()
         b.set(images, 21, [3, 5, 7], [1, 3], stats)   # images: (a1, a2, phase, ref, phase_filt, coh, h, w)
+        b.set_covmat(images, 3, 21, [3, 5, 7], [1, 3], stats)   # or (covmat, covmat_filt, h, w)
         b.enqueue(stream)          # or b.graph_launch(stream) to capture once and replay
     """
 
@@ -181,6 +182,21 @@ class NlsarBatch:
         st, ns, keep = _stats_structs(nlsar_stats)
         check(lib().dkx_nlsar_batch_set(self._h, arr, len(images), int(search_window_size), P, nP, S, nS,
                                         st, ns, float(h), float(c)))
+        self.n = len(images)
+
+    def set_covmat(self, images, dim, search_window_size, patch_sizes, scale_sizes, nlsar_stats, h=15.0, c=49.0):
+        """Covariance input of dimension 2 or 3: images (covmat, covmat_filt, h, w), device
+        pointers to 2*dim*dim interlaced planes (the layout of nlsar_covmat)."""
+        from . import _stats_structs
+        from ._lib import DkxCovmatImage
+        arr = (DkxCovmatImage * len(images))()
+        for i, (cov, out, hh, ww) in enumerate(images):
+            arr[i] = DkxCovmatImage(cov, out, int(hh), int(ww))
+        P, nP = int_array(patch_sizes)
+        S, nS = int_array(scale_sizes)
+        st, ns, keep = _stats_structs(nlsar_stats)
+        check(lib().dkx_nlsar_batch_set_covmat(self._h, arr, len(images), int(dim), int(search_window_size), P, nP,
+                                               S, nS, st, ns, float(h), float(c)))
         self.n = len(images)
 
     def enqueue(self, stream=0):

@@ -296,6 +296,15 @@ typedef struct {
     int width;
 } dkx_nlsar_image;
 
+/* One image of a covariance batch: device pointers to 2*dim*dim float32 planes of
+ * height x width, interlaced as for dkx_nlsar_covmat. */
+typedef struct {
+    const float* covmat;
+    float* covmat_filt;
+    int height;
+    int width;
+} dkx_nlsar_covmat_image;
+
 typedef struct dkx_nlsar_batch_s* dkx_nlsar_batch;
 
 int dkx_nlsar_batch_create(dkx_nlsar_batch* batch);
@@ -308,6 +317,14 @@ int dkx_nlsar_batch_set(dkx_nlsar_batch batch, const dkx_nlsar_image* images, in
                         int search_window_size, const int* patch_sizes, int n_patch_sizes,
                         const int* scale_sizes, int n_scale_sizes, const dkx_nlsar_stats* stats,
                         int n_stats, float h_param, float c_param);
+/* The same for covariance input of dimension 2 or 3 (dkx_nlsar_covmat's meaning and checks);
+ * every image of the batch has that dimension.  A covariance batch has one pass variant per
+ * dimension, so _set_tile and _autotune leave it unchanged (_autotune reports -1). */
+int dkx_nlsar_batch_set_covmat(dkx_nlsar_batch batch, const dkx_nlsar_covmat_image* images,
+                               int n_images, int dim, int search_window_size,
+                               const int* patch_sizes, int n_patch_sizes, const int* scale_sizes,
+                               int n_scale_sizes, const dkx_nlsar_stats* stats, int n_stats,
+                               float h_param, float c_param);
 /* Enqueues the whole batch on `stream` (a hipStream_t); capturable in a HIP graph. */
 int dkx_nlsar_batch_enqueue(dkx_nlsar_batch batch, void* stream);
 /* Captures the batch's stage sequence into a HIP graph on first use (after each _set), then

@@ -178,3 +178,59 @@ def test_covmat_device_entry_equals_host_entry(D):
     got = _deinterlace(dout.download(), 3)
     assert np.array_equal(got, host)
     stream.destroy()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_covmat_batch_equals_single_calls(D, dim):
+    """Covariance input in a batch (dkx_nlsar_batch_set_covmat): every image of uneven size
+    equals its single dkx_nlsar_covmat_device call bit for bit (NaN positions included),
+    eagerly and on two replays of the batch's HIP graph with eager work between them."""
+    from despeckcl_amd import device as DV
+    from despeckcl_amd import hip as HIP
+    st = _stats(D, dim)
+    shapes = [(1, 5), (7, 3), (40, 64), (65, 33), (96, 130), (33, 200)]
+    npl = 2 * dim * dim
+    imgs = []
+    try:
+        for i, (h, w) in enumerate(shapes):
+            c = _interlace(_data(dim, h, w, 8100 + 10 * dim + i))
+            imgs.append((HIP.DeviceArray.from_host(c), HIP.DeviceArray((npl, h, w)), h, w))
+        stream = HIP.Stream()
+        ref = []
+        for cin, cout, h, w in imgs:
+            cout.zero()
+            DV.nlsar_covmat(cin.ptr, cout.ptr, h, w, dim, 21, P, S_, st, stream=stream.handle)
+            stream.synchronize()
+            ref.append(cout.download())
+        b = DV.NlsarBatch()
+        try:
+            b.set_covmat([(cin.ptr, cout.ptr, h, w) for cin, cout, h, w in imgs], dim, 21, P, S_, st)
+            assert b.autotune(stream.handle)[0] == -1
+            for run in range(3):
+                for _cin, cout, _h, _w in imgs:
+                    cout.zero()
+                HIP.synchronize()
+                if run == 0:
+                    b.enqueue(stream.handle)
+                else:
+                    b.graph_launch(stream.handle)
+                stream.synchronize()
+                for i, (_cin, cout, h, w) in enumerate(imgs):
+                    assert np.array_equal(cout.download(), ref[i], equal_nan=True), \
+                        f"dim {dim}, run {run}, image {i} ({h}x{w})"
+        finally:
+            b.close()
+    finally:
+        for cin, cout, _h, _w in imgs:
+            cin.free()
+            cout.free()
+
+
+def test_covmat_batch_rejects_other_dimensions(D):
+    from despeckcl_amd import device as DV
+    b = DV.NlsarBatch()
+    try:
+        with pytest.raises(ValueError, match="dimension 2 or 3"):
+            b.set_covmat([(1, 1, 4, 4)], 4, 21, P, S_, _stats(D, 2))
+    finally:
+        b.close()
