@@ -777,7 +777,8 @@ def main():
                               "pass_ms": round(sel_ms, 4), "pass_tflops": round(sel_ach, 3),
                               "flop_per_px_pass": FLOP_PER_PX_LAST_KEY_PASS},
         "autotune": dict(tuned, note="pass variants 0 / 1 / 2 = 4 / 2 / 1 rows per thread, 32 x 32 tiles; "
-                                     "mean of 2 runs each on this image"),
+                                     "mean of 2 runs each on this image; the pick: the fastest or, within 2 % of it, "
+                                     "the fewest rows per thread"),
         "gpu": gpu_info(),
         "p99_latency_ms": None,
         "ub_pixels": int(ub),

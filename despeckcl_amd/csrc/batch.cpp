@@ -397,8 +397,7 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch h, void* stream_, int reps, int* ti
         hipEvent_t e0, e1;
         DKX_HIP_CHECK(hipEventCreate(&e0));
         DKX_HIP_CHECK(hipEventCreate(&e1));
-        int best = 1;
-        float best_ms = 1e30f;
+        float ms_t[NLSAR_NTILES];
         for (int t = 0; t < NLSAR_NTILES; t++) {
             b->pp.tile = t;
             b->cm.tile = t;
@@ -413,14 +412,12 @@ int dkx_nlsar_batch_autotune(dkx_nlsar_batch h, void* stream_, int reps, int* ti
             DKX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             ms /= reps;
             if (ms_out) ms_out[t] = ms;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
+            ms_t[t] = ms;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc) return rc;
+        const int best = nlsar_pick_variant(ms_t);
         record_tuned_tile(0, b->diagnostics, b->pp.sw, b->P.data(), (int)b->P.size(), b->S.data(), (int)b->S.size(),
                           batch_pixels(*b), best);
         if (tile_out) *tile_out = best;

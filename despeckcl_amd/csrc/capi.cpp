@@ -598,8 +598,7 @@ int dkx_nlsar_insar_device_autotune(const float* a1, const float* a2, const floa
         if (!w) { set_error("no HIP device"); return DKX_EHIP; }
         const hipStream_t stream = (hipStream_t)stream_;
         reps = std::max(reps, 1);
-        int best = -1;
-        float best_ms = 0.f;
+        float ms_t[NLSAR_NTILES];
         for (int t = 0; t < NLSAR_NTILES; t++) {
             call.pp.tile = t;
             call.cm.tile = t;
@@ -615,11 +614,9 @@ int dkx_nlsar_insar_device_autotune(const float* a1, const float* a2, const floa
             }
             if (rc) return rc;
             if (ms_out) ms_out[t] = ms;
-            if (best < 0 || ms < best_ms) {
-                best = t;
-                best_ms = ms;
-            }
+            ms_t[t] = ms;
         }
+        const int best = nlsar_pick_variant(ms_t);
         record_tuned_tile(0, every_key ? 1 : 0, sw, P, nP, S, nS, (double)height * width, best);
         if (tile_out) *tile_out = best;
         return 0;

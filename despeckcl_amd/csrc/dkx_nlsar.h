@@ -15,6 +15,22 @@ constexpr int NLSAR_MAX_PATCHES = 16;  // patch sizes per call
 constexpr int NLSAR_MAX_NS = 8;
 constexpr int NLSAR_TILE = 32;         // every pass variant computes 32 x 32 output tiles
 constexpr int NLSAR_NTILES = 3;        // pass variants: 4, 2 or 1 rows per thread
+
+// The autotuner's choice among the pass variants' mean times: the fastest, or, among the
+// variants within 2 % of it (the spread of such timings from run to run), the one with the
+// fewest rows per thread.  Its blocks hold the smallest neighbourhood each, so the blocks
+// resident on an XCD keep theirs in its L2: at 2048^2 the every-key last pass fetches
+// 1.9e9 B from beyond L2 with one row per thread against 1.07e10 B with two, in the same time.
+inline int nlsar_pick_variant(const float (&ms)[NLSAR_NTILES])
+{
+    int fastest = 0;
+    for (int t = 1; t < NLSAR_NTILES; t++)
+        if (ms[t] < ms[fastest]) fastest = t;
+    int pick = fastest;
+    for (int t = fastest + 1; t < NLSAR_NTILES; t++)
+        if (ms[t] <= 1.02f * ms[fastest]) pick = t;
+    return pick;
+}
 constexpr int NLSAR_TILE_AUTO = -1;    // per pass: 1 row per thread for one patch size, else 2
 
 // Records of the grids, per mode (0: InSAR input; 2, 3: covariance input of that dimension).

@@ -234,7 +234,9 @@ int dkx_get_nlinsar_weight_store(void);
  * picks host tile dimensions from the device memory size alone; here every image is one
  * device call and the choice that matters is the kernels' tile variant.  The entry points
  * below run the filter on the caller's image once per variant (one warm-up run, then `reps`
- * timed runs each), record the fastest for the parameter set and the image size (log2 of the
+ * timed runs each), record the fastest (NL-SAR: among variants within 2 % of it, the one with
+ * the fewest rows per thread, which moves the fewest bytes from beyond L2) for the parameter
+ * set and the image size (log2 of the
  * pixel count) and return it; ms (optional, 3 entries) receives each variant's mean time, -1
  * for a variant the parameters do not allow.  While dkx_set_tile holds -1 for the filter,
  * single calls, batches (keyed by their total pixel count) and the server use a recorded
@@ -340,7 +342,8 @@ int dkx_nlsar_batch_set_tile(dkx_nlsar_batch batch, int tile);
 int dkx_nlsar_batch_set_diagnostics(dkx_nlsar_batch batch, int on);
 int dkx_nlsar_batch_get_tile(dkx_nlsar_batch batch);
 /* Times every pass variant on the batch (reps runs each, after one warm-up run), records the
- * fastest for this parameter set and total pixel count, and selects it.  ms (optional)
+ * fastest (within 2 %, the fewest rows per thread) for this parameter set and total pixel
+ * count, and selects it.  ms (optional)
  * receives the mean time of each variant. */
 int dkx_nlsar_batch_autotune(dkx_nlsar_batch batch, void* stream, int reps, int* tile,
                              float* ms);

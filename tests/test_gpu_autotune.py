@@ -53,6 +53,9 @@ def test_nlsar_autotune(D, stats, every_key):
                                         stats, every_key=every_key, reps=1)
         print("variant", t, "ms", ms)
         assert t in (0, 1, 2) and all(m > 0 for m in ms)
+        # the fastest variant, or within 2 % of it the one with the fewest rows per thread
+        fastest = min(range(3), key=lambda i: ms[i])
+        assert t == max(i for i in range(fastest, 3) if ms[i] <= 1.02 * ms[fastest])
         assert DV.tuned_tile(0, h, w, 21, PATCHES, SCALES, every_key) == t
         # another size in the same power-of-two bucket finds the record, another bucket not
         assert DV.tuned_tile(0, h + 10, w, 21, PATCHES, SCALES, every_key) == t
