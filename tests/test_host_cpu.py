@@ -110,6 +110,19 @@ def test_argument_errors_without_device(D):
         D.nlinsar(a, a, a, 21, 7, 2, 0)
 
 
+def test_images_beyond_32_bit_grid_offsets_are_rejected(D):
+    """The kernels address their padded grids with 32-bit offsets: images beyond that are
+    rejected with a message to split them into bands (the reference tiles any size on the
+    host), before any device call.  The pointers are never dereferenced."""
+    from despeckcl_amd import device as DV
+    st = _stats_for(D, [(3, 1), (5, 1), (7, 1), (3, 3), (5, 3), (7, 3)])
+    for n in (20000, 40000):
+        with pytest.raises(ValueError, match="too large for one call"):
+            DV.nlsar_insar(8, 8, 8, 8, 8, 8, n, n, 21, [3, 5, 7], [1, 3], st)
+    with pytest.raises(ValueError, match="too large for one call"):
+        DV.nlinsar(8, 8, 8, 8, 8, 8, 40000, 40000, 21, 7, 2, 10)
+
+
 def test_statistics_collections_with_extra_keys(D):
     """A collection trained on more (patch, scale) keys than the call filters with passes the
     host checks, as in the reference (only the device step fails here, without a GPU); a
