@@ -41,8 +41,8 @@ TRAFFIC_LAST_PASS_BYTES = {1: round((5141421 * 2.000 + 156482 * 1.000) * 1024),
                            2: round((864173 * 2.000 + 148032 * 1.000) * 1024)}
 # the same for a later NL-InSAR weights launch at 4096^2 (18 x 54 blocks): the mean over
 # iterations 2-10 of one call (profiles/nlinsar_per_call.md)
-TRAFFIC_NLINSAR_WEIGHTS_BYTES = round((6.15e10 + 6.14e10 + 6.17e10 + 3.95e10 + 3.55e10 + 3.44e10 + 3.38e10 +
-                                       3.34e10 + 3.32e10) / 9)
+TRAFFIC_NLINSAR_WEIGHTS_BYTES = round((6.12e10 + 6.12e10 + 6.14e10 + 3.92e10 + 3.54e10 + 3.44e10 + 3.39e10 +
+                                       3.35e10 + 3.33e10) / 9)
 # algorithmic FLOP per output pixel (each transcendental / divide = 1, FMA = 2):
 # pixel dissimilarity 220*27 per scale, patch sums 220*2*(p-1) summed over p, LUT weights
 # 220*8 per (p, s), nonlocal statistics 441*11 per (p, s), alpha/ENL 22 per (p, s),

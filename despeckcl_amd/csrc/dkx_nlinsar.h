@@ -54,7 +54,7 @@ struct NlinsarPlan {
     int wsh, psh, overlap, step;   // step = ref_tile - 2*overlap (interior of a reference tile)
     int eh, ew;                    // virtual distance of the last row / column
     Grid R, E, F;                  // float4 grids: raw (a1, a2, phase), estimates (ref, phase,
-                                   // coh), filter values (a, Re x, Im x)
+                                   // coh); 12-byte records: filter values (a, Re x, Im x)
     int heap_block;                // threads per block of the smoothing kernel
     int blocks[NLI_SHAPES];        // blocks of the weights kernel per shape (weight / cache chunks)
     size_t off_w[NLI_SHAPES], off_s[NLI_SHAPES];   // weight and similarity-cache regions per shape
