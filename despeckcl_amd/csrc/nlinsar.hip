@@ -1222,7 +1222,8 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 // search_window_smoothing (smoothing.cpp:78-129) and compute_insar for the queued pixels,
 // one pixel per thread.  Offsets are visited in the reference's row-major order, CH at a time
 // (NLI_SCH: 8 ran 3 % faster than 4 in the heaviest iteration at 4096², at 3 waves per SIMD
-// instead of 4): their loads are issued together and then used in order.  Each variant
+// instead of 4; 8 with the registers capped for 4 waves per SIMD ran 5 % slower, 19.7 against
+// 18.8 ms): their loads are issued together and then used in order.  Each variant
 // serves the queue lengths in [qlo, qhi) and returns otherwise.  Each XCD takes a contiguous
 // eighth of the queue (nl_xcd_items).
 //
