@@ -1075,7 +1075,10 @@ __device__ void hp_pop(float* hw, int* hi, int bs, int& n)
     n = len;
 }
 
-constexpr int NLI_SCH = 8;    // offsets whose loads the register selection issues together
+constexpr int NLI_SCH = 16;   // offsets whose loads the register selection issues together
+// the candidate flags of one run of CH offsets, one bit per offset
+template <int CH>
+using NlFlags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
 constexpr int XSCH = 8;       // the same for the exact queue (fewer live values)
 // Queues shorter than NLI_QWAVE pixels run one wave per pixel (nlinsar_smooth_wave).
 constexpr unsigned int NLI_QWAVE = 1u << 15;
@@ -1220,10 +1223,11 @@ __device__ __forceinline__ SmoothPixel nl_smooth_pixel(const SmoothArgs& a, unsi
 }
 
 // search_window_smoothing (smoothing.cpp:78-129) and compute_insar for the queued pixels,
-// one pixel per thread.  Offsets are visited in the reference's row-major order, CH at a time
-// (NLI_SCH: 8 ran 3 % faster than 4 in the heaviest iteration at 4096², at 3 waves per SIMD
-// instead of 4; 8 with the registers capped for 4 waves per SIMD ran 5 % slower, 19.7 against
-// 18.8 ms): their loads are issued together and then used in order.  Each variant
+// one pixel per thread.  Offsets are visited in the reference's row-major order, CH at a time:
+// their loads are issued together and then used in order.  In the heaviest iteration at
+// 4096²: CH = 4 at 4 waves per SIMD, 8 at 3 waves (3 % faster), 16 at 2 waves (NLI_SCH, 3 %
+// faster again: 18.2-18.4 against 18.8-18.9 ms); 24 within 0.5 % of 16 at 248 registers; 8
+// with the registers capped for 4 waves per SIMD 5 % slower than 8 at 3.  Each variant
 // serves the queue lengths in [qlo, qhi) and returns otherwise.  Each XCD takes a contiguous
 // eighth of the queue (nl_xcd_items).
 //
@@ -1244,7 +1248,7 @@ template <int CH>
 __device__ __forceinline__ void nl_smooth_image(const SmoothArgs& a, unsigned int qlo, unsigned int qhi,
                                                 const XcdItems& xs, unsigned long long& base)
 {
-    using Flags = typename std::conditional<(CH <= 8), unsigned char, unsigned int>::type;
+    using Flags = NlFlags<CH>;
     static_assert(CH <= 32, "candidate flags of a run are kept in one 32-bit word");
     extern __shared__ float4 smem_flags[];
     const unsigned int total = a.cnt[0];
@@ -2427,9 +2431,9 @@ static int launch_smoothing(const SmoothBatch& sb, int sw, int lmin, int heap_bl
 {
     const int nfull = sw * sw;
     const size_t lds = (size_t)heap_block * 2 * lmin * 4;
-    // candidate flags: one byte per run of NLI_SCH offsets; blocks are narrowed until they
-    // fit 48 KB
-    const size_t fl = (size_t)(nfull + NLI_SCH - 1) / NLI_SCH;
+    // candidate flags: per thread one NlFlags word per run of NLI_SCH offsets; blocks are
+    // narrowed until they fit 48 KB
+    const size_t fl = (size_t)(nfull + NLI_SCH - 1) / NLI_SCH * sizeof(NlFlags<NLI_SCH>);
     int fbs = 256;
     while (fbs > 64 && fbs * fl > 48 * 1024) fbs /= 2;
     // multiples of 8 blocks: each XCD takes a contiguous eighth of the queue (nl_xcd_items)
